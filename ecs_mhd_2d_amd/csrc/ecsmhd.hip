@@ -1,0 +1,573 @@
+// Host orchestration and C-ABI of the fused MHD kernels (gfx950, FP64).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include "rhs_kernels.h"
+#include "vec_kernels.h"
+#include <map>
+#include <mutex>
+#include "../../include/ecsmhd.h"
+
+using namespace ecs;
+
+static thread_local std::string g_err;
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+#define HIPCHK(x)                                                                           \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess) return fail(2, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define LAUNCHCHK() HIPCHK(hipGetLastError())
+
+constexpr int NT = 256;
+
+// Per-N launch tiles: columns per column-kernel block (TY) and rows per
+// row-kernel block (TX).  LDS per block stays at or below 64 KiB.
+template <int N> struct Tiles;
+template <> struct Tiles<16> { static constexpr int TY = 2, TX = 2; };
+template <> struct Tiles<32> { static constexpr int TY = 4, TX = 2; };
+template <> struct Tiles<64> { static constexpr int TY = 4, TX = 2; };
+template <> struct Tiles<128> { static constexpr int TY = 4, TX = 2; };
+template <> struct Tiles<256> { static constexpr int TY = 2, TX = 1; };
+
+struct ecs_ctx {
+  int n = 0, dev = 0;
+  double nu = 0, eta = 0, b0x = 0, b0y = 0;
+  double2* tw = nullptr;
+  double2* fhat = nullptr;
+  int cap = 0;  // batch capacity of the workspace
+  double2 *I = nullptr, *O = nullptr;
+  double2 *x0 = nullptr, *acc = nullptr, *v1 = nullptr, *v4 = nullptr, *v5 = nullptr, *xf = nullptr, *out = nullptr;
+  double2 *dx0 = nullptr, *dacc = nullptr, *dv1 = nullptr, *dv4 = nullptr, *dxf = nullptr, *dout = nullptr;
+  double* scal = nullptr;      // device [4][cap]: dt, ddt, T, dT
+  double* hscal = nullptr;     // pinned host staging [4][cap]
+  Hist* hist = nullptr;
+  size_t hist_cap = 0;
+  // adaptive tangent schedule
+  double* s_h = nullptr;
+  int* s_rs = nullptr;
+  int* s_rc = nullptr;
+  double* s_rh = nullptr;
+  int* s_nacc = nullptr;
+  size_t sched_cap = 0;
+  // adjoint workspace
+  double2* ckpt = nullptr;     // [nck][cap][2][R][KC]
+  size_t ckpt_cap = 0;
+  double2* tape = nullptr;     // [steps_per_segment][4 stages][cap][2][R][KC]
+  size_t tape_cap = 0;
+  double* red = nullptr;       // reductions [cap][64]
+  double* hred = nullptr;      // pinned [cap]
+  size_t compact_elems() const {
+    int kmax = (n - 1) / 3;
+    return (size_t)2 * (2 * kmax + 1) * (kmax + 1);
+  }
+};
+
+static int ensure_cap(ecs_ctx* c, int B) {
+  if (B <= c->cap) return 0;
+  int nc = std::max(B, c->cap * 2);
+  const int kmax = (c->n - 1) / 3, KC = kmax + 1;
+  const size_t comp = c->compact_elems();
+  const size_t inter = (size_t)16 * c->n * KC;
+  const size_t rowo = (size_t)4 * c->n * KC;
+  double2** arrs[] = {&c->x0, &c->acc, &c->v1, &c->v4, &c->v5, &c->xf, &c->out,
+                      &c->dx0, &c->dacc, &c->dv1, &c->dv4, &c->dxf, &c->dout};
+  for (auto p : arrs) {
+    if (*p) HIPCHK(hipFree(*p));
+    HIPCHK(hipMalloc((void**)p, comp * nc * sizeof(double2)));
+    HIPCHK(hipMemset(*p, 0, comp * nc * sizeof(double2)));
+  }
+  if (c->I) HIPCHK(hipFree(c->I));
+  if (c->O) HIPCHK(hipFree(c->O));
+  HIPCHK(hipMalloc((void**)&c->I, inter * nc * sizeof(double2)));
+  HIPCHK(hipMalloc((void**)&c->O, rowo * nc * sizeof(double2)));
+  if (c->scal) HIPCHK(hipFree(c->scal));
+  if (c->hscal) HIPCHK(hipHostFree(c->hscal));
+  HIPCHK(hipMalloc((void**)&c->scal, 4 * nc * sizeof(double)));
+  HIPCHK(hipHostMalloc((void**)&c->hscal, 4 * nc * sizeof(double), hipHostMallocDefault));
+  if (c->red) HIPCHK(hipFree(c->red));
+  if (c->hred) HIPCHK(hipHostFree(c->hred));
+  HIPCHK(hipMalloc((void**)&c->red, (size_t)64 * nc * sizeof(double)));
+  HIPCHK(hipHostMalloc((void**)&c->hred, (size_t)64 * nc * sizeof(double), hipHostMallocDefault));
+  c->cap = nc;
+  return 0;
+}
+
+static Args base_args(ecs_ctx* c) {
+  Args a;
+  memset(&a, 0, sizeof(a));
+  a.tw = c->tw;
+  a.fhat = c->fhat;
+  a.nu[0] = c->nu;
+  a.nu[1] = c->eta;
+  a.b0x = c->b0x;
+  a.b0y = c->b0y;
+  a.rowout = c->O;
+  a.inter = c->I;
+  a.x0 = c->x0; a.acc = c->acc; a.v1 = c->v1; a.v4 = c->v4; a.v5 = c->v5; a.xf = c->xf; a.out = c->out;
+  a.dx0 = c->dx0; a.dacc = c->dacc; a.dv1 = c->dv1; a.dv4 = c->dv4; a.dxf = c->dxf; a.dout = c->dout;
+  const int cap = c->cap;
+  a.dt = c->scal;
+  a.ddt = c->scal + cap;
+  a.Tb = c->scal + 2 * cap;
+  a.dTb = c->scal + 3 * cap;
+  a.hist = c->hist;
+  a.masked_e = 1;
+  return a;
+}
+
+// ------------------------------------------------------------------ launches
+template <int N, int MODE, bool TAN>
+static int col(const Args& a, int B, hipStream_t st) {
+  constexpr int TY = Tiles<N>::TY;
+  dim3 grid((Geo<N>::KC + TY - 1) / TY, 2, B);
+  hipLaunchKernelGGL((col_kernel<N, TY, NT, MODE, TAN>), grid, dim3(NT), 0, st, a);
+  LAUNCHCHK();
+  return 0;
+}
+template <int N, int RMODE, bool TAN>
+static int row(const Args& a, int B, hipStream_t st) {
+  constexpr int TX = Tiles<N>::TX;
+  dim3 grid(N / TX, 1, B);
+  hipLaunchKernelGGL((row_kernel<N, TX, NT, RMODE, TAN>), grid, dim3(NT), 0, st, a);
+  LAUNCHCHK();
+  return 0;
+}
+template <int N>
+static int pack(const double* full, double2* comp, int B, hipStream_t st) {
+  const size_t tot = (size_t)B * 2 * Geo<N>::R * Geo<N>::KC;
+  int blocks = (int)std::min<size_t>((tot + 255) / 256, 4096);
+  hipLaunchKernelGGL((pack_kernel<N>), dim3(blocks), dim3(256), 0, st, (const double2*)full, comp, B);
+  LAUNCHCHK();
+  return 0;
+}
+template <int N>
+static int unpack(const double2* comp, double* full, int B, int zero_outside, hipStream_t st) {
+  const size_t tot = zero_outside ? (size_t)B * 2 * N * Geo<N>::NH : (size_t)B * 2 * Geo<N>::R * Geo<N>::KC;
+  int blocks = (int)std::min<size_t>((tot + 255) / 256, 4096);
+  hipLaunchKernelGGL((unpack_kernel<N>), dim3(blocks), dim3(256), 0, st, comp, (double2*)full, B, zero_outside);
+  LAUNCHCHK();
+  return 0;
+}
+
+#define TRY(x)              \
+  do {                      \
+    int r_ = (x);           \
+    if (r_) return r_;      \
+  } while (0)
+
+static int upload_scalars(ecs_ctx* c, int slot, const double* vals, int B, hipStream_t st) {
+  double* h = c->hscal + (size_t)slot * c->cap;
+  for (int b = 0; b < B; ++b) h[b] = vals[b];
+  HIPCHK(hipMemcpyAsync(c->scal + (size_t)slot * c->cap, h, B * sizeof(double), hipMemcpyHostToDevice, st));
+  return 0;
+}
+
+// ------------------------------------------------------------------ state_vel
+template <int N>
+static int run_state_vel(ecs_ctx* c, const double* in, const double* din, double* out, double* dout, int B,
+                         int diss, hipStream_t st) {
+  TRY(ensure_cap(c, B));
+  Args a = base_args(c);
+  a.include_diss = diss;
+  TRY(pack<N>(in, c->x0, B, st));
+  if (din) {
+    TRY(pack<N>(din, c->dx0, B, st));
+    TRY((col<N, C_LOAD, true>(a, B, st)));
+    TRY((row<N, R_RHS, true>(a, B, st)));
+    TRY((col<N, C_RHS, true>(a, B, st)));
+    TRY(unpack<N>(c->dout, dout, B, 1, st));
+  } else {
+    TRY((col<N, C_LOAD, false>(a, B, st)));
+    TRY((row<N, R_RHS, false>(a, B, st)));
+    TRY((col<N, C_RHS, false>(a, B, st)));
+  }
+  TRY(unpack<N>(c->out, out, B, 1, st));
+  return 0;
+}
+
+// ------------------------------------------------------------------ Lawson RK4
+template <int N, bool TAN>
+static int run_rk4(ecs_ctx* c, double* f, double* df, int B, const double* dt, const double* ddt, int steps,
+                   int masked, hipStream_t st) {
+  TRY(ensure_cap(c, B));
+  Args a = base_args(c);
+  a.masked_e = masked;
+  TRY(upload_scalars(c, 0, dt, B, st));
+  if (TAN) TRY(upload_scalars(c, 1, ddt, B, st));
+  TRY(pack<N>(f, c->x0, B, st));
+  if (TAN) TRY(pack<N>(df, c->dx0, B, st));
+  if (steps > 0) {
+    TRY((col<N, C_LOAD, TAN>(a, B, st)));
+    for (int s = 0; s < steps; ++s) {
+      TRY((row<N, R_RHS, TAN>(a, B, st)));
+      TRY((col<N, C_S1F, TAN>(a, B, st)));
+      TRY((row<N, R_RHS, TAN>(a, B, st)));
+      TRY((col<N, C_S2, TAN>(a, B, st)));
+      TRY((row<N, R_RHS, TAN>(a, B, st)));
+      TRY((col<N, C_S3, TAN>(a, B, st)));
+      TRY((row<N, R_RHS, TAN>(a, B, st)));
+      Args a4 = a;
+      a4.skip_inv = (s == steps - 1);
+      TRY((col<N, C_S4, TAN>(a4, B, st)));
+    }
+  }
+  // unmasked e keeps the modes outside the 2/3 region unchanged (L = 0 there)
+  TRY(unpack<N>(c->x0, f, B, masked ? 1 : 0, st));
+  if (TAN) TRY(unpack<N>(c->dx0, df, B, masked ? 1 : 0, st));
+  return 0;
+}
+
+// ------------------------------------------------------------------ Lawson RK4(3)
+static int ensure_hist(ecs_ctx* c, size_t need) {
+  if (need <= c->hist_cap) return 0;
+  if (c->hist) HIPCHK(hipFree(c->hist));
+  HIPCHK(hipMalloc((void**)&c->hist, need * sizeof(Hist)));
+  c->hist_cap = need;
+  return 0;
+}
+
+template <int N>
+static int run_rk43(ecs_ctx* c, double* x, int B, const double* T, double h0, double atol, int maxs, int nck,
+                    ecs_rk43_info* info, double* hs, int8_t* accf, hipStream_t st) {
+  TRY(ensure_cap(c, B));
+  const int budget = maxs * nck;
+  const int hstride = budget + 1;
+  TRY(ensure_hist(c, (size_t)B * hstride));
+  HIPCHK(hipMemsetAsync(c->hist, 0, (size_t)B * hstride * sizeof(Hist), st));
+  TRY(upload_scalars(c, 2, T, B, st));
+  Args a = base_args(c);
+  a.hist = c->hist;
+  a.hstride = hstride;
+  a.atol = atol;
+  a.h0 = h0;
+  TRY(pack<N>(x, c->x0, B, st));
+  // k1 = t v(x) (timestepping.py:255)
+  Args al = a;
+  al.adaptive = AD_FIXED;
+  TRY((col<N, C_LOAD, false>(al, B, st)));
+  TRY((row<N, R_RHS, false>(al, B, st)));
+  a.adaptive = AD_PRIMAL;
+  std::vector<Hist> hh(B);
+  int launched = 0;
+  const int chunk = 32;
+  while (launched < budget) {
+    const int end = std::min(budget, launched + chunk);
+    for (int i = launched; i < end; ++i) {
+      Args ai = a;
+      ai.attempt = i;
+      if (i == 0) TRY((col<N, C_A0, false>(ai, B, st)));
+      else TRY((col<N, C_A, false>(ai, B, st)));
+      TRY((row<N, R_RHS, false>(ai, B, st)));
+      TRY((col<N, C_S2, false>(ai, B, st)));
+      TRY((row<N, R_RHS, false>(ai, B, st)));
+      TRY((col<N, C_S3, false>(ai, B, st)));
+      TRY((row<N, R_RHS, false>(ai, B, st)));
+      TRY((col<N, C_S4, false>(ai, B, st)));
+      TRY((row<N, R_RHS, false>(ai, B, st)));
+      TRY((col<N, C_S5, false>(ai, B, st)));
+      TRY((row<N, R_ERR, false>(ai, B, st)));
+    }
+    launched = end;
+    // poll: stop once every candidate's last attempt was a no-op
+    bool any = false;
+    for (int b = 0; b < B; ++b) {
+      HIPCHK(hipMemcpyAsync(&hh[b], c->hist + (size_t)b * hstride + (launched - 1), sizeof(Hist),
+                            hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    for (int b = 0; b < B; ++b) any = any || hh[b].active;
+    if (!any) break;
+  }
+  Args af = a;
+  af.attempt = launched;
+  TRY((col<N, C_FIN, false>(af, B, st)));
+  TRY(unpack<N>(c->x0, x, B, 0, st));
+  std::vector<Hist> all((size_t)B * hstride);
+  HIPCHK(hipMemcpyAsync(all.data(), c->hist, (size_t)B * hstride * sizeof(Hist), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int b = 0; b < B; ++b) {
+    const Hist* H = all.data() + (size_t)b * hstride;
+    const Hist& fin = H[launched];
+    if (info) {
+      info[b].s = fin.s;
+      info[b].completed = fabs(fin.s - 1.0) < 1e-10;
+      info[b].accepted = fin.nacc;
+      info[b].rejected = fin.nrej;
+      info[b].attempts = fin.nacc + fin.nrej;
+      info[b].fevals = 1 + 4 * (fin.nacc + fin.nrej);
+      info[b].pad = 0;
+    }
+    for (int i = 0; i < budget; ++i) {
+      const bool act = i < launched && H[i].active;
+      if (hs) hs[(size_t)b * budget + i] = act ? H[i].h : 0.0;
+      if (accf) accf[(size_t)b * budget + i] = (act && H[i + 1].prev_acc) ? 1 : 0;
+    }
+  }
+  return 0;
+}
+
+template <int N>
+static int run_rk43_jvp(ecs_ctx* c, double* x, double* dx, int B, const double* T, const double* dT,
+                        const double* hs, const int8_t* accf, const int* attempts, int budget, hipStream_t st) {
+  TRY(ensure_cap(c, B));
+  // build the accepted-step schedule with the rejections preceding each step
+  std::vector<double> sh((size_t)B * budget, 0.0), srh((size_t)B * budget, 0.0);
+  std::vector<int> srs((size_t)B * budget, 0), src((size_t)B * budget, 0), snacc(B, 0);
+  int maxacc = 0;
+  for (int b = 0; b < B; ++b) {
+    int m = 0, nr = 0, rstart = 0;
+    for (int i = 0; i < attempts[b]; ++i) {
+      const double h = hs[(size_t)b * budget + i];
+      if (accf[(size_t)b * budget + i]) {
+        sh[(size_t)b * budget + m] = h;
+        srs[(size_t)b * budget + m] = rstart;
+        src[(size_t)b * budget + m] = nr - rstart;
+        rstart = nr;
+        ++m;
+      } else {
+        srh[(size_t)b * budget + nr] = h;
+        ++nr;
+      }
+    }
+    snacc[b] = m;
+    maxacc = std::max(maxacc, m);
+  }
+  const size_t need = (size_t)B * budget;
+  if (need > c->sched_cap) {
+    if (c->s_h) { HIPCHK(hipFree(c->s_h)); HIPCHK(hipFree(c->s_rh)); HIPCHK(hipFree(c->s_rs)); HIPCHK(hipFree(c->s_rc)); }
+    HIPCHK(hipMalloc((void**)&c->s_h, need * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&c->s_rh, need * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&c->s_rs, need * sizeof(int)));
+    HIPCHK(hipMalloc((void**)&c->s_rc, need * sizeof(int)));
+    c->sched_cap = need;
+  }
+  if (c->s_nacc) HIPCHK(hipFree(c->s_nacc));
+  HIPCHK(hipMalloc((void**)&c->s_nacc, B * sizeof(int)));
+  HIPCHK(hipMemcpyAsync(c->s_h, sh.data(), need * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->s_rh, srh.data(), need * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->s_rs, srs.data(), need * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->s_rc, src.data(), need * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->s_nacc, snacc.data(), B * sizeof(int), hipMemcpyHostToDevice, st));
+  TRY(upload_scalars(c, 2, T, B, st));
+  TRY(upload_scalars(c, 3, dT, B, st));
+  Args a = base_args(c);
+  a.s_h = c->s_h; a.s_rh = c->s_rh; a.s_rs = c->s_rs; a.s_rc = c->s_rc; a.s_nacc = c->s_nacc;
+  a.sstride = budget;
+  a.rstride = budget;
+  TRY(pack<N>(x, c->x0, B, st));
+  TRY(pack<N>(dx, c->dx0, B, st));
+  Args al = a;
+  al.adaptive = AD_FIXED;
+  TRY((col<N, C_LOAD, true>(al, B, st)));
+  TRY((row<N, R_RHS, true>(al, B, st)));
+  a.adaptive = AD_TANGENT;
+  for (int m = 0; m < maxacc; ++m) {
+    Args am = a;
+    am.attempt = m;
+    if (m == 0) TRY((col<N, C_A0, true>(am, B, st)));
+    else TRY((col<N, C_A, true>(am, B, st)));
+    TRY((row<N, R_RHS, true>(am, B, st)));
+    TRY((col<N, C_S2, true>(am, B, st)));
+    TRY((row<N, R_RHS, true>(am, B, st)));
+    TRY((col<N, C_S3, true>(am, B, st)));
+    TRY((row<N, R_RHS, true>(am, B, st)));
+    TRY((col<N, C_S4, true>(am, B, st)));
+    TRY((row<N, R_RHS, true>(am, B, st)));
+    TRY((col<N, C_S5, true>(am, B, st)));
+  }
+  TRY(unpack<N>(c->x0, x, B, 0, st));
+  TRY(unpack<N>(c->dx0, dx, B, 0, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+// ------------------------------------------------------------------ dispatch
+#define DISPATCH_N(n, fn, ...)                            \
+  switch (n) {                                            \
+    case 16: return fn<16>(__VA_ARGS__);                  \
+    case 32: return fn<32>(__VA_ARGS__);                  \
+    case 64: return fn<64>(__VA_ARGS__);                  \
+    case 128: return fn<128>(__VA_ARGS__);                \
+    case 256: return fn<256>(__VA_ARGS__);                \
+    default: return fail(1, "unsupported grid size n");   \
+  }
+
+template <int N> static int rk4_p(ecs_ctx* c, double* f, int B, const double* dt, int steps, int masked, hipStream_t st) {
+  return run_rk4<N, false>(c, f, nullptr, B, dt, nullptr, steps, masked, st);
+}
+template <int N> static int rk4_t(ecs_ctx* c, double* f, double* df, int B, const double* dt, const double* ddt, int steps, int masked, hipStream_t st) {
+  return run_rk4<N, true>(c, f, df, B, dt, ddt, steps, masked, st);
+}
+
+extern "C" {
+
+const char* ecs_last_error(void) { return g_err.c_str(); }
+
+int ecs_ctx_create(ecs_ctx** out, int n, double nu, double eta, double b0x, double b0y, const double* fhat_full,
+                   int device) {
+  if (!out) return fail(1, "null ctx pointer");
+  if (n != 16 && n != 32 && n != 64 && n != 128 && n != 256) return fail(1, "unsupported grid size n");
+  HIPCHK(hipSetDevice(device));
+  ecs_ctx* c = new ecs_ctx();
+  c->n = n; c->dev = device; c->nu = nu; c->eta = eta; c->b0x = b0x; c->b0y = b0y;
+  std::vector<double2> tw(n);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int j = 0; j < n; ++j) {
+    const long double ang = -2.0L * pi * (long double)j / (long double)n;
+    tw[j] = make_double2((double)cosl(ang), (double)sinl(ang));
+  }
+  HIPCHK(hipMalloc((void**)&c->tw, n * sizeof(double2)));
+  HIPCHK(hipMemcpy(c->tw, tw.data(), n * sizeof(double2), hipMemcpyHostToDevice));
+  const int kmax = (n - 1) / 3, KC = kmax + 1, R = 2 * kmax + 1, NH = n / 2 + 1;
+  std::vector<double2> fc((size_t)R * KC, make_double2(0.0, 0.0));
+  if (fhat_full) {
+    for (int r = 0; r < R; ++r) {
+      const int x = r <= kmax ? r : r - R + n;
+      for (int ky = 0; ky < KC; ++ky) {
+        if (r == 0 && ky == 0) continue;  // mask(0,0) = False
+        fc[(size_t)r * KC + ky] = make_double2(fhat_full[2 * ((size_t)x * NH + ky)], fhat_full[2 * ((size_t)x * NH + ky) + 1]);
+      }
+    }
+  }
+  HIPCHK(hipMalloc((void**)&c->fhat, fc.size() * sizeof(double2)));
+  HIPCHK(hipMemcpy(c->fhat, fc.data(), fc.size() * sizeof(double2), hipMemcpyHostToDevice));
+  *out = c;
+  return 0;
+}
+
+int ecs_ctx_destroy(ecs_ctx* c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->dev);
+  void* ps[] = {c->tw, c->fhat, c->I, c->O, c->x0, c->acc, c->v1, c->v4, c->v5, c->xf, c->out, c->dx0, c->dacc,
+                c->dv1, c->dv4, c->dxf, c->dout, c->scal, c->hist, c->s_h, c->s_rs, c->s_rc, c->s_rh, c->s_nacc,
+                c->ckpt, c->tape, c->red};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->hred) (void)hipHostFree(c->hred);
+  delete c;
+  return 0;
+}
+
+int ecs_ctx_n(const ecs_ctx* c) { return c ? c->n : 0; }
+
+int ecs_state_vel(ecs_ctx* c, const double* in, double* out, int batch, int diss, void* stream) {
+  if (!c || !in || !out || batch < 1) return fail(1, "ecs_state_vel: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  DISPATCH_N(c->n, run_state_vel, c, in, nullptr, out, nullptr, batch, diss, st);
+}
+
+int ecs_state_vel_jvp(ecs_ctx* c, const double* in, const double* din, double* out, double* dout, int batch,
+                      int diss, void* stream) {
+  if (!c || !in || !din || !out || !dout || batch < 1) return fail(1, "ecs_state_vel_jvp: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  DISPATCH_N(c->n, run_state_vel, c, in, din, out, dout, batch, diss, st);
+}
+
+int ecs_lawson_rk4(ecs_ctx* c, double* f, int batch, const double* dt, int steps, int masked, void* stream) {
+  if (!c || !f || !dt || batch < 1 || steps < 0) return fail(1, "ecs_lawson_rk4: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+DISPATCH_N(c->n, rk4_p, c, f, batch, dt, steps, masked, st);
+
+}
+
+int ecs_lawson_rk4_jvp(ecs_ctx* c, double* f, double* df, int batch, const double* dt, const double* ddt, int steps,
+                       int masked, void* stream) {
+  if (!c || !f || !df || !dt || !ddt || batch < 1 || steps < 0) return fail(1, "ecs_lawson_rk4_jvp: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+DISPATCH_N(c->n, rk4_t, c, f, df, batch, dt, ddt, steps, masked, st);
+
+}
+
+int ecs_lawson_rk43(ecs_ctx* c, double* x, int batch, const double* T, double h0, double atol, int maxs, int nck,
+                    ecs_rk43_info* info, double* hs, int8_t* acc, void* stream) {
+  if (!c || !x || !T || batch < 1 || maxs < 1 || nck < 1) return fail(1, "ecs_lawson_rk43: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  DISPATCH_N(c->n, run_rk43, c, x, batch, T, h0, atol, maxs, nck, info, hs, acc, st);
+}
+
+int ecs_lawson_rk43_jvp(ecs_ctx* c, double* x, double* dx, int batch, const double* T, const double* dT,
+                        const double* hs, const int8_t* acc, const int* attempts, int budget, void* stream) {
+  if (!c || !x || !dx || !T || !dT || !hs || !acc || !attempts || batch < 1 || budget < 1)
+    return fail(1, "ecs_lawson_rk43_jvp: bad arguments");
+  for (int b = 0; b < batch; ++b)
+    if (attempts[b] < 0 || attempts[b] > budget) return fail(1, "ecs_lawson_rk43_jvp: attempts out of range");
+  hipStream_t st = (hipStream_t)stream;
+  DISPATCH_N(c->n, run_rk43_jvp, c, x, dx, batch, T, dT, hs, acc, attempts, budget, st);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ Krylov vectors
+static std::mutex g_part_mu;
+static std::map<int, double*> g_part;  // per device: 2 * VMAXG partial sums
+
+static int partials(double** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_part_mu);
+  auto it = g_part.find(dev);
+  if (it == g_part.end()) {
+    double* p = nullptr;
+    HIPCHK(hipMalloc((void**)&p, 2 * VMAXG * sizeof(double)));
+    it = g_part.emplace(dev, p).first;
+  }
+  *out = it->second;
+  return 0;
+}
+
+static int red_blocks(int n) { return std::max(1, std::min(VMAXG, (n + 4 * VNT - 1) / (4 * VNT))); }
+
+extern "C" int ecs_dot(const double* a, const double* b, int n, double* out, void* stream) {
+  if (!a || !b || !out || n < 1) return fail(1, "ecs_dot: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  double* part = nullptr;
+  TRY(partials(&part));
+  const int G = red_blocks(n);
+  hipLaunchKernelGGL(dot_partial_kernel, dim3(G), dim3(VNT), 0, st, a, b, n, part);
+  LAUNCHCHK();
+  hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(64), 0, st, part, G, out);
+  LAUNCHCHK();
+  return 0;
+}
+
+extern "C" int ecs_mgs(double* Q, int ldq, int n, int k, double* w, double* h, void* stream) {
+  if (!Q || !w || !h || n < 1 || k < 0 || ldq < n) return fail(1, "ecs_mgs: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  double* part = nullptr;
+  TRY(partials(&part));
+  const int G = red_blocks(n);
+  hipLaunchKernelGGL(dot_partial_kernel, dim3(G), dim3(VNT), 0, st, w, Q, n, part);
+  LAUNCHCHK();
+  for (int j = 0; j <= k; ++j) {
+    double* pin = part + (j & 1) * VMAXG;
+    double* pout = part + ((j + 1) & 1) * VMAXG;
+    hipLaunchKernelGGL(mgs_update_kernel, dim3(G), dim3(VNT), 0, st, w, Q, ldq, n, j, k, h, pin, pout);
+    LAUNCHCHK();
+  }
+  hipLaunchKernelGGL(mgs_finish_kernel, dim3(G), dim3(VNT), 0, st, w, Q, ldq, n, k, h,
+                     part + ((k + 1) & 1) * VMAXG);
+  LAUNCHCHK();
+  return 0;
+}
+
+extern "C" int ecs_state_vel_vjp(ecs_ctx* c, const double* in, const double* cot, double* grad, int batch,
+                                 int diss, void* stream) {
+  (void)c; (void)in; (void)cot; (void)grad; (void)batch; (void)diss; (void)stream;
+  return fail(3, "ecs_state_vel_vjp: adjoint kernels not built in this version");
+}
+
+extern "C" int ecs_lawson_rk4_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar, int batch,
+                                  const double* dt, int steps, int masked, int ckpt, void* stream) {
+  (void)c; (void)f0; (void)fbar; (void)hbar; (void)batch; (void)dt; (void)steps; (void)masked; (void)ckpt;
+  (void)stream;
+  return fail(3, "ecs_lawson_rk4_vjp: adjoint kernels not built in this version");
+}

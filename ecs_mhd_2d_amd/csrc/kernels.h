@@ -1,0 +1,114 @@
+// Fused pseudo-spectral MHD kernels for gfx950 (FP64).
+//
+// Data layouts (all double2 = complex128, row-major):
+//   compact spectral state  S[b][c][r][ky]   c in {omega, j}, r in [0,R), ky in [0,KC)
+//       kept rows r <= KMAX hold kx = r, rows r > KMAX hold kx = r - R (negative);
+//       KMAX = (N-1)/3, R = 2 KMAX + 1, KC = KMAX + 1.  These are exactly the
+//       modes of the 2/3 mask |kx|,|ky| < N/3 (mhd_jax.py:65-70); every
+//       derivative and the RHS vanish outside them (kx, ky, to_u, to_v are
+//       pre-masked, mhd_jax.py:87-90).
+//   column-pass output I[b][g][x][ky]  x in [0,N): 8 derived fields (16 with a
+//       tangent) after the inverse x-FFT, g = set*8 + 4c + d with
+//       d = 0: i kx f, 1: i ky f, 2: i to_u f, 3: i to_v f.
+//   row-pass output    O[b][o][x][ky]  o = set*2 + c: the y-r2c of the two
+//       real-space nonlinear terms (omega tendency, (u x B)_z).
+//
+// One RHS evaluation = one column kernel (x-FFT of the previous row output,
+// Lawson stage glue, derived spectra, inverse x-FFT) + one row kernel
+// (Hermitian c2r along y of 4 (8) packed pairs, products, r2c along y).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "fft_lds.h"
+
+namespace ecs {
+
+template <int N>
+struct Geo {
+  static constexpr int KMAX = (N - 1) / 3;
+  static constexpr int KC = KMAX + 1;
+  static constexpr int R = 2 * KMAX + 1;
+  static constexpr int NH = N / 2 + 1;
+  __device__ __host__ static constexpr int row_kx(int r) { return r <= KMAX ? r : r - R; }
+  __device__ __host__ static constexpr int row_x(int r) { return r <= KMAX ? r : r - R + N; }
+};
+
+// Per-attempt record of the adaptive controller, one per candidate and attempt.
+// Entry i describes attempt i; entry i+1 also records whether attempt i was
+// accepted (prev_acc), because that decision is taken by the first kernel of
+// attempt i+1.
+struct Hist {
+  double s;                 // pseudo-time at the start of the attempt
+  double h;                 // step tried by the attempt
+  unsigned long long err;   // bit pattern of max|err| (non-negative double, atomicMax)
+  int active;               // attempt performed (integration not yet complete)
+  int prev_acc;             // attempt i-1 was accepted
+  int nacc, nrej;           // counters before this attempt
+};
+
+// Column-kernel modes.
+enum ColMode : int {
+  C_LOAD = 0,   // y = x0
+  C_RHS = 1,    // out = v(x0) [+ L x0]; no inverse pass
+  C_S1F = 2,    // fixed step: v1 = v(x0) from the row output; stage-1 glue -> y2
+  C_S2 = 3,     // k2 -> y3
+  C_S3 = 4,     // k3 -> y4
+  C_S4 = 5,     // k4 -> xf (fixed step: x0 = xf) -> y = xf
+  C_S5 = 6,     // adaptive: k5, err spectrum (tangent replay: accept xf)
+  C_A0 = 7,     // adaptive, first attempt: v1 = v(x0) from the row output; stage-1 glue
+  C_A = 8,      // adaptive: apply previous decision, new h, stage-1 glue from stored v1
+  C_FIN = 9,    // adaptive: apply the last decision only
+};
+
+// Row-kernel modes.
+enum RowMode : int { R_RHS = 0, R_ERR = 1 };
+
+// adaptive flag values
+enum AdaptKind : int { AD_FIXED = 0, AD_PRIMAL = 1, AD_TANGENT = 2 };
+
+struct Args {
+  const double2* tw;       // twiddles [N]
+  const double2* fhat;     // masked forcing spectrum, compact [R][KC]
+  double nu[2];
+  double b0x, b0y;
+  double2* rowout;         // O
+  double2* inter;          // I
+  // compact state [B][2][R][KC]
+  double2* x0;
+  double2* acc;
+  double2* v1;
+  double2* v4;
+  double2* v5;
+  double2* xf;
+  double2* out;            // C_RHS output
+  // tangents
+  double2* dx0;
+  double2* dacc;
+  double2* dv1;
+  double2* dv4;
+  double2* dxf;
+  double2* dout;
+  // per-candidate scalars
+  const double* dt;        // fixed step: dt[b]; tangent: ddt[b]
+  const double* ddt;
+  const double* Tb;        // adaptive: T[b]
+  const double* dTb;       // adaptive tangent: dT[b]
+  double atol;
+  Hist* hist;              // [B][hstride]
+  int hstride;
+  int attempt;             // adaptive attempt index (primal) / accepted step (tangent)
+  // tangent replay schedule of the adaptive stepper, [B][sstride]
+  const double* s_h;       // step of the accepted attempt
+  const int* s_rs;         // start of its preceding rejections in s_rh
+  const int* s_rc;         // number of preceding rejections
+  const double* s_rh;      // rejected steps, [B][rstride]
+  const int* s_nacc;       // number of accepted attempts per candidate
+  int sstride, rstride;
+  double h0;               // adaptive: first step (pseudo-time)
+  int include_diss;
+  int adaptive;            // AdaptKind
+  int masked_e;            // fixed step: e = exp(dt/2 L) * mask
+  int skip_inv;            // 1: do not produce the inverse pass (last stage of a run)
+};
+
+}  // namespace ecs

@@ -1,0 +1,497 @@
+// Column and row kernels of the fused RHS / Lawson stepper (see kernels.h).
+#pragma once
+#include "kernels.h"
+
+namespace ecs {
+
+// Block-uniform control values.
+struct Blk {
+  bool run;      // block has any work
+  bool stage;    // stage glue runs and produces a next-stage input
+  bool pend;     // adaptive: a previous attempt awaits its decision
+  bool pacc;     // adaptive: that attempt is accepted
+  double hprev;  // adaptive: step of that attempt
+  double T, h;   // adaptive: period and pseudo-time step
+  double dt;     // stage scale (physical time step)
+  double ddt;    // tangent of dt per unit tangent
+};
+
+__device__ __forceinline__ double relu_nan(double x) { return (x != x) ? x : (x > 0.0 ? x : 0.0); }
+
+// Controller decision of timestepping.py:288-313, from the record of attempt i-1.
+template <int MODE, bool TAN>
+__device__ __forceinline__ Blk prologue(const Args& a, int b) {
+  Blk bk;
+  bk.run = true; bk.stage = true; bk.pend = false; bk.pacc = false;
+  bk.hprev = 0.0; bk.T = 0.0; bk.h = 0.0; bk.dt = 0.0; bk.ddt = 0.0;
+  if (a.adaptive == AD_FIXED) {
+    bk.dt = a.dt ? a.dt[b] : 0.0;
+    bk.ddt = (TAN && a.ddt) ? a.ddt[b] : 0.0;
+    return bk;
+  }
+  bk.T = a.Tb[b];
+  if (a.adaptive == AD_TANGENT) {
+    const int m = a.attempt;
+    bk.run = bk.stage = m < a.s_nacc[b];
+    if (!bk.run) return bk;
+    bk.h = a.s_h[(size_t)b * a.sstride + m];
+    bk.dt = bk.h * bk.T;
+    bk.ddt = bk.h * a.dTb[b];
+    return bk;
+  }
+  // AD_PRIMAL
+  const int i = a.attempt;
+  Hist* H = a.hist + (size_t)b * a.hstride;
+  if (MODE == C_A0) {
+    bk.h = a.h0;
+    bk.dt = bk.h * bk.T;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+      Hist w; w.s = 0.0; w.h = a.h0; w.err = 0ull; w.active = 1; w.prev_acc = 0; w.nacc = 0; w.nrej = 0;
+      H[0] = w;
+    }
+    return bk;
+  }
+  if (MODE == C_A || MODE == C_FIN) {
+    const Hist hp = H[i - 1];
+    double s = hp.s, hn = hp.h;
+    int nacc = hp.nacc, nrej = hp.nrej;
+    bk.pend = hp.active != 0;
+    bk.hprev = hp.h;
+    if (bk.pend) {
+      const double err = __longlong_as_double((long long)hp.err);
+      bk.pacc = err < a.atol;
+      if (bk.pacc) { s = hp.s + hp.h; nacc += 1; } else { nrej += 1; }
+      if (err != err) hn = hp.h / 2;
+      else hn = hp.h * 0.9 * pow(a.atol / err, 0.25);
+      hn = hn - relu_nan(s + hn - 1.0);
+    }
+    const bool act = (MODE == C_A) && !(fabs(s - 1.0) < 1e-10);
+    bk.stage = act;
+    bk.run = bk.pend || act;
+    bk.h = hn;
+    bk.dt = hn * bk.T;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+      Hist w; w.s = s; w.h = hn; w.err = 0ull; w.active = act ? 1 : 0; w.prev_acc = bk.pacc ? 1 : 0;
+      w.nacc = nacc; w.nrej = nrej;
+      H[i] = w;
+    }
+    return bk;
+  }
+  const Hist hc = H[i];
+  bk.run = bk.stage = hc.active != 0;
+  bk.h = hc.h;
+  bk.dt = bk.h * bk.T;
+  return bk;
+}
+
+__device__ __forceinline__ bool row_active(const Args& a, int b) {
+  if (a.adaptive == AD_PRIMAL) return a.hist[(size_t)b * a.hstride + a.attempt].active != 0;
+  if (a.adaptive == AD_TANGENT) return a.attempt < a.s_nacc[b];
+  return true;
+}
+
+__device__ __forceinline__ double2 cdivs(double2 a, double s) { return make_double2(a.x / s, a.y / s); }
+
+// Stage glue for one kept mode.  rh/drh: row-pass spectrum after the x-FFT
+// (nonlinear term before masking).  Produces the next stage input y (dy).
+// Lawson RK4 in the form of timestepping.py:70-80 and :263-282:
+//   y2 = e x0 + e k1/2, y3 = e x0 + k2/2, y4 = e e x0 + e k3,
+//   xf = e e x0 + e e k1/6 + e k2/3 + e k3/3 + k4/6,   k_s = dt v(y_s).
+template <int N, int MODE, bool TAN>
+__device__ __forceinline__ void glue(const Args& a, const Blk& bk, int bb, size_t ci, int r, int c, int kx, int ky,
+                                     double2 rh, double2 drh, double2& y, double2& dy) {
+  using G = Geo<N>;
+  const double k2 = (double)(kx * kx + ky * ky);
+  const bool dc = (kx == 0 && ky == 0);
+  // mask (omega) and k^2 mask (j) of mhd_jax.py:160,164; forcing added spectrally
+  const double m = dc ? 0.0 : (c == 0 ? 1.0 : k2);
+  const double L = -k2 * a.nu[c];
+  double2 F = make_double2(0.0, 0.0);
+  if (c == 0) F = a.fhat[r * G::KC + ky];
+  const double2 vh = make_double2(m * rh.x + F.x, m * rh.y + F.y);
+  const double2 dvh = cscale(m, drh);
+  y = make_double2(0.0, 0.0);
+  dy = y;
+  if (MODE == C_LOAD) {
+    y = a.x0[ci];
+    if (TAN) dy = a.dx0[ci];
+    return;
+  }
+  if (MODE == C_RHS) {
+    double2 o = vh;
+    if (a.include_diss) o = cadd(o, cscale(L, a.x0[ci]));
+    a.out[ci] = o;
+    if (TAN) {
+      double2 d = dvh;
+      if (a.include_diss) d = cadd(d, cscale(L, a.dx0[ci]));
+      a.dout[ci] = d;
+    }
+    return;
+  }
+  // v1 of the stage-1 glue
+  double2 v1 = make_double2(0.0, 0.0), dv1 = v1;
+  if (MODE == C_S1F) {
+    v1 = vh;
+    dv1 = dvh;
+  }
+  if (MODE == C_A0 || MODE == C_A || MODE == C_FIN) {
+    const double TL = bk.T * L;
+    if (a.adaptive == AD_PRIMAL) {
+      if (MODE == C_A0) {
+        v1 = vh;
+        a.v1[ci] = v1;
+      } else {
+        v1 = a.v1[ci];
+        if (bk.pend) {
+          if (bk.pacc) {
+            a.x0[ci] = a.xf[ci];
+            v1 = a.v5[ci];
+          } else {
+            // rejected: carried k1 is e (e k1) (timestepping.py:267,276,297)
+            const double ep = exp((TL * bk.hprev) * 0.5);
+            v1 = cscale(ep, cscale(ep, v1));
+          }
+          a.v1[ci] = v1;
+        }
+      }
+    } else {  // tangent replay: rejections preceding this accepted attempt
+      if (MODE == C_A0) {
+        v1 = vh;
+        dv1 = dvh;
+      } else {
+        v1 = a.v1[ci];
+        dv1 = a.dv1[ci];
+      }
+      const size_t so = (size_t)bb * a.sstride + a.attempt;
+      const int rs = a.s_rs[so], rc = a.s_rc[so];
+      const double dT = a.dTb[bb];
+      for (int q = 0; q < rc; ++q) {
+        const double hj = a.s_rh[(size_t)bb * a.rstride + rs + q];
+        const double ep = exp((TL * hj) * 0.5);
+        v1 = cscale(ep, cscale(ep, v1));
+        dv1 = cadd(cscale(ep, cscale(ep, dv1)), cscale(L * (hj * dT), v1));
+      }
+      a.v1[ci] = v1;
+      a.dv1[ci] = dv1;
+    }
+    if (!bk.stage) return;
+  }
+  // Lawson factor
+  double e;
+  if (a.adaptive == AD_FIXED) {
+    e = exp((bk.dt * 0.5) * L);
+    if (a.masked_e && dc) e = 0.0;
+  } else {
+    e = exp(((bk.T * L) * bk.h) * 0.5);
+  }
+  const double de = TAN ? (bk.ddt * 0.5) * L * e : 0.0;
+  const double de2 = 2.0 * e * de;
+  const double2 x0 = a.x0[ci];
+  const double2 dx0 = TAN ? a.dx0[ci] : make_double2(0.0, 0.0);
+  if (MODE == C_S1F || MODE == C_A0 || MODE == C_A) {
+    const double2 k1 = cscale(bk.dt, v1);
+    const double2 ek1 = cscale(e, k1);
+    const double2 ex0 = cscale(e, x0);
+    a.acc[ci] = cadd(cscale(e, ex0), cdivs(cscale(e, ek1), 6.0));
+    y = cadd(ex0, cdivs(ek1, 2.0));
+    if (TAN) {
+      const double2 dk1 = cadd(cscale(bk.ddt, v1), cscale(bk.dt, dv1));
+      a.dacc[ci] = cadd(cadd(cscale(de2, x0), cscale(e * e, dx0)),
+                        cdivs(cadd(cscale(de2, k1), cscale(e * e, dk1)), 6.0));
+      dy = cadd(cadd(cscale(de, x0), cscale(e, dx0)), cdivs(cadd(cscale(de, k1), cscale(e, dk1)), 2.0));
+    }
+    return;
+  }
+  const double2 ks = cscale(bk.dt, vh);
+  const double2 dks = TAN ? cadd(cscale(bk.ddt, vh), cscale(bk.dt, dvh)) : make_double2(0.0, 0.0);
+  if (MODE == C_S2) {
+    a.acc[ci] = cadd(a.acc[ci], cdivs(cscale(e, ks), 3.0));
+    y = cadd(cscale(e, x0), cdivs(ks, 2.0));
+    if (TAN) {
+      a.dacc[ci] = cadd(a.dacc[ci], cdivs(cadd(cscale(de, ks), cscale(e, dks)), 3.0));
+      dy = cadd(cadd(cscale(de, x0), cscale(e, dx0)), cdivs(dks, 2.0));
+    }
+    return;
+  }
+  if (MODE == C_S3) {
+    a.acc[ci] = cadd(a.acc[ci], cdivs(cscale(e, ks), 3.0));
+    y = cadd(cscale(e, cscale(e, x0)), cscale(e, ks));
+    if (TAN) {
+      a.dacc[ci] = cadd(a.dacc[ci], cdivs(cadd(cscale(de, ks), cscale(e, dks)), 3.0));
+      dy = cadd(cadd(cscale(de2, x0), cscale(e * e, dx0)), cadd(cscale(de, ks), cscale(e, dks)));
+    }
+    return;
+  }
+  if (MODE == C_S4) {
+    const double2 xf = cadd(a.acc[ci], cdivs(ks, 6.0));
+    y = xf;
+    double2 dxf = make_double2(0.0, 0.0);
+    if (TAN) {
+      dxf = cadd(a.dacc[ci], cdivs(dks, 6.0));
+      dy = dxf;
+    }
+    if (a.adaptive == AD_FIXED) {
+      a.x0[ci] = xf;
+      if (TAN) a.dx0[ci] = dxf;
+    } else {
+      a.xf[ci] = xf;
+      if (TAN) a.dxf[ci] = dxf;
+      else a.v4[ci] = vh;
+    }
+    return;
+  }
+  if (MODE == C_S5) {
+    if (TAN) {  // replay: this attempt is accepted; k1 <- k5, x0 <- xf
+      a.x0[ci] = a.xf[ci];
+      a.dx0[ci] = a.dxf[ci];
+      a.v1[ci] = vh;
+      a.dv1[ci] = dvh;
+    } else {
+      a.v5[ci] = vh;
+      // err = h (k5 - k4) / 6 (timestepping.py:288)
+      y = cdivs(cscale(bk.dt, csub(vh, a.v4[ci])), 6.0);
+    }
+    return;
+  }
+}
+
+template <int N, int TY, int NT, int MODE, bool TAN>
+__global__ __launch_bounds__(NT) void col_kernel(Args a) {
+  using G = Geo<N>;
+  constexpr int KC = G::KC, R = G::R, KMAX = G::KMAX;
+  constexpr int NF = TAN ? 2 : 1;
+  constexpr int NDO = (MODE == C_S5 && !TAN) ? 1 : 4;
+  constexpr bool FWD = MODE == C_RHS || MODE == C_S1F || MODE == C_S2 || MODE == C_S3 || MODE == C_S4 ||
+                       MODE == C_S5 || MODE == C_A0;
+  constexpr bool HASINV = !(MODE == C_RHS || MODE == C_FIN || (MODE == C_S5 && TAN));
+  constexpr int NSEQ = HASINV ? NDO * NF * TY : NF * TY;
+  constexpr int ND = (NDO == 1) ? 2 : 8 * NF;
+  __shared__ double2 sm[NSEQ * N];
+  const int b = blockIdx.z, c = blockIdx.y, ky0 = blockIdx.x * TY, tid = threadIdx.x;
+  const Blk bk = prologue<MODE, TAN>(a, b);
+  if (!bk.run) return;
+
+  if (FWD) {
+    const double2* O = a.rowout + (size_t)b * (2 * NF) * N * KC;
+    for (int idx = tid; idx < NF * N * TY; idx += NT) {
+      const int cy = idx % TY;
+      const int t = idx / TY;
+      const int x = t % N;
+      const int f = t / N;
+      const int ky = ky0 + cy;
+      double2 v = make_double2(0.0, 0.0);
+      if (ky < KC) v = O[((size_t)(f * 2 + c) * N + x) * KC + ky];
+      sm[(f * TY + cy) * N + x] = v;
+    }
+    __syncthreads();
+    fft_lds<N, NF * TY, N, NT, false>(sm, a.tw);
+  }
+
+  const bool inv = HASINV && bk.stage && !a.skip_inv;
+  constexpr double invN2 = 1.0 / ((double)N * (double)N);
+  for (int idx = tid; idx < R * TY; idx += NT) {
+    const int cy = idx % TY;
+    const int r = idx / TY;
+    const int ky = ky0 + cy;
+    if (ky >= KC) continue;
+    const int kx = G::row_kx(r);
+    const int xr = G::row_x(r);
+    const size_t ci = (((size_t)b * 2 + c) * R + r) * KC + ky;
+    double2 rh = make_double2(0.0, 0.0), drh = rh;
+    if (FWD) {
+      rh = sm[(0 * TY + cy) * N + xr];
+      if (TAN) drh = sm[(1 * TY + cy) * N + xr];
+    }
+    double2 y, dy;
+    glue<N, MODE, TAN>(a, bk, b, ci, r, c, kx, ky, rh, drh, y, dy);
+    if (HASINV && inv) {
+      if (NDO == 1) {
+        sm[(0 * TY + cy) * N + xr] = cscale(invN2, y);
+      } else {
+        const bool dc = (kx == 0 && ky == 0);
+        const double k2s = dc ? 1.0 : (double)(kx * kx + ky * ky);
+        const double mult[4] = {(double)kx, (double)ky, (double)ky / k2s, -(double)kx / k2s};
+        const double2 iy = cmuli(y);
+#pragma unroll
+        for (int d = 0; d < 4; ++d) sm[((d * NF + 0) * TY + cy) * N + xr] = cscale(mult[d] * invN2, iy);
+        if (TAN) {
+          const double2 idy = cmuli(dy);
+#pragma unroll
+          for (int d = 0; d < 4; ++d) sm[((d * NF + 1) * TY + cy) * N + xr] = cscale(mult[d] * invN2, idy);
+        }
+      }
+    }
+  }
+  if (!HASINV || !inv) return;
+  constexpr int GAP = N - R;
+  for (int idx = tid; idx < NSEQ * GAP; idx += NT) {
+    const int s = idx / GAP;
+    const int g = idx - s * GAP;
+    sm[s * N + KMAX + 1 + g] = make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  fft_lds<N, NSEQ, N, NT, true>(sm, a.tw);
+  for (int idx = tid; idx < NSEQ * N; idx += NT) {
+    const int cy = idx % TY;
+    const int t = idx / TY;
+    const int x = t % N;
+    const int s2 = t / N;
+    const int ky = ky0 + cy;
+    if (ky >= KC) continue;
+    const int d = s2 / NF, f = s2 % NF;
+    const int g = (NDO == 1) ? c : (f * 8 + 4 * c + d);
+    a.inter[(((size_t)b * ND + g) * N + x) * KC + ky] = sm[(s2 * TY + cy) * N + x];
+  }
+}
+
+template <int N, int TX, int NT, int RMODE, bool TAN>
+__global__ __launch_bounds__(NT) void row_kernel(Args a) {
+  using G = Geo<N>;
+  constexpr int KC = G::KC;
+  constexpr int NP = (RMODE == R_ERR) ? 1 : (TAN ? 8 : 4);
+  constexpr int NF = TAN ? 2 : 1;
+  constexpr int ND = (RMODE == R_ERR) ? 2 : 8 * NF;
+  constexpr int NHALF = N / 2 + 1;
+  __shared__ double2 sm[TX * NP * N];
+  const int b = blockIdx.z, xbase = blockIdx.x * TX, tid = threadIdx.x;
+  if (!row_active(a, b)) return;
+  const double2* I = a.inter + (size_t)b * ND * N * KC;
+  // Hermitian extension of the packed pairs (A + iB); the imaginary parts of
+  // the ky = 0 and ky = N/2 bins are dropped, as numpy's c2r does.
+  for (int idx = tid; idx < TX * NP * NHALF; idx += NT) {
+    const int k = idx % NHALF;
+    const int t = idx / NHALF;
+    const int p = t % NP;
+    const int xl = t / NP;
+    const int x = xbase + xl;
+    double2 A = make_double2(0.0, 0.0), B = A;
+    if (k < KC) {
+      A = I[((size_t)(2 * p) * N + x) * KC + k];
+      B = I[((size_t)(2 * p + 1) * N + x) * KC + k];
+    }
+    double2* z = sm + (xl * NP + p) * N;
+    if (k == 0 || k == N / 2) {
+      z[k] = make_double2(A.x, B.x);
+    } else {
+      z[k] = make_double2(A.x - B.y, A.y + B.x);
+      z[N - k] = make_double2(A.x + B.y, B.x - A.y);
+    }
+  }
+  __syncthreads();
+  fft_lds<N, TX * NP, N, NT, true>(sm, a.tw);
+
+  if (RMODE == R_ERR) {
+    unsigned long long mx = 0ull;
+    for (int idx = tid; idx < TX * N; idx += NT) {
+      const double2 z = sm[idx];
+      const unsigned long long u1 = (unsigned long long)__double_as_longlong(fabs(z.x));
+      const unsigned long long u2 = (unsigned long long)__double_as_longlong(fabs(z.y));
+      mx = u1 > mx ? u1 : mx;
+      mx = u2 > mx ? u2 : mx;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(mx, off, 64);
+      mx = o > mx ? o : mx;
+    }
+    __shared__ unsigned long long red[NT / 64];
+    if ((tid & 63) == 0) red[tid >> 6] = mx;
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 1; w < NT / 64; ++w) mx = red[w] > mx ? red[w] : mx;
+      atomicMax(&a.hist[(size_t)b * a.hstride + a.attempt].err, mx);
+    }
+    return;
+  }
+
+  constexpr int EPT = (TX * N + NT - 1) / NT;
+  double2 q[EPT], dq[EPT];
+  const double b0x = a.b0x, b0y = a.b0y;
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int idx = tid + e * NT;
+    if (idx < TX * N) {
+      const int xl = idx / N, yy = idx - xl * N;
+      const double2* zb = sm + xl * NP * N + yy;
+      const double2 z0 = zb[0], z1 = zb[N], z2 = zb[2 * N], z3 = zb[3 * N];
+      const double wx = z0.x, wy = z0.y, u = z1.x, v = z1.y, jx = z2.x, jy = z2.y;
+      const double Bx = z3.x + b0x, By = z3.y + b0y;
+      // -advection[0] + advection[1] and u x B (mhd_jax.py:152-163)
+      q[e] = make_double2(-(u * wx + v * wy) + (Bx * jx + By * jy), u * By - v * Bx);
+      if (TAN) {
+        const double2 z4 = zb[4 * N], z5 = zb[5 * N], z6 = zb[6 * N], z7 = zb[7 * N];
+        const double twx = z4.x, twy = z4.y, tu = z5.x, tv = z5.y, tjx = z6.x, tjy = z6.y, ta = z7.x, tb = z7.y;
+        dq[e] = make_double2(-(tu * wx + u * twx + tv * wy + v * twy) + (ta * jx + Bx * tjx + tb * jy + By * tjy),
+                             tu * By + u * tb - tv * Bx - v * ta);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int idx = tid + e * NT;
+    if (idx < TX * N) {
+      const int xl = idx / N, yy = idx - xl * N;
+      sm[(xl * NF + 0) * N + yy] = q[e];
+      if (TAN) sm[(xl * NF + 1) * N + yy] = dq[e];
+    }
+  }
+  __syncthreads();
+  fft_lds<N, TX * NF, N, NT, false>(sm, a.tw);
+  double2* O = a.rowout + (size_t)b * (2 * NF) * N * KC;
+  for (int idx = tid; idx < TX * NF * KC; idx += NT) {
+    const int k = idx % KC;
+    const int t = idx / KC;
+    const int f = t % NF;
+    const int xl = t / NF;
+    const int x = xbase + xl;
+    const double2* Z = sm + (xl * NF + f) * N;
+    const double2 z = Z[k], zm = Z[(N - k) & (N - 1)];
+    const double2 pw = make_double2(0.5 * (z.x + zm.x), 0.5 * (z.y - zm.y));
+    const double2 pj = make_double2(0.5 * (z.y + zm.y), -0.5 * (z.x - zm.x));
+    O[((size_t)(f * 2 + 0) * N + x) * KC + k] = pw;
+    O[((size_t)(f * 2 + 1) * N + x) * KC + k] = pj;
+  }
+}
+
+// full [B][2][N][N/2+1] <-> compact [B][2][R][KC]
+template <int N>
+__global__ void pack_kernel(const double2* __restrict__ full, double2* __restrict__ comp, int B) {
+  using G = Geo<N>;
+  const size_t tot = (size_t)B * 2 * G::R * G::KC;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+    const int ky = (int)(i % G::KC);
+    const size_t t = i / G::KC;
+    const int r = (int)(t % G::R);
+    const size_t bc = t / G::R;
+    comp[i] = full[(bc * N + G::row_x(r)) * G::NH + ky];
+  }
+}
+
+template <int N>
+__global__ void unpack_kernel(const double2* __restrict__ comp, double2* __restrict__ full, int B, int zero_outside) {
+  using G = Geo<N>;
+  if (zero_outside) {
+    const size_t tot = (size_t)B * 2 * N * G::NH;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+      const int ky = (int)(i % G::NH);
+      const size_t t = i / G::NH;
+      const int x = (int)(t % N);
+      const size_t bc = t / N;
+      double2 v = make_double2(0.0, 0.0);
+      const int r = x <= G::KMAX ? x : (x >= N - G::KMAX ? x - N + G::R : -1);
+      if (r >= 0 && ky < G::KC) v = comp[(bc * G::R + r) * G::KC + ky];
+      full[i] = v;
+    }
+  } else {
+    const size_t tot = (size_t)B * 2 * G::R * G::KC;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+      const int ky = (int)(i % G::KC);
+      const size_t t = i / G::KC;
+      const int r = (int)(t % G::R);
+      const size_t bc = t / G::R;
+      full[(bc * N + G::row_x(r)) * G::NH + ky] = comp[i];
+    }
+  }
+}
+
+}  // namespace ecs

@@ -1,0 +1,114 @@
+/* C-ABI of the MI355X-native periodic-orbit hot path of 2D MHD.
+ *
+ * All arrays are caller-owned device buffers, C-order, FP64.  A "full
+ * spectrum" is complex128 [batch][2][N][N/2+1] (interleaved re, im doubles),
+ * the layout of jnp.fft.rfft2 of real fields [batch][2][N][N] with field 0 =
+ * vorticity omega and field 1 = current j (mhd_jax.py:105-120).  Every call
+ * is asynchronous on `stream` (a hipStream_t, 0 = default) unless stated.
+ * Return value: 0 on success, otherwise a nonzero code; ecs_last_error()
+ * describes the last failure of the calling thread.
+ *
+ * The reference (mgolden30/ECS_MHD_2D) has no native boundary: its operator
+ * API is a set of JAX functions.  Each entry point below replaces the
+ * reference function it names.
+ */
+#ifndef ECSMHD_H
+#define ECSMHD_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ecs_ctx ecs_ctx;
+
+typedef struct ecs_rk43_info {
+  int completed;  /* |s - 1| < 1e-10 */
+  int accepted;
+  int rejected;
+  int fevals;     /* 1 + 4 (accepted + rejected) */
+  int attempts;   /* accepted + rejected */
+  int pad;
+  double s;       /* pseudo-time reached */
+} ecs_rk43_info;
+
+/* Domain tables and physics of one grid size.  Replaces construct_domain
+ * (jax_scripts/lib/mhd_jax.py:20-94) plus the nu, eta, b0, forcing entries of
+ * param_dict.  fhat_full: host pointer to the complex128 [N][N/2+1] rfft2 of
+ * the real-space forcing (it is masked inside).  n in {16,32,64,128,256}. */
+int ecs_ctx_create(ecs_ctx** out, int n, double nu, double eta, double b0x, double b0y,
+                   const double* fhat_full, int device);
+int ecs_ctx_destroy(ecs_ctx* ctx);
+const char* ecs_last_error(void);
+int ecs_ctx_n(const ecs_ctx* ctx);
+
+/* state_vel(fields, param_dict, include_dissipation)  (mhd_jax.py:105-175).
+ * in, out: full spectra [batch][2][N][N/2+1]. */
+int ecs_state_vel(ecs_ctx* ctx, const double* in, double* out, int batch, int include_dissipation,
+                  void* stream);
+
+/* Tangent of state_vel: out = v(in), dout = J(in) din  (linearisation of
+ * mhd_jax.py:105-175, as jax.jvp of state_vel would give). */
+int ecs_state_vel_jvp(ecs_ctx* ctx, const double* in, const double* din, double* out, double* dout,
+                      int batch, int include_dissipation, void* stream);
+
+/* Adjoint of state_vel: grad = J(in)^T cot  (what jax.vjp of state_vel gives,
+ * with spectral cotangents holding d/dRe in the real part and d/dIm in the
+ * imaginary part). */
+int ecs_state_vel_vjp(ecs_ctx* ctx, const double* in, const double* cot, double* grad, int batch,
+                      int include_dissipation, void* stream);
+
+/* lawson_rk4(f, t, steps, v_fn, L_diag, mask)  (timestepping.py:46-81) with
+ * v_fn = state_vel without dissipation and L_diag = dissipation(param_dict),
+ * as mismatch_RPO mode "Lawson_RK4" builds them (loss_functions.py:57-65).
+ * f: full spectra, updated in place.  dt: host array [batch] of steps h.
+ * masked != 0 multiplies e = exp(h/2 L) by the 2/3 mask (the mode's choice). */
+int ecs_lawson_rk4(ecs_ctx* ctx, double* f, int batch, const double* dt, int steps, int masked,
+                   void* stream);
+
+/* Forward-mode derivative of ecs_lawson_rk4 in (f, h): f and df advance
+ * together; ddt: host [batch] tangent of h (dT/steps for h = T/steps). */
+int ecs_lawson_rk4_jvp(ecs_ctx* ctx, double* f, double* df, int batch, const double* dt,
+                       const double* ddt, int steps, int masked, void* stream);
+
+/* Reverse-mode derivative of ecs_lawson_rk4 (what jax.vjp of the
+ * "Lawson_RK4" integration gives, loss_functions.py:57-65).  f0: initial
+ * spectra (not modified); fbar_inout: on entry the cotangent of the final
+ * spectra, on exit that of f0; hbar: host [batch] output, d/dh.  The forward
+ * trajectory is checkpointed every `ckpt` steps and recomputed segment by
+ * segment. */
+int ecs_lawson_rk4_vjp(ecs_ctx* ctx, const double* f0, double* fbar_inout, double* hbar, int batch,
+                       const double* dt, int steps, int masked, int ckpt, void* stream);
+
+/* lawson_rk43(x, vel_fn, diag_L, t, h, atol, max_steps_per_checkpoint,
+ * checkpoints)  (timestepping.py:151-335).  x: full spectra, updated in
+ * place.  T: host [batch].  Synchronous: returns when every candidate has
+ * completed or used its budget of checkpoints * max_steps attempts.
+ * info: host [batch]; hs: host [batch][budget] steps of each attempt (the
+ * reference's info["hs"]); acc: host [batch][budget] 1 if that attempt was
+ * accepted (may be NULL). */
+int ecs_lawson_rk43(ecs_ctx* ctx, double* x, int batch, const double* T, double h0, double atol,
+                    int max_steps_per_checkpoint, int checkpoints, ecs_rk43_info* info, double* hs,
+                    int8_t* acc, void* stream);
+
+/* Forward-mode derivative of lawson_rk43 in (x, T) along the schedule a
+ * previous ecs_lawson_rk43 call recorded (h is stop-gradient,
+ * timestepping.py:308, and decisions follow the primal).  x, dx: full
+ * spectra updated in place; T, dT: host [batch]; hs, acc: host
+ * [batch][budget] as returned; attempts: host [batch]. */
+int ecs_lawson_rk43_jvp(ecs_ctx* ctx, double* x, double* dx, int batch, const double* T,
+                        const double* dT, const double* hs, const int8_t* acc, const int* attempts,
+                        int budget, void* stream);
+
+/* GMRES vector kernels (linalg.py:58-97) on flat FP64 vectors of length n.
+ * ecs_mgs: one modified-Gram-Schmidt sweep of Arnoldi step k (linalg.py:70-85):
+ * for j = 0..k: h[j] = <w, Q[:, j]>, w -= h[j] Q[:, j]; then h[k+1] = |w| and
+ * Q[:, k+1] = w / h[k+1].  Q is column-major with leading dimension ldq
+ * (Q[:, j] = Q + j*ldq); h: device [k+2]. */
+int ecs_mgs(double* Q, int ldq, int n, int k, double* w, double* h, void* stream);
+/* out[0] = <a, b> */
+int ecs_dot(const double* a, const double* b, int n, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,52 @@
+"""Write the golden fixtures under tests/golden/ (run in the build container).
+
+1. Converged solutions shipped with the reference (data, read with
+   numpy.load(allow_pickle=False), no code executed): a subset of
+   /root/reference/solutions/Re40/*.npz (N=128, nu=eta=0.025) and
+   /root/reference/solutions/Re50/RPO2.npz (N=256, nu=eta=0.02).  Only the
+   numeric entries are kept (fields, T, sx, nu, eta, b0, steps, ministeps,
+   num_checkpoints, shift_reflect_ny, rot); the forcing of every file is
+   -4 cos(4 y) (its forcing_str).
+2. residuals.json: the relative RPO residual |irfft2(f0 - g Phi_T f0)| / |f0|
+   of each solution under the oracle's Lawson RK4 with dt = T / steps
+   (ministeps x num_checkpoints as stored).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+from oracle import mhd_oracle as O  # noqa: E402
+
+REF = "/root/reference/solutions"
+PICK = ["Re40/EQ1", "Re40/RPO4", "Re40/RPO5", "Re40/RPO6", "Re40/TW3", "Re50/RPO2"]
+KEYS = ["fields", "T", "sx", "nu", "eta", "b0", "steps", "ministeps", "num_checkpoints", "shift_reflect_ny", "rot"]
+
+
+def main():
+    os.makedirs(os.path.join(HERE, "solutions"), exist_ok=True)
+    res = {}
+    for name in PICK:
+        d = np.load(os.path.join(REF, name + ".npz"), allow_pickle=False)
+        assert str(d["forcing_str"]) == "lambda x,y : -4*jnp.cos(4*y)"
+        out = {k: np.asarray(d[k]) for k in KEYS}
+        tag = name.replace("/", "_")
+        np.savez(os.path.join(HERE, "solutions", tag + ".npz"), **out)
+        n = out["fields"].shape[-1]
+        pd = O.make_params(n, float(out["nu"]), float(out["eta"]), tuple(out["b0"]))
+        pd.update({"num_checkpoints": int(out["num_checkpoints"]), "ministeps": int(out["ministeps"]),
+                   "shift_reflect_ny": int(out["shift_reflect_ny"]), "rot": bool(out["rot"])})
+        inp = {"fields": out["fields"], "T": float(out["T"]), "sx": float(out["sx"])}
+        obj, _ = O.objective_RPO(inp, pd, "Lawson_RK4")
+        r = float(np.linalg.norm(obj["fields"]) / np.linalg.norm(out["fields"]))
+        res[tag] = r
+        print(tag, n, r, flush=True)
+    with open(os.path.join(HERE, "residuals.json"), "w") as fh:
+        json.dump(res, fh, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,165 @@
+"""Parity of the HIP kernels (through the C-ABI) against the CPU oracle.
+
+Tolerances: relative L2 in FP64.  The RHS is one pass of transforms, so it
+agrees to ~1e-14; trajectories are held to 1e-11 after 64 steps (the bar
+for states), tangents to 1e-10.
+"""
+import numpy as np
+import pytest
+import torch
+
+from ecs_mhd_2d_amd import _runtime
+from ecs_mhd_2d_amd.synthetic import spectrum
+from oracle import mhd_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NU = ETA = 1 / 40
+B0 = (0.0, 0.1)
+
+
+def setup(n):
+    pd = O.make_params(n, NU, ETA, B0)
+    ctx = _runtime.Context(n, NU, ETA, B0, pd["forcing"])
+    return pd, ctx
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("n", [16, 32, 64, 128, 256])
+@pytest.mark.parametrize("diss", [False, True])
+def test_state_vel(n, diss):
+    print(f"state_vel n={n} diss={diss}", flush=True)
+    pd, ctx = setup(n)
+    f = np.stack([spectrum(n, 1), spectrum(n, 2) * 0.5])
+    # include a non-dealiased component: state_vel must ignore it except via dissipation
+    rng = np.random.default_rng(3)
+    f = f + 1e-3 * np.fft.rfft2(rng.standard_normal((2, 2, n, n)))
+    out = host(ctx.state_vel(dev(f), diss))
+    ref = np.stack([O.state_vel(f[b], pd, diss) for b in range(2)])
+    assert rel(out, ref) < 1e-13
+
+
+@pytest.mark.parametrize("n", [32, 128])
+def test_state_vel_jvp(n):
+    print(f"state_vel_jvp n={n}", flush=True)
+    pd, ctx = setup(n)
+    f = spectrum(n, 4)
+    df = spectrum(n, 5)
+    out, dout = ctx.state_vel_jvp(dev(f), dev(df), True)
+    assert rel(host(out), O.state_vel(f, pd, True)) < 1e-13
+    assert rel(host(dout), O.state_vel_jvp(f, df, pd, True)) < 1e-13
+
+
+@pytest.mark.parametrize("n,steps", [(16, 64), (32, 64), (128, 8)])
+def test_lawson_rk4(n, steps):
+    print(f"lawson_rk4 n={n} steps={steps}", flush=True)
+    pd, ctx = setup(n)
+    f = np.stack([spectrum(n, 11), spectrum(n, 12)])
+    dts = [0.005, 0.004]
+    out = host(ctx.lawson_rk4(dev(f), dts, steps))
+    for b in range(2):
+        ref = O.lawson_rk4(f[b], dts[b] * steps, steps, pd, True)
+        assert rel(out[b], ref) < 1e-11
+
+
+def test_lawson_rk4_unmasked_keeps_outside_modes():
+    n = 32
+    pd, ctx = setup(n)
+    rng = np.random.default_rng(9)
+    f = np.fft.rfft2(rng.standard_normal((2, n, n)))
+    out = host(ctx.lawson_rk4(dev(f), 0.003, 5, masked=False))
+    ref = O.lawson_rk4(f, 0.015, 5, pd, False)
+    assert rel(out, ref) < 1e-11
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_lawson_rk4_jvp(n):
+    print(f"lawson_rk4_jvp n={n}", flush=True)
+    pd, ctx = setup(n)
+    f = spectrum(n, 21)
+    df = spectrum(n, 22)
+    steps, dt, ddt = 16, 0.005, 0.3 / 16
+    x, dx = ctx.lawson_rk4_jvp(dev(f), dev(df), dt, ddt, steps)
+    rx, rdx = O.lawson_rk4_jvp(f, df, dt * steps, ddt * steps, steps, pd, True)
+    assert rel(host(x), rx) < 1e-11
+    assert rel(host(dx), rdx) < 1e-10
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_lawson_rk43(n):
+    print(f"lawson_rk43 n={n}", flush=True)
+    pd, ctx = setup(n)
+    x0 = spectrum(n, 31) * 2
+    rng = np.random.default_rng(5)
+    x0 = x0 + 1e-3 * np.fft.rfft2(rng.standard_normal((2, n, n)))  # outside-mask modes are carried
+    T, atol = 1.3, 1e-6
+    x, infos = ctx.lawson_rk43(dev(x0), T, 1e-2, atol, 64, 8)
+    rx, rinfo = O.lawson_rk43(x0, pd, T, 1e-2, atol, 64, 8, record=True)
+    gi = infos[0]
+    sched = [(gi["hs"][i], bool(gi["acc"][i])) for i in range(gi["attempts"])]
+    print("gpu", gi["accepted"], gi["rejected"], "oracle", rinfo["accepted"], rinfo["rejected"], flush=True)
+    assert gi["completed"] and rinfo["completed"]
+    assert gi["fevals"] == 1 + 4 * (gi["accepted"] + gi["rejected"])
+    assert [a for _, a in sched] == [a for _, a in rinfo["schedule"]]
+    np.testing.assert_allclose([h for h, _ in sched], [h for h, _ in rinfo["schedule"]], rtol=1e-9)
+    assert rel(host(x), rx) < 1e-10
+
+
+def test_lawson_rk43_jvp():
+    n = 32
+    pd, ctx = setup(n)
+    x0 = spectrum(n, 41) * 2
+    dx0 = spectrum(n, 42)
+    T, dT, atol = 0.9, 0.4, 1e-6
+    x, infos = ctx.lawson_rk43(dev(x0), T, 1e-2, atol, 32, 8)
+    sched = [(infos[0]["hs"][i], bool(infos[0]["acc"][i])) for i in range(infos[0]["attempts"])]
+    xj, dxj = ctx.lawson_rk43_jvp(dev(x0), dev(dx0), T, dT, infos)
+    rx, rdx = O.lawson_rk43_jvp(x0, dx0, pd, T, dT, sched)
+    assert rel(host(xj), host(x)) < 1e-12
+    assert rel(host(xj), rx) < 1e-10
+    assert rel(host(dxj), rdx) < 1e-10
+
+
+def test_mgs_and_dot():
+    from ecs_mhd_2d_amd import _lib
+    lib = _lib.load()
+    rng = np.random.default_rng(0)
+    n, inner = 5000, 6
+    A = rng.standard_normal((n, n // 50))
+    Q = torch.zeros((inner + 1, n), dtype=torch.float64, device="cuda")
+    b = rng.standard_normal(n)
+    Q[0] = torch.as_tensor(b / np.linalg.norm(b))
+    H = torch.zeros((inner, inner + 2), dtype=torch.float64, device="cuda")
+    Qh = np.zeros((n, inner + 1))
+    Qh[:, 0] = b / np.linalg.norm(b)
+    Hh = np.zeros((inner + 1, inner))
+    for k in range(inner):
+        wv = A @ (A.T @ Qh[:, k])
+        w = torch.as_tensor(wv).cuda()
+        _lib.check(lib.ecs_mgs(Q.data_ptr(), n, n, k, w.data_ptr(), H[k].data_ptr(),
+                               _runtime._stream()), "ecs_mgs")
+        for j in range(k + 1):
+            Hh[j, k] = wv @ Qh[:, j]
+            wv = wv - Hh[j, k] * Qh[:, j]
+        Hh[k + 1, k] = np.linalg.norm(wv)
+        Qh[:, k + 1] = wv / Hh[k + 1, k]
+    Hd = host(H)
+    for k in range(inner):
+        np.testing.assert_allclose(Hd[k, :k + 2], Hh[:k + 2, k], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(host(Q).T, Qh, atol=1e-12)
+    out = torch.zeros(1, dtype=torch.float64, device="cuda")
+    a, c = torch.as_tensor(A[:, 0]).cuda(), torch.as_tensor(A[:, 1]).cuda()
+    _lib.check(lib.ecs_dot(a.data_ptr(), c.data_ptr(), n, out.data_ptr(), _runtime._stream()), "ecs_dot")
+    assert abs(host(out)[0] - A[:, 0] @ A[:, 1]) < 1e-10
