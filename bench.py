@@ -1,0 +1,170 @@
+"""Benchmark of the periodic-orbit hot path on one MI355X (FP64).
+
+Headline workload (the Newton-GMRES configuration): N=128, nu=eta=1/40,
+b0=(0,0.1), F=-4cos(4y); guess = synthetic spectrum G(128, seed 0) evolved
+8192 Lawson-RK4 steps of h=0.005; T=4.3, sx=0.1; adaptive Lawson RK4(3) with
+atol=1e-8, h0=1e-2 (budget 64 x 64 attempts).  One "step" is one GMRES
+matvec: the JVP of objective_RPO through the whole adaptive trajectory, on a
+random unit tangent.  value = matvecs/s.
+
+Also reported: Lawson RK4 steps/s at N=256 (batch 1 and batch 64), the
+roofline of the dominant kernel of the matvec (tangent row kernel), measured
+live with HIP events, and the CPU oracle's matvec rate on a bounded sample.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "FP64 RK steps/s at N=256 + GMRES matvecs/s; p50/p99 RPO solve latency"
+
+
+def zbytes(n):
+    """Z(N): bytes of one complex half-spectrum field."""
+    return n * (n // 2 + 1) * 16
+
+
+def build_guess(n, seed, steps, h, pd):
+    import torch
+    from ecs_mhd_2d_amd import mhd
+    from ecs_mhd_2d_amd.synthetic import spectrum
+    ctx = mhd.context(pd)
+    f = torch.as_tensor(spectrum(n, seed)).cuda()
+    f = ctx.lawson_rk4(f, h, steps)
+    return torch.fft.irfft2(f, s=(n, n)).contiguous()
+
+
+def cpu_baseline_matvec(fields_np, T, sx, lin_info, budget_s=15.0):
+    """Oracle (numpy, 1 thread of pocketfft) JVP through the first S accepted
+    steps of the same schedule; scaled to the full trajectory."""
+    from oracle import mhd_oracle as O
+    n = fields_np.shape[-1]
+    pd = O.make_params(n, 1 / 40, 1 / 40)
+    acc = lin_info["acc"][: lin_info["attempts"]]
+    hs = lin_info["hs"][: lin_info["attempts"]]
+    sched = [(float(h), bool(a)) for h, a in zip(hs, acc)]
+    f0 = np.fft.rfft2(fields_np)
+    rng = np.random.default_rng(5)
+    df0 = np.fft.rfft2(rng.standard_normal((2, n, n)))
+    # probe cost of a few steps, then size the sample to ~budget_s
+    t0 = time.perf_counter()
+    O.lawson_rk43_jvp(f0, df0, pd, T, 0.1, sched[:4])
+    per = (time.perf_counter() - t0) / 4
+    S = int(max(8, min(len(sched), budget_s / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    O.lawson_rk43_jvp(f0, df0, pd, T, 0.1, sched[:S])
+    t = time.perf_counter() - t0
+    full = t * len(sched) / S
+    return {"value": 1.0 / full, "unit": "matvecs/s", "cores": 1, "kind": "port",
+            "sample": f"oracle JVP through the first {S} of {len(sched)} attempts of the same N=128 schedule "
+                      f"({t:.1f} s), scaled linearly to the full trajectory"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build targets a single MI355X); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from ecs_mhd_2d_amd import _lib, _runtime, loss_functions as LF, mhd, utils
+
+    torch.cuda.init()
+    n = 128
+    pd = mhd.make_param_dict(n, 1 / 40, 1 / 40, (0.0, 0.1))
+    pd.update({"shift_reflect_ny": 0, "rot": False,
+               "adaptive_dict": {"atol": 1e-8, "checkpoints": 64, "max_steps_per_checkpoint": 64}})
+    fields = build_guess(n, 0, 8192, 0.005, pd)
+    inp = {"fields": fields, "T": 4.3, "sx": 0.1}
+    lin = LF.Linearization(inp, pd, "Lawson_RK43")
+    info = lin.info
+    gen = torch.Generator(device="cuda").manual_seed(1)
+
+    def tangent():
+        v = torch.randn(2 * n * n + 2, generator=gen, device="cuda", dtype=torch.float64)
+        return utils.unflatten(v / torch.linalg.vector_norm(v), n)
+
+    tans = [tangent() for _ in range(args.warmup + args.steps)]
+    for i in range(args.warmup):
+        lin.jvp(tans[i])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        out = lin.jvp(tans[args.warmup + i])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    value = args.steps / dt
+
+    # Lawson RK4 steps/s at N=256
+    from ecs_mhd_2d_amd.synthetic import spectrum
+    pd256 = mhd.make_param_dict(256, 1 / 40, 1 / 40, (0.0, 0.1))
+    ctx256 = mhd.context(pd256)
+    rk4 = {}
+    for B, steps in ((1, 256), (64, 32)):
+        f = torch.as_tensor(np.stack([spectrum(256, s) for s in range(B)])).cuda()
+        ctx256.lawson_rk4(f, 0.004, 4)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ctx256.lawson_rk4(f, 0.004, steps)
+        torch.cuda.synchronize()
+        rk4[B] = B * steps / (time.perf_counter() - t1)
+
+    # live roofline of the dominant kernel of the matvec (tangent row kernel, N=128, batch 1)
+    lib = _lib.load()
+    ctx = mhd.context(pd)
+    ms = ctypes.c_double()
+    _lib.check(lib.ecs_time_kernel(ctx.handle, 1, 3, 200, ctypes.byref(ms), _runtime._stream()),
+               "ecs_time_kernel")
+    row_t_ms = ms.value
+    bytes_row_t = 20 * zbytes(n)   # reads 16 Z (8 primal + 8 tangent derived fields), writes 4 Z
+    ach = bytes_row_t / (row_t_ms * 1e-3) / 1e9
+    # the same kernel family at full occupancy: primal row kernel, N=256, batch 64
+    _lib.check(lib.ecs_time_kernel(ctx256.handle, 64, 1, 50, ctypes.byref(ms), _runtime._stream()),
+               "ecs_time_kernel")
+    row_b_ms = ms.value
+    ach_b = 64 * 10 * zbytes(256) / (row_b_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_matvec(fields.cpu().numpy(), 4.3, 0.1, info)
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "matvecs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "GMRES matvec (JVP of objective_RPO through the adaptive Lawson RK4(3) "
+                               "trajectory) on one RPO guess, N=128, T=4.3, sx=0.1, atol=1e-8, Krylov dim 64",
+                   "n": 128, "attempts": info["attempts"], "accepted": info["accepted"],
+                   "rejected": info["rejected"], "parallelism": "single GPU"},
+        "rk4_steps_per_s_n256_b1": rk4[1], "rk4_steps_per_s_n256_b64": rk4[64],
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "row_kernel<128,2,256,R_RHS,tangent> (batch 1)", "avg_ms": row_t_ms,
+                     "algorithmic_bytes": bytes_row_t},
+        "roofline_batched": {"bound": "hbm", "achieved": ach_b, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": ach_b / HBM_PEAK_GBS, "kernel": "row_kernel<256,1,256,R_RHS> (batch 64)",
+                             "avg_ms": row_b_ms, "algorithmic_bytes": 64 * 10 * zbytes(256)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

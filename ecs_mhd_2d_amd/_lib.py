@@ -39,10 +39,16 @@ _SIGS = {
                                    c_void_p]),
     "ecs_lawson_rk4_vjp": (c_int, [c_void_p, c_void_p, c_void_p, P_double, c_int, P_double, c_int, c_int,
                                    c_int, c_void_p]),
+    "ecs_ckpt_doubles": (ctypes.c_longlong, [c_void_p, c_int, c_int, c_int]),
+    "ecs_lawson_rk4_fwd_ckpt": (c_int, [c_void_p, c_void_p, c_int, P_double, c_int, c_int, c_int, c_void_p,
+                                        c_void_p]),
+    "ecs_lawson_rk4_vjp_ckpt": (c_int, [c_void_p, c_void_p, c_void_p, P_double, c_int, P_double, c_int, c_int,
+                                        c_int, c_void_p]),
     "ecs_lawson_rk43": (c_int, [c_void_p, c_void_p, c_int, P_double, c_double, c_double, c_int, c_int,
                                 ctypes.POINTER(RK43Info), P_double, P_int8, c_void_p]),
     "ecs_lawson_rk43_jvp": (c_int, [c_void_p, c_void_p, c_void_p, c_int, P_double, P_double, P_double, P_int8,
                                     P_int, c_int, c_void_p]),
+    "ecs_time_kernel": (c_int, [c_void_p, c_int, c_int, c_int, P_double, c_void_p]),
     "ecs_mgs": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "ecs_dot": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
 }

@@ -1,0 +1,296 @@
+// Reverse-mode (adjoint) kernels of the fused RHS and of the Lawson RK4 step.
+//
+// Cotangent convention: a spectral cotangent holds d/dRe in its real part and
+// d/dIm in its imaginary part (the real inner product on stored entries).
+// The adjoint RHS  g -> J(y)^T g  is the transpose of the forward chain
+//   f -> (i m_d / N^2) f -> inverse x-FFT -> c2r along y -> products
+//     -> r2c along y -> x-FFT -> mask (omega) / k^2 mask (j),
+// i.e. (column-adjoint) m g -> inverse x-FFT -> (row-adjoint) weighted c2r
+// along y -> transposed products -> weighted r2c along y -> (column-adjoint)
+// x-FFT -> sum_d (-i m_d / N^2) G_d.
+//
+// One Lawson RK4 step  y2 = e x0 + e k1/2, y3 = e x0 + k2/2, y4 = e e x0 + e k3,
+// x' = e e x0 + e e k1/6 + e k2/3 + e k3/3 + k4/6,  k_s = dt v(y_s),
+// is reversed stage by stage; d/d(dt) collects <v_s, kbar_s> and the e terms
+// (de/d(dt) = L/2 e) per mode into hb, summed once at the end.
+#pragma once
+#include "rhs_kernels.h"
+
+namespace ecs {
+
+enum AdjMode : int {
+  B_G = 0,      // standalone: gbar input -> column-adjoint output
+  B_OUT = 1,    // standalone: ybar -> out (+ L g)
+  B_ENTRY = 2,  // entry of the last step of a segment (xb = cotangent of its output)
+  B_S4 = 3,     // ybar4 -> ..., next g = dt k3bar
+  B_S3 = 4,
+  B_S2 = 5,
+  B_S1E = 6,    // ybar1 -> xb; fused entry of the previous step
+  B_S1 = 7,     // ybar1 -> xb; no previous step in this segment
+};
+
+template <int N, int MODE>
+__device__ __forceinline__ void adj_glue(const Args& a, int b, size_t ci, int r, int c, int kx, int ky, double2 yb,
+                                         double2& g) {
+  const double k2 = (double)(kx * kx + ky * ky);
+  const bool dc = (kx == 0 && ky == 0);
+  const double L = -k2 * a.nu[c];
+  g = make_double2(0.0, 0.0);
+  if (MODE == B_G) {
+    g = a.gin[ci];
+    return;
+  }
+  if (MODE == B_OUT) {
+    double2 o = yb;
+    if (a.include_diss) o = cadd(o, cscale(L, a.gin[ci]));
+    a.out[ci] = o;
+    return;
+  }
+  const double dt = a.dt[b];
+  double e = exp((dt * 0.5) * L);
+  if (a.masked_e && dc) e = 0.0;
+  const double dedh = 0.5 * L * e;  // de/d(dt)
+  const size_t cs = a.cstride;
+  double2 x0 = make_double2(0.0, 0.0);
+  double hacc = 0.0;
+  if (MODE != B_ENTRY) x0 = a.tx[ci];
+  if (MODE == B_S4) {
+    const double2 k3 = cscale(dt, a.tv[2 * cs + ci]);
+    a.xb[ci] = cadd(a.xb[ci], cscale(e * e, yb));
+    const double2 k3b = cadd(a.k3b[ci], cscale(e, yb));
+    const double2 dy = cadd(cscale(2.0 * e, x0), k3);
+    hacc += (yb.x * dy.x + yb.y * dy.y) * dedh;
+    const double2 v3 = a.tv[2 * cs + ci];
+    hacc += v3.x * k3b.x + v3.y * k3b.y;
+    g = cscale(dt, k3b);
+  }
+  if (MODE == B_S3) {
+    a.xb[ci] = cadd(a.xb[ci], cscale(e, yb));
+    const double2 k2b = cadd(a.k2b[ci], cdivs(yb, 2.0));
+    hacc += (yb.x * x0.x + yb.y * x0.y) * dedh;
+    const double2 v2 = a.tv[1 * cs + ci];
+    hacc += v2.x * k2b.x + v2.y * k2b.y;
+    g = cscale(dt, k2b);
+  }
+  if (MODE == B_S2) {
+    const double2 k1 = cscale(dt, a.tv[ci]);
+    a.xb[ci] = cadd(a.xb[ci], cscale(e, yb));
+    const double2 k1b = cadd(a.k1b[ci], cdivs(cscale(e, yb), 2.0));
+    const double2 dy = cadd(x0, cdivs(k1, 2.0));
+    hacc += (yb.x * dy.x + yb.y * dy.y) * dedh;
+    const double2 v1 = a.tv[ci];
+    hacc += v1.x * k1b.x + v1.y * k1b.y;
+    g = cscale(dt, k1b);
+  }
+  if (MODE == B_S1 || MODE == B_S1E) {
+    a.xb[ci] = cadd(a.xb[ci], yb);
+  }
+  if (MODE != B_ENTRY) a.hb[ci] += hacc;
+  if (MODE == B_ENTRY || MODE == B_S1E) {
+    // entry of a step: xbar' known; tape of that step
+    const double2* tx = (MODE == B_ENTRY) ? a.tx : a.txp;
+    const double2* tv = (MODE == B_ENTRY) ? a.tv : a.tvp;
+    const double2 xbp = a.xb[ci];
+    const double2 x0s = tx[ci];
+    const double2 k1 = cscale(dt, tv[ci]);
+    const double2 k2 = cscale(dt, tv[cs + ci]);
+    const double2 k3 = cscale(dt, tv[2 * cs + ci]);
+    const double2 v4 = tv[3 * cs + ci];
+    const double2 k4b = cdivs(xbp, 6.0);
+    double h = v4.x * k4b.x + v4.y * k4b.y;
+    a.xb[ci] = cscale(e * e, xbp);
+    a.k1b[ci] = cdivs(cscale(e * e, xbp), 6.0);
+    a.k2b[ci] = cdivs(cscale(e, xbp), 3.0);
+    a.k3b[ci] = cdivs(cscale(e, xbp), 3.0);
+    // d x'/de = 2 e x0 + 2 e k1/6 + k2/3 + k3/3
+    const double2 dxe = cadd(cadd(cscale(2.0 * e, x0s), cdivs(cscale(2.0 * e, k1), 6.0)),
+                             cadd(cdivs(k2, 3.0), cdivs(k3, 3.0)));
+    h += (xbp.x * dxe.x + xbp.y * dxe.y) * dedh;
+    a.hb[ci] += h;
+    g = cscale(dt, k4b);
+  }
+}
+
+// Column-adjoint kernel: [x-FFT of the row-adjoint output -> ybar] -> glue ->
+// [m g -> inverse x-FFT -> AO].
+template <int N, int TY, int NT, int MODE>
+__global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
+  using G = Geo<N>;
+  constexpr int KC = G::KC, R = G::R, KMAX = G::KMAX;
+  constexpr bool FWD = !(MODE == B_G || MODE == B_ENTRY);
+  constexpr bool INV = !(MODE == B_OUT || MODE == B_S1);
+  constexpr int NSEQ = FWD ? 4 * TY : TY;
+  __shared__ double2 sm[NSEQ * N];
+  const int b = blockIdx.z, c = blockIdx.y, ky0 = blockIdx.x * TY, tid = threadIdx.x;
+  if (FWD) {
+    const double2* AI = a.ain + (size_t)b * 8 * N * KC;
+    for (int idx = tid; idx < 4 * N * TY; idx += NT) {
+      const int cy = idx % TY;
+      const int t = idx / TY;
+      const int x = t % N;
+      const int d = t / N;
+      const int ky = ky0 + cy;
+      double2 v = make_double2(0.0, 0.0);
+      if (ky < KC) v = AI[((size_t)(4 * c + d) * N + x) * KC + ky];
+      sm[(d * TY + cy) * N + x] = v;
+    }
+    __syncthreads();
+    fft_lds<N, 4 * TY, N, NT, false>(sm, a.tw);
+  }
+  constexpr double invN2 = 1.0 / ((double)N * (double)N);
+  for (int idx = tid; idx < R * TY; idx += NT) {
+    const int cy = idx % TY;
+    const int r = idx / TY;
+    const int ky = ky0 + cy;
+    if (ky >= KC) continue;
+    const int kx = G::row_kx(r);
+    const int xr = G::row_x(r);
+    const size_t ci = (((size_t)b * 2 + c) * R + r) * KC + ky;
+    double2 yb = make_double2(0.0, 0.0);
+    if (FWD) {
+      const bool dc = (kx == 0 && ky == 0);
+      const double k2s = dc ? 1.0 : (double)(kx * kx + ky * ky);
+      const double mult[4] = {(double)kx, (double)ky, (double)ky / k2s, -(double)kx / k2s};
+      // ybar = sum_d conj(i m_d / N^2) Gbar_d = sum_d (-i m_d / N^2) Gbar_d
+      double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        const double2 gd = sm[(d * TY + cy) * N + xr];
+        s = cadd(s, cscale(mult[d] * invN2, make_double2(gd.y, -gd.x)));
+      }
+      yb = s;
+    }
+    double2 g;
+    adj_glue<N, MODE>(a, b, ci, r, c, kx, ky, yb, g);
+    if (INV) {
+      const bool dc = (kx == 0 && ky == 0);
+      const double m = dc ? 0.0 : (c == 0 ? 1.0 : (double)(kx * kx + ky * ky));
+      sm[(0 * TY + cy) * N + xr] = cscale(m, g);
+    }
+  }
+  if (!INV) return;
+  // gap rows and the other 3 (unused) sequence slots are not needed: only slot 0
+  constexpr int GAP = N - R;
+  for (int idx = tid; idx < TY * GAP; idx += NT) {
+    const int s = idx / GAP;
+    const int gg = idx - s * GAP;
+    sm[s * N + KMAX + 1 + gg] = make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  fft_lds<N, TY, N, NT, true>(sm, a.tw);
+  double2* AO = a.aout + (size_t)b * 2 * N * KC;
+  for (int idx = tid; idx < N * TY; idx += NT) {
+    const int cy = idx % TY;
+    const int x = idx / TY;
+    const int ky = ky0 + cy;
+    if (ky >= KC) continue;
+    AO[((size_t)c * N + x) * KC + ky] = sm[cy * N + x];
+  }
+}
+
+// Row-adjoint kernel: weighted c2r of (Pbar_omega, Pbar_j), transposed
+// products with the primal real fields, weighted r2c of the 8 cotangents.
+template <int N, int TX, int NT>
+__global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
+  using G = Geo<N>;
+  constexpr int KC = G::KC;
+  constexpr int NP = 5;  // 4 primal pairs + 1 adjoint pair
+  constexpr int NHALF = N / 2 + 1;
+  __shared__ double2 sm[TX * NP * N];
+  const int b = blockIdx.z, xbase = blockIdx.x * TX, tid = threadIdx.x;
+  const double2* P = a.prim + (size_t)b * 8 * N * KC;
+  const double2* AO = a.aout + (size_t)b * 2 * N * KC;
+  for (int idx = tid; idx < TX * NP * NHALF; idx += NT) {
+    const int k = idx % NHALF;
+    const int t = idx / NHALF;
+    const int p = t % NP;
+    const int xl = t / NP;
+    const int x = xbase + xl;
+    double2 A = make_double2(0.0, 0.0), B = A;
+    if (k < KC) {
+      if (p < 4) {
+        A = P[((size_t)(2 * p) * N + x) * KC + k];
+        B = P[((size_t)(2 * p + 1) * N + x) * KC + k];
+      } else {
+        // r2c^T: pbar[y] = sum_k Re(Pbar_k e^{i theta}) = c2r of (Pbar_0, Pbar_k / 2)
+        const double w = (k == 0) ? 1.0 : 0.5;
+        A = cscale(w, AO[((size_t)0 * N + x) * KC + k]);
+        B = cscale(w, AO[((size_t)1 * N + x) * KC + k]);
+      }
+    }
+    double2* z = sm + (xl * NP + p) * N;
+    if (k == 0 || k == N / 2) {
+      z[k] = make_double2(A.x, B.x);
+    } else {
+      z[k] = make_double2(A.x - B.y, A.y + B.x);
+      z[N - k] = make_double2(A.x + B.y, B.x - A.y);
+    }
+  }
+  __syncthreads();
+  fft_lds<N, TX * NP, N, NT, true>(sm, a.tw);
+  constexpr int EPT = (TX * N + NT - 1) / NT;
+  double2 q[EPT][4];
+  const double b0x = a.b0x, b0y = a.b0y;
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int idx = tid + e * NT;
+    if (idx < TX * N) {
+      const int xl = idx / N, yy = idx - xl * N;
+      const double2* zb = sm + xl * NP * N + yy;
+      const double2 z0 = zb[0], z1 = zb[N], z2 = zb[2 * N], z3 = zb[3 * N], z4 = zb[4 * N];
+      const double wx = z0.x, wy = z0.y, u = z1.x, v = z1.y, jx = z2.x, jy = z2.y;
+      const double Bx = z3.x + b0x, By = z3.y + b0y;
+      const double pw = z4.x, pj = z4.y;
+      // p_w = -(u wx + v wy) + (Bx jx + By jy),  p_j = u By - v Bx
+      q[e][0] = make_double2(-pw * u, -pw * v);                       // (wx, wy)
+      q[e][1] = make_double2(-pw * wx + pj * By, -pw * wy - pj * Bx);  // (u, v)
+      q[e][2] = make_double2(pw * Bx, pw * By);                        // (jx, jy)
+      q[e][3] = make_double2(pw * jx - pj * v, pw * jy + pj * u);      // (a, b)
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int idx = tid + e * NT;
+    if (idx < TX * N) {
+      const int xl = idx / N, yy = idx - xl * N;
+#pragma unroll
+      for (int p = 0; p < 4; ++p) sm[(xl * 4 + p) * N + yy] = q[e][p];
+    }
+  }
+  __syncthreads();
+  fft_lds<N, TX * 4, N, NT, false>(sm, a.tw);
+  double2* AI = a.ain + (size_t)b * 8 * N * KC;
+  for (int idx = tid; idx < TX * 4 * KC; idx += NT) {
+    const int k = idx % KC;
+    const int t = idx / KC;
+    const int p = t % 4;
+    const int xl = t / 4;
+    const int x = xbase + xl;
+    const double2* Z = sm + (xl * 4 + p) * N;
+    const double2 z = Z[k], zm = Z[(N - k) & (N - 1)];
+    // c2r^T: Dbar_k = w'_k sum_y dbar e^{-i theta}, w'_0 = 1, w'_k = 2
+    const double w = (k == 0) ? 0.5 : 1.0;
+    const double2 da = make_double2(w * (z.x + zm.x), w * (z.y - zm.y));
+    const double2 db = make_double2(w * (z.y + zm.y), -w * (z.x - zm.x));
+    AI[((size_t)(2 * p) * N + x) * KC + k] = da;
+    AI[((size_t)(2 * p + 1) * N + x) * KC + k] = db;
+  }
+}
+
+// sum of hb over the compact modes of each candidate (fixed order)
+template <int N>
+__global__ __launch_bounds__(256) void hb_sum_kernel(const double* __restrict__ hb, double* __restrict__ out) {
+  using G = Geo<N>;
+  constexpr int M = 2 * G::R * G::KC;
+  const int b = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < M; i += 256) s += hb[(size_t)b * M + i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[b] = red[0] + red[1] + red[2] + red[3];
+}
+
+}  // namespace ecs

@@ -7,6 +7,7 @@
 #include <vector>
 #include <algorithm>
 #include "rhs_kernels.h"
+#include "adjoint_kernels.h"
 #include "vec_kernels.h"
 #include <map>
 #include <mutex>
@@ -58,10 +59,16 @@ struct ecs_ctx {
   int* s_nacc = nullptr;
   size_t sched_cap = 0;
   // adjoint workspace
-  double2* ckpt = nullptr;     // [nck][cap][2][R][KC]
+  double2* ckpt = nullptr;     // checkpoints [nck][B][2][R][KC]
   size_t ckpt_cap = 0;
-  double2* tape = nullptr;     // [steps_per_segment][4 stages][cap][2][R][KC]
-  size_t tape_cap = 0;
+  double2* xtape = nullptr;    // [ckpt][B][2][R][KC]
+  double2* vtape = nullptr;    // [ckpt][4][B][2][R][KC]
+  size_t vt_cap = 0;           // elements of xtape + vtape
+  double2* itape = nullptr;    // primal column outputs [ckpt][4][B][8][N][KC]
+  size_t it_cap = 0;
+  double2 *AO = nullptr, *AI = nullptr, *xb = nullptr, *k1b = nullptr, *k2b = nullptr, *k3b = nullptr;
+  double* hb = nullptr;
+  int adj_cap = 0;
   double* red = nullptr;       // reductions [cap][64]
   double* hred = nullptr;      // pinned [cap]
   size_t compact_elems() const {
@@ -448,12 +455,13 @@ int ecs_ctx_destroy(ecs_ctx* c) {
   (void)hipSetDevice(c->dev);
   void* ps[] = {c->tw, c->fhat, c->I, c->O, c->x0, c->acc, c->v1, c->v4, c->v5, c->xf, c->out, c->dx0, c->dacc,
                 c->dv1, c->dv4, c->dxf, c->dout, c->scal, c->hist, c->s_h, c->s_rs, c->s_rc, c->s_rh, c->s_nacc,
-                c->ckpt, c->tape, c->red};
+                c->ckpt, c->xtape, c->itape, c->red, c->AO, c->AI, c->xb, c->k1b, c->k2b, c->k3b, c->hb};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->hred) (void)hipHostFree(c->hred);
   delete c;
+  (void)hipGetLastError();  // leave no sticky error behind for the caller
   return 0;
 }
 
@@ -559,15 +567,317 @@ extern "C" int ecs_mgs(double* Q, int ldq, int n, int k, double* w, double* h, v
   return 0;
 }
 
+
+// ------------------------------------------------------------------ adjoint
+template <int N, int MODE>
+static int acol(const Args& a, int B, hipStream_t st) {
+  constexpr int TY = Tiles<N>::TY;
+  dim3 grid((Geo<N>::KC + TY - 1) / TY, 2, B);
+  hipLaunchKernelGGL((adj_col_kernel<N, TY, NT, MODE>), grid, dim3(NT), 0, st, a);
+  LAUNCHCHK();
+  return 0;
+}
+template <int N>
+static int arow(const Args& a, int B, hipStream_t st) {
+  constexpr int TX = Tiles<N>::TX;
+  dim3 grid(N / TX, 1, B);
+  hipLaunchKernelGGL((adj_row_kernel<N, TX, NT>), grid, dim3(NT), 0, st, a);
+  LAUNCHCHK();
+  return 0;
+}
+
+static int grow(double2** p, size_t* cap, size_t need) {
+  if (need <= *cap) return 0;
+  if (*p) HIPCHK(hipFree(*p));
+  *p = nullptr;
+  HIPCHK(hipMalloc((void**)p, need * sizeof(double2)));
+  *cap = need;
+  return 0;
+}
+
+static int ensure_adj(ecs_ctx* c, int B) {
+  TRY(ensure_cap(c, B));
+  if (B <= c->adj_cap) return 0;
+  const int nc = c->cap;
+  const int KC = (c->n - 1) / 3 + 1;
+  const size_t comp = c->compact_elems();
+  double2** arrs[] = {&c->xb, &c->k1b, &c->k2b, &c->k3b};
+  for (auto p : arrs) {
+    if (*p) HIPCHK(hipFree(*p));
+    HIPCHK(hipMalloc((void**)p, comp * nc * sizeof(double2)));
+  }
+  if (c->hb) HIPCHK(hipFree(c->hb));
+  HIPCHK(hipMalloc((void**)&c->hb, comp * nc * sizeof(double)));
+  if (c->AO) HIPCHK(hipFree(c->AO));
+  if (c->AI) HIPCHK(hipFree(c->AI));
+  HIPCHK(hipMalloc((void**)&c->AO, (size_t)2 * c->n * KC * nc * sizeof(double2)));
+  HIPCHK(hipMalloc((void**)&c->AI, (size_t)8 * c->n * KC * nc * sizeof(double2)));
+  c->adj_cap = nc;
+  return 0;
+}
+
+template <int N>
+static int run_state_vel_vjp(ecs_ctx* c, const double* in, const double* cot, double* grad, int B, int diss,
+                             hipStream_t st) {
+  TRY(ensure_adj(c, B));
+  Args a = base_args(c);
+  a.include_diss = diss;
+  a.aout = c->AO;
+  a.ain = c->AI;
+  a.prim = c->I;
+  a.gin = c->acc;  // compact cotangent
+  TRY(pack<N>(in, c->x0, B, st));
+  TRY(pack<N>(cot, c->acc, B, st));
+  TRY((col<N, C_LOAD, false>(a, B, st)));
+  TRY((acol<N, B_G>(a, B, st)));
+  TRY(arow<N>(a, B, st));
+  TRY((acol<N, B_OUT>(a, B, st)));
+  TRY(unpack<N>(c->out, grad, B, 1, st));
+  return 0;
+}
+
+template <int N>
+static int adj_buffers(ecs_ctx* c, int B, int ckpt) {
+  TRY(ensure_adj(c, B));
+  using G = Geo<N>;
+  const size_t cs = (size_t)B * 2 * G::R * G::KC;
+  const size_t is = (size_t)B * 8 * N * G::KC;
+  const size_t need = (size_t)ckpt * 5 * cs;
+  if (need > c->vt_cap) {
+    if (c->xtape) HIPCHK(hipFree(c->xtape));
+    c->xtape = nullptr;
+    size_t cap0 = 0;
+    TRY(grow(&c->xtape, &cap0, need));
+    c->vt_cap = need;
+  }
+  c->vtape = c->xtape + (size_t)ckpt * cs;
+  TRY(grow(&c->itape, &c->it_cap, (size_t)ckpt * 4 * is));
+  return 0;
+}
+
+// Forward pass of Lawson RK4 storing the compact state at the start of every
+// segment of ckpt steps into ck [nseg][B][2][R][KC]; f is advanced in place.
+template <int N>
+static int run_rk4_ckpt(ecs_ctx* c, double* f, int B, const double* dt, int steps, int masked, int ckpt,
+                        double2* ck, hipStream_t st) {
+  TRY(ensure_cap(c, B));
+  using G = Geo<N>;
+  const size_t cs = (size_t)B * 2 * G::R * G::KC;
+  TRY(upload_scalars(c, 0, dt, B, st));
+  Args a = base_args(c);
+  a.masked_e = masked;
+  TRY(pack<N>(f, c->x0, B, st));
+  HIPCHK(hipMemcpyAsync(ck, c->x0, cs * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  if (steps > 0) TRY((col<N, C_LOAD, false>(a, B, st)));
+  for (int s = 0; s < steps; ++s) {
+    TRY((row<N, R_RHS, false>(a, B, st)));
+    TRY((col<N, C_S1F, false>(a, B, st)));
+    TRY((row<N, R_RHS, false>(a, B, st)));
+    TRY((col<N, C_S2, false>(a, B, st)));
+    TRY((row<N, R_RHS, false>(a, B, st)));
+    TRY((col<N, C_S3, false>(a, B, st)));
+    TRY((row<N, R_RHS, false>(a, B, st)));
+    Args a4 = a;
+    a4.skip_inv = (s == steps - 1);
+    TRY((col<N, C_S4, false>(a4, B, st)));
+    if ((s + 1) % ckpt == 0 && s + 1 < steps)
+      HIPCHK(hipMemcpyAsync(ck + (size_t)((s + 1) / ckpt) * cs, c->x0, cs * sizeof(double2),
+                            hipMemcpyDeviceToDevice, st));
+  }
+  TRY(unpack<N>(c->x0, f, B, masked ? 1 : 0, st));
+  return 0;
+}
+
+// Reverse pass from the checkpoints: fbar (cotangent of the final spectra) is
+// replaced by the cotangent of the initial spectra; hbar[b] = d/dh.
+template <int N>
+static int run_rk4_rev(ecs_ctx* c, const double2* ck, double* fbar, double* hbar, int B, const double* dt, int steps,
+                       int masked, int ckpt, hipStream_t st) {
+  TRY(adj_buffers<N>(c, B, ckpt));
+  using G = Geo<N>;
+  const size_t cs = (size_t)B * 2 * G::R * G::KC;
+  const size_t is = (size_t)B * 8 * N * G::KC;
+  const int nseg = (steps + ckpt - 1) / ckpt;
+  TRY(upload_scalars(c, 0, dt, B, st));
+  Args a = base_args(c);
+  a.masked_e = masked;
+  a.aout = c->AO;
+  a.ain = c->AI;
+  a.xb = c->xb; a.k1b = c->k1b; a.k2b = c->k2b; a.k3b = c->k3b;
+  a.hb = c->hb;
+  a.cstride = cs;
+  TRY(pack<N>(fbar, c->xb, B, st));
+  HIPCHK(hipMemsetAsync(c->hb, 0, cs * sizeof(double), st));
+  auto slot = [&](int t, int s) { return c->itape + ((size_t)t * 4 + s) * is; };
+  auto xt = [&](int t) { return c->xtape + (size_t)t * cs; };
+  auto vt = [&](int t) { return c->vtape + (size_t)t * 4 * cs; };
+  for (int j = nseg - 1; j >= 0; --j) {
+    const int t0 = j * ckpt;
+    const int L = std::min(ckpt, steps - t0);
+    HIPCHK(hipMemcpyAsync(c->x0, ck + (size_t)j * cs, cs * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    Args f = a;
+    f.inter = slot(0, 0);
+    TRY((col<N, C_LOAD, false>(f, B, st)));
+    for (int t = 0; t < L; ++t) {
+      f.vtape = nullptr; f.xtape = nullptr;
+      f.inter = slot(t, 0);
+      TRY((row<N, R_RHS, false>(f, B, st)));
+      f.inter = slot(t, 1); f.vtape = vt(t); f.xtape = xt(t);
+      TRY((col<N, C_S1F, false>(f, B, st)));
+      f.xtape = nullptr;
+      TRY((row<N, R_RHS, false>(f, B, st)));
+      f.inter = slot(t, 2); f.vtape = vt(t) + cs;
+      TRY((col<N, C_S2, false>(f, B, st)));
+      TRY((row<N, R_RHS, false>(f, B, st)));
+      f.inter = slot(t, 3); f.vtape = vt(t) + 2 * cs;
+      TRY((col<N, C_S3, false>(f, B, st)));
+      TRY((row<N, R_RHS, false>(f, B, st)));
+      Args f4 = f;
+      f4.vtape = vt(t) + 3 * cs;
+      f4.inter = (t + 1 < L) ? slot(t + 1, 0) : slot(t, 3);
+      f4.skip_inv = (t + 1 == L);
+      TRY((col<N, C_S4, false>(f4, B, st)));
+    }
+    Args b = a;
+    b.tx = xt(L - 1);
+    b.tv = vt(L - 1);
+    TRY((acol<N, B_ENTRY>(b, B, st)));
+    for (int t = L - 1; t >= 0; --t) {
+      b.tx = xt(t);
+      b.tv = vt(t);
+      b.prim = slot(t, 3);
+      TRY(arow<N>(b, B, st));
+      TRY((acol<N, B_S4>(b, B, st)));
+      b.prim = slot(t, 2);
+      TRY(arow<N>(b, B, st));
+      TRY((acol<N, B_S3>(b, B, st)));
+      b.prim = slot(t, 1);
+      TRY(arow<N>(b, B, st));
+      TRY((acol<N, B_S2>(b, B, st)));
+      b.prim = slot(t, 0);
+      TRY(arow<N>(b, B, st));
+      if (t > 0) {
+        b.txp = xt(t - 1);
+        b.tvp = vt(t - 1);
+        TRY((acol<N, B_S1E>(b, B, st)));
+      } else {
+        TRY((acol<N, B_S1>(b, B, st)));
+      }
+    }
+  }
+  TRY(unpack<N>(c->xb, fbar, B, masked ? 1 : 0, st));
+  hipLaunchKernelGGL((hb_sum_kernel<N>), dim3(B), dim3(256), 0, st, c->hb, c->red);
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(c->hred, c->red, B * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int b2 = 0; b2 < B; ++b2) hbar[b2] = c->hred[b2];
+  return 0;
+}
+
+static size_t ckpt_elems(int n, int B, int steps, int ckpt) {
+  const int kmax = (n - 1) / 3;
+  const int nseg = (steps + ckpt - 1) / ckpt;
+  return (size_t)std::max(nseg, 1) * B * 2 * (2 * kmax + 1) * (kmax + 1);
+}
+
+template <int N>
+static int run_rk4_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar, int B, const double* dt, int steps,
+                       int masked, int ckpt, hipStream_t st) {
+  TRY(ensure_cap(c, B));
+  TRY(grow(&c->ckpt, &c->ckpt_cap, ckpt_elems(N, B, steps, ckpt)));
+  // the forward pass advances a private copy of f0 (held in c->out, full layout not needed)
+  double* fwork = nullptr;
+  const size_t full = (size_t)B * 2 * N * Geo<N>::NH * 2;
+  HIPCHK(hipMallocAsync((void**)&fwork, full * sizeof(double), st));
+  HIPCHK(hipMemcpyAsync(fwork, f0, full * sizeof(double), hipMemcpyDeviceToDevice, st));
+  int rc = run_rk4_ckpt<N>(c, fwork, B, dt, steps, masked, ckpt, c->ckpt, st);
+  if (rc == 0) rc = run_rk4_rev<N>(c, c->ckpt, fbar, hbar, B, dt, steps, masked, ckpt, st);
+  HIPCHK(hipFreeAsync(fwork, st));
+  return rc;
+}
+
 extern "C" int ecs_state_vel_vjp(ecs_ctx* c, const double* in, const double* cot, double* grad, int batch,
                                  int diss, void* stream) {
-  (void)c; (void)in; (void)cot; (void)grad; (void)batch; (void)diss; (void)stream;
-  return fail(3, "ecs_state_vel_vjp: adjoint kernels not built in this version");
+  if (!c || !in || !cot || !grad || batch < 1) return fail(1, "ecs_state_vel_vjp: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  DISPATCH_N(c->n, run_state_vel_vjp, c, in, cot, grad, batch, diss, st);
 }
 
 extern "C" int ecs_lawson_rk4_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar, int batch,
                                   const double* dt, int steps, int masked, int ckpt, void* stream) {
-  (void)c; (void)f0; (void)fbar; (void)hbar; (void)batch; (void)dt; (void)steps; (void)masked; (void)ckpt;
-  (void)stream;
-  return fail(3, "ecs_lawson_rk4_vjp: adjoint kernels not built in this version");
+  if (!c || !f0 || !fbar || !hbar || !dt || batch < 1 || steps < 1 || ckpt < 1)
+    return fail(1, "ecs_lawson_rk4_vjp: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  DISPATCH_N(c->n, run_rk4_vjp, c, f0, fbar, hbar, batch, dt, steps, masked, ckpt, st);
+}
+
+extern "C" long long ecs_ckpt_doubles(const ecs_ctx* c, int batch, int steps, int ckpt) {
+  if (!c || batch < 1 || steps < 1 || ckpt < 1) return -1;
+  return (long long)(2 * ckpt_elems(c->n, batch, steps, ckpt));
+}
+
+extern "C" int ecs_lawson_rk4_fwd_ckpt(ecs_ctx* c, double* f, int batch, const double* dt, int steps, int masked,
+                                       int ckpt, double* ck, void* stream) {
+  if (!c || !f || !dt || !ck || batch < 1 || steps < 1 || ckpt < 1)
+    return fail(1, "ecs_lawson_rk4_fwd_ckpt: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  DISPATCH_N(c->n, run_rk4_ckpt, c, f, batch, dt, steps, masked, ckpt, (double2*)ck, st);
+}
+
+extern "C" int ecs_lawson_rk4_vjp_ckpt(ecs_ctx* c, const double* ck, double* fbar, double* hbar, int batch,
+                                       const double* dt, int steps, int masked, int ckpt, void* stream) {
+  if (!c || !ck || !fbar || !hbar || !dt || batch < 1 || steps < 1 || ckpt < 1)
+    return fail(1, "ecs_lawson_rk4_vjp_ckpt: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  DISPATCH_N(c->n, run_rk4_rev, c, (const double2*)ck, fbar, hbar, batch, dt, steps, masked, ckpt, st);
+}
+
+// ------------------------------------------------------------------ timing hook
+// Average duration (ms) of one launch of a fused RHS kernel, timed with HIP
+// events on `stream` over `reps` back-to-back launches of the same kernel on
+// batch spectra already resident in the context's workspace.
+// which: 0 = column kernel (stage 2 glue), 1 = row kernel, 2 = tangent column
+// kernel, 3 = tangent row kernel.
+template <int N>
+static int run_time_kernel(ecs_ctx* c, int B, int which, int reps, double* ms, hipStream_t st) {
+  TRY(ensure_cap(c, B));
+  std::vector<double> dt(B, 1e-3);
+  TRY(upload_scalars(c, 0, dt.data(), B, st));
+  TRY(upload_scalars(c, 1, dt.data(), B, st));
+  Args a = base_args(c);
+  // defined inputs: zero state, one load + row pass to fill I and O
+  HIPCHK(hipMemsetAsync(c->x0, 0, c->compact_elems() * B * sizeof(double2), st));
+  HIPCHK(hipMemsetAsync(c->dx0, 0, c->compact_elems() * B * sizeof(double2), st));
+  HIPCHK(hipMemsetAsync(c->acc, 0, c->compact_elems() * B * sizeof(double2), st));
+  HIPCHK(hipMemsetAsync(c->dacc, 0, c->compact_elems() * B * sizeof(double2), st));
+  TRY((col<N, C_LOAD, true>(a, B, st)));
+  TRY((row<N, R_RHS, true>(a, B, st)));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  auto launch = [&]() -> int {
+    switch (which) {
+      case 0: return col<N, C_S2, false>(a, B, st);
+      case 1: return row<N, R_RHS, false>(a, B, st);
+      case 2: return col<N, C_S2, true>(a, B, st);
+      default: return row<N, R_RHS, true>(a, B, st);
+    }
+  };
+  for (int i = 0; i < 3; ++i) TRY(launch());
+  HIPCHK(hipEventRecord(e0, st));
+  for (int i = 0; i < reps; ++i) TRY(launch());
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  *ms = (double)t / reps;
+  return 0;
+}
+
+extern "C" int ecs_time_kernel(ecs_ctx* c, int batch, int which, int reps, double* ms, void* stream) {
+  if (!c || !ms || batch < 1 || reps < 1 || which < 0 || which > 3) return fail(1, "ecs_time_kernel: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  DISPATCH_N(c->n, run_time_kernel, c, batch, which, reps, ms, st);
 }

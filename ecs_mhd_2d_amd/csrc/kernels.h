@@ -105,6 +105,25 @@ struct Args {
   const int* s_nacc;       // number of accepted attempts per candidate
   int sstride, rstride;
   double h0;               // adaptive: first step (pseudo-time)
+  // tape mode of the forward (checkpointed reverse pass): when non-null the
+  // stage glue stores v(y_s) at vtape[ci] and the step's x0 at xtape[ci]
+  double2* vtape;
+  double2* xtape;
+  // reverse pass (adjoint kernels)
+  double2* xb;             // state cotangent
+  double2* k1b;            // stage-increment cotangents
+  double2* k2b;
+  double2* k3b;
+  double* hb;              // per-mode accumulator of d/dh, compact real [B][2][R][KC]
+  const double2* tx;       // tape: x0 of the current step
+  const double2* tv;       // tape: v1..v4 of the current step, stage stride cstride
+  const double2* txp;      // tape of the previous step (fused entry)
+  const double2* tvp;
+  size_t cstride;          // elements of one compact batch array
+  double2* aout;           // adjoint column output AO [B][2][N][KC]
+  double2* ain;            // adjoint row output AI [B][8][N][KC]
+  const double2* prim;     // primal column output for the row-adjoint [B][8][N][KC]
+  const double2* gin;      // cotangent input of a standalone RHS adjoint
   int include_diss;
   int adaptive;            // AdaptKind
   int masked_e;            // fixed step: e = exp(dt/2 L) * mask

@@ -188,6 +188,8 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, int bb, size_
   const double de2 = 2.0 * e * de;
   const double2 x0 = a.x0[ci];
   const double2 dx0 = TAN ? a.dx0[ci] : make_double2(0.0, 0.0);
+  if (MODE == C_S1F && a.xtape) a.xtape[ci] = x0;
+  if ((MODE == C_S1F || MODE == C_S2 || MODE == C_S3 || MODE == C_S4) && a.vtape) a.vtape[ci] = vh;
   if (MODE == C_S1F || MODE == C_A0 || MODE == C_A) {
     const double2 k1 = cscale(bk.dt, v1);
     const double2 ek1 = cscale(e, k1);

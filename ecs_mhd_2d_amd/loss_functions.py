@@ -1,0 +1,270 @@
+"""RPO objective, loss and their derivatives (jax_scripts/lib/loss_functions.py:9-148).
+
+Inputs follow the reference: ``input_dict = {"fields": real [2, N, N],
+"T": scalar, "sx": scalar}`` (float64 CUDA tensors or Python floats) and a
+``param_dict`` from ``mhd.make_param_dict`` plus the mode items
+(``num_checkpoints``/``ministeps`` for "Lawson_RK4", ``adaptive_dict`` for
+"Lawson_RK43", and ``shift_reflect_ny``, ``rot``).  Time integration runs in
+the fused HIP steppers; the spectral transforms at the boundary of an
+objective evaluation (rfft2 of the input fields, irfft2 of the output) use
+torch.fft on the GPU.
+
+Derivatives replace jax.jvp / jax.vjp / jax.value_and_grad of these
+functions: ``objective_RPO_jvp`` (utils.py:108), ``objective_RPO_vjp``
+(newton.py:168-169) and ``RPOLoss`` / ``loss_RPO_value_and_grad``
+(adjoint_descent.py:90-96), the last a torch.autograd.Function whose backward
+is the hand-written adjoint kernel with checkpoint/recompute every 64 steps.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib, _runtime, mhd, symmetry
+
+MODES = ("Lawson_RK4", "Lawson_RK43")
+CKPT = 64  # steps per reverse-pass segment
+
+
+def _f(v) -> float:
+    return float(v.item() if isinstance(v, torch.Tensor) else v)
+
+
+def _fields(inp) -> torch.Tensor:
+    f = inp["fields"]
+    if not isinstance(f, torch.Tensor):
+        f = torch.as_tensor(f)
+    if not f.is_cuda:
+        f = f.cuda()
+    return f.to(torch.float64).contiguous()
+
+
+def _rfft2(x):
+    return torch.fft.rfft2(x).contiguous()
+
+
+def _irfft2(z, n):
+    return torch.fft.irfft2(z, s=(n, n))
+
+
+def _col_weight(n, inner, edge, device):
+    w = torch.full((1, n // 2 + 1), inner, dtype=torch.float64, device=device)
+    w[0, 0] = edge
+    w[0, -1] = edge
+    return w
+
+
+def irfft2_T(g, n):
+    """Real transpose of irfft2: (w'/N^2) rfft2(g), w' = 1 on ky = 0, N/2 and 2 elsewhere."""
+    return _col_weight(n, 2.0, 1.0, g.device) / (n * n) * _rfft2(g)
+
+
+def rfft2_T(c, n):
+    """Real transpose of rfft2: N^2 irfft2(w c), w = 1 on ky = 0, N/2 and 1/2 elsewhere."""
+    return n * n * _irfft2(_col_weight(n, 0.5, 1.0, c.device) * c, n)
+
+
+def _rk4_steps(pd):
+    nc, ms = int(pd["num_checkpoints"]), int(pd["ministeps"])
+    return nc, ms, nc * ms
+
+
+def _rk4_h(T, pd):
+    nc, ms, _ = _rk4_steps(pd)
+    return (T / nc) / ms  # loss_functions.py:63 then timestepping.py:69
+
+
+def integrate(f0, T, param_dict, mode):
+    """The time integration of mismatch_RPO (loss_functions.py:48-88)."""
+    ctx = mhd.context(param_dict)
+    if mode == "Lawson_RK4":
+        _, _, steps = _rk4_steps(param_dict)
+        return ctx.lawson_rk4(f0, _rk4_h(T, param_dict), steps, masked=True), None
+    if mode == "Lawson_RK43":
+        ad = param_dict["adaptive_dict"]
+        f, infos = ctx.lawson_rk43(f0, T, 1e-2, ad["atol"], ad["max_steps_per_checkpoint"], ad["checkpoints"])
+        return f, infos[0]
+    raise ValueError(f"You selected mode = {mode}, which does not exist.")
+
+
+def mismatch_RPO(input_dict, param_dict, mode):
+    """(f0, f, info): f0 = rfft2(fields) (unmasked), f = g(Phi_T f0) (loss_functions.py:9-96)."""
+    f0 = _rfft2(_fields(input_dict))
+    f, info = integrate(f0, _f(input_dict["T"]), param_dict, mode)
+    f = symmetry.apply(f, _f(input_dict["sx"]), param_dict)
+    return f0, f, info
+
+
+def loss_RPO(input_dict, param_dict, mode):
+    """(mean(irfft2(f - f0)^2), info)  (loss_functions.py:99-111)."""
+    f0, f, info = mismatch_RPO(input_dict, param_dict, mode)
+    n = f0.shape[-2]
+    return torch.mean(torch.square(_irfft2(f - f0, n))), info
+
+
+def phase_vectors(f0, param_dict):
+    """Stop-gradient phase vectors vt = irfft2(state_vel(f0, diss)), vx = irfft2(i kx f0)
+    (loss_functions.py:115-122)."""
+    n = f0.shape[-2]
+    vt = _irfft2(mhd.state_vel(f0, param_dict, True), n)
+    vx = _irfft2(1j * param_dict["kx"] * f0, n)
+    return vt, vx
+
+
+def phase_conditions(f0, param_dict):
+    """(pt, px): exactly zero in value (x - stop_gradient(x), loss_functions.py:127-128)."""
+    z = torch.zeros((), dtype=torch.float64, device=f0.device)
+    return z, z.clone()
+
+
+def objective_RPO(input_dict, param_dict, mode):
+    """{"fields": irfft2(f0 - f), "T": pt, "sx": px}  (loss_functions.py:132-148)."""
+    f0, f, _ = mismatch_RPO(input_dict, param_dict, mode)
+    pt, px = phase_conditions(f0, param_dict)
+    return {"fields": _irfft2(f0 - f, f0.shape[-2]), "T": pt, "sx": px}
+
+
+class Linearization:
+    """Primal data shared by every JVP at one input (one Newton iteration):
+    the adaptive schedule (decisions follow the primal) and the phase vectors."""
+
+    def __init__(self, input_dict, param_dict, mode):
+        if mode not in MODES:
+            raise ValueError(f"You selected mode = {mode}, which does not exist.")
+        self.pd, self.mode = param_dict, mode
+        self.fields = _fields(input_dict)
+        self.T, self.sx = _f(input_dict["T"]), _f(input_dict["sx"])
+        self.n = self.fields.shape[-1]
+        self.f0 = _rfft2(self.fields)
+        self.info = None
+        if mode == "Lawson_RK43":
+            _, self.info = integrate(self.f0, self.T, param_dict, mode)
+        self.vt, self.vx = phase_vectors(self.f0, param_dict)
+
+    def jvp(self, tangent):
+        """J(x) t for the tangent dict t (utils.py:108)."""
+        ctx = mhd.context(self.pd)
+        dfields = _fields(tangent)
+        dT, dsx = _f(tangent["T"]), _f(tangent["sx"])
+        df0 = _rfft2(dfields)
+        if self.mode == "Lawson_RK4":
+            _, _, steps = _rk4_steps(self.pd)
+            f, df = ctx.lawson_rk4_jvp(self.f0, df0, _rk4_h(self.T, self.pd), _rk4_h(dT, self.pd), steps, True)
+        else:
+            f, df = ctx.lawson_rk43_jvp(self.f0, df0, self.T, dT, [self.info])
+        dg = symmetry.apply_jvp(f, df, self.sx, dsx, self.pd)
+        return {"fields": _irfft2(df0 - dg, self.n),
+                "T": torch.mean(self.vt * dfields),
+                "sx": torch.mean(self.vx * dfields)}
+
+
+def objective_RPO_jvp(primal, tangent, param_dict, mode):
+    """jax.jvp(objective_RPO)(primal, tangent)[1] (utils.py:108)."""
+    return Linearization(primal, param_dict, mode).jvp(tangent)
+
+
+def _rk4_vjp_from(ctx, f0, fbar, h, steps, pd):
+    """Reverse pass of the Lawson RK4 integration from f0 (two-phase C-ABI)."""
+    lib = _lib.load()
+    nd = lib.ecs_ckpt_doubles(ctx.handle, 1, steps, CKPT)
+    ck = torch.empty(int(nd), dtype=torch.float64, device=f0.device)
+    fw = f0.clone().contiguous()
+    hp = _lib.dptr([h])
+    _lib.check(lib.ecs_lawson_rk4_fwd_ckpt(ctx.handle, fw.data_ptr(), 1, hp, steps, 1, CKPT, ck.data_ptr(),
+                                           _runtime._stream()), "ecs_lawson_rk4_fwd_ckpt")
+    g = fbar.clone().contiguous()
+    hb = (ctypes.c_double * 1)()
+    _lib.check(lib.ecs_lawson_rk4_vjp_ckpt(ctx.handle, ck.data_ptr(), g.data_ptr(), hb, 1, hp, steps, 1, CKPT,
+                                           _runtime._stream()), "ecs_lawson_rk4_vjp_ckpt")
+    return fw, g, hb[0]
+
+
+def objective_RPO_vjp(primal, cot, param_dict, mode="Lawson_RK4"):
+    """J(x)^T c for the cotangent dict c (newton.py:168-169), mode "Lawson_RK4"."""
+    if mode != "Lawson_RK4":
+        raise NotImplementedError("the reverse pass is built for mode 'Lawson_RK4'")
+    ctx = mhd.context(param_dict)
+    fields = _fields(primal)
+    n = fields.shape[-1]
+    T, sx = _f(primal["T"]), _f(primal["sx"])
+    nc, ms, steps = _rk4_steps(param_dict)
+    f0 = _rfft2(fields)
+    cbar = irfft2_T(_fields(cot), n)                 # cotangent of (f0 - g)
+    # g = sym(fT): the symmetry transpose of -cbar does not need fT; sxbar does
+    fT, f0bar, hbar = _rk4_vjp_from(ctx, f0, _sym_T(-cbar, sx, param_dict), _rk4_h(T, param_dict), steps,
+                                    param_dict)
+    _, sxbar = symmetry.apply_vjp(fT, sx, param_dict, -cbar)
+    vt, vx = phase_vectors(f0, param_dict)
+    n2 = 2 * n * n
+    gfields = rfft2_T(cbar + f0bar, n) + (_f(cot["T"]) * vt + _f(cot["sx"]) * vx) / n2
+    return {"fields": gfields, "T": torch.tensor(hbar / steps, dtype=torch.float64, device=fields.device),
+            "sx": sxbar}
+
+
+def _sym_T(gbar, sx, pd):
+    """fbar part of symmetry.apply_vjp (independent of the state)."""
+    if pd.get("rot", False):
+        gbar = symmetry._sign(gbar) * torch.conj(gbar)
+    gbar = symmetry.shift_reflect_T(gbar, pd.get("shift_reflect_ny", 0), pd)
+    return torch.conj(torch.exp(1j * pd["kx"] * sx)) * gbar
+
+
+class RPOLoss(torch.autograd.Function):
+    """loss_RPO in mode "Lawson_RK4" with a hand-written backward.
+
+    forward: one fused trajectory that also stores the state every 64 steps;
+    backward: the adjoint kernels recompute each 64-step segment from its
+    checkpoint (keeping the segment's stage data on the device) and run the
+    transposed stages in reverse.
+    """
+
+    @staticmethod
+    def forward(ctx_, fields, T, sx, param_dict):
+        ctx = mhd.context(param_dict)
+        n = fields.shape[-1]
+        nc, ms, steps = _rk4_steps(param_dict)
+        Tf, sxf = float(T), float(sx)
+        h = _rk4_h(Tf, param_dict)
+        f0 = _rfft2(fields.detach().to(torch.float64))
+        lib = _lib.load()
+        nd = lib.ecs_ckpt_doubles(ctx.handle, 1, steps, CKPT)
+        ck = torch.empty(int(nd), dtype=torch.float64, device=f0.device)
+        fT = f0.clone().contiguous()
+        _lib.check(lib.ecs_lawson_rk4_fwd_ckpt(ctx.handle, fT.data_ptr(), 1, _lib.dptr([h]), steps, 1, CKPT,
+                                               ck.data_ptr(), _runtime._stream()), "ecs_lawson_rk4_fwd_ckpt")
+        g = symmetry.apply(fT, sxf, param_dict)
+        diff = _irfft2(g - f0, n)
+        ctx_.save_for_backward(ck, fT, diff)
+        ctx_.pd, ctx_.h, ctx_.steps, ctx_.sx, ctx_.n = param_dict, h, steps, sxf, n
+        return torch.mean(diff * diff)
+
+    @staticmethod
+    def backward(ctx_, gl):
+        ck, fT, diff = ctx_.saved_tensors
+        pd, n = ctx_.pd, ctx_.n
+        ctx = mhd.context(pd)
+        dbar = irfft2_T(diff * (gl / (n * n)), n)         # cotangent of g - f0
+        fTbar, sxbar = symmetry.apply_vjp(fT, ctx_.sx, pd, dbar)
+        g = fTbar.clone().contiguous()
+        hb = (ctypes.c_double * 1)()
+        lib = _lib.load()
+        _lib.check(lib.ecs_lawson_rk4_vjp_ckpt(ctx.handle, ck.data_ptr(), g.data_ptr(), hb, 1,
+                                               _lib.dptr([ctx_.h]), ctx_.steps, 1, CKPT, _runtime._stream()),
+                   "ecs_lawson_rk4_vjp_ckpt")
+        gfields = rfft2_T(g - dbar, n)
+        Tbar = torch.tensor(hb[0] / ctx_.steps, dtype=torch.float64, device=diff.device)
+        return gfields, Tbar, sxbar, None
+
+
+def loss_RPO_value_and_grad(input_dict, param_dict, mode="Lawson_RK4"):
+    """jax.value_and_grad(loss_RPO, has_aux=True) for mode "Lawson_RK4":
+    returns ((loss, None), grad_dict)  (adjoint_descent.py:90-96)."""
+    if mode != "Lawson_RK4":
+        raise NotImplementedError("the reverse pass is built for mode 'Lawson_RK4'")
+    fields = _fields(input_dict).requires_grad_(True)
+    T = torch.tensor(_f(input_dict["T"]), dtype=torch.float64, device=fields.device, requires_grad=True)
+    sx = torch.tensor(_f(input_dict["sx"]), dtype=torch.float64, device=fields.device, requires_grad=True)
+    loss = RPOLoss.apply(fields, T, sx, param_dict)
+    gf, gT, gs = torch.autograd.grad(loss, (fields, T, sx))
+    return (loss.detach(), None), {"fields": gf, "T": gT, "sx": gs}

@@ -1,0 +1,83 @@
+"""Newton-GMRES and adjoint-descent iterations (jax_scripts/newton.py:151-215,
+jax_scripts/adjoint_descent.py:93-123) on the fused GPU path."""
+from __future__ import annotations
+
+import time
+
+import torch
+
+from . import adam, linalg, loss_functions, utils
+
+
+def dealias(fields, param_dict):
+    """irfft2(mask * rfft2(fields)) (newton.py:211-214, adjoint_descent.py:111-114)."""
+    n = fields.shape[-1]
+    return torch.fft.irfft2(param_dict["mask"] * torch.fft.rfft2(fields), s=(n, n))
+
+
+def newton_gmres_update(input_dict, param_dict, mode, inner=64, outer=1, do_line_search=True,
+                        default_damp=0.1, use_transpose=False):
+    """One Newton-GMRES iteration (newton.py:151-215).
+
+    Returns (new_input_dict, log) with the reference's logged quantities
+    (rel_err = |f| / |x_new.fields| as newton.py:195 computes it, |f|,
+    fwall, gmreswall, gmres_rel_res, damp, T, sx) plus the per-Krylov-step
+    least-squares residual history.
+    """
+    n = input_dict["fields"].shape[-1]
+    t0 = time.perf_counter()
+    f = loss_functions.objective_RPO(input_dict, param_dict, mode)
+    b = utils.flatten(f)
+    torch.cuda.synchronize()
+    fwall = time.perf_counter() - t0
+    lin = loss_functions.Linearization(input_dict, param_dict, mode)
+    A = lambda v: utils.flatten(lin.jvp(utils.unflatten(v, n)))
+    precond = []
+    if use_transpose:
+        if mode != "Lawson_RK4":
+            raise NotImplementedError("use_transpose needs the reverse pass (mode 'Lawson_RK4')")
+        A_T = lambda v: utils.flatten(loss_functions.objective_RPO_vjp(input_dict, utils.unflatten(v, n),
+                                                                        param_dict, mode))
+        precond = [A_T]
+    t1 = time.perf_counter()
+    step, gres, hist = linalg.gmres(A, b, inner, outer, preconditioner_list=precond, return_history=True)
+    torch.cuda.synchronize()
+    gwall = time.perf_counter() - t1
+    x = utils.flatten(input_dict)
+    obj = lambda d: loss_functions.objective_RPO(d, param_dict, mode)
+    unravel = lambda v: utils.unflatten(v, n)
+    if do_line_search:
+        x, damp = utils.line_search_unravel(x, step, obj, unravel, b, max_iters=20)
+    else:
+        damp = default_damp
+        x = x - damp * step
+    new = unravel(x)
+    rel_err = float(torch.linalg.vector_norm(f["fields"]) / torch.linalg.vector_norm(new["fields"]))
+    new = {"fields": dealias(new["fields"].contiguous(), param_dict), "T": new["T"].clone(), "sx": new["sx"].clone()}
+    log = {"rel_err": rel_err, "f_norm": float(torch.linalg.vector_norm(b)), "fwall": fwall, "gmreswall": gwall,
+           "gmres_rel_res": gres, "damp": damp, "T": float(new["T"]), "sx": float(new["sx"]),
+           "krylov_residuals": hist["krylov_residuals"], "matvecs": hist["matvecs"]}
+    return new, log
+
+
+class AdjointDescent:
+    """ADAM on the loss_RPO gradient (adjoint_descent.py:79-123): lr 1e-2,
+    betas (0.9, 0.999), eps 1e-6, fields dealiased after every step."""
+
+    def __init__(self, input_dict, param_dict, lr=1e-2, beta1=0.9, beta2=0.999, eps=1e-6):
+        self.params = {"fields": loss_functions._fields(input_dict),
+                       "T": torch.as_tensor(input_dict["T"], dtype=torch.float64, device="cuda").reshape(()),
+                       "sx": torch.as_tensor(input_dict["sx"], dtype=torch.float64, device="cuda").reshape(())}
+        self.pd = param_dict
+        self.m, self.v = adam.init_adam(self.params)
+        self.t = 0
+        self.hyper = (lr, beta1, beta2, eps)
+
+    def step(self):
+        (loss, _), grad = loss_functions.loss_RPO_value_and_grad(self.params, self.pd, "Lawson_RK4")
+        lr, b1, b2, eps = self.hyper
+        self.params, self.m, self.v = adam.adam_update(self.params, grad, self.m, self.v, self.t + 1, lr=lr,
+                                                       beta1=b1, beta2=b2, eps=eps)
+        self.params["fields"] = dealias(self.params["fields"], self.pd)
+        self.t += 1
+        return float(loss)

@@ -79,6 +79,19 @@ int ecs_lawson_rk4_jvp(ecs_ctx* ctx, double* f, double* df, int batch, const dou
 int ecs_lawson_rk4_vjp(ecs_ctx* ctx, const double* f0, double* fbar_inout, double* hbar, int batch,
                        const double* dt, int steps, int masked, int ckpt, void* stream);
 
+/* Two-phase form of ecs_lawson_rk4_vjp for callers that keep the forward
+ * result (torch.autograd.Function): ecs_lawson_rk4_fwd_ckpt advances f in
+ * place like ecs_lawson_rk4 and stores the compact state at the start of
+ * every segment of `ckpt` steps into the caller's device buffer ck of
+ * ecs_ckpt_doubles() doubles; ecs_lawson_rk4_vjp_ckpt then runs the reverse
+ * pass from ck (recomputing each segment), replacing fbar_inout (cotangent
+ * of the final spectra) by that of the initial spectra, and hbar = d/dh. */
+long long ecs_ckpt_doubles(const ecs_ctx* ctx, int batch, int steps, int ckpt);
+int ecs_lawson_rk4_fwd_ckpt(ecs_ctx* ctx, double* f, int batch, const double* dt, int steps, int masked, int ckpt,
+                            double* ck, void* stream);
+int ecs_lawson_rk4_vjp_ckpt(ecs_ctx* ctx, const double* ck, double* fbar_inout, double* hbar, int batch,
+                            const double* dt, int steps, int masked, int ckpt, void* stream);
+
 /* lawson_rk43(x, vel_fn, diag_L, t, h, atol, max_steps_per_checkpoint,
  * checkpoints)  (timestepping.py:151-335).  x: full spectra, updated in
  * place.  T: host [batch].  Synchronous: returns when every candidate has
@@ -107,6 +120,13 @@ int ecs_lawson_rk43_jvp(ecs_ctx* ctx, double* x, double* dx, int batch, const do
 int ecs_mgs(double* Q, int ldq, int n, int k, double* w, double* h, void* stream);
 /* out[0] = <a, b> */
 int ecs_dot(const double* a, const double* b, int n, double* out, void* stream);
+
+/* Measurement hook (no counterpart in the reference, which has no native
+ * code): average duration in ms of one launch of a fused RHS kernel over
+ * `reps` back-to-back launches on `batch` resident spectra, timed with HIP
+ * events on `stream`.  which: 0 column kernel, 1 row kernel, 2 tangent
+ * column kernel, 3 tangent row kernel.  Synchronous. */
+int ecs_time_kernel(ecs_ctx* ctx, int batch, int which, int reps, double* ms, void* stream);
 
 #ifdef __cplusplus
 }
