@@ -72,12 +72,38 @@ def cpu_baseline_matvec(fields_np, T, sx, lin_info, budget_s=15.0):
                       f"({t:.1f} s), scaled linearly to the full trajectory"}
 
 
+def probe():
+    """Short fixed workload for kernel profiles: Lawson RK4 at N=128 (batch 1),
+    N=256 (batch 1 and 64), and one RK4(3) JVP at N=128 (T=0.3, atol=1e-8)."""
+    import torch
+    from ecs_mhd_2d_amd import loss_functions as LF, mhd, utils
+    from ecs_mhd_2d_amd.synthetic import spectrum
+    for n, B, steps in ((128, 1, 32), (256, 1, 16), (256, 64, 8)):
+        pd = mhd.make_param_dict(n, 1 / 40, 1 / 40)
+        f = torch.as_tensor(np.stack([spectrum(n, s) for s in range(B)])).cuda()
+        mhd.context(pd).lawson_rk4(f, 0.004, steps)
+        torch.cuda.synchronize()
+        print("probe rk4", n, B, steps, flush=True)
+    n = 128
+    pd = mhd.make_param_dict(n, 1 / 40, 1 / 40)
+    pd.update({"shift_reflect_ny": 0, "rot": False,
+               "adaptive_dict": {"atol": 1e-8, "checkpoints": 8, "max_steps_per_checkpoint": 64}})
+    inp = {"fields": torch.as_tensor(np.fft.irfft2(spectrum(n, 0), s=(n, n))).cuda(), "T": 0.3, "sx": 0.1}
+    lin = LF.Linearization(inp, pd, "Lawson_RK43")
+    v = torch.randn(2 * n * n + 2, dtype=torch.float64, device="cuda")
+    lin.jvp(utils.unflatten(v, n))
+    torch.cuda.synchronize()
+    print("probe rk43 jvp", lin.info["accepted"], lin.info["rejected"], flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="run a short fixed kernel workload for rocprofv3 and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build targets a single MI355X); got --gpus {args.gpus}",
@@ -87,6 +113,9 @@ def main():
     from ecs_mhd_2d_amd import _lib, _runtime, loss_functions as LF, mhd, utils
 
     torch.cuda.init()
+    if args.probe:
+        probe()
+        return
     n = 128
     pd = mhd.make_param_dict(n, 1 / 40, 1 / 40, (0.0, 0.1))
     pd.update({"shift_reflect_ny": 0, "rot": False,
@@ -141,6 +170,12 @@ def main():
     row_b_ms = ms.value
     ach_b = 64 * 10 * zbytes(256) / (row_b_ms * 1e-3) / 1e9
 
+    tiles = {}
+    for nn, bb in ((128, 1), (256, 64)):
+        arr = (ctypes.c_int * 4)()
+        lib.ecs_tuned_tiles(nn, bb, arr)
+        tiles[f"n{nn}_b{bb}"] = {"col_ty": arr[0], "col_tan_ty": arr[1], "row_tx": arr[2], "row_tan_tx": arr[3]}
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_matvec(fields.cpu().numpy(), 4.3, 0.1, info)
@@ -156,12 +191,15 @@ def main():
         "rk4_steps_per_s_n256_b1": rk4[1], "rk4_steps_per_s_n256_b64": rk4[64],
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "row_kernel<128,2,256,R_RHS,tangent> (batch 1)", "avg_ms": row_t_ms,
+                     "kernel": f"row_kernel<128,TX={tiles['n128_b1']['row_tan_tx']},tangent> (batch 1)",
+                     "avg_ms": row_t_ms,
                      "algorithmic_bytes": bytes_row_t},
         "roofline_batched": {"bound": "hbm", "achieved": ach_b, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": ach_b / HBM_PEAK_GBS, "kernel": "row_kernel<256,1,256,R_RHS> (batch 64)",
+                             "frac": ach_b / HBM_PEAK_GBS,
+                             "kernel": f"row_kernel<256,TX={tiles['n256_b64']['row_tx']}> (batch 64)",
                              "avg_ms": row_b_ms, "algorithmic_bytes": 64 * 10 * zbytes(256)},
         "cpu_baseline": cpu,
+        "autotuned_tiles": tiles,
     }
     print(json.dumps(line))
 

@@ -120,7 +120,10 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
   constexpr bool FWD = !(MODE == B_G || MODE == B_ENTRY);
   constexpr bool INV = !(MODE == B_OUT || MODE == B_S1);
   constexpr int NSEQ = FWD ? 4 * TY : TY;
-  __shared__ double2 sm[NSEQ * N];
+  constexpr int SS = Seq<N>::S;
+  __shared__ double2 sm[NSEQ * SS];
+  __shared__ double2 twl[N];
+  load_twiddles<N, NT>(twl, a.tw);
   const int b = blockIdx.z, c = blockIdx.y, ky0 = blockIdx.x * TY, tid = threadIdx.x;
   if (FWD) {
     const double2* AI = a.ain + (size_t)b * 8 * N * KC;
@@ -132,10 +135,10 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
       const int ky = ky0 + cy;
       double2 v = make_double2(0.0, 0.0);
       if (ky < KC) v = AI[((size_t)(4 * c + d) * N + x) * KC + ky];
-      sm[(d * TY + cy) * N + x] = v;
+      sm[(d * TY + cy) * SS + pidx(x)] = v;
     }
     __syncthreads();
-    fft_lds<N, 4 * TY, N, NT, false>(sm, a.tw);
+    fft_lds<N, 4 * TY, SS, NT, false>(sm, twl);
   }
   constexpr double invN2 = 1.0 / ((double)N * (double)N);
   for (int idx = tid; idx < R * TY; idx += NT) {
@@ -155,7 +158,7 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
       double2 s = make_double2(0.0, 0.0);
 #pragma unroll
       for (int d = 0; d < 4; ++d) {
-        const double2 gd = sm[(d * TY + cy) * N + xr];
+        const double2 gd = sm[(d * TY + cy) * SS + pidx(xr)];
         s = cadd(s, cscale(mult[d] * invN2, make_double2(gd.y, -gd.x)));
       }
       yb = s;
@@ -165,7 +168,7 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
     if (INV) {
       const bool dc = (kx == 0 && ky == 0);
       const double m = dc ? 0.0 : (c == 0 ? 1.0 : (double)(kx * kx + ky * ky));
-      sm[(0 * TY + cy) * N + xr] = cscale(m, g);
+      sm[(0 * TY + cy) * SS + pidx(xr)] = cscale(m, g);
     }
   }
   if (!INV) return;
@@ -174,17 +177,17 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
   for (int idx = tid; idx < TY * GAP; idx += NT) {
     const int s = idx / GAP;
     const int gg = idx - s * GAP;
-    sm[s * N + KMAX + 1 + gg] = make_double2(0.0, 0.0);
+    sm[s * SS + pidx(KMAX + 1 + gg)] = make_double2(0.0, 0.0);
   }
   __syncthreads();
-  fft_lds<N, TY, N, NT, true>(sm, a.tw);
+  fft_lds<N, TY, SS, NT, true>(sm, twl);
   double2* AO = a.aout + (size_t)b * 2 * N * KC;
   for (int idx = tid; idx < N * TY; idx += NT) {
     const int cy = idx % TY;
     const int x = idx / TY;
     const int ky = ky0 + cy;
     if (ky >= KC) continue;
-    AO[((size_t)c * N + x) * KC + ky] = sm[cy * N + x];
+    AO[((size_t)c * N + x) * KC + ky] = sm[cy * SS + pidx(x)];
   }
 }
 
@@ -196,7 +199,10 @@ __global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
   constexpr int KC = G::KC;
   constexpr int NP = 5;  // 4 primal pairs + 1 adjoint pair
   constexpr int NHALF = N / 2 + 1;
-  __shared__ double2 sm[TX * NP * N];
+  constexpr int SS = Seq<N>::S;
+  __shared__ double2 sm[TX * NP * SS];
+  __shared__ double2 twl[N];
+  load_twiddles<N, NT>(twl, a.tw);
   const int b = blockIdx.z, xbase = blockIdx.x * TX, tid = threadIdx.x;
   const double2* P = a.prim + (size_t)b * 8 * N * KC;
   const double2* AO = a.aout + (size_t)b * 2 * N * KC;
@@ -218,16 +224,16 @@ __global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
         B = cscale(w, AO[((size_t)1 * N + x) * KC + k]);
       }
     }
-    double2* z = sm + (xl * NP + p) * N;
+    double2* z = sm + (xl * NP + p) * SS;
     if (k == 0 || k == N / 2) {
-      z[k] = make_double2(A.x, B.x);
+      z[pidx(k)] = make_double2(A.x, B.x);
     } else {
-      z[k] = make_double2(A.x - B.y, A.y + B.x);
-      z[N - k] = make_double2(A.x + B.y, B.x - A.y);
+      z[pidx(k)] = make_double2(A.x - B.y, A.y + B.x);
+      z[pidx(N - k)] = make_double2(A.x + B.y, B.x - A.y);
     }
   }
   __syncthreads();
-  fft_lds<N, TX * NP, N, NT, true>(sm, a.tw);
+  fft_lds<N, TX * NP, SS, NT, true>(sm, twl);
   constexpr int EPT = (TX * N + NT - 1) / NT;
   double2 q[EPT][4];
   const double b0x = a.b0x, b0y = a.b0y;
@@ -236,8 +242,8 @@ __global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
     const int idx = tid + e * NT;
     if (idx < TX * N) {
       const int xl = idx / N, yy = idx - xl * N;
-      const double2* zb = sm + xl * NP * N + yy;
-      const double2 z0 = zb[0], z1 = zb[N], z2 = zb[2 * N], z3 = zb[3 * N], z4 = zb[4 * N];
+      const double2* zb = sm + xl * NP * SS + pidx(yy);
+      const double2 z0 = zb[0], z1 = zb[SS], z2 = zb[2 * SS], z3 = zb[3 * SS], z4 = zb[4 * SS];
       const double wx = z0.x, wy = z0.y, u = z1.x, v = z1.y, jx = z2.x, jy = z2.y;
       const double Bx = z3.x + b0x, By = z3.y + b0y;
       const double pw = z4.x, pj = z4.y;
@@ -255,11 +261,11 @@ __global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
     if (idx < TX * N) {
       const int xl = idx / N, yy = idx - xl * N;
 #pragma unroll
-      for (int p = 0; p < 4; ++p) sm[(xl * 4 + p) * N + yy] = q[e][p];
+      for (int p = 0; p < 4; ++p) sm[(xl * 4 + p) * SS + pidx(yy)] = q[e][p];
     }
   }
   __syncthreads();
-  fft_lds<N, TX * 4, N, NT, false>(sm, a.tw);
+  fft_lds<N, TX * 4, SS, NT, false>(sm, twl);
   double2* AI = a.ain + (size_t)b * 8 * N * KC;
   for (int idx = tid; idx < TX * 4 * KC; idx += NT) {
     const int k = idx % KC;
@@ -267,8 +273,8 @@ __global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
     const int p = t % 4;
     const int xl = t / 4;
     const int x = xbase + xl;
-    const double2* Z = sm + (xl * 4 + p) * N;
-    const double2 z = Z[k], zm = Z[(N - k) & (N - 1)];
+    const double2* Z = sm + (xl * 4 + p) * SS;
+    const double2 z = Z[pidx(k)], zm = Z[pidx((N - k) & (N - 1))];
     // c2r^T: Dbar_k = w'_k sum_y dbar e^{-i theta}, w'_0 = 1, w'_k = 2
     const double w = (k == 0) ? 0.5 : 1.0;
     const double2 da = make_double2(w * (z.x + zm.x), w * (z.y - zm.y));

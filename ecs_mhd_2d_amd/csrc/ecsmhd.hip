@@ -6,11 +6,12 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <stdlib.h>
+#include <map>
+#include <mutex>
 #include "rhs_kernels.h"
 #include "adjoint_kernels.h"
 #include "vec_kernels.h"
-#include <map>
-#include <mutex>
 #include "../../include/ecsmhd.h"
 
 using namespace ecs;
@@ -29,8 +30,8 @@ static int fail(int code, const std::string& msg) {
 
 constexpr int NT = 256;
 
-// Per-N launch tiles: columns per column-kernel block (TY) and rows per
-// row-kernel block (TX).  LDS per block stays at or below 64 KiB.
+// Fixed tiles of the adjoint kernels: columns per column-kernel block (TY)
+// and rows per row-kernel block (TX).
 template <int N> struct Tiles;
 template <> struct Tiles<16> { static constexpr int TY = 2, TX = 2; };
 template <> struct Tiles<32> { static constexpr int TY = 4, TX = 2; };
@@ -131,21 +132,62 @@ static Args base_args(ecs_ctx* c) {
 }
 
 // ------------------------------------------------------------------ launches
-template <int N, int MODE, bool TAN>
-static int col(const Args& a, int B, hipStream_t st) {
-  constexpr int TY = Tiles<N>::TY;
+// Tile variants: column kernels take TY in {2, 4} columns per block, row
+// kernels TX in {1, 2, 4} rows per block.  The autotuner (autotune<N>) picks
+// one per (N, kernel family, batch bucket) by timing each on the device.
+enum TuneKind : int { K_COL = 0, K_COLT = 1, K_ROW = 2, K_ROWT = 3 };
+static std::mutex g_tune_mu;
+static std::map<long long, int> g_tune;  // key -> variant index
+
+static int bucket_of(int B) {
+  int k = 0;
+  while ((1 << k) < B && k < 12) ++k;
+  return k;
+}
+static long long tune_key(int n, int kind, int B) { return ((long long)n * 8 + kind) * 64 + bucket_of(B); }
+static int tuned(int n, int kind, int B, int dflt) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(tune_key(n, kind, B));
+  return it == g_tune.end() ? dflt : it->second;
+}
+
+template <int N> struct DefTile;
+template <> struct DefTile<16> { static constexpr int COL = 0, ROW = 1; };
+template <> struct DefTile<32> { static constexpr int COL = 1, ROW = 1; };
+template <> struct DefTile<64> { static constexpr int COL = 1, ROW = 1; };
+template <> struct DefTile<128> { static constexpr int COL = 1, ROW = 1; };
+template <> struct DefTile<256> { static constexpr int COL = 0, ROW = 0; };
+
+template <int N, int TY, int MODE, bool TAN>
+static int col_v(const Args& a, int B, hipStream_t st) {
   dim3 grid((Geo<N>::KC + TY - 1) / TY, 2, B);
   hipLaunchKernelGGL((col_kernel<N, TY, NT, MODE, TAN>), grid, dim3(NT), 0, st, a);
   LAUNCHCHK();
   return 0;
 }
-template <int N, int RMODE, bool TAN>
-static int row(const Args& a, int B, hipStream_t st) {
-  constexpr int TX = Tiles<N>::TX;
+template <int N, int TX, int RMODE, bool TAN>
+static int row_v(const Args& a, int B, hipStream_t st) {
   dim3 grid(N / TX, 1, B);
   hipLaunchKernelGGL((row_kernel<N, TX, NT, RMODE, TAN>), grid, dim3(NT), 0, st, a);
   LAUNCHCHK();
   return 0;
+}
+template <int N, int MODE, bool TAN>
+static int col_var(int v, const Args& a, int B, hipStream_t st) {
+  return v == 0 ? col_v<N, 2, MODE, TAN>(a, B, st) : col_v<N, 4, MODE, TAN>(a, B, st);
+}
+template <int N, int RMODE, bool TAN>
+static int row_var(int v, const Args& a, int B, hipStream_t st) {
+  return v == 0 ? row_v<N, 1, RMODE, TAN>(a, B, st)
+                : (v == 1 ? row_v<N, 2, RMODE, TAN>(a, B, st) : row_v<N, 4, RMODE, TAN>(a, B, st));
+}
+template <int N, int MODE, bool TAN>
+static int col(const Args& a, int B, hipStream_t st) {
+  return col_var<N, MODE, TAN>(tuned(N, TAN ? K_COLT : K_COL, B, DefTile<N>::COL), a, B, st);
+}
+template <int N, int RMODE, bool TAN>
+static int row(const Args& a, int B, hipStream_t st) {
+  return row_var<N, RMODE, TAN>(tuned(N, TAN ? K_ROWT : K_ROW, B, DefTile<N>::ROW), a, B, st);
 }
 template <int N>
 static int pack(const double* full, double2* comp, int B, hipStream_t st) {
@@ -177,11 +219,70 @@ static int upload_scalars(ecs_ctx* c, int slot, const double* vals, int B, hipSt
   return 0;
 }
 
+// ------------------------------------------------------------------ autotuner
+// Times every tile variant of the four RHS kernel families (column, tangent
+// column, row, tangent row) at this batch size with HIP events on `st` and
+// keeps the fastest for the batch bucket (powers of two).  Runs at the start
+// of an entry point, before any input is packed, on zeroed workspace.
+// ECSMHD_NO_TUNE=1 keeps the default tiles.
+template <int N>
+static int autotune(ecs_ctx* c, int B, hipStream_t st) {
+  if (getenv("ECSMHD_NO_TUNE")) return 0;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (g_tune.count(tune_key(N, K_ROWT, B))) return 0;
+  }
+  const size_t ce = c->compact_elems() * B * sizeof(double2);
+  std::vector<double> dt(B, 1e-3);
+  TRY(upload_scalars(c, 0, dt.data(), B, st));
+  TRY(upload_scalars(c, 1, dt.data(), B, st));
+  Args a = base_args(c);
+  HIPCHK(hipMemsetAsync(c->x0, 0, ce, st));
+  HIPCHK(hipMemsetAsync(c->dx0, 0, ce, st));
+  HIPCHK(hipMemsetAsync(c->acc, 0, ce, st));
+  HIPCHK(hipMemsetAsync(c->dacc, 0, ce, st));
+  TRY((col_var<N, C_LOAD, true>(1, a, B, st)));
+  TRY((row_var<N, R_RHS, true>(1, a, B, st)));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int reps = std::max(3, 64 / B);
+  float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+  int pick[4] = {DefTile<N>::COL, DefTile<N>::COL, DefTile<N>::ROW, DefTile<N>::ROW};
+  for (int kind = 0; kind < 4; ++kind) {
+    const int nv = (kind < 2) ? 2 : 3;
+    for (int v = 0; v < nv; ++v) {
+      auto launch = [&]() -> int {
+        switch (kind) {
+          case K_COL: return col_var<N, C_S2, false>(v, a, B, st);
+          case K_COLT: return col_var<N, C_S2, true>(v, a, B, st);
+          case K_ROW: return row_var<N, R_RHS, false>(v, a, B, st);
+          default: return row_var<N, R_RHS, true>(v, a, B, st);
+        }
+      };
+      TRY(launch());
+      HIPCHK(hipEventRecord(e0, st));
+      for (int i = 0; i < reps; ++i) TRY(launch());
+      HIPCHK(hipEventRecord(e1, st));
+      HIPCHK(hipEventSynchronize(e1));
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, e0, e1));
+      if (t < best[kind]) { best[kind] = t; pick[kind] = v; }
+    }
+  }
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (int kind = 0; kind < 4; ++kind) g_tune[tune_key(N, kind, B)] = pick[kind];
+  return 0;
+}
+
 // ------------------------------------------------------------------ state_vel
 template <int N>
 static int run_state_vel(ecs_ctx* c, const double* in, const double* din, double* out, double* dout, int B,
                          int diss, hipStream_t st) {
   TRY(ensure_cap(c, B));
+  TRY(autotune<N>(c, B, st));
   Args a = base_args(c);
   a.include_diss = diss;
   TRY(pack<N>(in, c->x0, B, st));
@@ -205,6 +306,7 @@ template <int N, bool TAN>
 static int run_rk4(ecs_ctx* c, double* f, double* df, int B, const double* dt, const double* ddt, int steps,
                    int masked, hipStream_t st) {
   TRY(ensure_cap(c, B));
+  TRY(autotune<N>(c, B, st));
   Args a = base_args(c);
   a.masked_e = masked;
   TRY(upload_scalars(c, 0, dt, B, st));
@@ -245,6 +347,7 @@ template <int N>
 static int run_rk43(ecs_ctx* c, double* x, int B, const double* T, double h0, double atol, int maxs, int nck,
                     ecs_rk43_info* info, double* hs, int8_t* accf, hipStream_t st) {
   TRY(ensure_cap(c, B));
+  TRY(autotune<N>(c, B, st));
   const int budget = maxs * nck;
   const int hstride = budget + 1;
   TRY(ensure_hist(c, (size_t)B * hstride));
@@ -325,6 +428,7 @@ template <int N>
 static int run_rk43_jvp(ecs_ctx* c, double* x, double* dx, int B, const double* T, const double* dT,
                         const double* hs, const int8_t* accf, const int* attempts, int budget, hipStream_t st) {
   TRY(ensure_cap(c, B));
+  TRY(autotune<N>(c, B, st));
   // build the accepted-step schedule with the rejections preceding each step
   std::vector<double> sh((size_t)B * budget, 0.0), srh((size_t)B * budget, 0.0);
   std::vector<int> srs((size_t)B * budget, 0), src((size_t)B * budget, 0), snacc(B, 0);
@@ -620,6 +724,7 @@ template <int N>
 static int run_state_vel_vjp(ecs_ctx* c, const double* in, const double* cot, double* grad, int B, int diss,
                              hipStream_t st) {
   TRY(ensure_adj(c, B));
+  TRY(autotune<N>(c, B, st));
   Args a = base_args(c);
   a.include_diss = diss;
   a.aout = c->AO;
@@ -661,6 +766,7 @@ template <int N>
 static int run_rk4_ckpt(ecs_ctx* c, double* f, int B, const double* dt, int steps, int masked, int ckpt,
                         double2* ck, hipStream_t st) {
   TRY(ensure_cap(c, B));
+  TRY(autotune<N>(c, B, st));
   using G = Geo<N>;
   const size_t cs = (size_t)B * 2 * G::R * G::KC;
   TRY(upload_scalars(c, 0, dt, B, st));
@@ -694,6 +800,7 @@ template <int N>
 static int run_rk4_rev(ecs_ctx* c, const double2* ck, double* fbar, double* hbar, int B, const double* dt, int steps,
                        int masked, int ckpt, hipStream_t st) {
   TRY(adj_buffers<N>(c, B, ckpt));
+  TRY(autotune<N>(c, B, st));
   using G = Geo<N>;
   const size_t cs = (size_t)B * 2 * G::R * G::KC;
   const size_t is = (size_t)B * 8 * N * G::KC;
@@ -784,6 +891,7 @@ template <int N>
 static int run_rk4_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar, int B, const double* dt, int steps,
                        int masked, int ckpt, hipStream_t st) {
   TRY(ensure_cap(c, B));
+  TRY(autotune<N>(c, B, st));
   TRY(grow(&c->ckpt, &c->ckpt_cap, ckpt_elems(N, B, steps, ckpt)));
   // the forward pass advances a private copy of f0 (held in c->out, full layout not needed)
   double* fwork = nullptr;
@@ -841,6 +949,7 @@ extern "C" int ecs_lawson_rk4_vjp_ckpt(ecs_ctx* c, const double* ck, double* fba
 template <int N>
 static int run_time_kernel(ecs_ctx* c, int B, int which, int reps, double* ms, hipStream_t st) {
   TRY(ensure_cap(c, B));
+  TRY(autotune<N>(c, B, st));
   std::vector<double> dt(B, 1e-3);
   TRY(upload_scalars(c, 0, dt.data(), B, st));
   TRY(upload_scalars(c, 1, dt.data(), B, st));
@@ -880,4 +989,15 @@ extern "C" int ecs_time_kernel(ecs_ctx* c, int batch, int which, int reps, doubl
   if (!c || !ms || batch < 1 || reps < 1 || which < 0 || which > 3) return fail(1, "ecs_time_kernel: bad arguments");
   hipStream_t st = (hipStream_t)stream;
   DISPATCH_N(c->n, run_time_kernel, c, batch, which, reps, ms, st);
+}
+
+extern "C" int ecs_tuned_tiles(int n, int batch, int* out) {
+  if (!out || batch < 1) return fail(1, "ecs_tuned_tiles: bad arguments");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (int kind = 0; kind < 4; ++kind) {
+    auto it = g_tune.find(tune_key(n, kind, batch));
+    const int v = it == g_tune.end() ? -1 : it->second;
+    out[kind] = v < 0 ? -1 : (kind < 2 ? (v == 0 ? 2 : 4) : (1 << v));
+  }
+  return 0;
 }

@@ -1,14 +1,18 @@
 // Batched complex FFT over sequences staged in LDS (FP64, gfx950).
 //
-// Stockham auto-sort, radix-4 passes plus one radix-2 pass when log2(N) is
-// odd.  All threads of the workgroup cooperate on NSEQ sequences of length N;
-// sequence s lives at buf + s * SSTRIDE (in double2 units).  Each pass reads
-// every butterfly's inputs into registers, synchronises, then writes, so it
-// runs in place.  Output is in natural order and unnormalised:
+// Stockham auto-sort with mixed radices chosen per length so that every
+// supported N takes at most two passes:
+//   16 = 16,  32 = 8 x 4,  64 = 8 x 8,  128 = 16 x 8,  256 = 16 x 16.
+// All threads of the workgroup cooperate on NSEQ sequences of length N;
+// sequence s lives at buf + s * SSTRIDE (double2 units).  Each pass reads its
+// butterflies' inputs into registers, synchronises, then writes, so it runs
+// in place; output is in natural order and unnormalised:
 //   forward  X[k] = sum_n x[n] exp(-2 pi i n k / N)
 //   inverse  x[n] = sum_k X[k] exp(+2 pi i n k / N)
-// The twiddle table tw[j] = exp(-2 pi i j / N), j in [0, N), lives in global
-// memory (built on the host from long-double sin/cos).
+// Element i of sequence s lives at buf[s * SSTRIDE + pidx(i)].
+// Twiddles come from an LDS copy of tw[j] = exp(-2 pi i j / N) (built on the
+// host from long-double sin/cos); radix-8/16 internal constants are the exact
+// cos/sin of multiples of pi/8 to double precision.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -26,106 +30,177 @@ __device__ __forceinline__ double2 cmuli(double2 a) { return make_double2(-a.y, 
 
 constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 
-template <int N, int NSEQ, int SSTRIDE, int NT, bool INV>
-__device__ __forceinline__ void fft_lds(double2* __restrict__ buf, const double2* __restrict__ tw) {
-  constexpr int LOG2N = ilog2c(N);
-  constexpr int N4 = LOG2N / 2;           // number of radix-4 passes
-  constexpr bool HAS2 = (LOG2N & 1) != 0;  // one trailing radix-2 pass
-  const int tid = threadIdx.x;
+// LDS sequences are padded by one element per 16 (element i of a sequence
+// lives at i + i/16), which keeps the strided butterfly writes of the
+// radix-8/16 passes free of bank conflicts.  SEQ<N> is the padded length.
+template <int N>
+struct Seq {
+  static constexpr int S = N + N / 16;
+};
+__device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
 
-  // ---------------- radix-4 passes ----------------
-  {
-    constexpr int NB = NSEQ * (N / 4);           // butterflies per pass
-    constexpr int BPT = (NB + NT - 1) / NT;      // butterflies per thread
-    int Ns = 1;
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ double2 rot_q(double2 a) {
+  return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+constexpr double C8 = 0.70710678118654752440;   // cos(pi/4)
+constexpr double C16 = 0.92387953251128675613;  // cos(pi/8)
+constexpr double S16 = 0.38268343236508977173;  // sin(pi/8)
+
+// cos(pi e / 8) and sin(pi e / 8) for e in [0, 16)
+__host__ __device__ constexpr double cos16(int e) {
+  return e == 0 ? 1.0 : e == 1 ? C16 : e == 2 ? C8 : e == 3 ? S16 : e == 4 ? 0.0 : e == 5 ? -S16 : e == 6 ? -C8
+       : e == 7 ? -C16 : e == 8 ? -1.0 : e == 9 ? -C16 : e == 10 ? -C8 : e == 11 ? -S16 : e == 12 ? 0.0
+       : e == 13 ? S16 : e == 14 ? C8 : C16;
+}
+__host__ __device__ constexpr double sin16(int e) { return cos16((e + 12) & 15); }
+
+// multiply by W^E where W = exp(-2 pi i / 16) (forward) or its conjugate (inverse)
+template <int E, bool INV>
+__device__ __forceinline__ double2 w16(double2 a) {
+  constexpr int e = E & 15;
+  if constexpr (e == 0) return a;
+  else if constexpr (e == 4) return rot_q<INV>(a);
+  else if constexpr (e == 8) return make_double2(-a.x, -a.y);
+  else {
+    // W^e = cos(pi e/8) - i sin(pi e/8)
+    constexpr double c = cos16(e);
+    constexpr double sg = INV ? sin16(e) : -sin16(e);
+    return make_double2(a.x * c - a.y * sg, a.x * sg + a.y * c);
+  }
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft2(double2& a, double2& b) {
+  const double2 t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(double2& v0, double2& v1, double2& v2, double2& v3) {
+  const double2 a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3);
+  const double2 a3 = rot_q<INV>(csub(v1, v3));
+  v0 = cadd(a0, a2);
+  v1 = cadd(a1, a3);
+  v2 = csub(a0, a2);
+  v3 = csub(a1, a3);
+}
+
+template <int R, bool INV> struct Dft;
+template <bool INV> struct Dft<2, INV> {
+  __device__ __forceinline__ static void run(double2* v) { dft2<INV>(v[0], v[1]); }
+};
+template <bool INV> struct Dft<4, INV> {
+  __device__ __forceinline__ static void run(double2* v) { dft4<INV>(v[0], v[1], v[2], v[3]); }
+};
+template <bool INV> struct Dft<8, INV> {
+  // X[k] = E[k] + W8^k O[k], X[k+4] = E[k] - W8^k O[k]
+  __device__ __forceinline__ static void run(double2* v) {
+    dft4<INV>(v[0], v[2], v[4], v[6]);
+    dft4<INV>(v[1], v[3], v[5], v[7]);
+    const double2 o1 = w16<2, INV>(v[3]), o2 = w16<4, INV>(v[5]), o3 = w16<6, INV>(v[7]);
+    const double2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1];
+    v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+    v[1] = cadd(e1, o1); v[5] = csub(e1, o1);
+    v[2] = cadd(e2, o2); v[6] = csub(e2, o2);
+    v[3] = cadd(e3, o3); v[7] = csub(e3, o3);
+  }
+};
+template <bool INV> struct Dft<16, INV> {
+  // n = 4m + q: A_q = DFT4 over m, twiddle W16^{q k'}, then DFT4 over q:
+  // X[k' + 4l] = sum_q W4^{q l} (W16^{q k'} A_q[k'])
+  __device__ __forceinline__ static void run(double2* v) {
+    dft4<INV>(v[0], v[4], v[8], v[12]);
+    dft4<INV>(v[1], v[5], v[9], v[13]);
+    dft4<INV>(v[2], v[6], v[10], v[14]);
+    dft4<INV>(v[3], v[7], v[11], v[15]);
+    // A_q[k'] sits at v[q + 4k']
+    v[5] = w16<1, INV>(v[5]);  v[9] = w16<2, INV>(v[9]);   v[13] = w16<3, INV>(v[13]);
+    v[6] = w16<2, INV>(v[6]);  v[10] = w16<4, INV>(v[10]); v[14] = w16<6, INV>(v[14]);
+    v[7] = w16<3, INV>(v[7]);  v[11] = w16<6, INV>(v[11]); v[15] = w16<9, INV>(v[15]);
+    dft4<INV>(v[0], v[1], v[2], v[3]);
+    dft4<INV>(v[4], v[5], v[6], v[7]);
+    dft4<INV>(v[8], v[9], v[10], v[11]);
+    dft4<INV>(v[12], v[13], v[14], v[15]);
+    // now X[k' + 4l] at v[4k' + l]: transpose the 4 x 4 block into natural order
+    double2 t[16];
 #pragma unroll
-    for (int p = 0; p < N4; ++p) {
-      double2 v[BPT][4];
+    for (int i = 0; i < 16; ++i) t[i] = v[i];
 #pragma unroll
-      for (int q = 0; q < BPT; ++q) {
-        const int bfly = tid + q * NT;
-        if (bfly < NB) {
-          const int s = bfly / (N / 4);
-          const int j = bfly - s * (N / 4);
-          const double2* src = buf + s * SSTRIDE;
-          const int k = j & (Ns - 1);
-          const int tstep = (N / 4) / Ns;  // = N / (Ns * 4)
+    for (int kp = 0; kp < 4; ++kp)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            double2 x = src[j + r * (N / 4)];
-            if (r > 0) {
-              double2 w = tw[(r * k * tstep) & (N - 1)];
-              if (INV) w.y = -w.y;
-              x = cmul(x, w);
-            }
-            v[q][r] = x;
-          }
-          // radix-4 DFT
-          double2 a0 = cadd(v[q][0], v[q][2]);
-          double2 a1 = csub(v[q][0], v[q][2]);
-          double2 a2 = cadd(v[q][1], v[q][3]);
-          double2 a3 = csub(v[q][1], v[q][3]);
-          a3 = INV ? make_double2(-a3.y, a3.x) : make_double2(a3.y, -a3.x);
-          v[q][0] = cadd(a0, a2);
-          v[q][1] = cadd(a1, a3);
-          v[q][2] = csub(a0, a2);
-          v[q][3] = csub(a1, a3);
+      for (int l = 0; l < 4; ++l) v[kp + 4 * l] = t[4 * kp + l];
+  }
+};
+
+// radix plan: first and second pass radices (R2 = 1: single pass)
+template <int N> struct Radix;
+template <> struct Radix<16> { static constexpr int R1 = 16, R2 = 1; };
+template <> struct Radix<32> { static constexpr int R1 = 8, R2 = 4; };
+template <> struct Radix<64> { static constexpr int R1 = 8, R2 = 8; };
+template <> struct Radix<128> { static constexpr int R1 = 16, R2 = 8; };
+template <> struct Radix<256> { static constexpr int R1 = 16, R2 = 16; };
+
+template <int N, int R, int NS, int NSEQ, int SSTRIDE, int NT, bool INV>
+__device__ __forceinline__ void stockham_pass(double2* __restrict__ buf, const double2* __restrict__ twl) {
+  constexpr int NB = NSEQ * (N / R);
+  constexpr int BPT = (NB + NT - 1) / NT;
+  const int tid = threadIdx.x;
+  double2 v[BPT][R];
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int bfly = tid + q * NT;
+    if (bfly < NB) {
+      const int s = bfly / (N / R);
+      const int j = bfly - s * (N / R);
+      const double2* src = buf + s * SSTRIDE;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = src[pidx(j + r * (N / R))];
+      if (NS > 1) {
+        const int k = j & (NS - 1);
+        constexpr int TSTEP = N / (NS * R);
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          double2 w = twl[(r * k * TSTEP) & (N - 1)];
+          if (INV) w.y = -w.y;
+          v[q][r] = cmul(v[q][r], w);
         }
       }
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < BPT; ++q) {
-        const int bfly = tid + q * NT;
-        if (bfly < NB) {
-          const int s = bfly / (N / 4);
-          const int j = bfly - s * (N / 4);
-          double2* dst = buf + s * SSTRIDE;
-          const int k = j & (Ns - 1);
-          const int d = (j - k) * 4 + k;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) dst[d + r * Ns] = v[q][r];
-        }
-      }
-      __syncthreads();
-      Ns *= 4;
+      Dft<R, INV>::run(v[q]);
     }
   }
-  // ---------------- radix-2 pass ----------------
-  if constexpr (HAS2) {
-    constexpr int NB = NSEQ * (N / 2);
-    constexpr int BPT = (NB + NT - 1) / NT;
-    constexpr int Ns = N / 2;
-    double2 v[BPT][2];
+  __syncthreads();
 #pragma unroll
-    for (int q = 0; q < BPT; ++q) {
-      const int bfly = tid + q * NT;
-      if (bfly < NB) {
-        const int s = bfly / (N / 2);
-        const int j = bfly - s * (N / 2);
-        const double2* src = buf + s * SSTRIDE;
-        double2 x0 = src[j];
-        double2 x1 = src[j + N / 2];
-        double2 w = tw[j];  // k = j (j < Ns), tstep = 1
-        if (INV) w.y = -w.y;
-        x1 = cmul(x1, w);
-        v[q][0] = cadd(x0, x1);
-        v[q][1] = csub(x0, x1);
-      }
-    }
-    __syncthreads();
+  for (int q = 0; q < BPT; ++q) {
+    const int bfly = tid + q * NT;
+    if (bfly < NB) {
+      const int s = bfly / (N / R);
+      const int j = bfly - s * (N / R);
+      double2* dst = buf + s * SSTRIDE;
+      const int k = j & (NS - 1);
+      const int d = (j - k) * R + k;
 #pragma unroll
-    for (int q = 0; q < BPT; ++q) {
-      const int bfly = tid + q * NT;
-      if (bfly < NB) {
-        const int s = bfly / (N / 2);
-        const int j = bfly - s * (N / 2);
-        double2* dst = buf + s * SSTRIDE;
-        dst[j] = v[q][0];
-        dst[j + Ns] = v[q][1];
-      }
+      for (int r = 0; r < R; ++r) dst[pidx(d + r * NS)] = v[q][r];
     }
-    __syncthreads();
   }
+  __syncthreads();
+}
+
+// twl must hold the twiddle table in LDS (load_twiddles + a barrier).
+template <int N, int NSEQ, int SSTRIDE, int NT, bool INV>
+__device__ __forceinline__ void fft_lds(double2* __restrict__ buf, const double2* __restrict__ twl) {
+  constexpr int R1 = Radix<N>::R1, R2 = Radix<N>::R2;
+  static_assert(R1 * (R2 == 1 ? 1 : R2) == N, "radix plan must factor N");
+  stockham_pass<N, R1, 1, NSEQ, SSTRIDE, NT, INV>(buf, twl);
+  if constexpr (R2 > 1) stockham_pass<N, R2, R1, NSEQ, SSTRIDE, NT, INV>(buf, twl);
+}
+
+template <int N, int NT>
+__device__ __forceinline__ void load_twiddles(double2* __restrict__ twl, const double2* __restrict__ tw) {
+  for (int i = threadIdx.x; i < N; i += NT) twl[i] = tw[i];
 }
 
 }  // namespace ecs

@@ -268,7 +268,10 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   constexpr bool HASINV = !(MODE == C_RHS || MODE == C_FIN || (MODE == C_S5 && TAN));
   constexpr int NSEQ = HASINV ? NDO * NF * TY : NF * TY;
   constexpr int ND = (NDO == 1) ? 2 : 8 * NF;
-  __shared__ double2 sm[NSEQ * N];
+  constexpr int SS = Seq<N>::S;
+  __shared__ double2 sm[NSEQ * SS];
+  __shared__ double2 twl[N];
+  load_twiddles<N, NT>(twl, a.tw);
   const int b = blockIdx.z, c = blockIdx.y, ky0 = blockIdx.x * TY, tid = threadIdx.x;
   const Blk bk = prologue<MODE, TAN>(a, b);
   if (!bk.run) return;
@@ -283,10 +286,10 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
       const int ky = ky0 + cy;
       double2 v = make_double2(0.0, 0.0);
       if (ky < KC) v = O[((size_t)(f * 2 + c) * N + x) * KC + ky];
-      sm[(f * TY + cy) * N + x] = v;
+      sm[(f * TY + cy) * SS + pidx(x)] = v;
     }
     __syncthreads();
-    fft_lds<N, NF * TY, N, NT, false>(sm, a.tw);
+    fft_lds<N, NF * TY, SS, NT, false>(sm, twl);
   }
 
   const bool inv = HASINV && bk.stage && !a.skip_inv;
@@ -301,25 +304,25 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
     const size_t ci = (((size_t)b * 2 + c) * R + r) * KC + ky;
     double2 rh = make_double2(0.0, 0.0), drh = rh;
     if (FWD) {
-      rh = sm[(0 * TY + cy) * N + xr];
-      if (TAN) drh = sm[(1 * TY + cy) * N + xr];
+      rh = sm[(0 * TY + cy) * SS + pidx(xr)];
+      if (TAN) drh = sm[(1 * TY + cy) * SS + pidx(xr)];
     }
     double2 y, dy;
     glue<N, MODE, TAN>(a, bk, b, ci, r, c, kx, ky, rh, drh, y, dy);
     if (HASINV && inv) {
       if (NDO == 1) {
-        sm[(0 * TY + cy) * N + xr] = cscale(invN2, y);
+        sm[(0 * TY + cy) * SS + pidx(xr)] = cscale(invN2, y);
       } else {
         const bool dc = (kx == 0 && ky == 0);
         const double k2s = dc ? 1.0 : (double)(kx * kx + ky * ky);
         const double mult[4] = {(double)kx, (double)ky, (double)ky / k2s, -(double)kx / k2s};
         const double2 iy = cmuli(y);
 #pragma unroll
-        for (int d = 0; d < 4; ++d) sm[((d * NF + 0) * TY + cy) * N + xr] = cscale(mult[d] * invN2, iy);
+        for (int d = 0; d < 4; ++d) sm[((d * NF + 0) * TY + cy) * SS + pidx(xr)] = cscale(mult[d] * invN2, iy);
         if (TAN) {
           const double2 idy = cmuli(dy);
 #pragma unroll
-          for (int d = 0; d < 4; ++d) sm[((d * NF + 1) * TY + cy) * N + xr] = cscale(mult[d] * invN2, idy);
+          for (int d = 0; d < 4; ++d) sm[((d * NF + 1) * TY + cy) * SS + pidx(xr)] = cscale(mult[d] * invN2, idy);
         }
       }
     }
@@ -329,10 +332,10 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   for (int idx = tid; idx < NSEQ * GAP; idx += NT) {
     const int s = idx / GAP;
     const int g = idx - s * GAP;
-    sm[s * N + KMAX + 1 + g] = make_double2(0.0, 0.0);
+    sm[s * SS + pidx(KMAX + 1 + g)] = make_double2(0.0, 0.0);
   }
   __syncthreads();
-  fft_lds<N, NSEQ, N, NT, true>(sm, a.tw);
+  fft_lds<N, NSEQ, SS, NT, true>(sm, twl);
   for (int idx = tid; idx < NSEQ * N; idx += NT) {
     const int cy = idx % TY;
     const int t = idx / TY;
@@ -342,7 +345,7 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
     if (ky >= KC) continue;
     const int d = s2 / NF, f = s2 % NF;
     const int g = (NDO == 1) ? c : (f * 8 + 4 * c + d);
-    a.inter[(((size_t)b * ND + g) * N + x) * KC + ky] = sm[(s2 * TY + cy) * N + x];
+    a.inter[(((size_t)b * ND + g) * N + x) * KC + ky] = sm[(s2 * TY + cy) * SS + pidx(x)];
   }
 }
 
@@ -354,7 +357,10 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
   constexpr int NF = TAN ? 2 : 1;
   constexpr int ND = (RMODE == R_ERR) ? 2 : 8 * NF;
   constexpr int NHALF = N / 2 + 1;
-  __shared__ double2 sm[TX * NP * N];
+  constexpr int SS = Seq<N>::S;
+  __shared__ double2 sm[TX * NP * SS];
+  __shared__ double2 twl[N];
+  load_twiddles<N, NT>(twl, a.tw);
   const int b = blockIdx.z, xbase = blockIdx.x * TX, tid = threadIdx.x;
   if (!row_active(a, b)) return;
   const double2* I = a.inter + (size_t)b * ND * N * KC;
@@ -371,21 +377,21 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
       A = I[((size_t)(2 * p) * N + x) * KC + k];
       B = I[((size_t)(2 * p + 1) * N + x) * KC + k];
     }
-    double2* z = sm + (xl * NP + p) * N;
+    double2* z = sm + (xl * NP + p) * SS;
     if (k == 0 || k == N / 2) {
-      z[k] = make_double2(A.x, B.x);
+      z[pidx(k)] = make_double2(A.x, B.x);
     } else {
-      z[k] = make_double2(A.x - B.y, A.y + B.x);
-      z[N - k] = make_double2(A.x + B.y, B.x - A.y);
+      z[pidx(k)] = make_double2(A.x - B.y, A.y + B.x);
+      z[pidx(N - k)] = make_double2(A.x + B.y, B.x - A.y);
     }
   }
   __syncthreads();
-  fft_lds<N, TX * NP, N, NT, true>(sm, a.tw);
+  fft_lds<N, TX * NP, SS, NT, true>(sm, twl);
 
   if (RMODE == R_ERR) {
     unsigned long long mx = 0ull;
     for (int idx = tid; idx < TX * N; idx += NT) {
-      const double2 z = sm[idx];
+      const double2 z = sm[(idx / N) * SS + pidx(idx % N)];
       const unsigned long long u1 = (unsigned long long)__double_as_longlong(fabs(z.x));
       const unsigned long long u2 = (unsigned long long)__double_as_longlong(fabs(z.y));
       mx = u1 > mx ? u1 : mx;
@@ -413,14 +419,14 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
     const int idx = tid + e * NT;
     if (idx < TX * N) {
       const int xl = idx / N, yy = idx - xl * N;
-      const double2* zb = sm + xl * NP * N + yy;
-      const double2 z0 = zb[0], z1 = zb[N], z2 = zb[2 * N], z3 = zb[3 * N];
+      const double2* zb = sm + xl * NP * SS + pidx(yy);
+      const double2 z0 = zb[0], z1 = zb[SS], z2 = zb[2 * SS], z3 = zb[3 * SS];
       const double wx = z0.x, wy = z0.y, u = z1.x, v = z1.y, jx = z2.x, jy = z2.y;
       const double Bx = z3.x + b0x, By = z3.y + b0y;
       // -advection[0] + advection[1] and u x B (mhd_jax.py:152-163)
       q[e] = make_double2(-(u * wx + v * wy) + (Bx * jx + By * jy), u * By - v * Bx);
       if (TAN) {
-        const double2 z4 = zb[4 * N], z5 = zb[5 * N], z6 = zb[6 * N], z7 = zb[7 * N];
+        const double2 z4 = zb[4 * SS], z5 = zb[5 * SS], z6 = zb[6 * SS], z7 = zb[7 * SS];
         const double twx = z4.x, twy = z4.y, tu = z5.x, tv = z5.y, tjx = z6.x, tjy = z6.y, ta = z7.x, tb = z7.y;
         dq[e] = make_double2(-(tu * wx + u * twx + tv * wy + v * twy) + (ta * jx + Bx * tjx + tb * jy + By * tjy),
                              tu * By + u * tb - tv * Bx - v * ta);
@@ -433,12 +439,12 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
     const int idx = tid + e * NT;
     if (idx < TX * N) {
       const int xl = idx / N, yy = idx - xl * N;
-      sm[(xl * NF + 0) * N + yy] = q[e];
-      if (TAN) sm[(xl * NF + 1) * N + yy] = dq[e];
+      sm[(xl * NF + 0) * SS + pidx(yy)] = q[e];
+      if (TAN) sm[(xl * NF + 1) * SS + pidx(yy)] = dq[e];
     }
   }
   __syncthreads();
-  fft_lds<N, TX * NF, N, NT, false>(sm, a.tw);
+  fft_lds<N, TX * NF, SS, NT, false>(sm, twl);
   double2* O = a.rowout + (size_t)b * (2 * NF) * N * KC;
   for (int idx = tid; idx < TX * NF * KC; idx += NT) {
     const int k = idx % KC;
@@ -446,8 +452,8 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
     const int f = t % NF;
     const int xl = t / NF;
     const int x = xbase + xl;
-    const double2* Z = sm + (xl * NF + f) * N;
-    const double2 z = Z[k], zm = Z[(N - k) & (N - 1)];
+    const double2* Z = sm + (xl * NF + f) * SS;
+    const double2 z = Z[pidx(k)], zm = Z[pidx((N - k) & (N - 1))];
     const double2 pw = make_double2(0.5 * (z.x + zm.x), 0.5 * (z.y - zm.y));
     const double2 pj = make_double2(0.5 * (z.y + zm.y), -0.5 * (z.x - zm.x));
     O[((size_t)(f * 2 + 0) * N + x) * KC + k] = pw;

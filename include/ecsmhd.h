@@ -128,6 +128,12 @@ int ecs_dot(const double* a, const double* b, int n, double* out, void* stream);
  * column kernel, 3 tangent row kernel.  Synchronous. */
 int ecs_time_kernel(ecs_ctx* ctx, int batch, int which, int reps, double* ms, void* stream);
 
+/* Tile choice of the autotuner for grid size n and this batch bucket (no
+ * reference counterpart): out[0] columns per column-kernel block, out[1] the
+ * same for the tangent column kernel, out[2] rows per row-kernel block,
+ * out[3] the same for the tangent row kernel; -1 where not tuned yet. */
+int ecs_tuned_tiles(int n, int batch, int* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,8 +114,9 @@ def test_lawson_rk43(n):
     assert gi["completed"] and rinfo["completed"]
     assert gi["fevals"] == 1 + 4 * (gi["accepted"] + gi["rejected"])
     assert [a for _, a in sched] == [a for _, a in rinfo["schedule"]]
-    # the last step is clipped to h = 1 - s, a difference of two O(1) sums
-    np.testing.assert_allclose([h for h, _ in sched], [h for h, _ in rinfo["schedule"]], rtol=1e-9, atol=1e-13)
+    # err = max|irfft2(h (k5 - k4)/6)| is a difference of nearly equal stages, so
+    # its relative rounding is ~eps |k| / |k5 - k4|; h follows it as err^(-1/4)
+    np.testing.assert_allclose([h for h, _ in sched], [h for h, _ in rinfo["schedule"]], rtol=1e-7)
     assert rel(host(x), rx) < 1e-10
 
 
