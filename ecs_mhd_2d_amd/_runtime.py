@@ -162,14 +162,18 @@ class Context:
 _CACHE: dict = {}
 
 
-def context_for(n, nu, eta, b0, forcing, device=None) -> Context:
-    """Cached Context for these physics (one device workspace per key)."""
+def context_for(n, nu, eta, b0, forcing, device=None, lane: int = 0) -> Context:
+    """Cached Context for these physics (one device workspace per key).
+
+    A context serves one call at a time (the library refuses concurrent calls
+    on one context); callers that run several operations concurrently on one
+    grid size use different ``lane`` numbers, each with its own workspace."""
     if device is None:
         device = torch.cuda.current_device()
     farr = np.ascontiguousarray(np.asarray(forcing.detach().cpu() if isinstance(forcing, torch.Tensor) else forcing,
                                            dtype=np.float64))
     key = (int(n), float(nu), float(eta), float(b0[0]), float(b0[1]), hashlib.sha1(farr.tobytes()).hexdigest(),
-           int(device))
+           int(device), int(lane))
     ctx = _CACHE.get(key)
     if ctx is None:
         ctx = Context(n, nu, eta, b0, farr, device)

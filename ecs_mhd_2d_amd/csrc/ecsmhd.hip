@@ -7,8 +7,10 @@
 #include <vector>
 #include <algorithm>
 #include <stdlib.h>
+#include <atomic>
 #include <map>
 #include <mutex>
+#include <string>
 #include "rhs_kernels.h"
 #include "adjoint_kernels.h"
 #include "vec_kernels.h"
@@ -28,6 +30,21 @@ static int fail(int code, const std::string& msg) {
   } while (0)
 #define LAUNCHCHK() HIPCHK(hipGetLastError())
 
+// Marks a context busy for the duration of one entry point; a second,
+// concurrent call on the same context fails with code 4 instead of racing on
+// (or reallocating) a workspace that kernels of the first call still use.
+struct CtxGuard {
+  std::atomic<int>* flag;
+  bool ok;
+  explicit CtxGuard(std::atomic<int>* f) : flag(f), ok(f->exchange(1) == 0) {}
+  ~CtxGuard() {
+    if (ok) flag->store(0);
+  }
+};
+#define GUARD(c)                                                                              \
+  CtxGuard guard_(&(c)->busy);                                                                \
+  if (!guard_.ok) return fail(4, "context is in use by another call (use one context per thread)")
+
 constexpr int NT = 256;
 
 // Fixed tiles of the adjoint kernels: columns per column-kernel block (TY)
@@ -40,6 +57,7 @@ template <> struct Tiles<128> { static constexpr int TY = 4, TX = 2; };
 template <> struct Tiles<256> { static constexpr int TY = 2, TX = 1; };
 
 struct ecs_ctx {
+  std::atomic<int> busy{0};    // one call at a time: the workspace is shared by all entry points
   int n = 0, dev = 0;
   double nu = 0, eta = 0, b0x = 0, b0y = 0;
   double2* tw = nullptr;
@@ -59,6 +77,7 @@ struct ecs_ctx {
   double* s_rh = nullptr;
   int* s_nacc = nullptr;
   size_t sched_cap = 0;
+  int snacc_cap = 0;
   // adjoint workspace
   double2* ckpt = nullptr;     // checkpoints [nck][B][2][R][KC]
   size_t ckpt_cap = 0;
@@ -70,6 +89,12 @@ struct ecs_ctx {
   double2 *AO = nullptr, *AI = nullptr, *xb = nullptr, *k1b = nullptr, *k2b = nullptr, *k3b = nullptr;
   double* hb = nullptr;
   int adj_cap = 0;
+  // hipGraph capture/replay of fixed chunks of stages
+  hipStream_t gs = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  int* cnt = nullptr;          // device step counter [2]
+  int* hcnt = nullptr;         // pinned staging
+  std::map<std::string, hipGraphExec_t> graphs;
   double* red = nullptr;       // reductions [cap][64]
   double* hred = nullptr;      // pinned [cap]
   size_t compact_elems() const {
@@ -78,8 +103,20 @@ struct ecs_ctx {
   }
 };
 
+// Allocation, frees and synchronous copies must not run while another thread
+// captures a graph (they would join the capture's dependencies), so both take
+// this lock.
+static std::recursive_mutex g_alloc_mu;
+
+static void drop_graphs(ecs_ctx* c) {
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+}
+
 static int ensure_cap(ecs_ctx* c, int B) {
   if (B <= c->cap) return 0;
+  std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
+  drop_graphs(c);
   int nc = std::max(B, c->cap * 2);
   const int kmax = (c->n - 1) / 3, KC = kmax + 1;
   const size_t comp = c->compact_elems();
@@ -90,7 +127,6 @@ static int ensure_cap(ecs_ctx* c, int B) {
   for (auto p : arrs) {
     if (*p) HIPCHK(hipFree(*p));
     HIPCHK(hipMalloc((void**)p, comp * nc * sizeof(double2)));
-    HIPCHK(hipMemset(*p, 0, comp * nc * sizeof(double2)));
   }
   if (c->I) HIPCHK(hipFree(c->I));
   if (c->O) HIPCHK(hipFree(c->O));
@@ -219,6 +255,67 @@ static int upload_scalars(ecs_ctx* c, int slot, const double* vals, int B, hipSt
   return 0;
 }
 
+// ------------------------------------------------------------------ hipGraph chunks
+// Captures the launches `record` makes on the context's own stream once per
+// key (the bytes of the kernels' arguments plus a tag), then replays the
+// instantiated graph; the caller's stream is joined before and after.
+// Opt-in (ECSMHD_GRAPH=1): on this ROCm stack, eager launches of the same
+// chunk measured faster, so eager is the default.
+static bool use_graphs() { return getenv("ECSMHD_GRAPH") != nullptr; }
+
+static std::string graph_key(const char* tag, const Args& a, int B, int K) {
+  std::string k(tag);
+  k.append((const char*)&a, sizeof(Args));
+  k.append((const char*)&B, sizeof(int));
+  k.append((const char*)&K, sizeof(int));
+  return k;
+}
+
+template <class F>
+static int graph_run(ecs_ctx* c, const std::string& key, hipStream_t st, F&& record) {
+  if (!c->gs) {
+    HIPCHK(hipStreamCreateWithFlags(&c->gs, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming));
+  }
+  HIPCHK(hipEventRecord(c->ev_in, st));
+  HIPCHK(hipStreamWaitEvent(c->gs, c->ev_in, 0));
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
+    HIPCHK(hipStreamBeginCapture(c->gs, hipStreamCaptureModeThreadLocal));
+    const int rc = record(c->gs);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->gs, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(2, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e2 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) return fail(2, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+    it = c->graphs.emplace(key, ex).first;
+  }
+  HIPCHK(hipGraphLaunch(it->second, c->gs));
+  HIPCHK(hipEventRecord(c->ev_out, c->gs));
+  HIPCHK(hipStreamWaitEvent(st, c->ev_out, 0));
+  return 0;
+}
+
+static int set_counter(ecs_ctx* c, int v, hipStream_t st) {
+  if (!c->cnt) {
+    HIPCHK(hipMalloc((void**)&c->cnt, 2 * sizeof(int)));
+    HIPCHK(hipHostMalloc((void**)&c->hcnt, 2 * sizeof(int), hipHostMallocDefault));
+  }
+  HIPCHK(hipStreamSynchronize(st));  // the pinned word may still feed an earlier copy
+  c->hcnt[0] = v;
+  c->hcnt[1] = v;
+  HIPCHK(hipMemcpyAsync(c->cnt, c->hcnt, 2 * sizeof(int), hipMemcpyHostToDevice, st));
+  return 0;
+}
+
 // ------------------------------------------------------------------ autotuner
 // Times every tile variant of the four RHS kernel families (column, tangent
 // column, row, tangent row) at this batch size with HIP events on `st` and
@@ -303,6 +400,21 @@ static int run_state_vel(ecs_ctx* c, const double* in, const double* din, double
 
 // ------------------------------------------------------------------ Lawson RK4
 template <int N, bool TAN>
+static int rk4_step(const Args& a, int B, hipStream_t st, bool last) {
+  TRY((row<N, R_RHS, TAN>(a, B, st)));
+  TRY((col<N, C_S1F, TAN>(a, B, st)));
+  TRY((row<N, R_RHS, TAN>(a, B, st)));
+  TRY((col<N, C_S2, TAN>(a, B, st)));
+  TRY((row<N, R_RHS, TAN>(a, B, st)));
+  TRY((col<N, C_S3, TAN>(a, B, st)));
+  TRY((row<N, R_RHS, TAN>(a, B, st)));
+  Args a4 = a;
+  a4.skip_inv = last ? 1 : 0;
+  TRY((col<N, C_S4, TAN>(a4, B, st)));
+  return 0;
+}
+
+template <int N, bool TAN>
 static int run_rk4(ecs_ctx* c, double* f, double* df, int B, const double* dt, const double* ddt, int steps,
                    int masked, hipStream_t st) {
   TRY(ensure_cap(c, B));
@@ -315,18 +427,19 @@ static int run_rk4(ecs_ctx* c, double* f, double* df, int B, const double* dt, c
   if (TAN) TRY(pack<N>(df, c->dx0, B, st));
   if (steps > 0) {
     TRY((col<N, C_LOAD, TAN>(a, B, st)));
-    for (int s = 0; s < steps; ++s) {
-      TRY((row<N, R_RHS, TAN>(a, B, st)));
-      TRY((col<N, C_S1F, TAN>(a, B, st)));
-      TRY((row<N, R_RHS, TAN>(a, B, st)));
-      TRY((col<N, C_S2, TAN>(a, B, st)));
-      TRY((row<N, R_RHS, TAN>(a, B, st)));
-      TRY((col<N, C_S3, TAN>(a, B, st)));
-      TRY((row<N, R_RHS, TAN>(a, B, st)));
-      Args a4 = a;
-      a4.skip_inv = (s == steps - 1);
-      TRY((col<N, C_S4, TAN>(a4, B, st)));
+    // full chunks of K steps replay one captured graph; the tail runs eagerly
+    constexpr int K = 16;
+    int s = 0;
+    if (use_graphs() && steps - 1 >= K) {
+      const std::string key = graph_key(TAN ? "rk4t" : "rk4", a, B, K);
+      for (; s + K <= steps - 1; s += K) {
+        TRY(graph_run(c, key, st, [&](hipStream_t gs) -> int {
+          for (int j = 0; j < K; ++j) TRY((rk4_step<N, TAN>(a, B, gs, false)));
+          return 0;
+        }));
+      }
     }
+    for (; s < steps; ++s) TRY((rk4_step<N, TAN>(a, B, st, s == steps - 1)));
   }
   // unmasked e keeps the modes outside the 2/3 region unchanged (L = 0 there)
   TRY(unpack<N>(c->x0, f, B, masked ? 1 : 0, st));
@@ -337,9 +450,27 @@ static int run_rk4(ecs_ctx* c, double* f, double* df, int B, const double* dt, c
 // ------------------------------------------------------------------ Lawson RK4(3)
 static int ensure_hist(ecs_ctx* c, size_t need) {
   if (need <= c->hist_cap) return 0;
+  std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
+  drop_graphs(c);
   if (c->hist) HIPCHK(hipFree(c->hist));
   HIPCHK(hipMalloc((void**)&c->hist, need * sizeof(Hist)));
   c->hist_cap = need;
+  return 0;
+}
+
+template <int N>
+static int rk43_attempt(const Args& ai, int B, hipStream_t st, bool first) {
+  if (first) TRY((col<N, C_A0, false>(ai, B, st)));
+  else TRY((col<N, C_A, false>(ai, B, st)));
+  TRY((row<N, R_RHS, false>(ai, B, st)));
+  TRY((col<N, C_S2, false>(ai, B, st)));
+  TRY((row<N, R_RHS, false>(ai, B, st)));
+  TRY((col<N, C_S3, false>(ai, B, st)));
+  TRY((row<N, R_RHS, false>(ai, B, st)));
+  TRY((col<N, C_S4, false>(ai, B, st)));
+  TRY((row<N, R_RHS, false>(ai, B, st)));
+  TRY((col<N, C_S5, false>(ai, B, st)));
+  TRY((row<N, R_ERR, false>(ai, B, st)));
   return 0;
 }
 
@@ -348,8 +479,9 @@ static int run_rk43(ecs_ctx* c, double* x, int B, const double* T, double h0, do
                     ecs_rk43_info* info, double* hs, int8_t* accf, hipStream_t st) {
   TRY(ensure_cap(c, B));
   TRY(autotune<N>(c, B, st));
+  constexpr int K = 32;  // attempts per captured chunk (even: the counter parity repeats)
   const int budget = maxs * nck;
-  const int hstride = budget + 1;
+  const int hstride = budget + K + 2;
   TRY(ensure_hist(c, (size_t)B * hstride));
   HIPCHK(hipMemsetAsync(c->hist, 0, (size_t)B * hstride * sizeof(Hist), st));
   TRY(upload_scalars(c, 2, T, B, st));
@@ -358,6 +490,7 @@ static int run_rk43(ecs_ctx* c, double* x, int B, const double* T, double h0, do
   a.hstride = hstride;
   a.atol = atol;
   a.h0 = h0;
+  a.budget = budget;
   TRY(pack<N>(x, c->x0, B, st));
   // k1 = t v(x) (timestepping.py:255)
   Args al = a;
@@ -365,33 +498,39 @@ static int run_rk43(ecs_ctx* c, double* x, int B, const double* T, double h0, do
   TRY((col<N, C_LOAD, false>(al, B, st)));
   TRY((row<N, R_RHS, false>(al, B, st)));
   a.adaptive = AD_PRIMAL;
+  Args a0 = a;
+  a0.attempt = 0;
+  TRY(rk43_attempt<N>(a0, B, st, true));
+  int launched = 1;
+  const bool graphs = use_graphs();
+  if (graphs) TRY(set_counter(c, 1, st));
   std::vector<Hist> hh(B);
-  int launched = 0;
-  const int chunk = 32;
   while (launched < budget) {
-    const int end = std::min(budget, launched + chunk);
-    for (int i = launched; i < end; ++i) {
-      Args ai = a;
-      ai.attempt = i;
-      if (i == 0) TRY((col<N, C_A0, false>(ai, B, st)));
-      else TRY((col<N, C_A, false>(ai, B, st)));
-      TRY((row<N, R_RHS, false>(ai, B, st)));
-      TRY((col<N, C_S2, false>(ai, B, st)));
-      TRY((row<N, R_RHS, false>(ai, B, st)));
-      TRY((col<N, C_S3, false>(ai, B, st)));
-      TRY((row<N, R_RHS, false>(ai, B, st)));
-      TRY((col<N, C_S4, false>(ai, B, st)));
-      TRY((row<N, R_RHS, false>(ai, B, st)));
-      TRY((col<N, C_S5, false>(ai, B, st)));
-      TRY((row<N, R_ERR, false>(ai, B, st)));
+    if (graphs) {
+      Args ag = a;
+      ag.cnt = c->cnt;
+      const std::string key = graph_key("rk43", ag, B, K);
+      TRY(graph_run(c, key, st, [&](hipStream_t gs) -> int {
+        for (int j = 0; j < K; ++j) {
+          Args aj = ag;
+          aj.cslot = j & 1;
+          TRY(rk43_attempt<N>(aj, B, gs, false));
+        }
+        return 0;
+      }));
+    } else {
+      for (int j = 0; j < K; ++j) {
+        Args aj = a;
+        aj.attempt = launched + j;
+        TRY(rk43_attempt<N>(aj, B, st, false));
+      }
     }
-    launched = end;
+    launched += K;
     // poll: stop once every candidate's last attempt was a no-op
     bool any = false;
-    for (int b = 0; b < B; ++b) {
+    for (int b = 0; b < B; ++b)
       HIPCHK(hipMemcpyAsync(&hh[b], c->hist + (size_t)b * hstride + (launched - 1), sizeof(Hist),
                             hipMemcpyDeviceToHost, st));
-    }
     HIPCHK(hipStreamSynchronize(st));
     for (int b = 0; b < B; ++b) any = any || hh[b].active;
     if (!any) break;
@@ -452,6 +591,7 @@ static int run_rk43_jvp(ecs_ctx* c, double* x, double* dx, int B, const double* 
     maxacc = std::max(maxacc, m);
   }
   const size_t need = (size_t)B * budget;
+  std::unique_lock<std::recursive_mutex> slk(g_alloc_mu);
   if (need > c->sched_cap) {
     if (c->s_h) { HIPCHK(hipFree(c->s_h)); HIPCHK(hipFree(c->s_rh)); HIPCHK(hipFree(c->s_rs)); HIPCHK(hipFree(c->s_rc)); }
     HIPCHK(hipMalloc((void**)&c->s_h, need * sizeof(double)));
@@ -460,13 +600,18 @@ static int run_rk43_jvp(ecs_ctx* c, double* x, double* dx, int B, const double* 
     HIPCHK(hipMalloc((void**)&c->s_rc, need * sizeof(int)));
     c->sched_cap = need;
   }
-  if (c->s_nacc) HIPCHK(hipFree(c->s_nacc));
-  HIPCHK(hipMalloc((void**)&c->s_nacc, B * sizeof(int)));
+  if (B > c->snacc_cap) {
+    if (c->s_nacc) HIPCHK(hipFree(c->s_nacc));
+    HIPCHK(hipMalloc((void**)&c->s_nacc, B * sizeof(int)));
+    c->snacc_cap = B;
+  }
   HIPCHK(hipMemcpyAsync(c->s_h, sh.data(), need * sizeof(double), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(c->s_rh, srh.data(), need * sizeof(double), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(c->s_rs, srs.data(), need * sizeof(int), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(c->s_rc, src.data(), need * sizeof(int), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(c->s_nacc, snacc.data(), B * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // the host vectors above go out of scope
+  slk.unlock();
   TRY(upload_scalars(c, 2, T, B, st));
   TRY(upload_scalars(c, 3, dT, B, st));
   Args a = base_args(c);
@@ -480,19 +625,47 @@ static int run_rk43_jvp(ecs_ctx* c, double* x, double* dx, int B, const double* 
   TRY((col<N, C_LOAD, true>(al, B, st)));
   TRY((row<N, R_RHS, true>(al, B, st)));
   a.adaptive = AD_TANGENT;
-  for (int m = 0; m < maxacc; ++m) {
-    Args am = a;
-    am.attempt = m;
-    if (m == 0) TRY((col<N, C_A0, true>(am, B, st)));
-    else TRY((col<N, C_A, true>(am, B, st)));
-    TRY((row<N, R_RHS, true>(am, B, st)));
-    TRY((col<N, C_S2, true>(am, B, st)));
-    TRY((row<N, R_RHS, true>(am, B, st)));
-    TRY((col<N, C_S3, true>(am, B, st)));
-    TRY((row<N, R_RHS, true>(am, B, st)));
-    TRY((col<N, C_S4, true>(am, B, st)));
-    TRY((row<N, R_RHS, true>(am, B, st)));
-    TRY((col<N, C_S5, true>(am, B, st)));
+  auto step = [&](const Args& am, hipStream_t s2, bool first) -> int {
+    if (first) TRY((col<N, C_A0, true>(am, B, s2)));
+    else TRY((col<N, C_A, true>(am, B, s2)));
+    TRY((row<N, R_RHS, true>(am, B, s2)));
+    TRY((col<N, C_S2, true>(am, B, s2)));
+    TRY((row<N, R_RHS, true>(am, B, s2)));
+    TRY((col<N, C_S3, true>(am, B, s2)));
+    TRY((row<N, R_RHS, true>(am, B, s2)));
+    TRY((col<N, C_S4, true>(am, B, s2)));
+    TRY((row<N, R_RHS, true>(am, B, s2)));
+    TRY((col<N, C_S5, true>(am, B, s2)));
+    return 0;
+  };
+  if (maxacc > 0) {
+    Args am0 = a;
+    am0.attempt = 0;
+    TRY(step(am0, st, true));
+  }
+  constexpr int K = 32;
+  int launched = 1;
+  if (use_graphs() && maxacc > 1) {
+    TRY(set_counter(c, 1, st));
+    Args ag = a;
+    ag.cnt = c->cnt;
+    const std::string key = graph_key("rk43t", ag, B, K);
+    for (; launched < maxacc; launched += K) {
+      TRY(graph_run(c, key, st, [&](hipStream_t gs) -> int {
+        for (int j = 0; j < K; ++j) {
+          Args aj = ag;
+          aj.cslot = j & 1;
+          TRY(step(aj, gs, false));
+        }
+        return 0;
+      }));
+    }
+  } else {
+    for (int m = 1; m < maxacc; ++m) {
+      Args am = a;
+      am.attempt = m;
+      TRY(step(am, st, false));
+    }
   }
   TRY(unpack<N>(c->x0, x, B, 0, st));
   TRY(unpack<N>(c->dx0, dx, B, 0, st));
@@ -526,6 +699,7 @@ int ecs_ctx_create(ecs_ctx** out, int n, double nu, double eta, double b0x, doub
                    int device) {
   if (!out) return fail(1, "null ctx pointer");
   if (n != 16 && n != 32 && n != 64 && n != 128 && n != 256) return fail(1, "unsupported grid size n");
+  std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
   HIPCHK(hipSetDevice(device));
   ecs_ctx* c = new ecs_ctx();
   c->n = n; c->dev = device; c->nu = nu; c->eta = eta; c->b0x = b0x; c->b0y = b0y;
@@ -556,12 +730,19 @@ int ecs_ctx_create(ecs_ctx** out, int n, double nu, double eta, double b0x, doub
 
 int ecs_ctx_destroy(ecs_ctx* c) {
   if (!c) return 0;
+  std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
   (void)hipSetDevice(c->dev);
   void* ps[] = {c->tw, c->fhat, c->I, c->O, c->x0, c->acc, c->v1, c->v4, c->v5, c->xf, c->out, c->dx0, c->dacc,
                 c->dv1, c->dv4, c->dxf, c->dout, c->scal, c->hist, c->s_h, c->s_rs, c->s_rc, c->s_rh, c->s_nacc,
                 c->ckpt, c->xtape, c->itape, c->red, c->AO, c->AI, c->xb, c->k1b, c->k2b, c->k3b, c->hb};
   for (void* p : ps)
     if (p) (void)hipFree(p);
+  drop_graphs(c);
+  if (c->gs) (void)hipStreamDestroy(c->gs);
+  if (c->ev_in) (void)hipEventDestroy(c->ev_in);
+  if (c->ev_out) (void)hipEventDestroy(c->ev_out);
+  if (c->cnt) (void)hipFree(c->cnt);
+  if (c->hcnt) (void)hipHostFree(c->hcnt);
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->hred) (void)hipHostFree(c->hred);
   delete c;
@@ -573,6 +754,7 @@ int ecs_ctx_n(const ecs_ctx* c) { return c ? c->n : 0; }
 
 int ecs_state_vel(ecs_ctx* c, const double* in, double* out, int batch, int diss, void* stream) {
   if (!c || !in || !out || batch < 1) return fail(1, "ecs_state_vel: bad arguments");
+  GUARD(c);
   hipStream_t st = (hipStream_t)stream;
   DISPATCH_N(c->n, run_state_vel, c, in, nullptr, out, nullptr, batch, diss, st);
 }
@@ -580,12 +762,14 @@ int ecs_state_vel(ecs_ctx* c, const double* in, double* out, int batch, int diss
 int ecs_state_vel_jvp(ecs_ctx* c, const double* in, const double* din, double* out, double* dout, int batch,
                       int diss, void* stream) {
   if (!c || !in || !din || !out || !dout || batch < 1) return fail(1, "ecs_state_vel_jvp: bad arguments");
+  GUARD(c);
   hipStream_t st = (hipStream_t)stream;
   DISPATCH_N(c->n, run_state_vel, c, in, din, out, dout, batch, diss, st);
 }
 
 int ecs_lawson_rk4(ecs_ctx* c, double* f, int batch, const double* dt, int steps, int masked, void* stream) {
   if (!c || !f || !dt || batch < 1 || steps < 0) return fail(1, "ecs_lawson_rk4: bad arguments");
+  GUARD(c);
   hipStream_t st = (hipStream_t)stream;
 DISPATCH_N(c->n, rk4_p, c, f, batch, dt, steps, masked, st);
 
@@ -594,6 +778,7 @@ DISPATCH_N(c->n, rk4_p, c, f, batch, dt, steps, masked, st);
 int ecs_lawson_rk4_jvp(ecs_ctx* c, double* f, double* df, int batch, const double* dt, const double* ddt, int steps,
                        int masked, void* stream) {
   if (!c || !f || !df || !dt || !ddt || batch < 1 || steps < 0) return fail(1, "ecs_lawson_rk4_jvp: bad arguments");
+  GUARD(c);
   hipStream_t st = (hipStream_t)stream;
 DISPATCH_N(c->n, rk4_t, c, f, df, batch, dt, ddt, steps, masked, st);
 
@@ -602,6 +787,7 @@ DISPATCH_N(c->n, rk4_t, c, f, df, batch, dt, ddt, steps, masked, st);
 int ecs_lawson_rk43(ecs_ctx* c, double* x, int batch, const double* T, double h0, double atol, int maxs, int nck,
                     ecs_rk43_info* info, double* hs, int8_t* acc, void* stream) {
   if (!c || !x || !T || batch < 1 || maxs < 1 || nck < 1) return fail(1, "ecs_lawson_rk43: bad arguments");
+  GUARD(c);
   hipStream_t st = (hipStream_t)stream;
   DISPATCH_N(c->n, run_rk43, c, x, batch, T, h0, atol, maxs, nck, info, hs, acc, st);
 }
@@ -612,6 +798,7 @@ int ecs_lawson_rk43_jvp(ecs_ctx* c, double* x, double* dx, int batch, const doub
     return fail(1, "ecs_lawson_rk43_jvp: bad arguments");
   for (int b = 0; b < batch; ++b)
     if (attempts[b] < 0 || attempts[b] > budget) return fail(1, "ecs_lawson_rk43_jvp: attempts out of range");
+  GUARD(c);
   hipStream_t st = (hipStream_t)stream;
   DISPATCH_N(c->n, run_rk43_jvp, c, x, dx, batch, T, dT, hs, acc, attempts, budget, st);
 }
@@ -625,6 +812,7 @@ static std::map<int, double*> g_part;  // per device: 2 * VMAXG partial sums
 static int partials(double** out) {
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::recursive_mutex> alk(g_alloc_mu);
   std::lock_guard<std::mutex> lk(g_part_mu);
   auto it = g_part.find(dev);
   if (it == g_part.end()) {
@@ -692,6 +880,7 @@ static int arow(const Args& a, int B, hipStream_t st) {
 
 static int grow(double2** p, size_t* cap, size_t need) {
   if (need <= *cap) return 0;
+  std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
   if (*p) HIPCHK(hipFree(*p));
   *p = nullptr;
   HIPCHK(hipMalloc((void**)p, need * sizeof(double2)));
@@ -702,6 +891,7 @@ static int grow(double2** p, size_t* cap, size_t need) {
 static int ensure_adj(ecs_ctx* c, int B) {
   TRY(ensure_cap(c, B));
   if (B <= c->adj_cap) return 0;
+  std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
   const int nc = c->cap;
   const int KC = (c->n - 1) / 3 + 1;
   const size_t comp = c->compact_elems();
@@ -907,6 +1097,7 @@ static int run_rk4_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
 extern "C" int ecs_state_vel_vjp(ecs_ctx* c, const double* in, const double* cot, double* grad, int batch,
                                  int diss, void* stream) {
   if (!c || !in || !cot || !grad || batch < 1) return fail(1, "ecs_state_vel_vjp: bad arguments");
+  GUARD(c);
   hipStream_t st = (hipStream_t)stream;
   DISPATCH_N(c->n, run_state_vel_vjp, c, in, cot, grad, batch, diss, st);
 }
@@ -915,6 +1106,7 @@ extern "C" int ecs_lawson_rk4_vjp(ecs_ctx* c, const double* f0, double* fbar, do
                                   const double* dt, int steps, int masked, int ckpt, void* stream) {
   if (!c || !f0 || !fbar || !hbar || !dt || batch < 1 || steps < 1 || ckpt < 1)
     return fail(1, "ecs_lawson_rk4_vjp: bad arguments");
+  GUARD(c);
   hipStream_t st = (hipStream_t)stream;
   DISPATCH_N(c->n, run_rk4_vjp, c, f0, fbar, hbar, batch, dt, steps, masked, ckpt, st);
 }
@@ -928,6 +1120,7 @@ extern "C" int ecs_lawson_rk4_fwd_ckpt(ecs_ctx* c, double* f, int batch, const d
                                        int ckpt, double* ck, void* stream) {
   if (!c || !f || !dt || !ck || batch < 1 || steps < 1 || ckpt < 1)
     return fail(1, "ecs_lawson_rk4_fwd_ckpt: bad arguments");
+  GUARD(c);
   hipStream_t st = (hipStream_t)stream;
   DISPATCH_N(c->n, run_rk4_ckpt, c, f, batch, dt, steps, masked, ckpt, (double2*)ck, st);
 }
@@ -936,6 +1129,7 @@ extern "C" int ecs_lawson_rk4_vjp_ckpt(ecs_ctx* c, const double* ck, double* fba
                                        const double* dt, int steps, int masked, int ckpt, void* stream) {
   if (!c || !ck || !fbar || !hbar || !dt || batch < 1 || steps < 1 || ckpt < 1)
     return fail(1, "ecs_lawson_rk4_vjp_ckpt: bad arguments");
+  GUARD(c);
   hipStream_t st = (hipStream_t)stream;
   DISPATCH_N(c->n, run_rk4_rev, c, (const double2*)ck, fbar, hbar, batch, dt, steps, masked, ckpt, st);
 }
@@ -987,6 +1181,7 @@ static int run_time_kernel(ecs_ctx* c, int B, int which, int reps, double* ms, h
 
 extern "C" int ecs_time_kernel(ecs_ctx* c, int batch, int which, int reps, double* ms, void* stream) {
   if (!c || !ms || batch < 1 || reps < 1 || which < 0 || which > 3) return fail(1, "ecs_time_kernel: bad arguments");
+  GUARD(c);
   hipStream_t st = (hipStream_t)stream;
   DISPATCH_N(c->n, run_time_kernel, c, batch, which, reps, ms, st);
 }

@@ -97,6 +97,12 @@ struct Args {
   Hist* hist;              // [B][hstride]
   int hstride;
   int attempt;             // adaptive attempt index (primal) / accepted step (tangent)
+  // graph-replayable index: when cnt is set, the index is cnt[cslot] and the
+  // first kernel of an attempt writes cnt[cslot ^ 1] = index + 1 (a captured
+  // chunk of an even number of attempts then replays unchanged)
+  int* cnt;
+  int cslot;
+  int budget;              // adaptive primal: attempts at or beyond it only apply the last decision
   // tangent replay schedule of the adaptive stepper, [B][sstride]
   const double* s_h;       // step of the accepted attempt
   const int* s_rs;         // start of its preceding rejections in s_rh

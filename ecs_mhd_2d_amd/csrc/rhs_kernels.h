@@ -16,6 +16,8 @@ struct Blk {
   double ddt;    // tangent of dt per unit tangent
 };
 
+__device__ __forceinline__ int attempt_of(const Args& a) { return a.cnt ? a.cnt[a.cslot] : a.attempt; }
+
 __device__ __forceinline__ double relu_nan(double x) { return (x != x) ? x : (x > 0.0 ? x : 0.0); }
 
 // Controller decision of timestepping.py:288-313, from the record of attempt i-1.
@@ -30,8 +32,12 @@ __device__ __forceinline__ Blk prologue(const Args& a, int b) {
     return bk;
   }
   bk.T = a.Tb[b];
+  const int att = attempt_of(a);
+  if ((MODE == C_A || MODE == C_A0) && a.cnt && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 &&
+      threadIdx.x == 0)
+    a.cnt[a.cslot ^ 1] = att + 1;
   if (a.adaptive == AD_TANGENT) {
-    const int m = a.attempt;
+    const int m = att;
     bk.run = bk.stage = m < a.s_nacc[b];
     if (!bk.run) return bk;
     bk.h = a.s_h[(size_t)b * a.sstride + m];
@@ -40,7 +46,7 @@ __device__ __forceinline__ Blk prologue(const Args& a, int b) {
     return bk;
   }
   // AD_PRIMAL
-  const int i = a.attempt;
+  const int i = att;
   Hist* H = a.hist + (size_t)b * a.hstride;
   if (MODE == C_A0) {
     bk.h = a.h0;
@@ -65,7 +71,7 @@ __device__ __forceinline__ Blk prologue(const Args& a, int b) {
       else hn = hp.h * 0.9 * pow(a.atol / err, 0.25);
       hn = hn - relu_nan(s + hn - 1.0);
     }
-    const bool act = (MODE == C_A) && !(fabs(s - 1.0) < 1e-10);
+    const bool act = (MODE == C_A) && (a.budget <= 0 || i < a.budget) && !(fabs(s - 1.0) < 1e-10);
     bk.stage = act;
     bk.run = bk.pend || act;
     bk.h = hn;
@@ -85,8 +91,8 @@ __device__ __forceinline__ Blk prologue(const Args& a, int b) {
 }
 
 __device__ __forceinline__ bool row_active(const Args& a, int b) {
-  if (a.adaptive == AD_PRIMAL) return a.hist[(size_t)b * a.hstride + a.attempt].active != 0;
-  if (a.adaptive == AD_TANGENT) return a.attempt < a.s_nacc[b];
+  if (a.adaptive == AD_PRIMAL) return a.hist[(size_t)b * a.hstride + attempt_of(a)].active != 0;
+  if (a.adaptive == AD_TANGENT) return attempt_of(a) < a.s_nacc[b];
   return true;
 }
 
@@ -162,7 +168,7 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, int bb, size_
         v1 = a.v1[ci];
         dv1 = a.dv1[ci];
       }
-      const size_t so = (size_t)bb * a.sstride + a.attempt;
+      const size_t so = (size_t)bb * a.sstride + attempt_of(a);
       const int rs = a.s_rs[so], rc = a.s_rc[so];
       const double dT = a.dTb[bb];
       for (int q = 0; q < rc; ++q) {
@@ -406,7 +412,7 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
     __syncthreads();
     if (tid == 0) {
       for (int w = 1; w < NT / 64; ++w) mx = red[w] > mx ? red[w] : mx;
-      atomicMax(&a.hist[(size_t)b * a.hstride + a.attempt].err, mx);
+      atomicMax(&a.hist[(size_t)b * a.hstride + attempt_of(a)].err, mx);
     }
     return;
   }

@@ -52,10 +52,12 @@ def make_param_dict(n: int, nu: float, eta: float, b0=(0.0, 0.1), forcing_amp: f
     return pd
 
 
-def context(param_dict: dict) -> _runtime.Context:
-    """The device context (tables, workspace) for this param_dict's physics."""
+def context(param_dict: dict, lane: int = 0) -> _runtime.Context:
+    """The device context (tables, workspace) for this param_dict's physics;
+    concurrent callers on one grid size use distinct lanes."""
     n = param_dict["x"].shape[-1]
-    return _runtime.context_for(n, param_dict["nu"], param_dict["eta"], param_dict["b0"], param_dict["forcing"])
+    return _runtime.context_for(n, param_dict["nu"], param_dict["eta"], param_dict["b0"], param_dict["forcing"],
+                                lane=lane)
 
 
 def dissipation(param_dict: dict) -> torch.Tensor:
