@@ -96,12 +96,72 @@ def probe():
     print("probe rk43 jvp", lin.info["accepted"], lin.info["rejected"], flush=True)
 
 
+def batched_leg():
+    """64 RPO candidates of uneven size (N in {64,128,256} at 16:32:16, T ~ U[2,8],
+    sx ~ U[-0.5,0.5], spectra G(N, i)), adaptive RK4(3) atol 1e-8 over each period:
+    accepted RK steps/s batched (grouped by N, groups concurrent) vs one at a time."""
+    import torch
+    from ecs_mhd_2d_amd import batch
+    cands = batch.make_candidates()
+    warm = [dict(c, T=0.05) for c in cands]
+    batch.run_batched(warm)
+    batch.run_unbatched(warm[:3])
+    rb, tb = batch.run_batched(cands)
+    ru, tu = batch.run_unbatched(cands)
+    sb, su = batch.accepted_steps(rb) / tb, batch.accepted_steps(ru) / tu
+    same = all(torch.equal(a[0], b[0]) for a, b in zip(rb, ru))
+    return {"workload": "64 candidates, N in {64,128,256} (16/32/16), T~U[2,8], RK4(3) atol 1e-8",
+            "accepted_steps": batch.accepted_steps(rb), "batched_s": tb, "unbatched_s": tu,
+            "batched_steps_per_s": sb, "unbatched_steps_per_s": su, "speedup": sb / su,
+            "bitwise_equal_to_unbatched": bool(same)}
+
+
+def serving_leg():
+    """Poisson arrivals (4 req/s, rng seed 7) of mixed jobs, 70% adjoint descent /
+    30% Newton-GMRES, admitted against a 200 GB HBM budget; reduced in size so the
+    leg runs in seconds: N in {32, 64}, T ~ U[0.5, 1.5], 8 descent steps, 2 Newton
+    iterations with Krylov dimension 16, 24 jobs."""
+    from ecs_mhd_2d_amd import serve
+    w = serve.make_workload(4, rate=100.0, seed=1, sizes=(32, 64), T_range=(0.3, 0.4), descent_iters=1,
+                            newton_iters=1, inner=2, atol=1e-6, adaptive_budget=(64, 16))
+    serve.Server().run(w)
+    wl = serve.make_workload(24, rate=4.0, seed=7, sizes=(32, 64), T_range=(0.5, 1.5), descent_iters=8,
+                             newton_iters=2, inner=16, atol=1e-8, adaptive_budget=(64, 32))
+    _, stats = serve.Server().run(wl)
+    stats["workload"] = ("24 jobs, Poisson 4/s, 70% adjoint descent (8 ADAM steps, dt 0.004) / 30% Newton "
+                         "(2 iterations, Krylov 16, RK4(3) atol 1e-8), N in {32,64}, T~U[0.5,1.5]")
+    return stats
+
+
+def probe_roofline():
+    """The tangent row kernel at N=128, batch 1 (50 launches through ecs_time_kernel)
+    and a plain device copy of 64 MiB (read 64 MiB, write 64 MiB), whose counter
+    readings scale FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+    from ecs_mhd_2d_amd import _lib, _runtime, mhd
+    pd = mhd.make_param_dict(128, 1 / 40, 1 / 40)
+    ctx = mhd.context(pd)
+    ms = ctypes.c_double()
+    lib = _lib.load()
+    _lib.check(lib.ecs_time_kernel(ctx.handle, 1, 3, 50, ctypes.byref(ms), _runtime._stream()), "ecs_time_kernel")
+    a = torch.ones(8 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("probe-roofline", ms.value, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched-candidates and serving legs")
+    ap.add_argument("--probe-roofline", action="store_true",
+                    help="launch only the matvec's dominant kernel (50x) and a 64 MiB copy, for rocprofv3 --pmc")
     ap.add_argument("--probe", action="store_true",
                     help="run a short fixed kernel workload for rocprofv3 and exit")
     args = ap.parse_args()
@@ -115,6 +175,9 @@ def main():
     torch.cuda.init()
     if args.probe:
         probe()
+        return
+    if args.probe_roofline:
+        probe_roofline()
         return
     n = 128
     pd = mhd.make_param_dict(n, 1 / 40, 1 / 40, (0.0, 0.1))
@@ -176,6 +239,11 @@ def main():
         lib.ecs_tuned_tiles(nn, bb, arr)
         tiles[f"n{nn}_b{bb}"] = {"col_ty": arr[0], "col_tan_ty": arr[1], "row_tx": arr[2], "row_tan_tx": arr[3]}
 
+    extras = {}
+    if not args.no_extras:
+        extras["batched_candidates"] = batched_leg()
+        extras["serving"] = serving_leg()
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_matvec(fields.cpu().numpy(), 4.3, 0.1, info)
@@ -201,6 +269,7 @@ def main():
         "cpu_baseline": cpu,
         "autotuned_tiles": tiles,
     }
+    line.update(extras)
     print(json.dumps(line))
 
 
