@@ -168,7 +168,7 @@ static Args base_args(ecs_ctx* c) {
 }
 
 // ------------------------------------------------------------------ launches
-// Tile variants: column kernels take TY in {2, 4} columns per block, row
+// Tile variants: column kernels take TY in {1, 2, 4} columns per block, row
 // kernels TX in {1, 2, 4} rows per block.  The autotuner (autotune<N>) picks
 // one per (N, kernel family, batch bucket) by timing each on the device.
 enum TuneKind : int { K_COL = 0, K_COLT = 1, K_ROW = 2, K_ROWT = 3 };
@@ -187,12 +187,13 @@ static int tuned(int n, int kind, int B, int dflt) {
   return it == g_tune.end() ? dflt : it->second;
 }
 
+// default variant indices (column: TY = 1 << v, row: TX = 1 << v)
 template <int N> struct DefTile;
-template <> struct DefTile<16> { static constexpr int COL = 0, ROW = 1; };
-template <> struct DefTile<32> { static constexpr int COL = 1, ROW = 1; };
-template <> struct DefTile<64> { static constexpr int COL = 1, ROW = 1; };
-template <> struct DefTile<128> { static constexpr int COL = 1, ROW = 1; };
-template <> struct DefTile<256> { static constexpr int COL = 0, ROW = 0; };
+template <> struct DefTile<16> { static constexpr int COL = 1, ROW = 1; };
+template <> struct DefTile<32> { static constexpr int COL = 2, ROW = 1; };
+template <> struct DefTile<64> { static constexpr int COL = 2, ROW = 1; };
+template <> struct DefTile<128> { static constexpr int COL = 1, ROW = 0; };
+template <> struct DefTile<256> { static constexpr int COL = 1, ROW = 0; };
 
 template <int N, int TY, int MODE, bool TAN>
 static int col_v(const Args& a, int B, hipStream_t st) {
@@ -210,7 +211,8 @@ static int row_v(const Args& a, int B, hipStream_t st) {
 }
 template <int N, int MODE, bool TAN>
 static int col_var(int v, const Args& a, int B, hipStream_t st) {
-  return v == 0 ? col_v<N, 2, MODE, TAN>(a, B, st) : col_v<N, 4, MODE, TAN>(a, B, st);
+  return v == 0 ? col_v<N, 1, MODE, TAN>(a, B, st)
+                : (v == 1 ? col_v<N, 2, MODE, TAN>(a, B, st) : col_v<N, 4, MODE, TAN>(a, B, st));
 }
 template <int N, int RMODE, bool TAN>
 static int row_var(int v, const Args& a, int B, hipStream_t st) {
@@ -347,7 +349,7 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
   float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
   int pick[4] = {DefTile<N>::COL, DefTile<N>::COL, DefTile<N>::ROW, DefTile<N>::ROW};
   for (int kind = 0; kind < 4; ++kind) {
-    const int nv = (kind < 2) ? 2 : 3;
+    const int nv = 3;
     for (int v = 0; v < nv; ++v) {
       auto launch = [&]() -> int {
         switch (kind) {
@@ -625,9 +627,13 @@ static int run_rk43_jvp(ecs_ctx* c, double* x, double* dx, int B, const double* 
   TRY((col<N, C_LOAD, true>(al, B, st)));
   TRY((row<N, R_RHS, true>(al, B, st)));
   a.adaptive = AD_TANGENT;
-  auto step = [&](const Args& am, hipStream_t s2, bool first) -> int {
-    if (first) TRY((col<N, C_A0, true>(am, B, s2)));
-    else TRY((col<N, C_A, true>(am, B, s2)));
+  // one accepted step: [A] row S2 row S3 row S4 row S5; eagerly, S5 of step m
+  // is fused with the first kernel (A) of step m+1 (C_S5A)
+  auto step = [&](const Args& am, hipStream_t s2, bool first, bool with_a, bool fuse_next) -> int {
+    if (with_a) {
+      if (first) TRY((col<N, C_A0, true>(am, B, s2)));
+      else TRY((col<N, C_A, true>(am, B, s2)));
+    }
     TRY((row<N, R_RHS, true>(am, B, s2)));
     TRY((col<N, C_S2, true>(am, B, s2)));
     TRY((row<N, R_RHS, true>(am, B, s2)));
@@ -635,36 +641,34 @@ static int run_rk43_jvp(ecs_ctx* c, double* x, double* dx, int B, const double* 
     TRY((row<N, R_RHS, true>(am, B, s2)));
     TRY((col<N, C_S4, true>(am, B, s2)));
     TRY((row<N, R_RHS, true>(am, B, s2)));
-    TRY((col<N, C_S5, true>(am, B, s2)));
+    if (fuse_next) TRY((col<N, C_S5A, true>(am, B, s2)));
+    else TRY((col<N, C_S5, true>(am, B, s2)));
     return 0;
   };
-  if (maxacc > 0) {
+  constexpr int K = 32;
+  if (use_graphs() && maxacc > 1) {
     Args am0 = a;
     am0.attempt = 0;
-    TRY(step(am0, st, true));
-  }
-  constexpr int K = 32;
-  int launched = 1;
-  if (use_graphs() && maxacc > 1) {
+    TRY(step(am0, st, true, true, false));
     TRY(set_counter(c, 1, st));
     Args ag = a;
     ag.cnt = c->cnt;
     const std::string key = graph_key("rk43t", ag, B, K);
-    for (; launched < maxacc; launched += K) {
+    for (int launched = 1; launched < maxacc; launched += K) {
       TRY(graph_run(c, key, st, [&](hipStream_t gs) -> int {
         for (int j = 0; j < K; ++j) {
           Args aj = ag;
           aj.cslot = j & 1;
-          TRY(step(aj, gs, false));
+          TRY(step(aj, gs, false, true, false));
         }
         return 0;
       }));
     }
   } else {
-    for (int m = 1; m < maxacc; ++m) {
+    for (int m = 0; m < maxacc; ++m) {
       Args am = a;
       am.attempt = m;
-      TRY(step(am, st, false));
+      TRY(step(am, st, m == 0, m == 0, m + 1 < maxacc));
     }
   }
   TRY(unpack<N>(c->x0, x, B, 0, st));
@@ -1192,7 +1196,7 @@ extern "C" int ecs_tuned_tiles(int n, int batch, int* out) {
   for (int kind = 0; kind < 4; ++kind) {
     auto it = g_tune.find(tune_key(n, kind, batch));
     const int v = it == g_tune.end() ? -1 : it->second;
-    out[kind] = v < 0 ? -1 : (kind < 2 ? (v == 0 ? 2 : 4) : (1 << v));
+    out[kind] = v < 0 ? -1 : (1 << v);
   }
   return 0;
 }

@@ -58,6 +58,7 @@ enum ColMode : int {
   C_A0 = 7,     // adaptive, first attempt: v1 = v(x0) from the row output; stage-1 glue
   C_A = 8,      // adaptive: apply previous decision, new h, stage-1 glue from stored v1
   C_FIN = 9,    // adaptive: apply the last decision only
+  C_S5A = 10,   // tangent replay: accept this attempt (C_S5) and run the next step's C_A in the same kernel
 };
 
 // Row-kernel modes.

@@ -14,6 +14,8 @@ struct Blk {
   double T, h;   // adaptive: period and pseudo-time step
   double dt;     // stage scale (physical time step)
   double ddt;    // tangent of dt per unit tangent
+  bool next;     // C_S5A: the next accepted step exists (its stage-1 glue runs)
+  double h2, dt2, ddt2;  // C_S5A: its step, scale and tangent
 };
 
 __device__ __forceinline__ int attempt_of(const Args& a) { return a.cnt ? a.cnt[a.cslot] : a.attempt; }
@@ -26,6 +28,7 @@ __device__ __forceinline__ Blk prologue(const Args& a, int b) {
   Blk bk;
   bk.run = true; bk.stage = true; bk.pend = false; bk.pacc = false;
   bk.hprev = 0.0; bk.T = 0.0; bk.h = 0.0; bk.dt = 0.0; bk.ddt = 0.0;
+  bk.next = false; bk.h2 = 0.0; bk.dt2 = 0.0; bk.ddt2 = 0.0;
   if (a.adaptive == AD_FIXED) {
     bk.dt = a.dt ? a.dt[b] : 0.0;
     bk.ddt = (TAN && a.ddt) ? a.ddt[b] : 0.0;
@@ -43,6 +46,12 @@ __device__ __forceinline__ Blk prologue(const Args& a, int b) {
     bk.h = a.s_h[(size_t)b * a.sstride + m];
     bk.dt = bk.h * bk.T;
     bk.ddt = bk.h * a.dTb[b];
+    if (MODE == C_S5A && m + 1 < a.s_nacc[b]) {
+      bk.next = true;
+      bk.h2 = a.s_h[(size_t)b * a.sstride + m + 1];
+      bk.dt2 = bk.h2 * bk.T;
+      bk.ddt2 = bk.h2 * a.dTb[b];
+    }
     return bk;
   }
   // AD_PRIMAL
@@ -98,40 +107,109 @@ __device__ __forceinline__ bool row_active(const Args& a, int b) {
 
 __device__ __forceinline__ double2 cdivs(double2 a, double s) { return make_double2(a.x / s, a.y / s); }
 
+// Glue operands loaded before the column FFT so their latency overlaps it.
+struct Pre {
+  double2 F;              // masked forcing spectrum (omega only)
+  double2 x0, dx0;        // stage-1 state (modes that read it before writing)
+  double2 acc, dacc;      // RK accumulator
+};
+
+template <int MODE, bool TAN>
+__device__ __forceinline__ Pre prefetch(const Args& a, size_t ci, size_t fi, int c) {
+  Pre p;
+  const double2 z = make_double2(0.0, 0.0);
+  p.F = (c == 0 && MODE != C_LOAD) ? a.fhat[fi] : z;
+  constexpr bool X0 = MODE == C_LOAD || MODE == C_RHS || MODE == C_S1F || MODE == C_S2 || MODE == C_S3 ||
+                      MODE == C_A0;
+  constexpr bool ACC = MODE == C_S2 || MODE == C_S3 || MODE == C_S4;
+  p.x0 = X0 ? a.x0[ci] : z;
+  p.dx0 = (X0 && TAN) ? a.dx0[ci] : z;
+  p.acc = ACC ? a.acc[ci] : z;
+  p.dacc = (ACC && TAN) ? a.dacc[ci] : z;
+  return p;
+}
+
+// Stage-1 glue of one Lawson step (timestepping.py:70-74, :263-270):
+// acc = e e x0 + e e k1/6, y2 = e x0 + e k1/2 with k1 = dt v1, and tangents.
+template <bool TAN>
+__device__ __forceinline__ void stage1(const Args& a, size_t ci, double e, double de, double dt, double ddt,
+                                       double2 x0, double2 dx0, double2 v1, double2 dv1, double2& y, double2& dy) {
+  const double de2 = 2.0 * e * de;
+  const double2 k1 = cscale(dt, v1);
+  const double2 ek1 = cscale(e, k1);
+  const double2 ex0 = cscale(e, x0);
+  a.acc[ci] = cadd(cscale(e, ex0), cdivs(cscale(e, ek1), 6.0));
+  y = cadd(ex0, cdivs(ek1, 2.0));
+  if (TAN) {
+    const double2 dk1 = cadd(cscale(ddt, v1), cscale(dt, dv1));
+    a.dacc[ci] = cadd(cadd(cscale(de2, x0), cscale(e * e, dx0)),
+                      cdivs(cadd(cscale(de2, k1), cscale(e * e, dk1)), 6.0));
+    dy = cadd(cadd(cscale(de, x0), cscale(e, dx0)), cdivs(cadd(cscale(de, k1), cscale(e, dk1)), 2.0));
+  }
+}
+
+// Rejections preceding accepted step m of the tangent replay: the carried k1
+// becomes e (e k1) per rejected step h_j (timestepping.py:267,276,297).
+__device__ __forceinline__ void replay_rejections(const Args& a, int bb, int m, double TL, double L, double2& v1,
+                                                  double2& dv1) {
+  const size_t so = (size_t)bb * a.sstride + m;
+  const int rs = a.s_rs[so], rc = a.s_rc[so];
+  const double dT = a.dTb[bb];
+  for (int q = 0; q < rc; ++q) {
+    const double hj = a.s_rh[(size_t)bb * a.rstride + rs + q];
+    const double ep = exp((TL * hj) * 0.5);
+    v1 = cscale(ep, cscale(ep, v1));
+    dv1 = cadd(cscale(ep, cscale(ep, dv1)), cscale(L * (hj * dT), v1));
+  }
+}
+
 // Stage glue for one kept mode.  rh/drh: row-pass spectrum after the x-FFT
 // (nonlinear term before masking).  Produces the next stage input y (dy).
 // Lawson RK4 in the form of timestepping.py:70-80 and :263-282:
 //   y2 = e x0 + e k1/2, y3 = e x0 + k2/2, y4 = e e x0 + e k3,
 //   xf = e e x0 + e e k1/6 + e k2/3 + e k3/3 + k4/6,   k_s = dt v(y_s).
 template <int N, int MODE, bool TAN>
-__device__ __forceinline__ void glue(const Args& a, const Blk& bk, int bb, size_t ci, int r, int c, int kx, int ky,
-                                     double2 rh, double2 drh, double2& y, double2& dy) {
-  using G = Geo<N>;
+__device__ __forceinline__ void glue(const Args& a, const Blk& bk, const Pre& p, int bb, size_t ci, int r, int c,
+                                     int kx, int ky, double2 rh, double2 drh, double2& y, double2& dy) {
   const double k2 = (double)(kx * kx + ky * ky);
   const bool dc = (kx == 0 && ky == 0);
   // mask (omega) and k^2 mask (j) of mhd_jax.py:160,164; forcing added spectrally
   const double m = dc ? 0.0 : (c == 0 ? 1.0 : k2);
   const double L = -k2 * a.nu[c];
-  double2 F = make_double2(0.0, 0.0);
-  if (c == 0) F = a.fhat[r * G::KC + ky];
-  const double2 vh = make_double2(m * rh.x + F.x, m * rh.y + F.y);
+  const double2 vh = make_double2(m * rh.x + p.F.x, m * rh.y + p.F.y);
   const double2 dvh = cscale(m, drh);
   y = make_double2(0.0, 0.0);
   dy = y;
   if (MODE == C_LOAD) {
-    y = a.x0[ci];
-    if (TAN) dy = a.dx0[ci];
+    y = p.x0;
+    if (TAN) dy = p.dx0;
     return;
   }
   if (MODE == C_RHS) {
     double2 o = vh;
-    if (a.include_diss) o = cadd(o, cscale(L, a.x0[ci]));
+    if (a.include_diss) o = cadd(o, cscale(L, p.x0));
     a.out[ci] = o;
     if (TAN) {
       double2 d = dvh;
-      if (a.include_diss) d = cadd(d, cscale(L, a.dx0[ci]));
+      if (a.include_diss) d = cadd(d, cscale(L, p.dx0));
       a.dout[ci] = d;
     }
+    return;
+  }
+  const double TL = bk.T * L;
+  if (MODE == C_S5A) {
+    // accept this attempt: k1 <- k5, x0 <- xf; then the next step's C_A glue
+    const double2 x0 = a.xf[ci], dx0 = a.dxf[ci];
+    a.x0[ci] = x0;
+    a.dx0[ci] = dx0;
+    double2 v1 = vh, dv1 = dvh;
+    if (bk.next) replay_rejections(a, bb, attempt_of(a) + 1, TL, L, v1, dv1);
+    a.v1[ci] = v1;
+    a.dv1[ci] = dv1;
+    if (!bk.next) return;
+    const double e = exp((TL * bk.h2) * 0.5);
+    const double de = (bk.ddt2 * 0.5) * L * e;
+    stage1<TAN>(a, ci, e, de, bk.dt2, bk.ddt2, x0, dx0, v1, dv1, y, dy);
     return;
   }
   // v1 of the stage-1 glue
@@ -141,7 +219,6 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, int bb, size_
     dv1 = dvh;
   }
   if (MODE == C_A0 || MODE == C_A || MODE == C_FIN) {
-    const double TL = bk.T * L;
     if (a.adaptive == AD_PRIMAL) {
       if (MODE == C_A0) {
         v1 = vh;
@@ -168,15 +245,7 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, int bb, size_
         v1 = a.v1[ci];
         dv1 = a.dv1[ci];
       }
-      const size_t so = (size_t)bb * a.sstride + attempt_of(a);
-      const int rs = a.s_rs[so], rc = a.s_rc[so];
-      const double dT = a.dTb[bb];
-      for (int q = 0; q < rc; ++q) {
-        const double hj = a.s_rh[(size_t)bb * a.rstride + rs + q];
-        const double ep = exp((TL * hj) * 0.5);
-        v1 = cscale(ep, cscale(ep, v1));
-        dv1 = cadd(cscale(ep, cscale(ep, dv1)), cscale(L * (hj * dT), v1));
-      }
+      replay_rejections(a, bb, attempt_of(a), TL, L, v1, dv1);
       a.v1[ci] = v1;
       a.dv1[ci] = dv1;
     }
@@ -188,54 +257,45 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, int bb, size_
     e = exp((bk.dt * 0.5) * L);
     if (a.masked_e && dc) e = 0.0;
   } else {
-    e = exp(((bk.T * L) * bk.h) * 0.5);
+    e = exp((TL * bk.h) * 0.5);
   }
   const double de = TAN ? (bk.ddt * 0.5) * L * e : 0.0;
   const double de2 = 2.0 * e * de;
-  const double2 x0 = a.x0[ci];
-  const double2 dx0 = TAN ? a.dx0[ci] : make_double2(0.0, 0.0);
+  // C_A may have just replaced x0 (accepted previous attempt): read it back
+  const double2 x0 = (MODE == C_A) ? a.x0[ci] : p.x0;
+  const double2 dx0 = (MODE == C_A && TAN) ? a.dx0[ci] : p.dx0;
   if (MODE == C_S1F && a.xtape) a.xtape[ci] = x0;
   if ((MODE == C_S1F || MODE == C_S2 || MODE == C_S3 || MODE == C_S4) && a.vtape) a.vtape[ci] = vh;
   if (MODE == C_S1F || MODE == C_A0 || MODE == C_A) {
-    const double2 k1 = cscale(bk.dt, v1);
-    const double2 ek1 = cscale(e, k1);
-    const double2 ex0 = cscale(e, x0);
-    a.acc[ci] = cadd(cscale(e, ex0), cdivs(cscale(e, ek1), 6.0));
-    y = cadd(ex0, cdivs(ek1, 2.0));
-    if (TAN) {
-      const double2 dk1 = cadd(cscale(bk.ddt, v1), cscale(bk.dt, dv1));
-      a.dacc[ci] = cadd(cadd(cscale(de2, x0), cscale(e * e, dx0)),
-                        cdivs(cadd(cscale(de2, k1), cscale(e * e, dk1)), 6.0));
-      dy = cadd(cadd(cscale(de, x0), cscale(e, dx0)), cdivs(cadd(cscale(de, k1), cscale(e, dk1)), 2.0));
-    }
+    stage1<TAN>(a, ci, e, de, bk.dt, bk.ddt, x0, dx0, v1, dv1, y, dy);
     return;
   }
   const double2 ks = cscale(bk.dt, vh);
   const double2 dks = TAN ? cadd(cscale(bk.ddt, vh), cscale(bk.dt, dvh)) : make_double2(0.0, 0.0);
   if (MODE == C_S2) {
-    a.acc[ci] = cadd(a.acc[ci], cdivs(cscale(e, ks), 3.0));
+    a.acc[ci] = cadd(p.acc, cdivs(cscale(e, ks), 3.0));
     y = cadd(cscale(e, x0), cdivs(ks, 2.0));
     if (TAN) {
-      a.dacc[ci] = cadd(a.dacc[ci], cdivs(cadd(cscale(de, ks), cscale(e, dks)), 3.0));
+      a.dacc[ci] = cadd(p.dacc, cdivs(cadd(cscale(de, ks), cscale(e, dks)), 3.0));
       dy = cadd(cadd(cscale(de, x0), cscale(e, dx0)), cdivs(dks, 2.0));
     }
     return;
   }
   if (MODE == C_S3) {
-    a.acc[ci] = cadd(a.acc[ci], cdivs(cscale(e, ks), 3.0));
+    a.acc[ci] = cadd(p.acc, cdivs(cscale(e, ks), 3.0));
     y = cadd(cscale(e, cscale(e, x0)), cscale(e, ks));
     if (TAN) {
-      a.dacc[ci] = cadd(a.dacc[ci], cdivs(cadd(cscale(de, ks), cscale(e, dks)), 3.0));
+      a.dacc[ci] = cadd(p.dacc, cdivs(cadd(cscale(de, ks), cscale(e, dks)), 3.0));
       dy = cadd(cadd(cscale(de2, x0), cscale(e * e, dx0)), cadd(cscale(de, ks), cscale(e, dks)));
     }
     return;
   }
   if (MODE == C_S4) {
-    const double2 xf = cadd(a.acc[ci], cdivs(ks, 6.0));
+    const double2 xf = cadd(p.acc, cdivs(ks, 6.0));
     y = xf;
     double2 dxf = make_double2(0.0, 0.0);
     if (TAN) {
-      dxf = cadd(a.dacc[ci], cdivs(dks, 6.0));
+      dxf = cadd(p.dacc, cdivs(dks, 6.0));
       dy = dxf;
     }
     if (a.adaptive == AD_FIXED) {
@@ -270,8 +330,9 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   constexpr int NF = TAN ? 2 : 1;
   constexpr int NDO = (MODE == C_S5 && !TAN) ? 1 : 4;
   constexpr bool FWD = MODE == C_RHS || MODE == C_S1F || MODE == C_S2 || MODE == C_S3 || MODE == C_S4 ||
-                       MODE == C_S5 || MODE == C_A0;
+                       MODE == C_S5 || MODE == C_A0 || MODE == C_S5A;
   constexpr bool HASINV = !(MODE == C_RHS || MODE == C_FIN || (MODE == C_S5 && TAN));
+  constexpr int EPT = (R * TY + NT - 1) / NT;  // glue elements per thread
   constexpr int NSEQ = HASINV ? NDO * NF * TY : NF * TY;
   constexpr int ND = (NDO == 1) ? 2 : 8 * NF;
   constexpr int SS = Seq<N>::S;
@@ -281,6 +342,19 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   const int b = blockIdx.z, c = blockIdx.y, ky0 = blockIdx.x * TY, tid = threadIdx.x;
   const Blk bk = prologue<MODE, TAN>(a, b);
   if (!bk.run) return;
+  // issue the glue operand loads first: their latency overlaps the x-FFT
+  Pre pre[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int idx = tid + q * NT;
+    const int cy = idx % TY;
+    const int r = idx / TY;
+    const int ky = ky0 + cy;
+    if (idx < R * TY && ky < KC) {
+      const size_t ci = (((size_t)b * 2 + c) * R + r) * KC + ky;
+      pre[q] = prefetch<MODE, TAN>(a, ci, (size_t)r * KC + ky, c);
+    }
+  }
 
   if (FWD) {
     const double2* O = a.rowout + (size_t)b * (2 * NF) * N * KC;
@@ -298,13 +372,15 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
     fft_lds<N, NF * TY, SS, NT, false>(sm, twl);
   }
 
-  const bool inv = HASINV && bk.stage && !a.skip_inv;
+  const bool inv = HASINV && (MODE == C_S5A ? bk.next : bk.stage) && !a.skip_inv;
   constexpr double invN2 = 1.0 / ((double)N * (double)N);
-  for (int idx = tid; idx < R * TY; idx += NT) {
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int idx = tid + q * NT;
     const int cy = idx % TY;
     const int r = idx / TY;
     const int ky = ky0 + cy;
-    if (ky >= KC) continue;
+    if (idx >= R * TY || ky >= KC) continue;
     const int kx = G::row_kx(r);
     const int xr = G::row_x(r);
     const size_t ci = (((size_t)b * 2 + c) * R + r) * KC + ky;
@@ -314,7 +390,7 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
       if (TAN) drh = sm[(1 * TY + cy) * SS + pidx(xr)];
     }
     double2 y, dy;
-    glue<N, MODE, TAN>(a, bk, b, ci, r, c, kx, ky, rh, drh, y, dy);
+    glue<N, MODE, TAN>(a, bk, pre[q], b, ci, r, c, kx, ky, rh, drh, y, dy);
     if (HASINV && inv) {
       if (NDO == 1) {
         sm[(0 * TY + cy) * SS + pidx(xr)] = cscale(invN2, y);

@@ -47,11 +47,18 @@ def test_graph_replays_match_eager_rk4_and_rk43():
         assert ig[0]["accepted"] == ir[0]["accepted"] and ig[0]["rejected"] == ir[0]["rejected"]
         assert np.array_equal(ig[0]["hs"], ir[0]["hs"])
     dx0 = torch.as_tensor(spectrum(n, 4)).cuda()
+    # eager launches fuse each step's last kernel with the next step's first
+    # (C_S5A); the captured chunk keeps them apart, so the two agree to
+    # rounding (FMA contraction differs between the kernels), not bitwise
     _, dxr = ctx.lawson_rk43_jvp(x0, dx0, 0.7, 0.2, ir)
+    first = None
     for rep in range(3):
         _, dxg = graphs(lambda: ctx.lawson_rk43_jvp(x0, dx0, 0.7, 0.2, ir))
         torch.cuda.synchronize()
-        assert torch.equal(dxg, dxr), rep
+        assert float(torch.linalg.vector_norm(dxg - dxr) / torch.linalg.vector_norm(dxr)) < 1e-12, rep
+        if first is None:
+            first = dxg
+        assert torch.equal(dxg, first), rep   # replays of one graph are bitwise identical
 
 
 def test_budget_exhaustion_with_graph_chunks_matches_oracle_counts():
