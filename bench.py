@@ -96,6 +96,43 @@ def probe():
     print("probe rk43 jvp", lin.info["accepted"], lin.info["rejected"], flush=True)
 
 
+def newton_leg(inp, pd):
+    """One full Newton-GMRES iteration on the headline guess (Krylov dimension 64:
+    objective, linearisation, 65 matvecs, line search, dealias)."""
+    import torch
+    from ecs_mhd_2d_amd import solvers
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    new, log = solvers.newton_gmres_update(inp, pd, "Lawson_RK43", inner=64)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {"workload": "one Newton-GMRES iteration, inner 64, same guess as the headline", "wall_s": wall,
+            "f_norm": log["f_norm"], "gmres_rel_res": log["gmres_rel_res"], "damp": log["damp"],
+            "krylov_residual_last": log["krylov_residuals"][-1]}
+
+
+def gradient_leg(reps=3):
+    """Adjoint-descent gradient: loss_RPO value and gradient at N=256, T=4.3, 1088
+    Lawson-RK4 steps (17 x 64), reverse pass with checkpoint/recompute every 64."""
+    import torch
+    from ecs_mhd_2d_amd import loss_functions as LF, mhd
+    from ecs_mhd_2d_amd.synthetic import fields as synth_fields
+    n = 256
+    pd = mhd.make_param_dict(n, 1 / 40, 1 / 40)
+    pd.update({"num_checkpoints": 17, "ministeps": 64, "shift_reflect_ny": 0, "rot": False})
+    inp = {"fields": torch.as_tensor(synth_fields(n, 2)).cuda(), "T": 4.3, "sx": 0.1}
+    LF.loss_RPO_value_and_grad(inp, pd)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        LF.loss_RPO_value_and_grad(inp, pd)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"workload": "loss_RPO value+grad, N=256, T=4.3, 1088 steps, checkpoint every 64",
+            "seconds_per_gradient": dt, "gradient_evals_per_s": 1.0 / dt,
+            "rk4_steps_per_s_fwd_plus_bwd": 1088 / dt}
+
+
 def batched_leg():
     """64 RPO candidates of uneven size (N in {64,128,256} at 16:32:16, T ~ U[2,8],
     sx ~ U[-0.5,0.5], spectra G(N, i)), adaptive RK4(3) atol 1e-8 over each period:
@@ -241,6 +278,8 @@ def main():
 
     extras = {}
     if not args.no_extras:
+        extras["newton_iteration"] = newton_leg(inp, pd)
+        extras["adjoint_gradient"] = gradient_leg()
         extras["batched_candidates"] = batched_leg()
         extras["serving"] = serving_leg()
 

@@ -239,7 +239,11 @@ def job_memory(kind, n, inner=64, steps=0, budget=4096):
 
 # ------------------------------------------------------------------ scheduler
 class Server:
-    def __init__(self, hbm_budget=200e9, nu=1 / 40, eta=1 / 40, b0=(0.0, 0.1), max_batch=64):
+    """``executor(key, ops, lane) -> results`` runs one group of compatible ops;
+    the default runs them on the GPU through the fused kernels."""
+
+    def __init__(self, hbm_budget=200e9, nu=1 / 40, eta=1 / 40, b0=(0.0, 0.1), max_batch=64, executor=None):
+        self.executor = executor
         self.budget = hbm_budget
         self.nu, self.eta, self.b0 = nu, eta, b0
         self.max_batch = max_batch
@@ -310,6 +314,9 @@ class Server:
             def work(ti):
                 key, js = tasks[ti]
                 try:
+                    if self.executor is not None:
+                        outs[ti] = self.executor(key, [j.pending for j in js], lanes[ti])
+                        return
                     s = torch.cuda.Stream()
                     with torch.cuda.stream(s):
                         outs[ti] = self._run_group(key, [j.pending for j in js], lanes[ti])
@@ -328,7 +335,8 @@ class Server:
                     work(ti)
             if errs:
                 raise errs[0]
-            torch.cuda.synchronize()   # results cross from the worker streams to the jobs' stream
+            if self.executor is None:
+                torch.cuda.synchronize()   # results cross from the worker streams to the jobs' stream
             self.ticks += 1
             self.batched_ops += sum(len(js) for _, js in tasks)
             now = time.perf_counter() - t0
@@ -344,6 +352,7 @@ class Server:
                         used -= j.mem
             active = still
         wall = time.perf_counter() - t0
+        self.max_concurrent = getattr(self, "max_concurrent", 0)
         lat = np.array([j.finish - j.arrival for j in done])
         return done, {"jobs": len(done), "wall_s": wall, "throughput_jobs_per_s": len(done) / wall,
                       "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),

@@ -1,0 +1,74 @@
+"""Host logic of the serving loop (no GPU): admission against the memory
+budget, grouping of compatible operations across jobs, arrival order and the
+latency statistics, with a CPU executor standing in for the kernels."""
+import threading
+
+import numpy as np
+
+from ecs_mhd_2d_amd import serve
+
+
+def toy_job(n, steps, value):
+    """A job that asks for `steps` 'grad' ops and sums what it receives."""
+    total = 0.0
+    for _ in range(steps):
+        r = yield serve.Op("grad", n, steps=64, x=value)
+        total += r
+    return total
+
+
+class Exec:
+    def __init__(self):
+        self.calls = []
+        self.lock = threading.Lock()
+
+    def __call__(self, key, ops, lane):
+        with self.lock:
+            self.calls.append((key, len(ops), lane))
+        return [2.0 * o.kw["x"] for o in ops]
+
+
+def arrivals(k, mem=10.0, gap=0.0, steps=3):
+    return [(gap * i, "adjoint", (lambda i=i: toy_job(32 if i % 2 else 64, steps, float(i))), mem, 32 if i % 2 else 64)
+            for i in range(k)]
+
+
+def test_results_and_batching():
+    ex = Exec()
+    done, stats = serve.Server(executor=ex).run(arrivals(6))
+    assert stats["jobs"] == 6
+    res = {j.id: j.result for j in done}
+    for i in range(6):
+        assert res[i] == 3 * 2.0 * i
+    # all six jobs arrive at t=0: each tick has one group per grid size
+    assert all(c[1] == 3 for c in ex.calls)
+    assert {c[0] for c in ex.calls} == {("grad", 32, 64), ("grad", 64, 64)}
+    assert stats["mean_batch"] == 6.0
+
+
+def test_admission_budget_limits_residency():
+    ex = Exec()
+    done, stats = serve.Server(hbm_budget=25.0, executor=ex).run(arrivals(6, mem=10.0))
+    assert stats["jobs"] == 6
+    assert stats["peak_resident_bytes"] <= 25.0
+    assert max(c[1] for c in ex.calls) <= 1      # two resident jobs of different N: never batched together
+    assert stats["ticks"] > 6
+
+
+def test_latency_percentiles_follow_arrivals():
+    ex = Exec()
+    done, stats = serve.Server(executor=ex).run(arrivals(4, gap=0.05, steps=1))
+    lat = sorted(j.finish - j.arrival for j in done)
+    assert stats["p50_latency_s"] == np.percentile(lat, 50)
+    assert stats["p99_latency_s"] <= max(lat) + 1e-12
+    assert all(j.finish >= j.arrival for j in done)
+
+
+def test_max_batch_splits_groups_into_lanes():
+    ex = Exec()
+    srv = serve.Server(executor=ex, max_batch=2)
+    done, stats = srv.run([(0.0, "adjoint", (lambda i=i: toy_job(64, 1, float(i))), 1.0, 64) for i in range(5)])
+    assert stats["jobs"] == 5
+    sizes = sorted(c[1] for c in ex.calls)
+    assert sizes == [1, 2, 2]
+    assert sorted(c[2] for c in ex.calls) == [0, 1, 2]   # concurrent tasks on one N get distinct lanes
