@@ -274,7 +274,11 @@ def main():
     for nn, bb in ((128, 1), (256, 64)):
         arr = (ctypes.c_int * 4)()
         lib.ecs_tuned_tiles(nn, bb, arr)
-        tiles[f"n{nn}_b{bb}"] = {"col_ty": arr[0], "col_tan_ty": arr[1], "row_tx": arr[2], "row_tan_tx": arr[3]}
+        plans = (ctypes.c_int * 4)()
+        lib.ecs_tuned_plans(nn, plans)
+        tiles[f"n{nn}_b{bb}"] = {"col_ty": arr[0], "col_tan_ty": arr[1], "row_tx": arr[2], "row_tan_tx": arr[3],
+                                 "fft_plan_col": plans[0], "fft_plan_col_tan": plans[1], "fft_plan_row": plans[2],
+                                 "fft_plan_row_tan": plans[3]}
 
     extras = {}
     if not args.no_extras:

@@ -50,6 +50,7 @@ _SIGS = {
                                     P_int, c_int, c_void_p]),
     "ecs_time_kernel": (c_int, [c_void_p, c_int, c_int, c_int, P_double, c_void_p]),
     "ecs_tuned_tiles": (c_int, [c_int, c_int, P_int]),
+    "ecs_tuned_plans": (c_int, [c_int, P_int]),
     "ecs_mgs": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "ecs_dot": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
 }

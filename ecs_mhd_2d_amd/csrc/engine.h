@@ -1,0 +1,747 @@
+// Grid-size-templated orchestration of the fused kernels: launches, the
+// autotuner, and the stepper / tangent / adjoint drivers behind the C-ABI.
+// Included once per supported N by grid_n*.hip, which instantiate it through
+// ECS_GRID(N).
+#pragma once
+#include "rhs_kernels.h"
+#include "adjoint_kernels.h"
+#include "host.h"
+
+// ------------------------------------------------------------------ launches
+// default variant indices (column: TY = 1 << v, row: TX = 1 << v; wide plan)
+template <int N> struct DefTile;
+template <> struct DefTile<16> { static constexpr int COL = 1, ROW = 1; };
+template <> struct DefTile<32> { static constexpr int COL = 2, ROW = 1; };
+template <> struct DefTile<64> { static constexpr int COL = 2, ROW = 1; };
+template <> struct DefTile<128> { static constexpr int COL = 1, ROW = 0; };
+template <> struct DefTile<256> { static constexpr int COL = 1, ROW = 0; };
+
+template <int N, int TY, int MODE, bool TAN, int P>
+static int col_v(const Args& a, int B, hipStream_t st) {
+  dim3 grid((Geo<N>::KC + TY - 1) / TY, 2, B);
+  hipLaunchKernelGGL((col_kernel<N, TY, NT, MODE, TAN, P>), grid, dim3(NT), 0, st, a);
+  LAUNCHCHK();
+  return 0;
+}
+template <int N, int TX, int RMODE, bool TAN, int P>
+static int row_v(const Args& a, int B, hipStream_t st) {
+  dim3 grid(N / TX, 1, B);
+  hipLaunchKernelGGL((row_kernel<N, TX, NT, RMODE, TAN, P>), grid, dim3(NT), 0, st, a);
+  LAUNCHCHK();
+  return 0;
+}
+// variant v: tile 1 << (v % 3), plan wide (v < 3) or narrow (v >= 3)
+template <int N, int MODE, bool TAN>
+static int col_var(int v, const Args& a, int B, hipStream_t st) {
+  switch (v) {
+    case 0: return col_v<N, 1, MODE, TAN, 16>(a, B, st);
+    case 1: return col_v<N, 2, MODE, TAN, 16>(a, B, st);
+    case 2: return col_v<N, 4, MODE, TAN, 16>(a, B, st);
+    case 3: return col_v<N, 1, MODE, TAN, 8>(a, B, st);
+    case 4: return col_v<N, 2, MODE, TAN, 8>(a, B, st);
+    default: return col_v<N, 4, MODE, TAN, 8>(a, B, st);
+  }
+}
+template <int N, int RMODE, bool TAN>
+static int row_var(int v, const Args& a, int B, hipStream_t st) {
+  switch (v) {
+    case 0: return row_v<N, 1, RMODE, TAN, 16>(a, B, st);
+    case 1: return row_v<N, 2, RMODE, TAN, 16>(a, B, st);
+    case 2: return row_v<N, 4, RMODE, TAN, 16>(a, B, st);
+    case 3: return row_v<N, 1, RMODE, TAN, 8>(a, B, st);
+    case 4: return row_v<N, 2, RMODE, TAN, 8>(a, B, st);
+    default: return row_v<N, 4, RMODE, TAN, 8>(a, B, st);
+  }
+}
+template <int N, int MODE, bool TAN>
+static int col(const Args& a, int B, hipStream_t st) {
+  return col_var<N, MODE, TAN>(tuned(N, TAN ? K_COLT : K_COL, B, DefTile<N>::COL), a, B, st);
+}
+template <int N, int RMODE, bool TAN>
+static int row(const Args& a, int B, hipStream_t st) {
+  return row_var<N, RMODE, TAN>(tuned(N, TAN ? K_ROWT : K_ROW, B, DefTile<N>::ROW), a, B, st);
+}
+
+template <int N>
+static int pack(const double* full, double2* comp, int B, hipStream_t st) {
+  const size_t tot = (size_t)B * 2 * Geo<N>::R * Geo<N>::KC;
+  int blocks = (int)std::min<size_t>((tot + 255) / 256, 4096);
+  hipLaunchKernelGGL((pack_kernel<N>), dim3(blocks), dim3(256), 0, st, (const double2*)full, comp, B);
+  LAUNCHCHK();
+  return 0;
+}
+template <int N>
+static int unpack(const double2* comp, double* full, int B, int zero_outside, hipStream_t st) {
+  const size_t tot = zero_outside ? (size_t)B * 2 * N * Geo<N>::NH : (size_t)B * 2 * Geo<N>::R * Geo<N>::KC;
+  int blocks = (int)std::min<size_t>((tot + 255) / 256, 4096);
+  hipLaunchKernelGGL((unpack_kernel<N>), dim3(blocks), dim3(256), 0, st, comp, (double2*)full, B, zero_outside);
+  LAUNCHCHK();
+  return 0;
+}
+
+// ------------------------------------------------------------------ autotuner
+// Average time (ms) of variants [v0, v1) of one kernel family at batch B,
+// timed with HIP events on `st` over back-to-back launches on zeroed
+// workspace (the workspace must hold B candidates).
+template <int N>
+static int time_variants(ecs_ctx* c, int B, int kind, int v0, int v1, float* t, hipStream_t st) {
+  const size_t ce = c->compact_elems() * B * sizeof(double2);
+  std::vector<double> dt(B, 1e-3);
+  TRY(upload_scalars(c, 0, dt.data(), B, st));
+  TRY(upload_scalars(c, 1, dt.data(), B, st));
+  Args a = base_args(c);
+  HIPCHK(hipMemsetAsync(c->x0, 0, ce, st));
+  HIPCHK(hipMemsetAsync(c->dx0, 0, ce, st));
+  HIPCHK(hipMemsetAsync(c->acc, 0, ce, st));
+  HIPCHK(hipMemsetAsync(c->dacc, 0, ce, st));
+  TRY((col_var<N, C_LOAD, true>(1, a, B, st)));
+  TRY((row_var<N, R_RHS, true>(1, a, B, st)));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int reps = std::max(3, 64 / B);
+  int rc = 0;
+  for (int v = v0; v < v1 && !rc; ++v) {
+    auto launch = [&]() -> int {
+      switch (kind) {
+        case K_COL: return col_var<N, C_S2, false>(v, a, B, st);
+        case K_COLT: return col_var<N, C_S2, true>(v, a, B, st);
+        case K_ROW: return row_var<N, R_RHS, false>(v, a, B, st);
+        default: return row_var<N, R_RHS, true>(v, a, B, st);
+      }
+    };
+    rc = launch();
+    for (int i = 0; i < reps && !rc; ++i) {
+      if (i == 0 && hipEventRecord(e0, st) != hipSuccess) rc = fail(2, "hipEventRecord");
+      if (!rc) rc = launch();
+    }
+    if (!rc && hipEventRecord(e1, st) != hipSuccess) rc = fail(2, "hipEventRecord");
+    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = fail(2, "hipEventSynchronize");
+    float ms = 0.f;
+    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(2, "hipEventElapsedTime");
+    t[v - v0] = ms / reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+// Picks, per kernel family (column, tangent column, row, tangent row), the
+// FFT radix plan once per grid size and the tile per batch bucket (powers of
+// two).  The plan decides the arithmetic, so it is fixed at the first call
+// for a grid size, whatever that call's batch: both plans are timed at a
+// latency batch (1) and a throughput batch (16), and the plan whose worst
+// slowdown against the faster plan at either batch is smaller wins.  Every
+// batch size then computes with the same arithmetic, so a candidate's result
+// does not depend on the batch it rides in.  Tiles do not change the
+// arithmetic and are timed per bucket within the fixed plan.  Runs at the
+// start of an entry point, before any input is packed (it overwrites the
+// workspace and may grow it to 16 candidates).
+// ECSMHD_NO_TUNE=1 keeps the default tiles and the wide plan;
+// ECSMHD_PLAN=wide|narrow fixes the plan and tunes the tiles only.
+template <int N>
+static int autotune(ecs_ctx* c, int B, hipStream_t st) {
+  if (getenv("ECSMHD_NO_TUNE")) return 0;
+  static std::mutex run_mu;  // one tuning pass per grid size at a time
+  std::lock_guard<std::mutex> rl(run_mu);
+  const char* force = getenv("ECSMHD_PLAN");
+  const int forced = force ? (strcmp(force, "narrow") == 0 ? 1 : 0) : -1;
+  int plan[4] = {-1, -1, -1, -1};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (g_tune.count(tune_key(N, K_ROWT, B))) return 0;
+    for (int kind = 0; kind < 4; ++kind) {
+      auto it = g_plan.find(plan_key(N, kind));
+      if (it != g_plan.end()) plan[kind] = it->second;
+    }
+  }
+  if (plan[0] < 0 || plan[1] < 0 || plan[2] < 0 || plan[3] < 0) {
+    constexpr int PB[2] = {1, 16};
+    TRY(ensure_cap(c, PB[1]));
+    float t[2][4][6];
+    for (int p = 0; p < 2; ++p)
+      for (int kind = 0; kind < 4; ++kind) TRY(time_variants<N>(c, PB[p], kind, 0, 6, t[p][kind], st));
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (int kind = 0; kind < 4; ++kind) {
+      int pl = plan[kind];
+      if (pl < 0) pl = forced;
+      if (pl < 0) {
+        float worst[2] = {0.f, 0.f};
+        for (int p = 0; p < 2; ++p) {
+          float bp[2];
+          for (int q = 0; q < 2; ++q) bp[q] = std::min(t[p][kind][3 * q], std::min(t[p][kind][3 * q + 1], t[p][kind][3 * q + 2]));
+          for (int q = 0; q < 2; ++q) worst[q] = std::max(worst[q], bp[q] / std::min(bp[0], bp[1]));
+        }
+        pl = worst[1] < worst[0] ? 1 : 0;
+      }
+      plan[kind] = pl;
+      g_plan[plan_key(N, kind)] = pl;
+      for (int p = 0; p < 2; ++p) {
+        int v = 3 * pl;
+        for (int q = 3 * pl; q < 3 * pl + 3; ++q)
+          if (t[p][kind][q] < t[p][kind][v]) v = q;
+        if (!g_tune.count(tune_key(N, kind, PB[p]))) g_tune[tune_key(N, kind, PB[p])] = v;
+      }
+    }
+    if (g_tune.count(tune_key(N, K_ROWT, B))) return 0;
+  }
+  int pick[4];
+  for (int kind = 0; kind < 4; ++kind) {
+    float t[3];
+    TRY(time_variants<N>(c, B, kind, 3 * plan[kind], 3 * plan[kind] + 3, t, st));
+    pick[kind] = 3 * plan[kind] + (t[1] < t[0] ? (t[2] < t[1] ? 2 : 1) : (t[2] < t[0] ? 2 : 0));
+  }
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (int kind = 0; kind < 4; ++kind) g_tune[tune_key(N, kind, B)] = pick[kind];
+  return 0;
+}
+
+// ------------------------------------------------------------------ state_vel
+template <int N>
+static int run_state_vel(ecs_ctx* c, const double* in, const double* din, double* out, double* dout, int B,
+                         int diss, hipStream_t st) {
+  TRY(ensure_cap(c, B));
+  TRY(autotune<N>(c, B, st));
+  Args a = base_args(c);
+  a.include_diss = diss;
+  TRY(pack<N>(in, c->x0, B, st));
+  if (din) {
+    TRY(pack<N>(din, c->dx0, B, st));
+    TRY((col<N, C_LOAD, true>(a, B, st)));
+    TRY((row<N, R_RHS, true>(a, B, st)));
+    TRY((col<N, C_RHS, true>(a, B, st)));
+    TRY(unpack<N>(c->dout, dout, B, 1, st));
+  } else {
+    TRY((col<N, C_LOAD, false>(a, B, st)));
+    TRY((row<N, R_RHS, false>(a, B, st)));
+    TRY((col<N, C_RHS, false>(a, B, st)));
+  }
+  TRY(unpack<N>(c->out, out, B, 1, st));
+  return 0;
+}
+
+// ------------------------------------------------------------------ Lawson RK4
+template <int N, bool TAN>
+static int rk4_step(const Args& a, int B, hipStream_t st, bool last) {
+  TRY((row<N, R_RHS, TAN>(a, B, st)));
+  TRY((col<N, C_S1F, TAN>(a, B, st)));
+  TRY((row<N, R_RHS, TAN>(a, B, st)));
+  TRY((col<N, C_S2, TAN>(a, B, st)));
+  TRY((row<N, R_RHS, TAN>(a, B, st)));
+  TRY((col<N, C_S3, TAN>(a, B, st)));
+  TRY((row<N, R_RHS, TAN>(a, B, st)));
+  Args a4 = a;
+  a4.skip_inv = last ? 1 : 0;
+  TRY((col<N, C_S4, TAN>(a4, B, st)));
+  return 0;
+}
+
+template <int N, bool TAN>
+static int run_rk4(ecs_ctx* c, double* f, double* df, int B, const double* dt, const double* ddt, int steps,
+                   int masked, hipStream_t st) {
+  TRY(ensure_cap(c, B));
+  TRY(autotune<N>(c, B, st));
+  Args a = base_args(c);
+  a.masked_e = masked;
+  TRY(upload_scalars(c, 0, dt, B, st));
+  if (TAN) TRY(upload_scalars(c, 1, ddt, B, st));
+  TRY(pack<N>(f, c->x0, B, st));
+  if (TAN) TRY(pack<N>(df, c->dx0, B, st));
+  if (steps > 0) {
+    TRY((col<N, C_LOAD, TAN>(a, B, st)));
+    // full chunks of K steps replay one captured graph; the tail runs eagerly
+    constexpr int K = 16;
+    int s = 0;
+    if (use_graphs() && steps - 1 >= K) {
+      const std::string key = graph_key(TAN ? "rk4t" : "rk4", a, B, K);
+      for (; s + K <= steps - 1; s += K) {
+        TRY(graph_run(c, key, st, [&](hipStream_t gs) -> int {
+          for (int j = 0; j < K; ++j) TRY((rk4_step<N, TAN>(a, B, gs, false)));
+          return 0;
+        }));
+      }
+    }
+    for (; s < steps; ++s) TRY((rk4_step<N, TAN>(a, B, st, s == steps - 1)));
+  }
+  // unmasked e keeps the modes outside the 2/3 region unchanged (L = 0 there)
+  TRY(unpack<N>(c->x0, f, B, masked ? 1 : 0, st));
+  if (TAN) TRY(unpack<N>(c->dx0, df, B, masked ? 1 : 0, st));
+  return 0;
+}
+
+// ------------------------------------------------------------------ Lawson RK4(3)
+template <int N>
+static int rk43_attempt(const Args& ai, int B, hipStream_t st, bool first) {
+  if (first) TRY((col<N, C_A0, false>(ai, B, st)));
+  else TRY((col<N, C_A, false>(ai, B, st)));
+  TRY((row<N, R_RHS, false>(ai, B, st)));
+  TRY((col<N, C_S2, false>(ai, B, st)));
+  TRY((row<N, R_RHS, false>(ai, B, st)));
+  TRY((col<N, C_S3, false>(ai, B, st)));
+  TRY((row<N, R_RHS, false>(ai, B, st)));
+  TRY((col<N, C_S4, false>(ai, B, st)));
+  TRY((row<N, R_RHS, false>(ai, B, st)));
+  TRY((col<N, C_S5, false>(ai, B, st)));
+  TRY((row<N, R_ERR, false>(ai, B, st)));
+  return 0;
+}
+
+template <int N>
+static int run_rk43(ecs_ctx* c, double* x, int B, const double* T, double h0, double atol, int maxs, int nck,
+                    ecs_rk43_info* info, double* hs, int8_t* accf, hipStream_t st) {
+  TRY(ensure_cap(c, B));
+  TRY(autotune<N>(c, B, st));
+  constexpr int K = 32;  // attempts per captured chunk (even: the counter parity repeats)
+  const int budget = maxs * nck;
+  const int hstride = budget + K + 2;
+  TRY(ensure_hist(c, (size_t)B * hstride));
+  HIPCHK(hipMemsetAsync(c->hist, 0, (size_t)B * hstride * sizeof(Hist), st));
+  TRY(upload_scalars(c, 2, T, B, st));
+  Args a = base_args(c);
+  a.hist = c->hist;
+  a.hstride = hstride;
+  a.atol = atol;
+  a.h0 = h0;
+  a.budget = budget;
+  TRY(pack<N>(x, c->x0, B, st));
+  // k1 = t v(x) (timestepping.py:255)
+  Args al = a;
+  al.adaptive = AD_FIXED;
+  TRY((col<N, C_LOAD, false>(al, B, st)));
+  TRY((row<N, R_RHS, false>(al, B, st)));
+  a.adaptive = AD_PRIMAL;
+  Args a0 = a;
+  a0.attempt = 0;
+  TRY(rk43_attempt<N>(a0, B, st, true));
+  int launched = 1;
+  const bool graphs = use_graphs();
+  if (graphs) TRY(set_counter(c, 1, st));
+  std::vector<Hist> hh(B);
+  while (launched < budget) {
+    if (graphs) {
+      Args ag = a;
+      ag.cnt = c->cnt;
+      const std::string key = graph_key("rk43", ag, B, K);
+      TRY(graph_run(c, key, st, [&](hipStream_t gs) -> int {
+        for (int j = 0; j < K; ++j) {
+          Args aj = ag;
+          aj.cslot = j & 1;
+          TRY(rk43_attempt<N>(aj, B, gs, false));
+        }
+        return 0;
+      }));
+    } else {
+      for (int j = 0; j < K; ++j) {
+        Args aj = a;
+        aj.attempt = launched + j;
+        TRY(rk43_attempt<N>(aj, B, st, false));
+      }
+    }
+    launched += K;
+    // poll: stop once every candidate's last attempt was a no-op
+    bool any = false;
+    for (int b = 0; b < B; ++b)
+      HIPCHK(hipMemcpyAsync(&hh[b], c->hist + (size_t)b * hstride + (launched - 1), sizeof(Hist),
+                            hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int b = 0; b < B; ++b) any = any || hh[b].active;
+    if (!any) break;
+  }
+  Args af = a;
+  af.attempt = launched;
+  TRY((col<N, C_FIN, false>(af, B, st)));
+  TRY(unpack<N>(c->x0, x, B, 0, st));
+  std::vector<Hist> all((size_t)B * hstride);
+  HIPCHK(hipMemcpyAsync(all.data(), c->hist, (size_t)B * hstride * sizeof(Hist), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int b = 0; b < B; ++b) {
+    const Hist* H = all.data() + (size_t)b * hstride;
+    const Hist& fin = H[launched];
+    if (info) {
+      info[b].s = fin.s;
+      info[b].completed = fabs(fin.s - 1.0) < 1e-10;
+      info[b].accepted = fin.nacc;
+      info[b].rejected = fin.nrej;
+      info[b].attempts = fin.nacc + fin.nrej;
+      info[b].fevals = 1 + 4 * (fin.nacc + fin.nrej);
+      info[b].pad = 0;
+    }
+    for (int i = 0; i < budget; ++i) {
+      const bool act = i < launched && H[i].active;
+      if (hs) hs[(size_t)b * budget + i] = act ? H[i].h : 0.0;
+      if (accf) accf[(size_t)b * budget + i] = (act && H[i + 1].prev_acc) ? 1 : 0;
+    }
+  }
+  return 0;
+}
+
+template <int N>
+static int run_rk43_jvp(ecs_ctx* c, double* x, double* dx, int B, const double* T, const double* dT,
+                        const double* hs, const int8_t* accf, const int* attempts, int budget, hipStream_t st) {
+  TRY(ensure_cap(c, B));
+  TRY(autotune<N>(c, B, st));
+  // build the accepted-step schedule with the rejections preceding each step
+  std::vector<double> sh((size_t)B * budget, 0.0), srh((size_t)B * budget, 0.0);
+  std::vector<int> srs((size_t)B * budget, 0), src((size_t)B * budget, 0), snacc(B, 0);
+  int maxacc = 0;
+  for (int b = 0; b < B; ++b) {
+    int m = 0, nr = 0, rstart = 0;
+    for (int i = 0; i < attempts[b]; ++i) {
+      const double h = hs[(size_t)b * budget + i];
+      if (accf[(size_t)b * budget + i]) {
+        sh[(size_t)b * budget + m] = h;
+        srs[(size_t)b * budget + m] = rstart;
+        src[(size_t)b * budget + m] = nr - rstart;
+        rstart = nr;
+        ++m;
+      } else {
+        srh[(size_t)b * budget + nr] = h;
+        ++nr;
+      }
+    }
+    snacc[b] = m;
+    maxacc = std::max(maxacc, m);
+  }
+  const size_t need = (size_t)B * budget;
+  std::unique_lock<std::recursive_mutex> slk(g_alloc_mu);
+  if (need > c->sched_cap) {
+    if (c->s_h) { HIPCHK(hipFree(c->s_h)); HIPCHK(hipFree(c->s_rh)); HIPCHK(hipFree(c->s_rs)); HIPCHK(hipFree(c->s_rc)); }
+    HIPCHK(hipMalloc((void**)&c->s_h, need * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&c->s_rh, need * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&c->s_rs, need * sizeof(int)));
+    HIPCHK(hipMalloc((void**)&c->s_rc, need * sizeof(int)));
+    c->sched_cap = need;
+  }
+  if (B > c->snacc_cap) {
+    if (c->s_nacc) HIPCHK(hipFree(c->s_nacc));
+    HIPCHK(hipMalloc((void**)&c->s_nacc, B * sizeof(int)));
+    c->snacc_cap = B;
+  }
+  HIPCHK(hipMemcpyAsync(c->s_h, sh.data(), need * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->s_rh, srh.data(), need * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->s_rs, srs.data(), need * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->s_rc, src.data(), need * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(c->s_nacc, snacc.data(), B * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // the host vectors above go out of scope
+  slk.unlock();
+  TRY(upload_scalars(c, 2, T, B, st));
+  TRY(upload_scalars(c, 3, dT, B, st));
+  Args a = base_args(c);
+  a.s_h = c->s_h; a.s_rh = c->s_rh; a.s_rs = c->s_rs; a.s_rc = c->s_rc; a.s_nacc = c->s_nacc;
+  a.sstride = budget;
+  a.rstride = budget;
+  TRY(pack<N>(x, c->x0, B, st));
+  TRY(pack<N>(dx, c->dx0, B, st));
+  Args al = a;
+  al.adaptive = AD_FIXED;
+  TRY((col<N, C_LOAD, true>(al, B, st)));
+  TRY((row<N, R_RHS, true>(al, B, st)));
+  a.adaptive = AD_TANGENT;
+  // one accepted step: [A] row S2 row S3 row S4 row S5; eagerly, S5 of step m
+  // is fused with the first kernel (A) of step m+1 (C_S5A)
+  auto step = [&](const Args& am, hipStream_t s2, bool first, bool with_a, bool fuse_next) -> int {
+    if (with_a) {
+      if (first) TRY((col<N, C_A0, true>(am, B, s2)));
+      else TRY((col<N, C_A, true>(am, B, s2)));
+    }
+    TRY((row<N, R_RHS, true>(am, B, s2)));
+    TRY((col<N, C_S2, true>(am, B, s2)));
+    TRY((row<N, R_RHS, true>(am, B, s2)));
+    TRY((col<N, C_S3, true>(am, B, s2)));
+    TRY((row<N, R_RHS, true>(am, B, s2)));
+    TRY((col<N, C_S4, true>(am, B, s2)));
+    TRY((row<N, R_RHS, true>(am, B, s2)));
+    if (fuse_next) TRY((col<N, C_S5A, true>(am, B, s2)));
+    else TRY((col<N, C_S5, true>(am, B, s2)));
+    return 0;
+  };
+  constexpr int K = 32;
+  if (use_graphs() && maxacc > 1) {
+    Args am0 = a;
+    am0.attempt = 0;
+    TRY(step(am0, st, true, true, false));
+    TRY(set_counter(c, 1, st));
+    Args ag = a;
+    ag.cnt = c->cnt;
+    const std::string key = graph_key("rk43t", ag, B, K);
+    for (int launched = 1; launched < maxacc; launched += K) {
+      TRY(graph_run(c, key, st, [&](hipStream_t gs) -> int {
+        for (int j = 0; j < K; ++j) {
+          Args aj = ag;
+          aj.cslot = j & 1;
+          TRY(step(aj, gs, false, true, false));
+        }
+        return 0;
+      }));
+    }
+  } else {
+    for (int m = 0; m < maxacc; ++m) {
+      Args am = a;
+      am.attempt = m;
+      TRY(step(am, st, m == 0, m == 0, m + 1 < maxacc));
+    }
+  }
+  TRY(unpack<N>(c->x0, x, B, 0, st));
+  TRY(unpack<N>(c->dx0, dx, B, 0, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+
+template <int N> static int rk4_p(ecs_ctx* c, double* f, int B, const double* dt, int steps, int masked, hipStream_t st) {
+  return run_rk4<N, false>(c, f, nullptr, B, dt, nullptr, steps, masked, st);
+}
+template <int N> static int rk4_t(ecs_ctx* c, double* f, double* df, int B, const double* dt, const double* ddt, int steps, int masked, hipStream_t st) {
+  return run_rk4<N, true>(c, f, df, B, dt, ddt, steps, masked, st);
+}
+
+
+// ------------------------------------------------------------------ adjoint
+template <int N, int MODE>
+static int acol(const Args& a, int B, hipStream_t st) {
+  constexpr int TY = Tiles<N>::TY;
+  dim3 grid((Geo<N>::KC + TY - 1) / TY, 2, B);
+  hipLaunchKernelGGL((adj_col_kernel<N, TY, NT, MODE>), grid, dim3(NT), 0, st, a);
+  LAUNCHCHK();
+  return 0;
+}
+template <int N>
+static int arow(const Args& a, int B, hipStream_t st) {
+  constexpr int TX = Tiles<N>::TX;
+  dim3 grid(N / TX, 1, B);
+  hipLaunchKernelGGL((adj_row_kernel<N, TX, NT>), grid, dim3(NT), 0, st, a);
+  LAUNCHCHK();
+  return 0;
+}
+
+template <int N>
+static int run_state_vel_vjp(ecs_ctx* c, const double* in, const double* cot, double* grad, int B, int diss,
+                             hipStream_t st) {
+  TRY(ensure_adj(c, B));
+  TRY(autotune<N>(c, B, st));
+  Args a = base_args(c);
+  a.include_diss = diss;
+  a.aout = c->AO;
+  a.ain = c->AI;
+  a.prim = c->I;
+  a.gin = c->acc;  // compact cotangent
+  TRY(pack<N>(in, c->x0, B, st));
+  TRY(pack<N>(cot, c->acc, B, st));
+  TRY((col<N, C_LOAD, false>(a, B, st)));
+  TRY((acol<N, B_G>(a, B, st)));
+  TRY(arow<N>(a, B, st));
+  TRY((acol<N, B_OUT>(a, B, st)));
+  TRY(unpack<N>(c->out, grad, B, 1, st));
+  return 0;
+}
+
+template <int N>
+static int adj_buffers(ecs_ctx* c, int B, int ckpt) {
+  TRY(ensure_adj(c, B));
+  using G = Geo<N>;
+  const size_t cs = (size_t)B * 2 * G::R * G::KC;
+  const size_t is = (size_t)B * 8 * N * G::KC;
+  const size_t need = (size_t)ckpt * 5 * cs;
+  if (need > c->vt_cap) {
+    if (c->xtape) HIPCHK(hipFree(c->xtape));
+    c->xtape = nullptr;
+    size_t cap0 = 0;
+    TRY(grow(&c->xtape, &cap0, need));
+    c->vt_cap = need;
+  }
+  c->vtape = c->xtape + (size_t)ckpt * cs;
+  TRY(grow(&c->itape, &c->it_cap, (size_t)ckpt * 4 * is));
+  return 0;
+}
+
+// Forward pass of Lawson RK4 storing the compact state at the start of every
+// segment of ckpt steps into ck [nseg][B][2][R][KC]; f is advanced in place.
+template <int N>
+static int run_rk4_ckpt(ecs_ctx* c, double* f, int B, const double* dt, int steps, int masked, int ckpt,
+                        double2* ck, hipStream_t st) {
+  TRY(ensure_cap(c, B));
+  TRY(autotune<N>(c, B, st));
+  using G = Geo<N>;
+  const size_t cs = (size_t)B * 2 * G::R * G::KC;
+  TRY(upload_scalars(c, 0, dt, B, st));
+  Args a = base_args(c);
+  a.masked_e = masked;
+  TRY(pack<N>(f, c->x0, B, st));
+  HIPCHK(hipMemcpyAsync(ck, c->x0, cs * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  if (steps > 0) TRY((col<N, C_LOAD, false>(a, B, st)));
+  for (int s = 0; s < steps; ++s) {
+    TRY((row<N, R_RHS, false>(a, B, st)));
+    TRY((col<N, C_S1F, false>(a, B, st)));
+    TRY((row<N, R_RHS, false>(a, B, st)));
+    TRY((col<N, C_S2, false>(a, B, st)));
+    TRY((row<N, R_RHS, false>(a, B, st)));
+    TRY((col<N, C_S3, false>(a, B, st)));
+    TRY((row<N, R_RHS, false>(a, B, st)));
+    Args a4 = a;
+    a4.skip_inv = (s == steps - 1);
+    TRY((col<N, C_S4, false>(a4, B, st)));
+    if ((s + 1) % ckpt == 0 && s + 1 < steps)
+      HIPCHK(hipMemcpyAsync(ck + (size_t)((s + 1) / ckpt) * cs, c->x0, cs * sizeof(double2),
+                            hipMemcpyDeviceToDevice, st));
+  }
+  TRY(unpack<N>(c->x0, f, B, masked ? 1 : 0, st));
+  return 0;
+}
+
+// Reverse pass from the checkpoints: fbar (cotangent of the final spectra) is
+// replaced by the cotangent of the initial spectra; hbar[b] = d/dh.
+template <int N>
+static int run_rk4_rev(ecs_ctx* c, const double2* ck, double* fbar, double* hbar, int B, const double* dt, int steps,
+                       int masked, int ckpt, hipStream_t st) {
+  TRY(adj_buffers<N>(c, B, ckpt));
+  TRY(autotune<N>(c, B, st));
+  using G = Geo<N>;
+  const size_t cs = (size_t)B * 2 * G::R * G::KC;
+  const size_t is = (size_t)B * 8 * N * G::KC;
+  const int nseg = (steps + ckpt - 1) / ckpt;
+  TRY(upload_scalars(c, 0, dt, B, st));
+  Args a = base_args(c);
+  a.masked_e = masked;
+  a.aout = c->AO;
+  a.ain = c->AI;
+  a.xb = c->xb; a.k1b = c->k1b; a.k2b = c->k2b; a.k3b = c->k3b;
+  a.hb = c->hb;
+  a.cstride = cs;
+  TRY(pack<N>(fbar, c->xb, B, st));
+  HIPCHK(hipMemsetAsync(c->hb, 0, cs * sizeof(double), st));
+  auto slot = [&](int t, int s) { return c->itape + ((size_t)t * 4 + s) * is; };
+  auto xt = [&](int t) { return c->xtape + (size_t)t * cs; };
+  auto vt = [&](int t) { return c->vtape + (size_t)t * 4 * cs; };
+  for (int j = nseg - 1; j >= 0; --j) {
+    const int t0 = j * ckpt;
+    const int L = std::min(ckpt, steps - t0);
+    HIPCHK(hipMemcpyAsync(c->x0, ck + (size_t)j * cs, cs * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    Args f = a;
+    f.inter = slot(0, 0);
+    TRY((col<N, C_LOAD, false>(f, B, st)));
+    for (int t = 0; t < L; ++t) {
+      f.vtape = nullptr; f.xtape = nullptr;
+      f.inter = slot(t, 0);
+      TRY((row<N, R_RHS, false>(f, B, st)));
+      f.inter = slot(t, 1); f.vtape = vt(t); f.xtape = xt(t);
+      TRY((col<N, C_S1F, false>(f, B, st)));
+      f.xtape = nullptr;
+      TRY((row<N, R_RHS, false>(f, B, st)));
+      f.inter = slot(t, 2); f.vtape = vt(t) + cs;
+      TRY((col<N, C_S2, false>(f, B, st)));
+      TRY((row<N, R_RHS, false>(f, B, st)));
+      f.inter = slot(t, 3); f.vtape = vt(t) + 2 * cs;
+      TRY((col<N, C_S3, false>(f, B, st)));
+      TRY((row<N, R_RHS, false>(f, B, st)));
+      Args f4 = f;
+      f4.vtape = vt(t) + 3 * cs;
+      f4.inter = (t + 1 < L) ? slot(t + 1, 0) : slot(t, 3);
+      f4.skip_inv = (t + 1 == L);
+      TRY((col<N, C_S4, false>(f4, B, st)));
+    }
+    Args b = a;
+    b.tx = xt(L - 1);
+    b.tv = vt(L - 1);
+    TRY((acol<N, B_ENTRY>(b, B, st)));
+    for (int t = L - 1; t >= 0; --t) {
+      b.tx = xt(t);
+      b.tv = vt(t);
+      b.prim = slot(t, 3);
+      TRY(arow<N>(b, B, st));
+      TRY((acol<N, B_S4>(b, B, st)));
+      b.prim = slot(t, 2);
+      TRY(arow<N>(b, B, st));
+      TRY((acol<N, B_S3>(b, B, st)));
+      b.prim = slot(t, 1);
+      TRY(arow<N>(b, B, st));
+      TRY((acol<N, B_S2>(b, B, st)));
+      b.prim = slot(t, 0);
+      TRY(arow<N>(b, B, st));
+      if (t > 0) {
+        b.txp = xt(t - 1);
+        b.tvp = vt(t - 1);
+        TRY((acol<N, B_S1E>(b, B, st)));
+      } else {
+        TRY((acol<N, B_S1>(b, B, st)));
+      }
+    }
+  }
+  TRY(unpack<N>(c->xb, fbar, B, masked ? 1 : 0, st));
+  hipLaunchKernelGGL((hb_sum_kernel<N>), dim3(B), dim3(256), 0, st, c->hb, c->red);
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(c->hred, c->red, B * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int b2 = 0; b2 < B; ++b2) hbar[b2] = c->hred[b2];
+  return 0;
+}
+
+template <int N>
+static int run_rk4_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar, int B, const double* dt, int steps,
+                       int masked, int ckpt, hipStream_t st) {
+  TRY(ensure_cap(c, B));
+  TRY(autotune<N>(c, B, st));
+  TRY(grow(&c->ckpt, &c->ckpt_cap, ckpt_elems(N, B, steps, ckpt)));
+  // the forward pass advances a private copy of f0 (held in c->out, full layout not needed)
+  double* fwork = nullptr;
+  const size_t full = (size_t)B * 2 * N * Geo<N>::NH * 2;
+  HIPCHK(hipMallocAsync((void**)&fwork, full * sizeof(double), st));
+  HIPCHK(hipMemcpyAsync(fwork, f0, full * sizeof(double), hipMemcpyDeviceToDevice, st));
+  int rc = run_rk4_ckpt<N>(c, fwork, B, dt, steps, masked, ckpt, c->ckpt, st);
+  if (rc == 0) rc = run_rk4_rev<N>(c, c->ckpt, fbar, hbar, B, dt, steps, masked, ckpt, st);
+  HIPCHK(hipFreeAsync(fwork, st));
+  return rc;
+}
+
+// ------------------------------------------------------------------ timing hook
+// Average duration (ms) of one launch of a fused RHS kernel, timed with HIP
+// events on `stream` over `reps` back-to-back launches of the same kernel on
+// batch spectra already resident in the context's workspace.
+// which: 0 = column kernel (stage 2 glue), 1 = row kernel, 2 = tangent column
+// kernel, 3 = tangent row kernel.
+template <int N>
+static int run_time_kernel(ecs_ctx* c, int B, int which, int reps, double* ms, hipStream_t st) {
+  TRY(ensure_cap(c, B));
+  TRY(autotune<N>(c, B, st));
+  std::vector<double> dt(B, 1e-3);
+  TRY(upload_scalars(c, 0, dt.data(), B, st));
+  TRY(upload_scalars(c, 1, dt.data(), B, st));
+  Args a = base_args(c);
+  // defined inputs: zero state, one load + row pass to fill I and O
+  HIPCHK(hipMemsetAsync(c->x0, 0, c->compact_elems() * B * sizeof(double2), st));
+  HIPCHK(hipMemsetAsync(c->dx0, 0, c->compact_elems() * B * sizeof(double2), st));
+  HIPCHK(hipMemsetAsync(c->acc, 0, c->compact_elems() * B * sizeof(double2), st));
+  HIPCHK(hipMemsetAsync(c->dacc, 0, c->compact_elems() * B * sizeof(double2), st));
+  TRY((col<N, C_LOAD, true>(a, B, st)));
+  TRY((row<N, R_RHS, true>(a, B, st)));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  auto launch = [&]() -> int {
+    switch (which) {
+      case 0: return col<N, C_S2, false>(a, B, st);
+      case 1: return row<N, R_RHS, false>(a, B, st);
+      case 2: return col<N, C_S2, true>(a, B, st);
+      default: return row<N, R_RHS, true>(a, B, st);
+    }
+  };
+  for (int i = 0; i < 3; ++i) TRY(launch());
+  HIPCHK(hipEventRecord(e0, st));
+  for (int i = 0; i < reps; ++i) TRY(launch());
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  *ms = (double)t / reps;
+  return 0;
+}
+
+
+#define ECS_GRID(NV)                                                                                     \
+  const GridOps* grid_ops_##NV() {                                                                       \
+    static const GridOps ops = {&run_state_vel<NV>, &rk4_p<NV>,          &rk4_t<NV>,    &run_rk43<NV>,  \
+                                &run_rk43_jvp<NV>,  &run_state_vel_vjp<NV>, &run_rk4_ckpt<NV>,            \
+                                &run_rk4_rev<NV>,   &run_rk4_vjp<NV>,    &run_time_kernel<NV>};           \
+    return &ops;                                                                                         \
+  }

@@ -1,8 +1,6 @@
 // Batched complex FFT over sequences staged in LDS (FP64, gfx950).
 //
-// Stockham auto-sort with mixed radices chosen per length so that every
-// supported N takes at most two passes:
-//   16 = 16,  32 = 8 x 4,  64 = 8 x 8,  128 = 16 x 8,  256 = 16 x 16.
+// Stockham auto-sort with mixed radices 16/8/4/2 (plans listed at Plan).
 // All threads of the workgroup cooperate on NSEQ sequences of length N;
 // sequence s lives at buf + s * SSTRIDE (double2 units).  Each pass reads its
 // butterflies' inputs into registers, synchronises, then writes, so it runs
@@ -136,13 +134,23 @@ template <bool INV> struct Dft<16, INV> {
   }
 };
 
-// radix plan: first and second pass radices (R2 = 1: single pass)
-template <int N> struct Radix;
-template <> struct Radix<16> { static constexpr int R1 = 16, R2 = 1; };
-template <> struct Radix<32> { static constexpr int R1 = 8, R2 = 4; };
-template <> struct Radix<64> { static constexpr int R1 = 8, R2 = 8; };
-template <> struct Radix<128> { static constexpr int R1 = 16, R2 = 8; };
-template <> struct Radix<256> { static constexpr int R1 = 16, R2 = 16; };
+// Radix plans, selected per kernel by the template argument P:
+//   P = 16 (wide):   16 = 16, 32 = 8x4, 64 = 8x8, 128 = 16x8, 256 = 16x16
+//   P = 8 (narrow):  16 = 4x4, 32 = 4x4x2, 64 = 4x4x4, 128 = 8x4x4, 256 = 8x8x4
+// Wide plans take fewer passes (fewer LDS round trips and barriers); narrow
+// ones keep more threads busy with shorter butterflies when a block holds few
+// sequences.  The autotuner picks one per grid size and kernel family.
+template <int N, int P> struct Plan;
+template <> struct Plan<16, 16> { static constexpr int R[3] = {16, 1, 1}; };
+template <> struct Plan<32, 16> { static constexpr int R[3] = {8, 4, 1}; };
+template <> struct Plan<64, 16> { static constexpr int R[3] = {8, 8, 1}; };
+template <> struct Plan<128, 16> { static constexpr int R[3] = {16, 8, 1}; };
+template <> struct Plan<256, 16> { static constexpr int R[3] = {16, 16, 1}; };
+template <> struct Plan<16, 8> { static constexpr int R[3] = {4, 4, 1}; };
+template <> struct Plan<32, 8> { static constexpr int R[3] = {4, 4, 2}; };
+template <> struct Plan<64, 8> { static constexpr int R[3] = {4, 4, 4}; };
+template <> struct Plan<128, 8> { static constexpr int R[3] = {8, 4, 4}; };
+template <> struct Plan<256, 8> { static constexpr int R[3] = {8, 8, 4}; };
 
 template <int N, int R, int NS, int NSEQ, int SSTRIDE, int NT, bool INV>
 __device__ __forceinline__ void stockham_pass(double2* __restrict__ buf, const double2* __restrict__ twl) {
@@ -190,12 +198,13 @@ __device__ __forceinline__ void stockham_pass(double2* __restrict__ buf, const d
 }
 
 // twl must hold the twiddle table in LDS (load_twiddles + a barrier).
-template <int N, int NSEQ, int SSTRIDE, int NT, bool INV>
+template <int N, int NSEQ, int SSTRIDE, int NT, bool INV, int P = 16>
 __device__ __forceinline__ void fft_lds(double2* __restrict__ buf, const double2* __restrict__ twl) {
-  constexpr int R1 = Radix<N>::R1, R2 = Radix<N>::R2;
-  static_assert(R1 * (R2 == 1 ? 1 : R2) == N, "radix plan must factor N");
-  stockham_pass<N, R1, 1, NSEQ, SSTRIDE, NT, INV>(buf, twl);
-  if constexpr (R2 > 1) stockham_pass<N, R2, R1, NSEQ, SSTRIDE, NT, INV>(buf, twl);
+  constexpr int R0 = Plan<N, P>::R[0], R1 = Plan<N, P>::R[1], R2 = Plan<N, P>::R[2];
+  static_assert(R0 * R1 * R2 == N, "radix plan must factor N");
+  stockham_pass<N, R0, 1, NSEQ, SSTRIDE, NT, INV>(buf, twl);
+  if constexpr (R1 > 1) stockham_pass<N, R1, R0, NSEQ, SSTRIDE, NT, INV>(buf, twl);
+  if constexpr (R2 > 1) stockham_pass<N, R2, R0 * R1, NSEQ, SSTRIDE, NT, INV>(buf, twl);
 }
 
 template <int N, int NT>

@@ -1,0 +1,207 @@
+// Host-side state shared by the translation units of the HIP library:
+// error reporting, the context (device workspace) and its allocation helpers,
+// the autotuner's tables, hipGraph replay, and the per-grid-size dispatch
+// table.  The grid-size-templated orchestration lives in engine.h, which is
+// compiled once per supported N (grid_n*.hip) so the kernels build in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <algorithm>
+#include <atomic>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+#include "kernels.h"
+#include "../../include/ecsmhd.h"
+
+using namespace ecs;
+
+// Records msg as the calling thread's last error (ecs_last_error) and returns code.
+int fail(int code, const std::string& msg);
+#define HIPCHK(x)                                                                           \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess) return fail(2, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define LAUNCHCHK() HIPCHK(hipGetLastError())
+#define TRY(x)              \
+  do {                      \
+    int r_ = (x);           \
+    if (r_) return r_;      \
+  } while (0)
+
+// Marks a context busy for the duration of one entry point; a second,
+// concurrent call on the same context fails with code 4 instead of racing on
+// (or reallocating) a workspace that kernels of the first call still use.
+struct CtxGuard {
+  std::atomic<int>* flag;
+  bool ok;
+  explicit CtxGuard(std::atomic<int>* f) : flag(f), ok(f->exchange(1) == 0) {}
+  ~CtxGuard() {
+    if (ok) flag->store(0);
+  }
+};
+#define GUARD(c)                                                                              \
+  CtxGuard guard_(&(c)->busy);                                                                \
+  if (!guard_.ok) return fail(4, "context is in use by another call (use one context per thread)")
+
+constexpr int NT = 256;
+
+// Fixed tiles of the adjoint kernels: columns per column-kernel block (TY)
+// and rows per row-kernel block (TX).
+template <int N> struct Tiles;
+template <> struct Tiles<16> { static constexpr int TY = 2, TX = 2; };
+template <> struct Tiles<32> { static constexpr int TY = 4, TX = 2; };
+template <> struct Tiles<64> { static constexpr int TY = 4, TX = 2; };
+template <> struct Tiles<128> { static constexpr int TY = 4, TX = 2; };
+template <> struct Tiles<256> { static constexpr int TY = 2, TX = 1; };
+
+struct ecs_ctx {
+  std::atomic<int> busy{0};    // one call at a time: the workspace is shared by all entry points
+  int n = 0, dev = 0;
+  double nu = 0, eta = 0, b0x = 0, b0y = 0;
+  double2* tw = nullptr;
+  double2* fhat = nullptr;
+  int cap = 0;  // batch capacity of the workspace
+  double2 *I = nullptr, *O = nullptr;
+  double2 *x0 = nullptr, *acc = nullptr, *v1 = nullptr, *v4 = nullptr, *v5 = nullptr, *xf = nullptr, *out = nullptr;
+  double2 *dx0 = nullptr, *dacc = nullptr, *dv1 = nullptr, *dv4 = nullptr, *dxf = nullptr, *dout = nullptr;
+  double* scal = nullptr;      // device [4][cap]: dt, ddt, T, dT
+  double* hscal = nullptr;     // pinned host staging [4][cap]
+  hipEvent_t ev_scal[4] = {};  // completion of the last copy out of each staging slot
+  Hist* hist = nullptr;
+  size_t hist_cap = 0;
+  // adaptive tangent schedule
+  double* s_h = nullptr;
+  int* s_rs = nullptr;
+  int* s_rc = nullptr;
+  double* s_rh = nullptr;
+  int* s_nacc = nullptr;
+  size_t sched_cap = 0;
+  int snacc_cap = 0;
+  // adjoint workspace
+  double2* ckpt = nullptr;     // checkpoints [nck][B][2][R][KC]
+  size_t ckpt_cap = 0;
+  double2* xtape = nullptr;    // [ckpt][B][2][R][KC]
+  double2* vtape = nullptr;    // [ckpt][4][B][2][R][KC]
+  size_t vt_cap = 0;           // elements of xtape + vtape
+  double2* itape = nullptr;    // primal column outputs [ckpt][4][B][8][N][KC]
+  size_t it_cap = 0;
+  double2 *AO = nullptr, *AI = nullptr, *xb = nullptr, *k1b = nullptr, *k2b = nullptr, *k3b = nullptr;
+  double* hb = nullptr;
+  int adj_cap = 0;
+  // hipGraph capture/replay of fixed chunks of stages
+  hipStream_t gs = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  int* cnt = nullptr;          // device step counter [2]
+  int* hcnt = nullptr;         // pinned staging
+  std::map<std::string, hipGraphExec_t> graphs;
+  double* red = nullptr;       // reductions [cap][64]
+  double* hred = nullptr;      // pinned [cap]
+  size_t compact_elems() const {
+    int kmax = (n - 1) / 3;
+    return (size_t)2 * (2 * kmax + 1) * (kmax + 1);
+  }
+};
+
+// Allocation, frees and synchronous copies must not run while another thread
+// captures a graph (they would join the capture's dependencies), so both take
+// this lock.
+extern std::recursive_mutex g_alloc_mu;
+
+void drop_graphs(ecs_ctx* c);
+// Grows the batch workspace (state arrays, column/row intermediates, scalars) to hold B candidates.
+int ensure_cap(ecs_ctx* c, int B);
+int ensure_hist(ecs_ctx* c, size_t need);
+int ensure_adj(ecs_ctx* c, int B);
+int grow(double2** p, size_t* cap, size_t need);
+size_t ckpt_elems(int n, int B, int steps, int ckpt);
+Args base_args(ecs_ctx* c);
+// Copies B per-candidate scalars into slot (0 dt, 1 ddt, 2 T, 3 dT) of the device scalar table.
+int upload_scalars(ecs_ctx* c, int slot, const double* vals, int B, hipStream_t st);
+
+// ------------------------------------------------------------------ autotuner tables
+// Tile variants: column kernels take TY in {1, 2, 4} columns per block, row
+// kernels TX in {1, 2, 4} rows per block; both come in a wide and a narrow
+// FFT radix plan.  Variant v: tile index v % 3 (TY or TX = 1 << (v % 3)),
+// plan v / 3 (0: wide, radix 16/8; 1: narrow, radix 8/4/2).  The autotuner
+// picks the plan once per (N, kernel family), at the first batch it sees, so
+// every batch size computes with the same arithmetic, and the tile per
+// (N, kernel family, batch bucket).
+enum TuneKind : int { K_COL = 0, K_COLT = 1, K_ROW = 2, K_ROWT = 3 };
+extern std::mutex g_tune_mu;
+extern std::map<long long, int> g_tune;  // (n, kind, bucket) -> variant
+extern std::map<long long, int> g_plan;  // (n, kind) -> plan index
+long long tune_key(int n, int kind, int B);
+long long plan_key(int n, int kind);
+int tuned(int n, int kind, int B, int dflt);
+
+// ------------------------------------------------------------------ hipGraph chunks
+// Captures the launches `record` makes on the context's own stream once per
+// key (the bytes of the kernels' arguments plus a tag), then replays the
+// instantiated graph; the caller's stream is joined before and after.
+// Opt-in (ECSMHD_GRAPH=1): on this ROCm stack, eager launches of the same
+// chunk measured faster, so eager is the default.
+bool use_graphs();
+std::string graph_key(const char* tag, const Args& a, int B, int K);
+int set_counter(ecs_ctx* c, int v, hipStream_t st);
+
+template <class F>
+inline int graph_run(ecs_ctx* c, const std::string& key, hipStream_t st, F&& record) {
+  if (!c->gs) {
+    HIPCHK(hipStreamCreateWithFlags(&c->gs, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming));
+  }
+  HIPCHK(hipEventRecord(c->ev_in, st));
+  HIPCHK(hipStreamWaitEvent(c->gs, c->ev_in, 0));
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
+    HIPCHK(hipStreamBeginCapture(c->gs, hipStreamCaptureModeThreadLocal));
+    const int rc = record(c->gs);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->gs, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(2, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e2 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) return fail(2, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+    it = c->graphs.emplace(key, ex).first;
+  }
+  HIPCHK(hipGraphLaunch(it->second, c->gs));
+  HIPCHK(hipEventRecord(c->ev_out, c->gs));
+  HIPCHK(hipStreamWaitEvent(st, c->ev_out, 0));
+  return 0;
+}
+
+// ------------------------------------------------------------------ per-N dispatch
+// Entry points of one grid size; grid_ops(n) returns null for unsupported n.
+struct GridOps {
+  int (*state_vel)(ecs_ctx*, const double*, const double*, double*, double*, int, int, hipStream_t);
+  int (*rk4)(ecs_ctx*, double*, int, const double*, int, int, hipStream_t);
+  int (*rk4_jvp)(ecs_ctx*, double*, double*, int, const double*, const double*, int, int, hipStream_t);
+  int (*rk43)(ecs_ctx*, double*, int, const double*, double, double, int, int, ecs_rk43_info*, double*, int8_t*,
+              hipStream_t);
+  int (*rk43_jvp)(ecs_ctx*, double*, double*, int, const double*, const double*, const double*, const int8_t*,
+                  const int*, int, hipStream_t);
+  int (*state_vel_vjp)(ecs_ctx*, const double*, const double*, double*, int, int, hipStream_t);
+  int (*rk4_ckpt)(ecs_ctx*, double*, int, const double*, int, int, int, double2*, hipStream_t);
+  int (*rk4_rev)(ecs_ctx*, const double2*, double*, double*, int, const double*, int, int, int, hipStream_t);
+  int (*rk4_vjp)(ecs_ctx*, const double*, double*, double*, int, const double*, int, int, int, hipStream_t);
+  int (*time_kernel)(ecs_ctx*, int, int, int, double*, hipStream_t);
+};
+const GridOps* grid_ops(int n);
+const GridOps* grid_ops_16();
+const GridOps* grid_ops_32();
+const GridOps* grid_ops_64();
+const GridOps* grid_ops_128();
+const GridOps* grid_ops_256();

@@ -323,7 +323,7 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, const Pre& p,
   }
 }
 
-template <int N, int TY, int NT, int MODE, bool TAN>
+template <int N, int TY, int NT, int MODE, bool TAN, int P = 16>
 __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   using G = Geo<N>;
   constexpr int KC = G::KC, R = G::R, KMAX = G::KMAX;
@@ -369,7 +369,7 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
       sm[(f * TY + cy) * SS + pidx(x)] = v;
     }
     __syncthreads();
-    fft_lds<N, NF * TY, SS, NT, false>(sm, twl);
+    fft_lds<N, NF * TY, SS, NT, false, P>(sm, twl);
   }
 
   const bool inv = HASINV && (MODE == C_S5A ? bk.next : bk.stage) && !a.skip_inv;
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
     sm[s * SS + pidx(KMAX + 1 + g)] = make_double2(0.0, 0.0);
   }
   __syncthreads();
-  fft_lds<N, NSEQ, SS, NT, true>(sm, twl);
+  fft_lds<N, NSEQ, SS, NT, true, P>(sm, twl);
   for (int idx = tid; idx < NSEQ * N; idx += NT) {
     const int cy = idx % TY;
     const int t = idx / TY;
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   }
 }
 
-template <int N, int TX, int NT, int RMODE, bool TAN>
+template <int N, int TX, int NT, int RMODE, bool TAN, int P = 16>
 __global__ __launch_bounds__(NT) void row_kernel(Args a) {
   using G = Geo<N>;
   constexpr int KC = G::KC;
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
     }
   }
   __syncthreads();
-  fft_lds<N, TX * NP, SS, NT, true>(sm, twl);
+  fft_lds<N, TX * NP, SS, NT, true, P>(sm, twl);
 
   if (RMODE == R_ERR) {
     unsigned long long mx = 0ull;
@@ -526,7 +526,7 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
     }
   }
   __syncthreads();
-  fft_lds<N, TX * NF, SS, NT, false>(sm, twl);
+  fft_lds<N, TX * NF, SS, NT, false, P>(sm, twl);
   double2* O = a.rowout + (size_t)b * (2 * NF) * N * KC;
   for (int idx = tid; idx < TX * NF * KC; idx += NT) {
     const int k = idx % KC;

@@ -134,6 +134,12 @@ int ecs_time_kernel(ecs_ctx* ctx, int batch, int which, int reps, double* ms, vo
  * out[3] the same for the tangent row kernel; -1 where not tuned yet. */
 int ecs_tuned_tiles(int n, int batch, int* out);
 
+/* FFT radix plan the autotuner fixed for grid size n (no reference
+ * counterpart), per kernel family in the order of ecs_tuned_tiles: 16 for
+ * the wide plan (radix 16/8: 128 = 16x8, 256 = 16x16), 8 for the narrow one
+ * (radix 8/4/2: 128 = 8x4x4, 256 = 8x8x4); -1 where not tuned yet. */
+int ecs_tuned_plans(int n, int* out);
+
 #ifdef __cplusplus
 }
 #endif
