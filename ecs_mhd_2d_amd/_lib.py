@@ -49,6 +49,13 @@ _SIGS = {
     "ecs_lawson_rk43_jvp": (c_int, [c_void_p, c_void_p, c_void_p, c_int, P_double, P_double, P_double, P_int8,
                                     P_int, c_int, c_void_p]),
     "ecs_time_kernel": (c_int, [c_void_p, c_int, c_int, c_int, P_double, c_void_p]),
+    "ecs_rk4": (c_int, [c_void_p, c_void_p, c_int, P_double, c_int, c_void_p]),
+    "ecs_rk4_jvp": (c_int, [c_void_p, c_void_p, c_void_p, c_int, P_double, P_double, c_int, c_void_p]),
+    "ecs_lawson_rk6": (c_int, [c_void_p, c_void_p, c_int, P_double, c_int, c_int, c_void_p]),
+    "ecs_lawson_rk6_jvp": (c_int, [c_void_p, c_void_p, c_void_p, c_int, P_double, P_double, c_int, c_int,
+                                   c_void_p]),
+    "ecs_tdrk4": (c_int, [c_void_p, c_void_p, c_int, P_double, c_int, c_void_p]),
+    "ecs_tdrk4_jvp": (c_int, [c_void_p, c_void_p, c_void_p, c_int, P_double, P_double, c_int, c_void_p]),
     "ecs_tuned_tiles": (c_int, [c_int, c_int, P_int]),
     "ecs_tuned_plans": (c_int, [c_int, P_int]),
     "ecs_mgs": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),

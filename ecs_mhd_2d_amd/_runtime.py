@@ -123,6 +123,42 @@ class Context:
                                                int(ckpt), _stream()), "ecs_lawson_rk4_vjp")
         return g, list(hbar)
 
+    # ------------------------------------------------------------ RK4, Lawson RK6, TDRK4
+    def _fixed(self, name, f, dt, steps, *extra):
+        f = _as_spec(f, self.n).clone()
+        B = _batch(f)
+        _lib.check(getattr(self.lib, name)(self.handle, f.data_ptr(), B, _lib.dptr(_per_batch(dt, B)), int(steps),
+                                           *extra, _stream()), name)
+        return f
+
+    def _fixed_jvp(self, name, f, df, dt, ddt, steps, *extra):
+        f, df = _as_spec(f, self.n).clone(), _as_spec(df, self.n).clone()
+        B = _batch(f)
+        _lib.check(getattr(self.lib, name)(self.handle, f.data_ptr(), df.data_ptr(), B, _lib.dptr(_per_batch(dt, B)),
+                                           _lib.dptr(_per_batch(ddt, B)), int(steps), *extra, _stream()), name)
+        return f, df
+
+    def rk4(self, f, dt, steps: int) -> torch.Tensor:
+        """Classical RK4 on the dissipative RHS, step dt (per candidate)."""
+        return self._fixed("ecs_rk4", f, dt, steps)
+
+    def rk4_jvp(self, f, df, dt, ddt, steps: int):
+        return self._fixed_jvp("ecs_rk4_jvp", f, df, dt, ddt, steps)
+
+    def lawson_rk6(self, f, dt, steps: int, masked: bool = True) -> torch.Tensor:
+        """Lawson RK6, e = exp(dt/6 L) [* mask]."""
+        return self._fixed("ecs_lawson_rk6", f, dt, steps, int(bool(masked)))
+
+    def lawson_rk6_jvp(self, f, df, dt, ddt, steps: int, masked: bool = True):
+        return self._fixed_jvp("ecs_lawson_rk6_jvp", f, df, dt, ddt, steps, int(bool(masked)))
+
+    def tdrk4(self, f, dt, steps: int) -> torch.Tensor:
+        """Two-derivative RK4 on the dissipative RHS."""
+        return self._fixed("ecs_tdrk4", f, dt, steps)
+
+    def tdrk4_jvp(self, f, df, dt, ddt, steps: int):
+        return self._fixed_jvp("ecs_tdrk4_jvp", f, df, dt, ddt, steps)
+
     # ------------------------------------------------------------ Lawson RK4(3)
     def lawson_rk43(self, x, T, h0: float, atol: float, max_steps_per_checkpoint: int, checkpoints: int):
         """Returns (x_T, infos) with one info dict per candidate (keys of timestepping.py:327-333

@@ -1,0 +1,84 @@
+// Stage combinations of the generic explicit steppers (classical RK4,
+// Lawson RK6, two-derivative RK4) over compact spectra [B][2][R][KC].
+//
+// out = sum_t coef_t * h^hp_t * E^ep_t * in_t,   E = exp(h L / ediv)
+// with L = -(kx^2 + ky^2) [nu; eta] and h the candidate's step.  A Lawson
+// scheme that multiplies its state and stage increments by e = exp(h/m L)
+// after some stages (timestepping.py:83-125) puts the accumulated power of e
+// on each term here: stage s reads x0 E^{p_s} and k_j E^{p_s - p_j}.  With
+// `masked`, E^p for p > 0 carries the 2/3 mask, whose only zero inside the
+// compact layout is the mean mode (mhd_jax.py:65-70).
+// The tangent adds d/dh of every weight times dh:
+//   dout = sum_t w_t din_t + dh * sum_t (hp_t / h + ep_t L / ediv) w_t in_t.
+#pragma once
+#include "kernels.h"
+
+namespace ecs {
+
+constexpr int COMB_MAX = 10;
+
+struct Comb {
+  const double2* in[COMB_MAX];   // null: the term has no primal part (tangent-only)
+  const double2* din[COMB_MAX];  // tangent inputs (null: zero)
+  double coef[COMB_MAX];
+  int hp[COMB_MAX];              // power of h (0, 1 or 2)
+  int ep[COMB_MAX];              // power of E (0..6)
+  int nterm;
+  double ediv;
+  int masked;
+  const double* h;               // [B]
+  const double* dh;              // [B] (tangent)
+  double nu[2];
+  double2* out;
+  double2* dout;
+  size_t elems;                  // compact elements per candidate
+};
+
+template <int N, bool TAN>
+__global__ __launch_bounds__(256) void comb_kernel(Comb cb, int B) {
+  using G = Geo<N>;
+  constexpr int KC = G::KC, R = G::R;
+  const size_t tot = (size_t)B * cb.elems;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+    const int b = (int)(i / cb.elems);
+    const size_t e = i - (size_t)b * cb.elems;
+    const int ky = (int)(e % KC);
+    const int r = (int)((e / KC) % R);
+    const int c = (int)(e / ((size_t)KC * R));
+    const int kx = G::row_kx(r);
+    const bool dc = (kx == 0 && ky == 0);
+    const double L = -(double)(kx * kx + ky * ky) * cb.nu[c];
+    const double h = cb.h[b];
+    const double dh = TAN ? cb.dh[b] : 0.0;
+    const double E1 = exp(h / cb.ediv * L);
+    double2 acc = make_double2(0.0, 0.0), dacc = acc;
+    for (int t = 0; t < cb.nterm; ++t) {
+      double w = cb.coef[t];
+      for (int q = 0; q < cb.hp[t]; ++q) w *= h;
+      if (cb.ep[t] > 0) {
+        double Ep = E1;
+        for (int q = 1; q < cb.ep[t]; ++q) Ep *= E1;
+        w *= (cb.masked && dc) ? 0.0 : Ep;
+      }
+      const double2 v = cb.in[t] ? cb.in[t][i] : make_double2(0.0, 0.0);
+      acc = cadd(acc, cscale(w, v));
+      if (TAN) {
+        // d w / dh = w (hp / h + ep L / ediv); hp / h written without dividing by h
+        double dw = cb.coef[t] * (double)cb.hp[t];
+        for (int q = 1; q < cb.hp[t]; ++q) dw *= h;
+        if (cb.ep[t] > 0) {
+          double Ep = E1;
+          for (int q = 1; q < cb.ep[t]; ++q) Ep *= E1;
+          const double em = (cb.masked && dc) ? 0.0 : Ep;
+          dw = dw * em + w * ((double)cb.ep[t] * L / cb.ediv);
+        }
+        dacc = cadd(dacc, cscale(dh * dw, v));
+        if (cb.din[t]) dacc = cadd(dacc, cscale(w, cb.din[t][i]));
+      }
+    }
+    cb.out[i] = acc;
+    if (TAN) cb.dout[i] = dacc;
+  }
+}
+
+}  // namespace ecs

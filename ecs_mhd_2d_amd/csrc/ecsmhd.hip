@@ -87,7 +87,10 @@ long long plan_key(int n, int kind) { return ((long long)n * 8 + kind) * 64 + 63
 int tuned(int n, int kind, int B, int dflt) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto it = g_tune.find(tune_key(n, kind, B));
-  return it == g_tune.end() ? dflt : it->second;
+  if (it != g_tune.end()) return it->second;
+  // an untuned bucket keeps the default tile within the grid size's fixed plan
+  auto ip = g_plan.find(plan_key(n, kind));
+  return ip == g_plan.end() ? dflt : 3 * ip->second + dflt % 3;
 }
 
 int upload_scalars(ecs_ctx* c, int slot, const double* vals, int B, hipStream_t st) {
@@ -232,7 +235,8 @@ int ecs_ctx_destroy(ecs_ctx* c) {
   (void)hipSetDevice(c->dev);
   void* ps[] = {c->tw, c->fhat, c->I, c->O, c->x0, c->acc, c->v1, c->v4, c->v5, c->xf, c->out, c->dx0, c->dacc,
                 c->dv1, c->dv4, c->dxf, c->dout, c->scal, c->hist, c->s_h, c->s_rs, c->s_rc, c->s_rh, c->s_nacc,
-                c->ckpt, c->xtape, c->itape, c->red, c->AO, c->AI, c->xb, c->k1b, c->k2b, c->k3b, c->hb};
+                c->ckpt, c->xtape, c->itape, c->red, c->AO, c->AI, c->xb, c->k1b, c->k2b, c->k3b, c->hb,
+                c->gen};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   drop_graphs(c);
@@ -301,6 +305,45 @@ int ecs_lawson_rk43_jvp(ecs_ctx* c, double* x, double* dx, int batch, const doub
   GUARD(c);
   hipStream_t st = (hipStream_t)stream;
   return OPS(c->n)->rk43_jvp(c, x, dx, batch, T, dT, hs, acc, attempts, budget, st);
+}
+
+int ecs_rk4(ecs_ctx* c, double* f, int batch, const double* dt, int steps, void* stream) {
+  if (!c || !f || !dt || batch < 1 || steps < 0) return fail(1, "ecs_rk4: bad arguments");
+  GUARD(c);
+  return OPS(c->n)->gen(c, f, batch, dt, steps, 0, 0, (hipStream_t)stream);
+}
+
+int ecs_rk4_jvp(ecs_ctx* c, double* f, double* df, int batch, const double* dt, const double* ddt, int steps,
+                void* stream) {
+  if (!c || !f || !df || !dt || !ddt || batch < 1 || steps < 0) return fail(1, "ecs_rk4_jvp: bad arguments");
+  GUARD(c);
+  return OPS(c->n)->gen_jvp(c, f, df, batch, dt, ddt, steps, 0, 0, (hipStream_t)stream);
+}
+
+int ecs_lawson_rk6(ecs_ctx* c, double* f, int batch, const double* dt, int steps, int masked, void* stream) {
+  if (!c || !f || !dt || batch < 1 || steps < 0) return fail(1, "ecs_lawson_rk6: bad arguments");
+  GUARD(c);
+  return OPS(c->n)->gen(c, f, batch, dt, steps, 1, masked ? 1 : 0, (hipStream_t)stream);
+}
+
+int ecs_lawson_rk6_jvp(ecs_ctx* c, double* f, double* df, int batch, const double* dt, const double* ddt, int steps,
+                       int masked, void* stream) {
+  if (!c || !f || !df || !dt || !ddt || batch < 1 || steps < 0) return fail(1, "ecs_lawson_rk6_jvp: bad arguments");
+  GUARD(c);
+  return OPS(c->n)->gen_jvp(c, f, df, batch, dt, ddt, steps, 1, masked ? 1 : 0, (hipStream_t)stream);
+}
+
+int ecs_tdrk4(ecs_ctx* c, double* f, int batch, const double* dt, int steps, void* stream) {
+  if (!c || !f || !dt || batch < 1 || steps < 0) return fail(1, "ecs_tdrk4: bad arguments");
+  GUARD(c);
+  return OPS(c->n)->gen(c, f, batch, dt, steps, 2, 0, (hipStream_t)stream);
+}
+
+int ecs_tdrk4_jvp(ecs_ctx* c, double* f, double* df, int batch, const double* dt, const double* ddt, int steps,
+                  void* stream) {
+  if (!c || !f || !df || !dt || !ddt || batch < 1 || steps < 0) return fail(1, "ecs_tdrk4_jvp: bad arguments");
+  GUARD(c);
+  return OPS(c->n)->gen_jvp(c, f, df, batch, dt, ddt, steps, 2, 0, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -5,6 +5,7 @@
 #pragma once
 #include "rhs_kernels.h"
 #include "adjoint_kernels.h"
+#include "comb_kernels.h"
 #include "host.h"
 
 // ------------------------------------------------------------------ launches
@@ -738,10 +739,212 @@ static int run_time_kernel(ecs_ctx* c, int B, int which, int reps, double* ms, h
 }
 
 
+
+// ------------------------------------------------------------------ generic explicit steppers
+// Classical RK4 (timestepping.py:17-44), Lawson RK6 (:83-125) and two-derivative
+// RK4 (:127-149), run as RHS passes (column load, row pass, column output)
+// with the stage algebra in comb_kernel between them.
+enum GenScheme : int { G_RK4 = 0, G_LRK6 = 1, G_TDRK4 = 2 };
+
+// Butcher form of a (Lawson) scheme: stage s reads
+//   x0 E^{p_s} + h sum_j A[s][j] E^{p_s - p_j} v_j,   v_j = RHS(stage j input),
+// and the step ends at x0 E^{p_S} + h sum_j b_j E^{p_S - p_j} v_j, with
+// E = exp(h L / ediv).  For Lawson RK6, p counts the multiplications by
+// e = exp(h/6 L) that precede each stage (after stages 1, 3, 4, 5, 6, 7).
+struct Tableau {
+  int S;
+  double A[8][8];
+  double b[8];
+  int p[9];
+  double ediv;
+};
+
+static const Tableau& tableau(int scheme) {
+  static const Tableau rk4 = {4,
+                              {{0}, {0.5}, {0, 0.5}, {0, 0, 1.0}},
+                              {1.0 / 6, 1.0 / 3, 1.0 / 3, 1.0 / 6},
+                              {0, 0, 0, 0, 0},
+                              1.0};
+  static const Tableau lrk6 = {8,
+                               {{0},
+                                {1.0 / 6},
+                                {1.0 / 12, 1.0 / 12},
+                                {0, -4.0 / 33, 5.0 / 11},
+                                {-1.0 / 4, -29.0 / 44, 31.0 / 22, 0},
+                                {3.0 / 11, 8.0 / 33, -4.0 / 11, 1.0 / 11, 14.0 / 33},
+                                {-17.0 / 48, -5.0 / 12, 1.0, 1.0, -13.0 / 12, 11.0 / 16},
+                                {20.0 / 39, 12.0 / 39, -31.0 / 39, -1.0 / 39, 34.0 / 39, -11.0 / 39, 16.0 / 39}},
+                               {13.0 / 200, 0, 4.0 / 25, 11.0 / 40, 0, 11.0 / 40, 4.0 / 25, 13.0 / 200},
+                               {0, 1, 1, 2, 3, 4, 5, 6, 6},
+                               6.0};
+  return scheme == G_LRK6 ? lrk6 : rk4;
+}
+
+template <int N, bool TAN>
+static int comb_launch(const Comb& cb, int B, hipStream_t st) {
+  const size_t tot = (size_t)B * cb.elems;
+  const int blocks = (int)std::min<size_t>((tot + 255) / 256, 4096);
+  hipLaunchKernelGGL((comb_kernel<N, TAN>), dim3(blocks), dim3(256), 0, st, cb, B);
+  LAUNCHCHK();
+  return 0;
+}
+
+// RHS of compact spectra x into v, with the tangent J(x) dx into dv.  hess:
+// the nonlinear term's second derivative H[x, dx] (= J(x) dx with b0 = 0 and
+// no dissipation; the MHD nonlinearity is a symmetric bilinear form) into dv,
+// v being scratch.
+template <int N, bool TAN>
+static int rhs_pass(const Args& base, double2* x, double2* dx, double2* v, double2* dv, int B, int diss,
+                    hipStream_t st, bool hess = false) {
+  Args a = base;
+  a.x0 = x; a.dx0 = dx; a.out = v; a.dout = dv;
+  a.include_diss = hess ? 0 : diss;
+  a.adaptive = AD_FIXED;
+  if (hess) { a.b0x = 0.0; a.b0y = 0.0; }
+  TRY((col<N, C_LOAD, TAN>(a, B, st)));
+  TRY((row<N, R_RHS, TAN>(a, B, st)));
+  TRY((col<N, C_RHS, TAN>(a, B, st)));
+  return 0;
+}
+
+struct CombBuilder {
+  Comb cb;
+  CombBuilder(ecs_ctx* c, size_t elems, double ediv, int masked, double2* out, double2* dout) {
+    memset(&cb, 0, sizeof(cb));
+    cb.ediv = ediv; cb.masked = masked;
+    cb.h = c->scal; cb.dh = c->scal + c->cap;
+    cb.nu[0] = c->nu; cb.nu[1] = c->eta;
+    cb.out = out; cb.dout = dout; cb.elems = elems;
+  }
+  CombBuilder& term(const double2* in, const double2* din, double coef, int hp, int ep) {
+    const int t = cb.nterm++;
+    cb.in[t] = in; cb.din[t] = din; cb.coef[t] = coef; cb.hp[t] = hp; cb.ep[t] = ep;
+    return *this;
+  }
+};
+
+// f (full layout [B][2][N][N/2+1]) advanced by `steps` steps of h = dt[b]
+// (tangent df along ddt[b]).  The RHS includes dissipation for RK4 and TDRK4
+// (loss_functions.py:53,81) and excludes it for Lawson RK6 (:68).  Unmasked
+// schemes keep the modes outside the 2/3 region (their RHS is zero there).
+template <int N, bool TAN>
+static int run_gen(ecs_ctx* c, double* f, double* df, int B, const double* dt, const double* ddt, int steps,
+                   int scheme, int masked, hipStream_t st) {
+  const bool td = scheme == G_TDRK4;
+  TRY(ensure_cap(c, (td && TAN) ? 2 * B : B));
+  TRY(autotune<N>(c, B, st));
+  if (td && TAN) TRY(autotune<N>(c, 2 * B, st));
+  const size_t ce = c->compact_elems();
+  const size_t cs = ce * B;
+  const Tableau& tb = tableau(scheme);
+  const int units = td ? (TAN ? 18 : 7) : (TAN ? 4 + 2 * tb.S : 2 + tb.S);
+  TRY(grow(&c->gen, &c->gen_cap, (size_t)units * cs));
+  TRY(upload_scalars(c, 0, dt, B, st));
+  if (TAN) TRY(upload_scalars(c, 1, ddt, B, st));
+  Args a = base_args(c);
+  double2* g = c->gen;
+  auto slot = [&](int u) { return g + (size_t)u * cs; };
+  const int diss = scheme == G_LRK6 ? 0 : 1;
+  const int msk = scheme == G_LRK6 ? masked : 0;
+  double2* x = slot(0);
+  TRY(pack<N>(f, x, B, st));
+  if (!td) {
+    double2* y = slot(1);
+    double2* dx = TAN ? slot(2) : nullptr;
+    double2* dy = TAN ? slot(3) : nullptr;
+    const int v0 = TAN ? 4 : 2;
+    auto v = [&](int j) { return slot(v0 + j); };
+    auto dv = [&](int j) { return TAN ? slot(v0 + tb.S + j) : nullptr; };
+    if (TAN) TRY(pack<N>(df, dx, B, st));
+    for (int s = 0; s < steps; ++s) {
+      for (int k = 0; k < tb.S; ++k) {
+        double2* in = x;
+        double2* din = dx;
+        if (k > 0) {
+          CombBuilder cb(c, ce, tb.ediv, msk, y, dy);
+          cb.term(x, dx, 1.0, 0, tb.p[k]);
+          for (int j = 0; j < k; ++j)
+            if (tb.A[k][j] != 0.0) cb.term(v(j), dv(j), tb.A[k][j], 1, tb.p[k] - tb.p[j]);
+          TRY((comb_launch<N, TAN>(cb.cb, B, st)));
+          in = y;
+          din = dy;
+        }
+        TRY((rhs_pass<N, TAN>(a, in, din, v(k), dv(k), B, diss, st)));
+      }
+      CombBuilder cb(c, ce, tb.ediv, msk, x, dx);
+      cb.term(x, dx, 1.0, 0, tb.p[tb.S]);
+      for (int j = 0; j < tb.S; ++j)
+        if (tb.b[j] != 0.0) cb.term(v(j), dv(j), tb.b[j], 1, tb.p[tb.S] - tb.p[j]);
+      TRY((comb_launch<N, TAN>(cb.cb, B, st)));
+    }
+    TRY(unpack<N>(x, f, B, msk, st));
+    if (TAN) TRY(unpack<N>(dx, df, B, msk, st));
+    return 0;
+  }
+  // Two-derivative RK4 with v_j = RHS(stage input), w_j = J(stage input) v_j:
+  //   y = x + h/2 v1 + h^2/8 w1,   x' = x + h v1 + h^2/6 w1 + h^2/3 w2
+  // (a_j = h J k_j = h^2 w_j).  Tangents use J for the first-order terms and
+  // dw = H[dz, v] + J(z) dv; J(z) v and J(z) dv share one pass of batch 2B.
+  if (!TAN) {
+    double2 *y = slot(1), *v1 = slot(2), *w1 = slot(3), *v2 = slot(4), *w2 = slot(5), *scr = slot(6);
+    for (int s = 0; s < steps; ++s) {
+      TRY((rhs_pass<N, false>(a, x, nullptr, v1, nullptr, B, diss, st)));
+      TRY((rhs_pass<N, true>(a, x, v1, scr, w1, B, diss, st)));
+      CombBuilder cy(c, ce, 1.0, 0, y, nullptr);
+      cy.term(x, nullptr, 1.0, 0, 0).term(v1, nullptr, 0.5, 1, 0).term(w1, nullptr, 0.125, 2, 0);
+      TRY((comb_launch<N, false>(cy.cb, B, st)));
+      TRY((rhs_pass<N, false>(a, y, nullptr, v2, nullptr, B, diss, st)));
+      TRY((rhs_pass<N, true>(a, y, v2, scr, w2, B, diss, st)));
+      CombBuilder cx(c, ce, 1.0, 0, x, nullptr);
+      cx.term(x, nullptr, 1.0, 0, 0).term(v1, nullptr, 1.0, 1, 0).term(w1, nullptr, 1.0 / 6, 2, 0)
+        .term(w2, nullptr, 1.0 / 3, 2, 0);
+      TRY((comb_launch<N, false>(cx.cb, B, st)));
+    }
+    TRY(unpack<N>(x, f, B, 0, st));
+    return 0;
+  }
+  // slots (units of B): XX = [x | x] 0-1, DV = [v1 | dv1] 2-3, W = [w1 | J dv1] 4-5,
+  // SCR 6-7, H1 8, dx 9, YY 10-11, DV2 12-13, W2 14-15, H2 16, dy 17
+  double2 *XX = slot(0), *DV = slot(2), *W = slot(4), *SCR = slot(6), *H1 = slot(8), *dx = slot(9);
+  double2 *YY = slot(10), *DV2 = slot(12), *W2 = slot(14), *H2 = slot(16), *dy = slot(17);
+  TRY(pack<N>(df, dx, B, st));
+  for (int s = 0; s < steps; ++s) {
+    HIPCHK(hipMemcpyAsync(XX + cs, XX, cs * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    TRY((rhs_pass<N, true>(a, XX, dx, DV, DV + cs, B, diss, st)));          // v1, dv1 = J(x) dx
+    TRY((rhs_pass<N, true>(a, XX, DV, SCR, W, 2 * B, diss, st)));           // w1 = J(x) v1, J(x) dv1
+    TRY((rhs_pass<N, true>(a, dx, DV, SCR, H1, B, diss, st, true)));        // H[dx, v1]
+    CombBuilder cy(c, ce, 1.0, 0, YY, dy);
+    cy.term(XX, dx, 1.0, 0, 0).term(DV, DV + cs, 0.5, 1, 0).term(W, W + cs, 0.125, 2, 0)
+      .term(nullptr, H1, 0.125, 2, 0);
+    TRY((comb_launch<N, true>(cy.cb, B, st)));
+    HIPCHK(hipMemcpyAsync(YY + cs, YY, cs * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    TRY((rhs_pass<N, true>(a, YY, dy, DV2, DV2 + cs, B, diss, st)));
+    TRY((rhs_pass<N, true>(a, YY, DV2, SCR, W2, 2 * B, diss, st)));
+    TRY((rhs_pass<N, true>(a, dy, DV2, SCR, H2, B, diss, st, true)));
+    CombBuilder cx(c, ce, 1.0, 0, XX, dx);
+    cx.term(XX, dx, 1.0, 0, 0).term(DV, DV + cs, 1.0, 1, 0).term(W, W + cs, 1.0 / 6, 2, 0)
+      .term(nullptr, H1, 1.0 / 6, 2, 0).term(W2, W2 + cs, 1.0 / 3, 2, 0).term(nullptr, H2, 1.0 / 3, 2, 0);
+    TRY((comb_launch<N, true>(cx.cb, B, st)));
+  }
+  TRY(unpack<N>(XX, f, B, 0, st));
+  TRY(unpack<N>(dx, df, B, 0, st));
+  return 0;
+}
+
+template <int N> static int gen_p(ecs_ctx* c, double* f, int B, const double* dt, int steps, int scheme, int masked,
+                                  hipStream_t st) {
+  return run_gen<N, false>(c, f, nullptr, B, dt, nullptr, steps, scheme, masked, st);
+}
+template <int N> static int gen_t(ecs_ctx* c, double* f, double* df, int B, const double* dt, const double* ddt,
+                                  int steps, int scheme, int masked, hipStream_t st) {
+  return run_gen<N, true>(c, f, df, B, dt, ddt, steps, scheme, masked, st);
+}
+
 #define ECS_GRID(NV)                                                                                     \
   const GridOps* grid_ops_##NV() {                                                                       \
     static const GridOps ops = {&run_state_vel<NV>, &rk4_p<NV>,          &rk4_t<NV>,    &run_rk43<NV>,  \
                                 &run_rk43_jvp<NV>,  &run_state_vel_vjp<NV>, &run_rk4_ckpt<NV>,            \
-                                &run_rk4_rev<NV>,   &run_rk4_vjp<NV>,    &run_time_kernel<NV>};           \
+                                &run_rk4_rev<NV>,   &run_rk4_vjp<NV>,    &run_time_kernel<NV>,            \
+                                &gen_p<NV>,         &gen_t<NV>};                                          \
     return &ops;                                                                                         \
   }

@@ -91,6 +91,8 @@ struct ecs_ctx {
   size_t vt_cap = 0;           // elements of xtape + vtape
   double2* itape = nullptr;    // primal column outputs [ckpt][4][B][8][N][KC]
   size_t it_cap = 0;
+  double2* gen = nullptr;      // stage arrays of the generic steppers
+  size_t gen_cap = 0;
   double2 *AO = nullptr, *AI = nullptr, *xb = nullptr, *k1b = nullptr, *k2b = nullptr, *k3b = nullptr;
   double* hb = nullptr;
   int adj_cap = 0;
@@ -198,6 +200,8 @@ struct GridOps {
   int (*rk4_rev)(ecs_ctx*, const double2*, double*, double*, int, const double*, int, int, int, hipStream_t);
   int (*rk4_vjp)(ecs_ctx*, const double*, double*, double*, int, const double*, int, int, int, hipStream_t);
   int (*time_kernel)(ecs_ctx*, int, int, int, double*, hipStream_t);
+  int (*gen)(ecs_ctx*, double*, int, const double*, int, int, int, hipStream_t);
+  int (*gen_jvp)(ecs_ctx*, double*, double*, int, const double*, const double*, int, int, int, hipStream_t);
 };
 const GridOps* grid_ops(int n);
 const GridOps* grid_ops_16();

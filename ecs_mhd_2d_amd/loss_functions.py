@@ -23,7 +23,7 @@ import torch
 
 from . import _lib, _runtime, mhd, symmetry
 
-MODES = ("Lawson_RK4", "Lawson_RK43")
+MODES = ("RK4", "Lawson_RK4", "Lawson_RK6", "Lawson_RK43", "TDRK4")
 CKPT = 64  # steps per reverse-pass segment
 
 
@@ -85,6 +85,15 @@ def integrate(f0, T, param_dict, mode):
         ad = param_dict["adaptive_dict"]
         f, infos = ctx.lawson_rk43(f0, T, 1e-2, ad["atol"], ad["max_steps_per_checkpoint"], ad["checkpoints"])
         return f, infos[0]
+    if mode == "RK4":
+        _, _, steps = _rk4_steps(param_dict)
+        return ctx.rk4(f0, _rk4_h(T, param_dict), steps), None
+    if mode == "TDRK4":
+        _, _, steps = _rk4_steps(param_dict)
+        return ctx.tdrk4(f0, _rk4_h(T, param_dict), steps), None
+    if mode == "Lawson_RK6":
+        steps = int(param_dict["steps"])
+        return ctx.lawson_rk6(f0, T / steps, steps, masked=True), None
     raise ValueError(f"You selected mode = {mode}, which does not exist.")
 
 
@@ -148,9 +157,18 @@ class Linearization:
         dfields = _fields(tangent)
         dT, dsx = _f(tangent["T"]), _f(tangent["sx"])
         df0 = _rfft2(dfields)
-        if self.mode == "Lawson_RK4":
+        if self.mode in ("Lawson_RK4", "RK4", "TDRK4"):
             _, _, steps = _rk4_steps(self.pd)
-            f, df = ctx.lawson_rk4_jvp(self.f0, df0, _rk4_h(self.T, self.pd), _rk4_h(dT, self.pd), steps, True)
+            h, dh = _rk4_h(self.T, self.pd), _rk4_h(dT, self.pd)
+            if self.mode == "Lawson_RK4":
+                f, df = ctx.lawson_rk4_jvp(self.f0, df0, h, dh, steps, True)
+            elif self.mode == "RK4":
+                f, df = ctx.rk4_jvp(self.f0, df0, h, dh, steps)
+            else:
+                f, df = ctx.tdrk4_jvp(self.f0, df0, h, dh, steps)
+        elif self.mode == "Lawson_RK6":
+            steps = int(self.pd["steps"])
+            f, df = ctx.lawson_rk6_jvp(self.f0, df0, self.T / steps, dT / steps, steps, True)
         else:
             f, df = ctx.lawson_rk43_jvp(self.f0, df0, self.T, dT, [self.info])
         dg = symmetry.apply_jvp(f, df, self.sx, dsx, self.pd)

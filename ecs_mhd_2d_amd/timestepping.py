@@ -1,9 +1,10 @@
-"""Lawson steppers (jax_scripts/lib/timestepping.py), fused on the GPU.
+"""Time steppers of jax_scripts/lib/timestepping.py on the GPU.
 
-The fused kernels integrate the MHD RHS with its dissipation handled by the
-integrating factor, so ``v_fn`` must be the ``VelocityFn`` of
-``mhd.VelocityFn(param_dict, include_dissipation=False)`` and ``L_diag``
-must be ``mhd.dissipation(param_dict)``; anything else raises (there is no
+The kernels integrate the MHD RHS itself, so ``v_fn`` must be an
+``mhd.VelocityFn``: without dissipation for the Lawson schemes (whose
+``L_diag`` must be ``mhd.dissipation(param_dict)``), with dissipation for
+classical RK4 and TDRK4, as mismatch_RPO builds them
+(loss_functions.py:53,61,68,74,81).  Anything else raises (there is no
 generic-callable fallback).
 """
 from __future__ import annotations
@@ -13,9 +14,9 @@ import torch
 from . import mhd
 
 
-def _check(v_fn, L_diag):
-    if not isinstance(v_fn, mhd.VelocityFn) or v_fn.include_dissipation:
-        raise TypeError("v_fn must be mhd.VelocityFn(param_dict, include_dissipation=False)")
+def _check(v_fn, L_diag, dissipative=False):
+    if not isinstance(v_fn, mhd.VelocityFn) or v_fn.include_dissipation != dissipative:
+        raise TypeError(f"v_fn must be mhd.VelocityFn(param_dict, include_dissipation={dissipative})")
     if L_diag is not None:
         ref = mhd.dissipation(v_fn.param_dict)
         if tuple(L_diag.shape) != tuple(ref.shape) or not torch.equal(L_diag.to(ref.dtype), ref):
@@ -34,6 +35,30 @@ def lawson_rk4(f, t, steps, v_fn, L_diag, mask=None):
     t = torch.as_tensor(t, dtype=torch.float64).reshape(-1).expand(B)
     h = (t / steps).tolist()
     return ctx.lawson_rk4(f, h, int(steps), masked=mask is not None)
+
+
+def _steps_h(f, t, steps):
+    B = int(torch.tensor(f.shape[:-3]).prod()) if f.dim() > 3 else 1
+    t = torch.as_tensor(t, dtype=torch.float64).reshape(-1).expand(B)
+    return (t / steps).tolist()
+
+
+def rk4(f, t, steps, v_fn):
+    """Classical RK4 (timestepping.py:17-44), h = t/steps; v_fn with dissipation."""
+    ctx = _check(v_fn, None, dissipative=True)
+    return ctx.rk4(f, _steps_h(f, t, steps), int(steps))
+
+
+def lawson_rk6(f, t, steps, v_fn, L_diag, mask=None):
+    """Lawson RK6 (timestepping.py:83-125): h = t/steps, e = exp(h/6 L) [* mask]."""
+    ctx = _check(v_fn, L_diag)
+    return ctx.lawson_rk6(f, _steps_h(f, t, steps), int(steps), masked=mask is not None)
+
+
+def tdrk4(f, t, steps, v_fn):
+    """Two-derivative RK4 (timestepping.py:127-149); v_fn with dissipation."""
+    ctx = _check(v_fn, None, dissipative=True)
+    return ctx.tdrk4(f, _steps_h(f, t, steps), int(steps))
 
 
 def lawson_rk43(x, vel_fn, diag_L, t, h=1e-2, atol=1e-4, max_steps_per_checkpoint=1024, checkpoints=1):

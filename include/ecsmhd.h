@@ -112,6 +112,30 @@ int ecs_lawson_rk43_jvp(ecs_ctx* ctx, double* x, double* dx, int batch, const do
                         const double* dT, const double* hs, const int8_t* acc, const int* attempts,
                         int budget, void* stream);
 
+/* The other fixed-step integrators of mismatch_RPO, with their forward-mode
+ * derivatives (f, df full spectra updated in place; dt, ddt host [batch], as
+ * for ecs_lawson_rk4).  They run as RHS passes with separate stage
+ * combinations rather than the fused Lawson-RK4 glue.
+ * ecs_rk4:        classical RK4 on the RHS with dissipation
+ *                 (timestepping.py:17-44; mode "RK4", loss_functions.py:49-56).
+ * ecs_lawson_rk6: Lawson RK6 with e = exp(h/6 L) [* mask], RHS without
+ *                 dissipation (timestepping.py:83-125; mode "Lawson_RK6",
+ *                 loss_functions.py:66-71; guess generation,
+ *                 generate_turbulence.py:88,105).
+ * ecs_tdrk4:      two-derivative RK4 with a_j = h J(y_j) k_j, RHS with
+ *                 dissipation (timestepping.py:127-149; mode "TDRK4",
+ *                 loss_functions.py:78-84; default of newton.py:70).  Its
+ *                 tangent carries the second derivative of the RHS. */
+int ecs_rk4(ecs_ctx* ctx, double* f, int batch, const double* dt, int steps, void* stream);
+int ecs_rk4_jvp(ecs_ctx* ctx, double* f, double* df, int batch, const double* dt, const double* ddt,
+                int steps, void* stream);
+int ecs_lawson_rk6(ecs_ctx* ctx, double* f, int batch, const double* dt, int steps, int masked, void* stream);
+int ecs_lawson_rk6_jvp(ecs_ctx* ctx, double* f, double* df, int batch, const double* dt, const double* ddt,
+                       int steps, int masked, void* stream);
+int ecs_tdrk4(ecs_ctx* ctx, double* f, int batch, const double* dt, int steps, void* stream);
+int ecs_tdrk4_jvp(ecs_ctx* ctx, double* f, double* df, int batch, const double* dt, const double* ddt,
+                  int steps, void* stream);
+
 /* GMRES vector kernels (linalg.py:58-97) on flat FP64 vectors of length n.
  * ecs_mgs: one modified-Gram-Schmidt sweep of Arnoldi step k (linalg.py:70-85):
  * for j = 0..k: h[j] = <w, Q[:, j]>, w -= h[j] Q[:, j]; then h[k+1] = |w| and

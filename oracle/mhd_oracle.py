@@ -454,6 +454,167 @@ def lawson_rk43_jvp(x, dx, pd, t, dt_t, schedule):
 
 
 # --------------------------------------------------------------------------
+# the other fixed-step integrators of mismatch_RPO  (timestepping.py:17-149)
+# --------------------------------------------------------------------------
+def state_vel_hess(a, b, pd):
+    """Second derivative of state_vel: d/de state_vel_jvp(f + e a, b) at e = 0.
+
+    The nonlinearity of mhd_jax.py:136-164 is a bilinear form plus the b0
+    terms, which are linear, so this is state_vel_jvp at the point a with
+    b0 = 0 and no dissipation, and it does not depend on f.
+    """
+    pd0 = dict(pd)
+    pd0["b0"] = [0.0, 0.0]
+    return state_vel_jvp(a, b, pd0, False)
+
+
+def rk4(f, t, steps, pd):
+    """Classical RK4 (timestepping.py:17-44) on the RHS with dissipation
+    (mode "RK4", loss_functions.py:49-56)."""
+    h = t / steps
+    v = lambda z: state_vel(z, pd, True)
+    for _ in range(int(steps)):
+        k1 = h * v(f)
+        k2 = h * v(f + k1 / 2)
+        k3 = h * v(f + k2 / 2)
+        k4 = h * v(f + k3)
+        f = f + k1 / 6 + k2 / 3 + k3 / 3 + k4 / 6
+    return f
+
+
+def rk4_jvp(f, df, t, dt_t, steps, pd):
+    """Forward-mode derivative of rk4 w.r.t. (f, t); dh = dt_t / steps."""
+    h, dh = t / steps, dt_t / steps
+    v = lambda z: state_vel(z, pd, True)
+    jv = lambda z, dz: state_vel_jvp(z, dz, pd, True)
+
+    def stage(z, dz):
+        vz = v(z)
+        return h * vz, dh * vz + h * jv(z, dz)
+
+    for _ in range(int(steps)):
+        k1, dk1 = stage(f, df)
+        k2, dk2 = stage(f + k1 / 2, df + dk1 / 2)
+        k3, dk3 = stage(f + k2 / 2, df + dk2 / 2)
+        k4, dk4 = stage(f + k3, df + dk3)
+        f = f + k1 / 6 + k2 / 3 + k3 / 3 + k4 / 6
+        df = df + dk1 / 6 + dk2 / 3 + dk3 / 3 + dk4 / 6
+    return f, df
+
+
+# Lawson RK6 of timestepping.py:83-125, written as data: stage s (0-based)
+# evaluates h v(f + sum_j RK6_A[s][j] k_j) after the integrating factor
+# e = exp(h/6 L) [* mask] has been applied to f and all k_j whenever
+# RK6_E_BEFORE[s] is set; the step returns f + sum_j RK6_B[j] k_j.
+RK6_A = [
+    [],
+    [1 / 6],
+    [1 / 12, 1 / 12],
+    [0, -4 / 33, 5 / 11],
+    [-1 / 4, -29 / 44, 31 / 22, 0],
+    [3 / 11, 8 / 33, -4 / 11, 1 / 11, 14 / 33],
+    [-17 / 48, -5 / 12, 1, 1, -13 / 12, 11 / 16],
+    [20 / 39, 12 / 39, -31 / 39, -1 / 39, 34 / 39, -11 / 39, 16 / 39],
+]
+RK6_B = [13 / 200, 0, 4 / 25, 11 / 40, 0, 11 / 40, 4 / 25, 13 / 200]
+RK6_E_BEFORE = [False, True, False, True, True, True, True, True]
+
+
+def lawson_rk6(f, t, steps, pd, mask=True):
+    """Lawson RK6 (timestepping.py:83-125), e = exp(h/6 L) [* mask], RHS without
+    dissipation (mode "Lawson_RK6", loss_functions.py:66-71)."""
+    h = t / steps
+    L = dissipation(pd)
+    e = np.exp(h / 6 * L)
+    if mask:
+        e = e * pd["mask"]
+    v = lambda z: state_vel(z, pd, False)
+    for _ in range(int(steps)):
+        ks = []
+        for s in range(8):
+            if RK6_E_BEFORE[s]:
+                f = e * f
+                ks = [e * k for k in ks]
+            y = f
+            for a, k in zip(RK6_A[s], ks):
+                if a != 0:
+                    y = y + a * k
+            ks.append(h * v(y))
+        f = f + sum(bj * k for bj, k in zip(RK6_B, ks) if bj != 0)
+    return f
+
+
+def lawson_rk6_jvp(f, df, t, dt_t, steps, pd, mask=True):
+    """Forward-mode derivative of lawson_rk6 w.r.t. (f, t): dh = dt_t / steps
+    enters every k = h v and e (de = dh/6 L e)."""
+    h, dh = t / steps, dt_t / steps
+    L = dissipation(pd)
+    e = np.exp(h / 6 * L)
+    if mask:
+        e = e * pd["mask"]
+    de = dh / 6 * L * e
+    v = lambda z: state_vel(z, pd, False)
+    jv = lambda z, dz: state_vel_jvp(z, dz, pd, False)
+    for _ in range(int(steps)):
+        ks, dks = [], []
+        for s in range(8):
+            if RK6_E_BEFORE[s]:
+                f, df = e * f, de * f + e * df
+                dks = [de * k + e * dk for k, dk in zip(ks, dks)]
+                ks = [e * k for k in ks]
+            y, dy = f, df
+            for a, k, dk in zip(RK6_A[s], ks, dks):
+                if a != 0:
+                    y, dy = y + a * k, dy + a * dk
+            vy = v(y)
+            ks.append(h * vy)
+            dks.append(dh * vy + h * jv(y, dy))
+        f = f + sum(bj * k for bj, k in zip(RK6_B, ks) if bj != 0)
+        df = df + sum(bj * dk for bj, dk in zip(RK6_B, dks) if bj != 0)
+    return f, df
+
+
+def tdrk4(f, t, steps, pd):
+    """Two-derivative RK4 (timestepping.py:127-149): a_j = h J(y_j) k_j with J
+    the RHS Jacobian (jax.jvp of v_fn, :135); RHS with dissipation (mode
+    "TDRK4", loss_functions.py:78-84)."""
+    h = t / steps
+    v = lambda z: state_vel(z, pd, True)
+    jv = lambda z, dz: state_vel_jvp(z, dz, pd, True)
+    for _ in range(int(steps)):
+        k1 = h * v(f)
+        a1 = h * jv(f, k1)
+        y = f + k1 / 2 + a1 / 8
+        k2 = h * v(y)
+        a2 = h * jv(y, k2)
+        f = f + k1 + a1 / 6 + a2 / 3
+    return f
+
+
+def tdrk4_jvp(f, df, t, dt_t, steps, pd):
+    """Forward-mode derivative of tdrk4 w.r.t. (f, t).  The tangent of
+    a = h J(y) k is dh J(y) k + h (H[dy, k] + J(y) dk), H = state_vel_hess
+    (the second derivative jax.jvp of the nested jvp produces)."""
+    h, dh = t / steps, dt_t / steps
+    v = lambda z: state_vel(z, pd, True)
+    jv = lambda z, dz: state_vel_jvp(z, dz, pd, True)
+    hv = lambda a, b: state_vel_hess(a, b, pd)
+    for _ in range(int(steps)):
+        v1 = v(f)
+        k1, dk1 = h * v1, dh * v1 + h * jv(f, df)
+        j1 = jv(f, k1)
+        a1, da1 = h * j1, dh * j1 + h * (hv(df, k1) + jv(f, dk1))
+        y, dy = f + k1 / 2 + a1 / 8, df + dk1 / 2 + da1 / 8
+        v2 = v(y)
+        k2, dk2 = h * v2, dh * v2 + h * jv(y, dy)
+        j2 = jv(y, k2)
+        a2, da2 = h * j2, dh * j2 + h * (hv(dy, k2) + jv(y, dk2))
+        f = f + k1 + a1 / 6 + a2 / 3
+        df = df + dk1 + da1 / 6 + da2 / 3
+    return f, df
+
+
+# --------------------------------------------------------------------------
 # symmetries  (symmetry.py:4-25) with tangent and transpose
 # --------------------------------------------------------------------------
 def shift_x(f, s, pd):
@@ -542,6 +703,16 @@ def mismatch_RPO(inp, pd, mode):
         f, info = lawson_rk43(f0, pd, T, h=1e-2, atol=ad["atol"],
                               max_steps_per_checkpoint=ad["max_steps_per_checkpoint"],
                               checkpoints=ad["checkpoints"], record=True)
+    elif mode in ("RK4", "TDRK4"):
+        nc, ms = int(pd["num_checkpoints"]), int(pd["ministeps"])
+        step = rk4 if mode == "RK4" else tdrk4
+        f = f0
+        for _ in range(nc):
+            f = step(f, T / nc, ms, pd)
+        info = None
+    elif mode == "Lawson_RK6":
+        f = lawson_rk6(f0, T, int(pd["steps"]), pd, True)
+        info = None
     else:
         raise ValueError(f"You selected mode = {mode}, which does not exist.")
     f = apply_symmetry(f, float(inp["sx"]), pd)
@@ -590,6 +761,14 @@ def objective_RPO_jvp(inp, tan, pd, mode, schedule=None):
             _, _, info = mismatch_RPO(inp, pd, mode)
             schedule = info["schedule"]
         f, df = lawson_rk43_jvp(f0, df0, pd, T, dT, schedule)
+    elif mode in ("RK4", "TDRK4"):
+        nc, ms = int(pd["num_checkpoints"]), int(pd["ministeps"])
+        step = rk4_jvp if mode == "RK4" else tdrk4_jvp
+        f, df = f0, df0
+        for _ in range(nc):
+            f, df = step(f, df, T / nc, dT / nc, ms, pd)
+    elif mode == "Lawson_RK6":
+        f, df = lawson_rk6_jvp(f0, df0, T, dT, int(pd["steps"]), pd, True)
     else:
         raise ValueError(mode)
     _, dg = apply_symmetry_jvp(f, df, float(inp["sx"]), float(tan["sx"]), pd)

@@ -332,3 +332,94 @@ def test_adam_matches_closed_form_first_step():
     # first ADAM step moves each entry by lr * g/(|g| + eps)
     for k in p:
         np.testing.assert_allclose(p1[k], p[k] - 1e-2 * g[k] / (np.abs(g[k]) + 1e-6), rtol=1e-12)
+
+
+# ------------------------------------------------------------ the other fixed-step integrators
+def test_lawson_rk6_exact_linear_decay():
+    """Zero nonlinearity: the integrating factor alone must give exp(L T) f,
+    which pins the accumulated powers of e (six per step)."""
+    n = 32
+    pd = params(n, b0=(0.0, 0.0), amp=0.0)
+    x, y = pd["x"], pd["y"]
+    f = np.fft.rfft2(np.stack([np.cos(2 * x - 3 * y), np.sin(2 * x - 3 * y)]))
+    T = 0.7
+    out = O.lawson_rk6(f, T, 5, pd, True)
+    assert rel(out, np.exp(T * O.dissipation(pd)) * f) < 1e-13
+
+
+@pytest.mark.parametrize("scheme,order", [("rk4", 4), ("lawson_rk6", 6), ("tdrk4", 4)])
+def test_fixed_step_orders(scheme, order):
+    """Empirical order on a nonlinear run: a dropped or wrong tableau entry
+    lowers it."""
+    n = 16
+    pd = params(n)
+    f = spectrum(n, 7)
+    T = 0.4
+    step = {"rk4": lambda z, s: O.rk4(z, T, s, pd), "lawson_rk6": lambda z, s: O.lawson_rk6(z, T, s, pd, True),
+            "tdrk4": lambda z, s: O.tdrk4(z, T, s, pd)}[scheme]
+    ref = O.lawson_rk6(f, T, 400, pd, True)
+    errs = [np.linalg.norm(step(f, s) - ref) for s in (8, 16)]
+    got = np.log2(errs[0] / errs[1])
+    assert order - 0.3 < got < order + 0.3, (scheme, got, errs)
+
+
+def test_fixed_step_schemes_agree_at_small_steps():
+    n = 16
+    pd = params(n)
+    f = spectrum(n, 8)
+    T = 0.3
+    a = O.rk4(f, T, 120, pd)
+    b = O.lawson_rk6(f, T, 30, pd, True)
+    c = O.tdrk4(f, T, 120, pd)
+    d = O.lawson_rk4(f, T, 120, pd, True)
+    # 4th-order schemes at h = 0.0025 are ~1e-8 from the 6th-order one
+    assert rel(a, b) < 1e-7 and rel(c, b) < 1e-7 and rel(d, b) < 1e-7
+
+
+def test_state_vel_hess_is_the_derivative_of_the_tangent():
+    n = 16
+    pd = params(n)
+    f, a, b = spectrum(n, 1), spectrum(n, 2), spectrum(n, 3)
+    eps = 1e-5
+    fd = (O.state_vel_jvp(f + eps * a, b, pd, True) - O.state_vel_jvp(f - eps * a, b, pd, True)) / (2 * eps)
+    assert rel(O.state_vel_hess(a, b, pd), fd) < 1e-9
+    assert rel(O.state_vel_hess(a, b, pd), O.state_vel_hess(b, a, pd)) < 1e-13   # symmetric
+
+
+@pytest.mark.parametrize("scheme", ["rk4", "lawson_rk6", "tdrk4"])
+def test_fixed_step_jvps_match_torch_forward_mode(scheme):
+    """Hand-derived tangents (incl. TDRK4's second-derivative term) against
+    torch.func.jvp of the independent torch restatement, along (df, dT)."""
+    from tests import torch_restatement as TR
+    n = 16
+    pd = params(n)
+    f, df = spectrum(n, 4), spectrum(n, 5)
+    T, dT, steps = 0.3, 0.7, 3
+    d = TR.domain(n)
+    forcing = -4 * torch.cos(4 * d["y"])
+    fn = {"rk4": TR.rk4_classic, "lawson_rk6": TR.lawson_rk6, "tdrk4": TR.tdrk4}[scheme]
+    run = lambda z, t: fn(z, t, steps, d, 1 / 40, 1 / 40, (0.0, 0.1), forcing)
+    out, dout = torch.func.jvp(run, (torch.tensor(f), torch.tensor(T, dtype=torch.float64)),
+                               (torch.tensor(df), torch.tensor(dT, dtype=torch.float64)))
+    ofn = {"rk4": lambda: O.rk4_jvp(f, df, T, dT, steps, pd),
+           "lawson_rk6": lambda: O.lawson_rk6_jvp(f, df, T, dT, steps, pd, True),
+           "tdrk4": lambda: O.tdrk4_jvp(f, df, T, dT, steps, pd)}[scheme]
+    of, odf = ofn()
+    assert rel(of, out.numpy()) < 1e-13
+    assert rel(odf, dout.numpy()) < 1e-12
+
+
+@pytest.mark.parametrize("mode", ["RK4", "Lawson_RK6", "TDRK4"])
+def test_objective_jvp_fd_other_modes(mode):
+    n = 16
+    pd = _pd16()
+    pd.update({"steps": 6})
+    inp = {"fields": np.fft.irfft2(spectrum(n, 3)), "T": 0.45, "sx": 0.2}
+    rng = np.random.default_rng(5)
+    tan = {"fields": rng.standard_normal((2, n, n)), "T": 0.3, "sx": -0.2}
+    jv = O.objective_RPO_jvp(inp, tan, pd, mode)
+    eps = 1e-6
+    op, _ = O.objective_RPO({k: inp[k] + eps * tan[k] for k in inp}, pd, mode)
+    om, _ = O.objective_RPO({k: inp[k] - eps * tan[k] for k in inp}, pd, mode)
+    fd = (op["fields"] - om["fields"]) / (2 * eps)
+    assert rel(jv["fields"], fd) < 1e-7

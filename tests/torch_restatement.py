@@ -81,3 +81,61 @@ def loss_rpo(fields, T, sx, n, nu, eta, b0, nc, ms, ny, rot):
         f = lawson_rk4(f, T / nc, ms, d, nu, eta, b0, forcing)
     f = symmetry(f, sx, ny, rot, d)
     return torch.mean(torch.fft.irfft2(f - f0) ** 2)
+
+
+def _v(f, d, nu, eta, b0, forcing, diss):
+    return state_vel(f, d, nu, eta, b0, forcing, diss)
+
+
+def rk4_classic(f, t, steps, d, nu, eta, b0, forcing):
+    """timestepping.py:17-44 with the dissipative RHS (loss_functions.py:53)."""
+    h = t / steps
+    for _ in range(steps):
+        k1 = h * _v(f, d, nu, eta, b0, forcing, True)
+        k2 = h * _v(f + 0.5 * k1, d, nu, eta, b0, forcing, True)
+        k3 = h * _v(f + 0.5 * k2, d, nu, eta, b0, forcing, True)
+        k4 = h * _v(f + k3, d, nu, eta, b0, forcing, True)
+        f = f + (k1 + 2 * k2 + 2 * k3 + k4) / 6
+    return f
+
+
+def lawson_rk6(f, t, steps, d, nu, eta, b0, forcing):
+    """timestepping.py:83-125 written stage by stage."""
+    h = t / steps
+    ksq = d["kx"] ** 2 + d["ky"] ** 2
+    e = torch.exp(h / 6 * (-torch.stack([nu * ksq, eta * ksq]))) * d["mask"]
+    v = lambda z: h * _v(z, d, nu, eta, b0, forcing, False)
+    for _ in range(steps):
+        k1 = v(f)
+        f, k1 = e * f, e * k1
+        k2 = v(f + k1 / 6)
+        k3 = v(f + (k1 + k2) / 12)
+        f, k1, k2, k3 = e * f, e * k1, e * k2, e * k3
+        k4 = v(f + (-4 * k2 + 15 * k3) / 33)
+        f, k1, k2, k3, k4 = e * f, e * k1, e * k2, e * k3, e * k4
+        k5 = v(f + (-11 * k1 - 29 * k2 + 62 * k3) / 44)
+        f, k1, k2, k3, k4, k5 = e * f, e * k1, e * k2, e * k3, e * k4, e * k5
+        k6 = v(f + (9 * k1 + 8 * k2 - 12 * k3 + 3 * k4 + 14 * k5) / 33)
+        f, k1, k2, k3, k4, k5, k6 = e * f, e * k1, e * k2, e * k3, e * k4, e * k5, e * k6
+        k7 = v(f + (-17 * k1 - 20 * k2 + 48 * k3 + 48 * k4 - 52 * k5 + 33 * k6) / 48)
+        f, k1, k2, k3, k4, k5, k6, k7 = e * f, e * k1, e * k2, e * k3, e * k4, e * k5, e * k6, e * k7
+        k8 = v(f + (20 * k1 + 12 * k2 - 31 * k3 - k4 + 34 * k5 - 11 * k6 + 16 * k7) / 39)
+        f = f + (13 * (k1 + k8) / 200 + 4 * (k3 + k7) / 25 + 11 * (k4 + k6) / 40)
+    return f
+
+
+def tdrk4(f, t, steps, d, nu, eta, b0, forcing):
+    """timestepping.py:127-149: the second stage derivative is a forward-mode
+    derivative of the RHS, taken here with torch.func.jvp as the reference
+    takes it with jax.jvp."""
+    h = t / steps
+    vf = lambda z: _v(z, d, nu, eta, b0, forcing, True)
+    jac = lambda z, k: torch.func.jvp(vf, (z,), (k,))[1]
+    for _ in range(steps):
+        k1 = h * vf(f)
+        a1 = h * jac(f, k1)
+        g = f + k1 / 2 + a1 / 8
+        k2 = h * vf(g)
+        a2 = h * jac(g, k2)
+        f = f + k1 + a1 / 6 + a2 / 3
+    return f
