@@ -170,6 +170,27 @@ def serving_leg():
     return stats
 
 
+def serving_full_leg(jobs):
+    """The serving workload at its stated size: Poisson arrivals at 4 req/s (rng
+    seed 7), 70% adjoint descent (64 ADAM steps, Lawson RK4 with steps =
+    round(T/0.004) rounded up to a multiple of 64) / 30% Newton-GMRES (Krylov 64,
+    5 iterations, RK4(3) atol 1e-8), N in {128, 256}, T ~ U[2, 8], 200 GB budget."""
+    from ecs_mhd_2d_amd import serve
+    w = serve.make_workload(4, rate=100.0, seed=1, sizes=(128, 256), T_range=(0.3, 0.4), descent_iters=1,
+                            newton_iters=1, inner=2, atol=1e-6, adaptive_budget=(64, 16))
+    serve.Server().run(w)   # warm-up: contexts, workspaces, autotuner
+    wl = serve.make_workload(jobs, rate=4.0, seed=7, sizes=(128, 256), T_range=(2.0, 8.0), descent_iters=64,
+                             newton_iters=5, inner=64, atol=1e-8, adaptive_budget=(64, 64))
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    done, stats = serve.Server().run(wl, log=log)
+    kinds = [j.kind for j in done]
+    stats["workload"] = (f"{jobs} jobs, Poisson 4/s, 70% adjoint descent (64 ADAM steps, dt 0.004) / 30% Newton "
+                         "(5 iterations, Krylov 64, RK4(3) atol 1e-8), N in {128,256}, T~U[2,8], 200 GB budget")
+    stats["adjoint_jobs"] = kinds.count("adjoint")
+    stats["newton_jobs"] = kinds.count("newton")
+    return stats
+
+
 def probe_roofline():
     """The tangent row kernel at N=128, batch 1 (50 launches through ecs_time_kernel)
     and a plain device copy of 64 MiB (read 64 MiB, write 64 MiB), whose counter
@@ -197,6 +218,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched-candidates and serving legs")
+    ap.add_argument("--serving-full", type=int, default=0, metavar="JOBS",
+                    help="run only the serving workload at its full size with JOBS jobs and print its statistics")
     ap.add_argument("--probe-roofline", action="store_true",
                     help="launch only the matvec's dominant kernel (50x) and a 64 MiB copy, for rocprofv3 --pmc")
     ap.add_argument("--probe", action="store_true",
@@ -215,6 +238,12 @@ def main():
         return
     if args.probe_roofline:
         probe_roofline()
+        return
+    if args.serving_full:
+        stats = serving_full_leg(args.serving_full)
+        print(json.dumps({"metric": "p50/p99 RPO solve latency (serving, full size)", "unit": "s",
+                          "value": stats["p99_latency_s"], "higher_is_better": False, "n_gpus": 1,
+                          "dtype": "f64", "data": "synthetic", "serving": stats}))
         return
     n = 128
     pd = mhd.make_param_dict(n, 1 / 40, 1 / 40, (0.0, 0.1))

@@ -41,9 +41,9 @@ _SIGS = {
                                    c_int, c_void_p]),
     "ecs_ckpt_doubles": (ctypes.c_longlong, [c_void_p, c_int, c_int, c_int]),
     "ecs_lawson_rk4_fwd_ckpt": (c_int, [c_void_p, c_void_p, c_int, P_double, c_int, c_int, c_int, c_void_p,
-                                        c_void_p]),
+                                        P_int, c_void_p]),
     "ecs_lawson_rk4_vjp_ckpt": (c_int, [c_void_p, c_void_p, c_void_p, P_double, c_int, P_double, c_int, c_int,
-                                        c_int, c_void_p]),
+                                        c_int, P_int, c_void_p]),
     "ecs_lawson_rk43": (c_int, [c_void_p, c_void_p, c_int, P_double, c_double, c_double, c_int, c_int,
                                 ctypes.POINTER(RK43Info), P_double, P_int8, c_void_p]),
     "ecs_lawson_rk43_jvp": (c_int, [c_void_p, c_void_p, c_void_p, c_int, P_double, P_double, P_double, P_int8,

@@ -125,6 +125,7 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
   __shared__ double2 twl[N];
   load_twiddles<N, NT>(twl, a.tw);
   const int b = blockIdx.z, c = blockIdx.y, ky0 = blockIdx.x * TY, tid = threadIdx.x;
+  if (a.nstep && a.step >= a.nstep[b]) return;  // candidate past its own step count
   if (FWD) {
     const double2* AI = a.ain + (size_t)b * 8 * N * KC;
     for (int idx = tid; idx < 4 * N * TY; idx += NT) {
@@ -204,6 +205,7 @@ __global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
   __shared__ double2 twl[N];
   load_twiddles<N, NT>(twl, a.tw);
   const int b = blockIdx.z, xbase = blockIdx.x * TX, tid = threadIdx.x;
+  if (a.nstep && a.step >= a.nstep[b]) return;  // candidate past its own step count
   const double2* P = a.prim + (size_t)b * 8 * N * KC;
   const double2* AO = a.aout + (size_t)b * 2 * N * KC;
   for (int idx = tid; idx < TX * NP * NHALF; idx += NT) {

@@ -41,6 +41,10 @@ int ensure_cap(ecs_ctx* c, int B) {
   if (c->hscal) HIPCHK(hipHostFree(c->hscal));
   HIPCHK(hipMalloc((void**)&c->scal, 4 * nc * sizeof(double)));
   HIPCHK(hipHostMalloc((void**)&c->hscal, 4 * nc * sizeof(double), hipHostMallocDefault));
+  if (c->nstep) HIPCHK(hipFree(c->nstep));
+  if (c->hnstep) HIPCHK(hipHostFree(c->hnstep));
+  HIPCHK(hipMalloc((void**)&c->nstep, nc * sizeof(int)));
+  HIPCHK(hipHostMalloc((void**)&c->hnstep, nc * sizeof(int), hipHostMallocDefault));
   if (c->red) HIPCHK(hipFree(c->red));
   if (c->hred) HIPCHK(hipHostFree(c->hred));
   HIPCHK(hipMalloc((void**)&c->red, (size_t)64 * nc * sizeof(double)));
@@ -102,6 +106,15 @@ int upload_scalars(ecs_ctx* c, int slot, const double* vals, int B, hipStream_t 
   for (int b = 0; b < B; ++b) h[b] = vals[b];
   HIPCHK(hipMemcpyAsync(c->scal + (size_t)slot * c->cap, h, B * sizeof(double), hipMemcpyHostToDevice, st));
   HIPCHK(hipEventRecord(c->ev_scal[slot], st));
+  return 0;
+}
+
+int upload_ints(ecs_ctx* c, const int* vals, int B, hipStream_t st) {
+  if (!c->ev_nstep) HIPCHK(hipEventCreateWithFlags(&c->ev_nstep, hipEventDisableTiming));
+  else HIPCHK(hipEventSynchronize(c->ev_nstep));
+  for (int b = 0; b < B; ++b) c->hnstep[b] = vals[b];
+  HIPCHK(hipMemcpyAsync(c->nstep, c->hnstep, B * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(hipEventRecord(c->ev_nstep, st));
   return 0;
 }
 
@@ -247,6 +260,9 @@ int ecs_ctx_destroy(ecs_ctx* c) {
   if (c->hcnt) (void)hipHostFree(c->hcnt);
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->hred) (void)hipHostFree(c->hred);
+  if (c->nstep) (void)hipFree(c->nstep);
+  if (c->hnstep) (void)hipHostFree(c->hnstep);
+  if (c->ev_nstep) (void)hipEventDestroy(c->ev_nstep);
   for (hipEvent_t e : c->ev_scal)
     if (e) (void)hipEventDestroy(e);
   delete c;
@@ -425,22 +441,38 @@ extern "C" long long ecs_ckpt_doubles(const ecs_ctx* c, int batch, int steps, in
   return (long long)(2 * ckpt_elems(c->n, batch, steps, ckpt));
 }
 
+// nsteps (host [batch], optional): per-candidate step counts, each in
+// [1, steps]; in the reverse pass each must be a multiple of ckpt or equal
+// steps, so that a candidate is active for whole segments only.
+static int check_nsteps(const char* who, const int* nsteps, int batch, int steps, int ckpt) {
+  if (!nsteps) return 0;
+  for (int b = 0; b < batch; ++b) {
+    const int k = nsteps[b];
+    if (k < 1 || k > steps || (k != steps && k % ckpt != 0))
+      return fail(1, std::string(who) + ": nsteps[b] must be in [1, steps] and a multiple of ckpt or equal to steps");
+  }
+  return 0;
+}
+
 extern "C" int ecs_lawson_rk4_fwd_ckpt(ecs_ctx* c, double* f, int batch, const double* dt, int steps, int masked,
-                                       int ckpt, double* ck, void* stream) {
+                                       int ckpt, double* ck, const int* nsteps, void* stream) {
   if (!c || !f || !dt || !ck || batch < 1 || steps < 1 || ckpt < 1)
     return fail(1, "ecs_lawson_rk4_fwd_ckpt: bad arguments");
+  TRY(check_nsteps("ecs_lawson_rk4_fwd_ckpt", nsteps, batch, steps, ckpt));
   GUARD(c);
   hipStream_t st = (hipStream_t)stream;
-  return OPS(c->n)->rk4_ckpt(c, f, batch, dt, steps, masked, ckpt, (double2*)ck, st);
+  return OPS(c->n)->rk4_ckpt(c, f, batch, dt, steps, masked, ckpt, (double2*)ck, nsteps, st);
 }
 
 extern "C" int ecs_lawson_rk4_vjp_ckpt(ecs_ctx* c, const double* ck, double* fbar, double* hbar, int batch,
-                                       const double* dt, int steps, int masked, int ckpt, void* stream) {
+                                       const double* dt, int steps, int masked, int ckpt, const int* nsteps,
+                                       void* stream) {
   if (!c || !ck || !fbar || !hbar || !dt || batch < 1 || steps < 1 || ckpt < 1)
     return fail(1, "ecs_lawson_rk4_vjp_ckpt: bad arguments");
+  TRY(check_nsteps("ecs_lawson_rk4_vjp_ckpt", nsteps, batch, steps, ckpt));
   GUARD(c);
   hipStream_t st = (hipStream_t)stream;
-  return OPS(c->n)->rk4_rev(c, (const double2*)ck, fbar, hbar, batch, dt, steps, masked, ckpt, st);
+  return OPS(c->n)->rk4_rev(c, (const double2*)ck, fbar, hbar, batch, dt, steps, masked, ckpt, nsteps, st);
 }
 
 extern "C" int ecs_time_kernel(ecs_ctx* c, int batch, int which, int reps, double* ms, void* stream) {

@@ -559,7 +559,7 @@ static int adj_buffers(ecs_ctx* c, int B, int ckpt) {
 // segment of ckpt steps into ck [nseg][B][2][R][KC]; f is advanced in place.
 template <int N>
 static int run_rk4_ckpt(ecs_ctx* c, double* f, int B, const double* dt, int steps, int masked, int ckpt,
-                        double2* ck, hipStream_t st) {
+                        double2* ck, const int* nsteps, hipStream_t st) {
   TRY(ensure_cap(c, B));
   TRY(autotune<N>(c, B, st));
   using G = Geo<N>;
@@ -567,10 +567,15 @@ static int run_rk4_ckpt(ecs_ctx* c, double* f, int B, const double* dt, int step
   TRY(upload_scalars(c, 0, dt, B, st));
   Args a = base_args(c);
   a.masked_e = masked;
+  if (nsteps) {
+    TRY(upload_ints(c, nsteps, B, st));
+    a.nstep = c->nstep;
+  }
   TRY(pack<N>(f, c->x0, B, st));
   HIPCHK(hipMemcpyAsync(ck, c->x0, cs * sizeof(double2), hipMemcpyDeviceToDevice, st));
   if (steps > 0) TRY((col<N, C_LOAD, false>(a, B, st)));
   for (int s = 0; s < steps; ++s) {
+    a.step = s;
     TRY((row<N, R_RHS, false>(a, B, st)));
     TRY((col<N, C_S1F, false>(a, B, st)));
     TRY((row<N, R_RHS, false>(a, B, st)));
@@ -593,7 +598,7 @@ static int run_rk4_ckpt(ecs_ctx* c, double* f, int B, const double* dt, int step
 // replaced by the cotangent of the initial spectra; hbar[b] = d/dh.
 template <int N>
 static int run_rk4_rev(ecs_ctx* c, const double2* ck, double* fbar, double* hbar, int B, const double* dt, int steps,
-                       int masked, int ckpt, hipStream_t st) {
+                       int masked, int ckpt, const int* nsteps, hipStream_t st) {
   TRY(adj_buffers<N>(c, B, ckpt));
   TRY(autotune<N>(c, B, st));
   using G = Geo<N>;
@@ -608,6 +613,10 @@ static int run_rk4_rev(ecs_ctx* c, const double2* ck, double* fbar, double* hbar
   a.xb = c->xb; a.k1b = c->k1b; a.k2b = c->k2b; a.k3b = c->k3b;
   a.hb = c->hb;
   a.cstride = cs;
+  if (nsteps) {
+    TRY(upload_ints(c, nsteps, B, st));
+    a.nstep = c->nstep;
+  }
   TRY(pack<N>(fbar, c->xb, B, st));
   HIPCHK(hipMemsetAsync(c->hb, 0, cs * sizeof(double), st));
   auto slot = [&](int t, int s) { return c->itape + ((size_t)t * 4 + s) * is; };
@@ -619,8 +628,10 @@ static int run_rk4_rev(ecs_ctx* c, const double2* ck, double* fbar, double* hbar
     HIPCHK(hipMemcpyAsync(c->x0, ck + (size_t)j * cs, cs * sizeof(double2), hipMemcpyDeviceToDevice, st));
     Args f = a;
     f.inter = slot(0, 0);
+    f.step = t0;
     TRY((col<N, C_LOAD, false>(f, B, st)));
     for (int t = 0; t < L; ++t) {
+      f.step = t0 + t;
       f.vtape = nullptr; f.xtape = nullptr;
       f.inter = slot(t, 0);
       TRY((row<N, R_RHS, false>(f, B, st)));
@@ -643,8 +654,10 @@ static int run_rk4_rev(ecs_ctx* c, const double2* ck, double* fbar, double* hbar
     Args b = a;
     b.tx = xt(L - 1);
     b.tv = vt(L - 1);
+    b.step = t0 + L - 1;
     TRY((acol<N, B_ENTRY>(b, B, st)));
     for (int t = L - 1; t >= 0; --t) {
+      b.step = t0 + t;
       b.tx = xt(t);
       b.tv = vt(t);
       b.prim = slot(t, 3);
@@ -687,8 +700,8 @@ static int run_rk4_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   const size_t full = (size_t)B * 2 * N * Geo<N>::NH * 2;
   HIPCHK(hipMallocAsync((void**)&fwork, full * sizeof(double), st));
   HIPCHK(hipMemcpyAsync(fwork, f0, full * sizeof(double), hipMemcpyDeviceToDevice, st));
-  int rc = run_rk4_ckpt<N>(c, fwork, B, dt, steps, masked, ckpt, c->ckpt, st);
-  if (rc == 0) rc = run_rk4_rev<N>(c, c->ckpt, fbar, hbar, B, dt, steps, masked, ckpt, st);
+  int rc = run_rk4_ckpt<N>(c, fwork, B, dt, steps, masked, ckpt, c->ckpt, nullptr, st);
+  if (rc == 0) rc = run_rk4_rev<N>(c, c->ckpt, fbar, hbar, B, dt, steps, masked, ckpt, nullptr, st);
   HIPCHK(hipFreeAsync(fwork, st));
   return rc;
 }

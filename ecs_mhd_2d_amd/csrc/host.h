@@ -73,6 +73,9 @@ struct ecs_ctx {
   double* scal = nullptr;      // device [4][cap]: dt, ddt, T, dT
   double* hscal = nullptr;     // pinned host staging [4][cap]
   hipEvent_t ev_scal[4] = {};  // completion of the last copy out of each staging slot
+  int* nstep = nullptr;        // device [cap]: per-candidate step counts
+  int* hnstep = nullptr;       // pinned staging [cap]
+  hipEvent_t ev_nstep = nullptr;
   Hist* hist = nullptr;
   size_t hist_cap = 0;
   // adaptive tangent schedule
@@ -125,6 +128,8 @@ size_t ckpt_elems(int n, int B, int steps, int ckpt);
 Args base_args(ecs_ctx* c);
 // Copies B per-candidate scalars into slot (0 dt, 1 ddt, 2 T, 3 dT) of the device scalar table.
 int upload_scalars(ecs_ctx* c, int slot, const double* vals, int B, hipStream_t st);
+// Copies B per-candidate step counts into the device table c->nstep.
+int upload_ints(ecs_ctx* c, const int* vals, int B, hipStream_t st);
 
 // ------------------------------------------------------------------ autotuner tables
 // Tile variants: column kernels take TY in {1, 2, 4} columns per block, row
@@ -196,8 +201,9 @@ struct GridOps {
   int (*rk43_jvp)(ecs_ctx*, double*, double*, int, const double*, const double*, const double*, const int8_t*,
                   const int*, int, hipStream_t);
   int (*state_vel_vjp)(ecs_ctx*, const double*, const double*, double*, int, int, hipStream_t);
-  int (*rk4_ckpt)(ecs_ctx*, double*, int, const double*, int, int, int, double2*, hipStream_t);
-  int (*rk4_rev)(ecs_ctx*, const double2*, double*, double*, int, const double*, int, int, int, hipStream_t);
+  int (*rk4_ckpt)(ecs_ctx*, double*, int, const double*, int, int, int, double2*, const int*, hipStream_t);
+  int (*rk4_rev)(ecs_ctx*, const double2*, double*, double*, int, const double*, int, int, int, const int*,
+                 hipStream_t);
   int (*rk4_vjp)(ecs_ctx*, const double*, double*, double*, int, const double*, int, int, int, hipStream_t);
   int (*time_kernel)(ecs_ctx*, int, int, int, double*, hipStream_t);
   int (*gen)(ecs_ctx*, double*, int, const double*, int, int, int, hipStream_t);

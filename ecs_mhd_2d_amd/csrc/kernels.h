@@ -131,6 +131,10 @@ struct Args {
   double2* ain;            // adjoint row output AI [B][8][N][KC]
   const double2* prim;     // primal column output for the row-adjoint [B][8][N][KC]
   const double2* gin;      // cotangent input of a standalone RHS adjoint
+  // fixed step with per-candidate step counts: candidate b takes part in the
+  // launches of global step `step` only while step < nstep[b] (null: all do)
+  const int* nstep;
+  int step;
   int include_diss;
   int adaptive;            // AdaptKind
   int masked_e;            // fixed step: e = exp(dt/2 L) * mask

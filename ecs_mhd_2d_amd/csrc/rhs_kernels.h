@@ -30,6 +30,10 @@ __device__ __forceinline__ Blk prologue(const Args& a, int b) {
   bk.hprev = 0.0; bk.T = 0.0; bk.h = 0.0; bk.dt = 0.0; bk.ddt = 0.0;
   bk.next = false; bk.h2 = 0.0; bk.dt2 = 0.0; bk.ddt2 = 0.0;
   if (a.adaptive == AD_FIXED) {
+    if (a.nstep && a.step >= a.nstep[b]) {
+      bk.run = bk.stage = false;
+      return bk;
+    }
     bk.dt = a.dt ? a.dt[b] : 0.0;
     bk.ddt = (TAN && a.ddt) ? a.ddt[b] : 0.0;
     return bk;
@@ -100,6 +104,7 @@ __device__ __forceinline__ Blk prologue(const Args& a, int b) {
 }
 
 __device__ __forceinline__ bool row_active(const Args& a, int b) {
+  if (a.adaptive == AD_FIXED) return !(a.nstep && a.step >= a.nstep[b]);
   if (a.adaptive == AD_PRIMAL) return a.hist[(size_t)b * a.hstride + attempt_of(a)].active != 0;
   if (a.adaptive == AD_TANGENT) return attempt_of(a) < a.s_nacc[b];
   return true;

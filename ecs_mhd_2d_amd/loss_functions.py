@@ -189,12 +189,12 @@ def _rk4_vjp_from(ctx, f0, fbar, h, steps, pd):
     ck = torch.empty(int(nd), dtype=torch.float64, device=f0.device)
     fw = f0.clone().contiguous()
     hp = _lib.dptr([h])
-    _lib.check(lib.ecs_lawson_rk4_fwd_ckpt(ctx.handle, fw.data_ptr(), 1, hp, steps, 1, CKPT, ck.data_ptr(),
+    _lib.check(lib.ecs_lawson_rk4_fwd_ckpt(ctx.handle, fw.data_ptr(), 1, hp, steps, 1, CKPT, ck.data_ptr(), None,
                                            _runtime._stream()), "ecs_lawson_rk4_fwd_ckpt")
     g = fbar.clone().contiguous()
     hb = (ctypes.c_double * 1)()
     _lib.check(lib.ecs_lawson_rk4_vjp_ckpt(ctx.handle, ck.data_ptr(), g.data_ptr(), hb, 1, hp, steps, 1, CKPT,
-                                           _runtime._stream()), "ecs_lawson_rk4_vjp_ckpt")
+                                           None, _runtime._stream()), "ecs_lawson_rk4_vjp_ckpt")
     return fw, g, hb[0]
 
 
@@ -250,7 +250,7 @@ class RPOLoss(torch.autograd.Function):
         ck = torch.empty(int(nd), dtype=torch.float64, device=f0.device)
         fT = f0.clone().contiguous()
         _lib.check(lib.ecs_lawson_rk4_fwd_ckpt(ctx.handle, fT.data_ptr(), 1, _lib.dptr([h]), steps, 1, CKPT,
-                                               ck.data_ptr(), _runtime._stream()), "ecs_lawson_rk4_fwd_ckpt")
+                                               ck.data_ptr(), None, _runtime._stream()), "ecs_lawson_rk4_fwd_ckpt")
         g = symmetry.apply(fT, sxf, param_dict)
         diff = _irfft2(g - f0, n)
         ctx_.save_for_backward(ck, fT, diff)
@@ -268,7 +268,7 @@ class RPOLoss(torch.autograd.Function):
         hb = (ctypes.c_double * 1)()
         lib = _lib.load()
         _lib.check(lib.ecs_lawson_rk4_vjp_ckpt(ctx.handle, ck.data_ptr(), g.data_ptr(), hb, 1,
-                                               _lib.dptr([ctx_.h]), ctx_.steps, 1, CKPT, _runtime._stream()),
+                                               _lib.dptr([ctx_.h]), ctx_.steps, 1, CKPT, None, _runtime._stream()),
                    "ecs_lawson_rk4_vjp_ckpt")
         gfields = rfft2_T(g - dbar, n)
         Tbar = torch.tensor(hb[0] / ctx_.steps, dtype=torch.float64, device=diff.device)

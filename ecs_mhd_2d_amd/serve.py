@@ -36,7 +36,10 @@ class Op:
 
     def group_key(self):
         if self.kind == "grad":
-            return ("grad", self.n, self.kw["steps"])
+            # step counts that are whole 64-step segments may differ within a
+            # batch (each candidate runs its own count); others batch alike only
+            steps = self.kw["steps"]
+            return ("grad", self.n) if steps % 64 == 0 else ("grad", self.n, steps)
         if self.kind in ("objective", "linearize"):
             ad = self.kw["adaptive"]
             return (self.kind, self.n, ad["atol"], ad["max_steps_per_checkpoint"], ad["checkpoints"])
@@ -50,23 +53,26 @@ def _pd(n, nu, eta, b0):
 
 
 def _batch_grad(ops, pd, lane):
-    """loss_RPO value and gradient (mode Lawson_RK4) for a batch with equal step counts."""
+    """loss_RPO value and gradient (mode Lawson_RK4) for a batch; each candidate
+    takes its own step count (a multiple of the 64-step segment)."""
     import ctypes
     from . import _lib, _runtime
     ctx = mhd.context(pd, lane)
     n = ops[0].n
-    steps = ops[0].kw["steps"]
+    nst = [int(o.kw["steps"]) for o in ops]
+    steps = max(nst)
+    nsteps = (ctypes.c_int * len(ops))(*nst) if len(set(nst)) > 1 else None
     B = len(ops)
     fields = torch.stack([o.kw["fields"] for o in ops]).contiguous()
     T = [float(o.kw["T"]) for o in ops]
     sx = [float(o.kw["sx"]) for o in ops]
-    h = [t / steps for t in T]
+    h = [t / k for t, k in zip(T, nst)]
     f0 = torch.fft.rfft2(fields).contiguous()
     lib = _lib.load()
     ck = torch.empty(int(lib.ecs_ckpt_doubles(ctx.handle, B, steps, LF.CKPT)), dtype=torch.float64, device="cuda")
     fT = f0.clone()
     _lib.check(lib.ecs_lawson_rk4_fwd_ckpt(ctx.handle, fT.data_ptr(), B, _lib.dptr(h), steps, 1, LF.CKPT,
-                                           ck.data_ptr(), _runtime._stream()), "ecs_lawson_rk4_fwd_ckpt")
+                                           ck.data_ptr(), nsteps, _runtime._stream()), "ecs_lawson_rk4_fwd_ckpt")
     out = []
     gbar = torch.empty_like(f0)
     sxb = []
@@ -83,12 +89,12 @@ def _batch_grad(ops, pd, lane):
         out.append(dbar)
     hb = (ctypes.c_double * B)()
     _lib.check(lib.ecs_lawson_rk4_vjp_ckpt(ctx.handle, ck.data_ptr(), gbar.data_ptr(), hb, B, _lib.dptr(h), steps,
-                                           1, LF.CKPT, _runtime._stream()), "ecs_lawson_rk4_vjp_ckpt")
+                                           1, LF.CKPT, nsteps, _runtime._stream()), "ecs_lawson_rk4_vjp_ckpt")
     res = []
     for b in range(B):
         loss = torch.mean(diffs[b] * diffs[b])
         gf = LF.rfft2_T(gbar[b] - out[b], n)
-        res.append((loss, {"fields": gf, "T": torch.tensor(hb[b] / steps, dtype=torch.float64, device="cuda"),
+        res.append((loss, {"fields": gf, "T": torch.tensor(hb[b] / nst[b], dtype=torch.float64, device="cuda"),
                            "sx": sxb[b]}))
     return res
 
@@ -269,14 +275,16 @@ class Server:
             return _batch_jvp(ops, pd, lane)
         raise ValueError(kind)
 
-    def run(self, arrivals, concurrent=True):
+    def run(self, arrivals, concurrent=True, log=None):
         """arrivals: list of (t_arrival_seconds, kind, make_generator, mem_bytes, n).
-        Returns the finished jobs and summary statistics."""
+        Returns the finished jobs and summary statistics; `log(msg)`, if given,
+        hears of every finished job and, every 10 s, of the ticks run so far."""
         t0 = time.perf_counter()
         waiting = [Job(i, k, t, g(), m, n) for i, (t, k, g, m, n) in enumerate(sorted(arrivals, key=lambda a: a[0]))]
         active, done = [], []
         used = 0.0
         peak = 0.0
+        last_log = 0.0
         while waiting or active:
             now = time.perf_counter() - t0
             while waiting and waiting[0].arrival <= now and used + waiting[0].mem <= self.budget:
@@ -340,6 +348,10 @@ class Server:
             self.ticks += 1
             self.batched_ops += sum(len(js) for _, js in tasks)
             now = time.perf_counter() - t0
+            if log is not None and now - last_log >= 10.0:
+                last_log = now
+                kinds = ", ".join(f"{k[0]} N={k[1]} x{len(js)}" for k, js in tasks)
+                log(f"{now:.1f} s: tick {self.ticks}, {len(active)} active jobs, {len(waiting)} waiting; groups: {kinds}")
             still = []
             for (key, js), res in zip(tasks, outs):
                 for j, r in zip(js, res):
@@ -350,6 +362,8 @@ class Server:
                         j.result, j.finish = e.value, now
                         done.append(j)
                         used -= j.mem
+                        if log is not None:
+                            log(f"job {j.id} ({j.kind}, N={j.n}) done at {now:.1f} s, latency {now - j.arrival:.1f} s")
             active = still
         wall = time.perf_counter() - t0
         self.max_concurrent = getattr(self, "max_concurrent", 0)

@@ -85,12 +85,17 @@ int ecs_lawson_rk4_vjp(ecs_ctx* ctx, const double* f0, double* fbar_inout, doubl
  * every segment of `ckpt` steps into the caller's device buffer ck of
  * ecs_ckpt_doubles() doubles; ecs_lawson_rk4_vjp_ckpt then runs the reverse
  * pass from ck (recomputing each segment), replacing fbar_inout (cotangent
- * of the final spectra) by that of the initial spectra, and hbar = d/dh. */
+ * of the final spectra) by that of the initial spectra, and hbar = d/dh.
+ * nsteps (host [batch], or NULL): per-candidate step counts, each in
+ * [1, steps] and a multiple of ckpt or equal to steps; candidate b stops
+ * after nsteps[b] steps of its own dt[b] (serving batches gradient jobs of
+ * different periods, adjoint_descent.py:93-123 with steps from T).  Without
+ * it every candidate takes `steps`. */
 long long ecs_ckpt_doubles(const ecs_ctx* ctx, int batch, int steps, int ckpt);
 int ecs_lawson_rk4_fwd_ckpt(ecs_ctx* ctx, double* f, int batch, const double* dt, int steps, int masked, int ckpt,
-                            double* ck, void* stream);
+                            double* ck, const int* nsteps, void* stream);
 int ecs_lawson_rk4_vjp_ckpt(ecs_ctx* ctx, const double* ck, double* fbar_inout, double* hbar, int batch,
-                            const double* dt, int steps, int masked, int ckpt, void* stream);
+                            const double* dt, int steps, int masked, int ckpt, const int* nsteps, void* stream);
 
 /* lawson_rk43(x, vel_fn, diag_L, t, h, atol, max_steps_per_checkpoint,
  * checkpoints)  (timestepping.py:151-335).  x: full spectra, updated in

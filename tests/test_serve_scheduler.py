@@ -8,11 +8,12 @@ import numpy as np
 from ecs_mhd_2d_amd import serve
 
 
-def toy_job(n, steps, value):
-    """A job that asks for `steps` 'grad' ops and sums what it receives."""
+def toy_job(n, steps, value, traj=64):
+    """A job that asks for `steps` 'grad' ops (trajectories of `traj` steps)
+    and sums what it receives."""
     total = 0.0
     for _ in range(steps):
-        r = yield serve.Op("grad", n, steps=64, x=value)
+        r = yield serve.Op("grad", n, steps=traj, x=value)
         total += r
     return total
 
@@ -42,8 +43,18 @@ def test_results_and_batching():
         assert res[i] == 3 * 2.0 * i
     # all six jobs arrive at t=0: each tick has one group per grid size
     assert all(c[1] == 3 for c in ex.calls)
-    assert {c[0] for c in ex.calls} == {("grad", 32, 64), ("grad", 64, 64)}
+    assert {c[0] for c in ex.calls} == {("grad", 32), ("grad", 64)}
     assert stats["mean_batch"] == 6.0
+
+
+def test_gradient_jobs_of_different_segment_counts_batch_together():
+    """Trajectories of whole 64-step segments batch across step counts (each
+    candidate runs its own count); other counts batch only with equal ones."""
+    ex = Exec()
+    arr = [(0.0, "adjoint", (lambda k=k: toy_job(32, 2, 1.0, traj=k)), 1.0, 32) for k in (64, 128, 192, 100, 100, 90)]
+    serve.Server(executor=ex).run(arr)
+    keys = sorted({(c[0], c[1]) for c in ex.calls}, key=str)
+    assert keys == sorted({(("grad", 32), 3), (("grad", 32, 100), 2), (("grad", 32, 90), 1)}, key=str)
 
 
 def test_admission_budget_limits_residency():
