@@ -191,17 +191,21 @@ def serving_full_leg(jobs):
     return stats
 
 
-def probe_roofline():
-    """The tangent row kernel at N=128, batch 1 (50 launches through ecs_time_kernel)
-    and a plain device copy of 64 MiB (read 64 MiB, write 64 MiB), whose counter
-    readings scale FETCH_SIZE / WRITE_SIZE to bytes."""
+def probe_roofline(batched=False):
+    """The matvec's dominant kernel (tangent row kernel, N=128, batch 1) or,
+    with ``batched``, the batched roofline kernel (row kernel, N=256, batch
+    64): 50 launches through ecs_time_kernel; then a plain device copy of 64
+    MiB (read 64 MiB, write 64 MiB), whose counter readings scale FETCH_SIZE /
+    WRITE_SIZE to bytes."""
     import torch
     from ecs_mhd_2d_amd import _lib, _runtime, mhd
-    pd = mhd.make_param_dict(128, 1 / 40, 1 / 40)
+    n, B, which = (256, 64, 1) if batched else (128, 1, 3)
+    pd = mhd.make_param_dict(n, 1 / 40, 1 / 40)
     ctx = mhd.context(pd)
     ms = ctypes.c_double()
     lib = _lib.load()
-    _lib.check(lib.ecs_time_kernel(ctx.handle, 1, 3, 50, ctypes.byref(ms), _runtime._stream()), "ecs_time_kernel")
+    _lib.check(lib.ecs_time_kernel(ctx.handle, B, which, 50, ctypes.byref(ms), _runtime._stream()),
+               "ecs_time_kernel")
     a = torch.ones(8 * 1024 * 1024, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
     torch.cuda.synchronize()
@@ -222,6 +226,8 @@ def main():
                     help="run only the serving workload at its full size with JOBS jobs and print its statistics")
     ap.add_argument("--probe-roofline", action="store_true",
                     help="launch only the matvec's dominant kernel (50x) and a 64 MiB copy, for rocprofv3 --pmc")
+    ap.add_argument("--probe-roofline-batched", action="store_true",
+                    help="the same for the batched row kernel (N=256, batch 64)")
     ap.add_argument("--probe", action="store_true",
                     help="run a short fixed kernel workload for rocprofv3 and exit")
     args = ap.parse_args()
@@ -236,8 +242,8 @@ def main():
     if args.probe:
         probe()
         return
-    if args.probe_roofline:
-        probe_roofline()
+    if args.probe_roofline or args.probe_roofline_batched:
+        probe_roofline(batched=args.probe_roofline_batched)
         return
     if args.serving_full:
         stats = serving_full_leg(args.serving_full)

@@ -4,6 +4,7 @@
 // drivers compiled in grid_n*.hip.
 #include "host.h"
 #include "vec_kernels.h"
+#include "recurrence_kernels.h"
 
 static thread_local std::string g_err;
 int fail(int code, const std::string& msg) {
@@ -365,19 +366,22 @@ int ecs_tdrk4_jvp(ecs_ctx* c, double* f, double* df, int batch, const double* dt
 }  // extern "C"
 
 // ------------------------------------------------------------------ Krylov vectors
+// Partial sums of the two-level reductions, one buffer per (device, stream)
+// so that calls on different streams never share one (2 * VMAXG doubles,
+// kept until the process ends).
 static std::mutex g_part_mu;
-static std::map<int, double*> g_part;  // per device: 2 * VMAXG partial sums
+static std::map<std::pair<int, hipStream_t>, double*> g_part;
 
-static int partials(double** out) {
+static int partials(double** out, hipStream_t st) {
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   std::lock_guard<std::recursive_mutex> alk(g_alloc_mu);
   std::lock_guard<std::mutex> lk(g_part_mu);
-  auto it = g_part.find(dev);
+  auto it = g_part.find({dev, st});
   if (it == g_part.end()) {
     double* p = nullptr;
     HIPCHK(hipMalloc((void**)&p, 2 * VMAXG * sizeof(double)));
-    it = g_part.emplace(dev, p).first;
+    it = g_part.emplace(std::make_pair(dev, st), p).first;
   }
   *out = it->second;
   return 0;
@@ -389,7 +393,7 @@ extern "C" int ecs_dot(const double* a, const double* b, int n, double* out, voi
   if (!a || !b || !out || n < 1) return fail(1, "ecs_dot: bad arguments");
   hipStream_t st = (hipStream_t)stream;
   double* part = nullptr;
-  TRY(partials(&part));
+  TRY(partials(&part, st));
   const int G = red_blocks(n);
   hipLaunchKernelGGL(dot_partial_kernel, dim3(G), dim3(VNT), 0, st, a, b, n, part);
   LAUNCHCHK();
@@ -402,7 +406,7 @@ extern "C" int ecs_mgs(double* Q, int ldq, int n, int k, double* w, double* h, v
   if (!Q || !w || !h || n < 1 || k < 0 || ldq < n) return fail(1, "ecs_mgs: bad arguments");
   hipStream_t st = (hipStream_t)stream;
   double* part = nullptr;
-  TRY(partials(&part));
+  TRY(partials(&part, st));
   const int G = red_blocks(n);
   hipLaunchKernelGGL(dot_partial_kernel, dim3(G), dim3(VNT), 0, st, w, Q, n, part);
   LAUNCHCHK();
@@ -418,6 +422,59 @@ extern "C" int ecs_mgs(double* Q, int ldq, int n, int k, double* w, double* h, v
   return 0;
 }
 
+// Scratch for the discarded dots of reorthogonalisation passes, per (device, stream).
+static std::map<std::pair<int, hipStream_t>, std::pair<double*, int>> g_hscr;
+
+extern "C" int ecs_mgs_reorth(double* Q, int ldq, int n, int k, double* w, double* h, int passes, void* stream) {
+  if (!Q || !w || !h || n < 1 || k < 0 || ldq < n || passes < 0) return fail(1, "ecs_mgs_reorth: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  double* part = nullptr;
+  TRY(partials(&part, st));
+  double* scr = nullptr;
+  {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::recursive_mutex> alk(g_alloc_mu);
+    std::lock_guard<std::mutex> lk(g_part_mu);
+    auto& e = g_hscr[std::make_pair(dev, st)];
+    if (e.second < k + 2) {
+      if (e.first) HIPCHK(hipFree(e.first));
+      e.first = nullptr;
+      e.second = 0;
+      HIPCHK(hipMalloc((void**)&e.first, (size_t)(k + 2) * sizeof(double)));
+      e.second = k + 2;
+    }
+    scr = e.first;
+  }
+  const int G = red_blocks(n);
+  int slot = 0;
+  for (int p = 0; p <= passes; ++p) {
+    double* hp = p == 0 ? h : scr;   // only the first pass's dots enter H
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(G), dim3(VNT), 0, st, w, Q, n, part + slot * VMAXG);
+    LAUNCHCHK();
+    for (int j = 0; j <= k; ++j) {
+      double* pin = part + slot * VMAXG;
+      slot ^= 1;
+      double* pout = part + slot * VMAXG;
+      hipLaunchKernelGGL(mgs_update_kernel, dim3(G), dim3(VNT), 0, st, w, Q, ldq, n, j, k, hp, pin, pout);
+      LAUNCHCHK();
+    }
+  }
+  hipLaunchKernelGGL(mgs_finish_kernel, dim3(G), dim3(VNT), 0, st, w, Q, ldq, n, k, h, part + slot * VMAXG);
+  LAUNCHCHK();
+  return 0;
+}
+
+
+
+extern "C" int ecs_recurrence(const double* fs, int nsnap, long long modes, double* dist, void* stream) {
+  if (!fs || !dist || nsnap < 1 || modes < 1) return fail(1, "ecs_recurrence: bad arguments");
+  const int nt = (nsnap + REC_TS - 1) / REC_TS;
+  hipLaunchKernelGGL(recurrence_kernel, dim3(nt, nt), dim3(256), 0, (hipStream_t)stream, (const double2*)fs, nsnap,
+                     modes, dist);
+  LAUNCHCHK();
+  return 0;
+}
 
 extern "C" int ecs_state_vel_vjp(ecs_ctx* c, const double* in, const double* cot, double* grad, int batch,
                                  int diss, void* stream) {

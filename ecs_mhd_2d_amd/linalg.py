@@ -63,3 +63,113 @@ def gmres(A, b, inner, outer=1, preconditioner_list=(), return_history=False):
         yk = np.linalg.lstsq(H[: k + 1, :k], b2[: k + 1], rcond=None)[0]
         hist.append(float(np.linalg.norm(H[: k + 1, :k] @ yk - b2[: k + 1]) / nbh))
     return x, rel_res, {"H": H, "krylov_residuals": hist, "matvecs": inner + 1}
+
+
+# ------------------------------------------------------------------ hookstep
+def _mgs_reorth(Q, k, w, hcol, passes=2):
+    lib = _lib.load()
+    n = Q.shape[1]
+    _lib.check(lib.ecs_mgs_reorth(Q.data_ptr(), n, n, k, w.data_ptr(), hcol.data_ptr(), int(passes),
+                                  _runtime._stream()), "ecs_mgs_reorth")
+
+
+def _arnoldi_reorth(A, b, m):
+    """Arnoldi with one recorded MGS pass and two discarded reorthogonalisation
+    passes per step (linalg.py:137-153, :255-268), basis on the device.
+    Returns Q [(m+1), n] (rows are basis vectors) and H (m+1, m) on the host."""
+    n = b.numel()
+    Q = torch.zeros((m + 1, n), dtype=torch.float64, device=b.device)
+    Hd = torch.zeros((m, m + 2), dtype=torch.float64, device=b.device)
+    Q[0] = b / torch.linalg.vector_norm(b)
+    for k in range(m):
+        w = A(Q[k]).contiguous().clone()
+        _mgs_reorth(Q, k, w, Hd[k])
+    H = np.zeros((m + 1, m))
+    Hh = Hd.cpu().numpy()
+    for k in range(m):
+        H[: k + 2, k] = Hh[k, : k + 2]
+    return Q, H
+
+
+def _bisection(g, lower, upper, maxit=128):
+    """linalg.py:165-177 / :289-301: t = 0 if g(0) > 0, else the last midpoint."""
+    if g(0.0) > 0:
+        return 0.0
+    t = lower
+    for _ in range(maxit):
+        t = (lower + upper) / 2
+        if g(t) > 0:
+            upper = t
+        else:
+            lower = t
+    return t
+
+
+def _flat_objective(objective, input_dict):
+    from . import utils
+    n = input_dict["fields"].shape[-1]
+    x = utils.flatten(input_dict)
+    return x, (lambda v: utils.flatten(objective(utils.unflatten(v, n)))), (lambda v: utils.unflatten(v, n))
+
+
+def newton_gmres_hookstep_v2(A, b, m, s, objective, input_dict, max_halvings=64):
+    """Newton step with a bound on its size (linalg.py:217-337).
+
+    Krylov space of A of dimension m from b, SVD of the (m+1) x m Hessenberg
+    matrix, least-squares step y; while |y| > s*s the step is regularised to
+    |y| = s by bisection on t in (H^T H + t I) y = H^T Q^T b; the step is kept
+    when |objective(x - dx)| < |b| (then s doubles), otherwise s halves.
+    Returns (new_input_dict, s, rel_res) with rel_res = |A dx - b| / |b|.
+    The reference loops until a step decreases |f|; here more than
+    ``max_halvings`` halvings raise RuntimeError.
+    """
+    Q, H = _arnoldi_reorth(A, b, m)
+    b2 = torch.mv(Q, b).cpu().numpy()
+    U, S, Vh = np.linalg.svd(H, full_matrices=False)
+    y = Vh.T @ ((U.T @ b2) / S)
+    x, f_vec, unravel = _flat_objective(objective, input_dict)
+    nb = float(torch.linalg.vector_norm(b))
+    for _ in range(max_halvings):
+        if np.linalg.norm(y) > s * s:
+            b4 = Vh @ (H.T @ b2)
+            g = lambda t: s - np.linalg.norm(b4 / (S * S + t))
+            t = _bisection(g, 0.0, np.sqrt(m) * np.linalg.norm(b4) / s)
+            y = Vh.T @ (b4 / (S * S + t))
+        dx = torch.mv(Q[:m].T.contiguous(), torch.as_tensor(y, device=b.device))
+        if float(torch.linalg.vector_norm(f_vec(x - dx))) < nb:
+            s = s * 2
+            break
+        s = s / 2
+    else:
+        raise RuntimeError("hookstep: no step decreased |f|")
+    rel_res = float(torch.linalg.vector_norm(A(dx) - b)) / nb
+    return unravel(x - dx), s, rel_res
+
+
+def newton_gmres_hookstep(AtA, Atb, m, s, f, f0, J, b, input_dict, max_halvings=64):
+    """Hookstep on the normal equations (linalg.py:102-213): Krylov space of
+    A^T A from A^T b, eigendecomposition of the m x m Hessenberg matrix (lower
+    triangle, as jnp/numpy eigh read it), bisection for a step of norm s on
+    [0, |b| / s].  f maps input dicts to output dicts, f0 is the flattened
+    objective at input_dict.  Returns (new_input_dict, s, rel_res_1,
+    rel_res_2) = (..., |A^T A step - A^T b| / |A^T b|, |J step + b| / |b|)."""
+    Q, Hf = _arnoldi_reorth(AtA, Atb, m)
+    H = Hf[:m, :m]
+    D, V = np.linalg.eigh(H)
+    b2 = V.T @ torch.mv(Q[:m], Atb).cpu().numpy()
+    x0, f_vec, unravel = _flat_objective(f, input_dict)
+    nf0 = float(torch.linalg.vector_norm(f0))
+    nb = float(torch.linalg.vector_norm(b))
+    for _ in range(max_halvings):
+        g = lambda t: s - np.linalg.norm(b2 / (D + t))
+        t = 0.0 if g(0.0) > 0 else _bisection(g, 0.0, nb / s)
+        step = torch.mv(Q[:m].T.contiguous(), torch.as_tensor(V @ (b2 / (D + t)), device=Atb.device))
+        if float(torch.linalg.vector_norm(f_vec(x0 - step))) < nf0:
+            s = s * 2
+            break
+        s = s / 2
+    else:
+        raise RuntimeError("hookstep: no step decreased |f|")
+    rr1 = float(torch.linalg.vector_norm(AtA(step) - Atb) / torch.linalg.vector_norm(Atb))
+    rr2 = float(torch.linalg.vector_norm(J(step) + b)) / nb
+    return unravel(x0 - step), s, rr1, rr2

@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import ctypes
 
+import numpy as np
 import torch
 
 from . import _lib, _runtime, mhd, symmetry
@@ -97,6 +98,29 @@ def integrate(f0, T, param_dict, mode):
     raise ValueError(f"You selected mode = {mode}, which does not exist.")
 
 
+def integrate_jvp(f0, df0, T, dT, param_dict, mode, info=None):
+    """Forward-mode derivative of the integration of mismatch_RPO for spectra
+    with leading batch dimensions (all candidates share T and dT; mode
+    "Lawson_RK43" replays ``info``, the schedule of the primal, for every
+    candidate).  Returns (f, df)."""
+    ctx = mhd.context(param_dict)
+    if mode in ("Lawson_RK4", "RK4", "TDRK4"):
+        _, _, steps = _rk4_steps(param_dict)
+        h, dh = _rk4_h(T, param_dict), _rk4_h(dT, param_dict)
+        if mode == "Lawson_RK4":
+            return ctx.lawson_rk4_jvp(f0, df0, h, dh, steps, True)
+        if mode == "RK4":
+            return ctx.rk4_jvp(f0, df0, h, dh, steps)
+        return ctx.tdrk4_jvp(f0, df0, h, dh, steps)
+    if mode == "Lawson_RK6":
+        steps = int(param_dict["steps"])
+        return ctx.lawson_rk6_jvp(f0, df0, T / steps, dT / steps, steps, True)
+    if mode == "Lawson_RK43":
+        B = int(np.prod(f0.shape[:-3])) if f0.dim() > 3 else 1
+        return ctx.lawson_rk43_jvp(f0, df0, T, dT, [info] * B)
+    raise ValueError(f"You selected mode = {mode}, which does not exist.")
+
+
 def mismatch_RPO(input_dict, param_dict, mode):
     """(f0, f, info): f0 = rfft2(fields) (unmasked), f = g(Phi_T f0) (loss_functions.py:9-96)."""
     f0 = _rfft2(_fields(input_dict))
@@ -153,24 +177,10 @@ class Linearization:
 
     def jvp(self, tangent):
         """J(x) t for the tangent dict t (utils.py:108)."""
-        ctx = mhd.context(self.pd)
         dfields = _fields(tangent)
         dT, dsx = _f(tangent["T"]), _f(tangent["sx"])
         df0 = _rfft2(dfields)
-        if self.mode in ("Lawson_RK4", "RK4", "TDRK4"):
-            _, _, steps = _rk4_steps(self.pd)
-            h, dh = _rk4_h(self.T, self.pd), _rk4_h(dT, self.pd)
-            if self.mode == "Lawson_RK4":
-                f, df = ctx.lawson_rk4_jvp(self.f0, df0, h, dh, steps, True)
-            elif self.mode == "RK4":
-                f, df = ctx.rk4_jvp(self.f0, df0, h, dh, steps)
-            else:
-                f, df = ctx.tdrk4_jvp(self.f0, df0, h, dh, steps)
-        elif self.mode == "Lawson_RK6":
-            steps = int(self.pd["steps"])
-            f, df = ctx.lawson_rk6_jvp(self.f0, df0, self.T / steps, dT / steps, steps, True)
-        else:
-            f, df = ctx.lawson_rk43_jvp(self.f0, df0, self.T, dT, [self.info])
+        f, df = integrate_jvp(self.f0, df0, self.T, dT, self.pd, self.mode, self.info)
         dg = symmetry.apply_jvp(f, df, self.sx, dsx, self.pd)
         return {"fields": _irfft2(df0 - dg, self.n),
                 "T": torch.mean(self.vt * dfields),

@@ -60,6 +60,30 @@ def newton_gmres_update(input_dict, param_dict, mode, inner=64, outer=1, do_line
     return new, log
 
 
+def newton_hookstep_update(input_dict, param_dict, mode, s, inner=64):
+    """One iteration of newton_hookstep_v2.py:95-121: objective, Jacobian
+    matvec, newton_gmres_hookstep_v2 with trust-region size s, dealias.
+    Returns (new_input_dict, s, log)."""
+    n = input_dict["fields"].shape[-1]
+    t0 = time.perf_counter()
+    f = loss_functions.objective_RPO(input_dict, param_dict, mode)
+    b = utils.flatten(f)
+    torch.cuda.synchronize()
+    fwall = time.perf_counter() - t0
+    lin = loss_functions.Linearization(input_dict, param_dict, mode)
+    A = lambda v: utils.flatten(lin.jvp(utils.unflatten(v, n)))
+    obj = lambda d: loss_functions.objective_RPO(d, param_dict, mode)
+    t1 = time.perf_counter()
+    new, s, rel_res = linalg.newton_gmres_hookstep_v2(A, b, inner, s, obj, input_dict)
+    torch.cuda.synchronize()
+    gwall = time.perf_counter() - t1
+    rel_err = float(torch.linalg.vector_norm(f["fields"]) / torch.linalg.vector_norm(new["fields"]))
+    new = {"fields": dealias(new["fields"].contiguous(), param_dict), "T": new["T"].clone(), "sx": new["sx"].clone()}
+    log = {"rel_err": rel_err, "f_norm": float(torch.linalg.vector_norm(b)), "fwall": fwall, "gmreswall": gwall,
+           "rel_res": rel_res, "s": s, "T": float(new["T"]), "sx": float(new["sx"])}
+    return new, s, log
+
+
 class AdjointDescent:
     """ADAM on the loss_RPO gradient (adjoint_descent.py:79-123): lr 1e-2,
     betas (0.9, 0.999), eps 1e-6, fields dealiased after every step."""

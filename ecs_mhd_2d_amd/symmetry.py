@@ -70,3 +70,27 @@ def apply_vjp(f, sx, param_dict, gbar):
     t = 1j * param_dict["kx"] * a * f
     sxbar = torch.sum(gbar.real * t.real + gbar.imag * t.imag)
     return fbar, sxbar
+
+
+def find_optimal_symmetry(f1, f2, param_dict, nx):
+    """The symmetry g = (sx, ny, rot) minimising mean(|f1 - g(f2)|^2)
+    (symmetry.py:28-56): sx = 2 pi i / nx for i < nx, ny in 0..7, rot in
+    (True, False), searched in that order with the first minimum kept.  The
+    nx shifts of one (ny, rot) are evaluated as one batch on the device."""
+    shifts = torch.arange(nx, dtype=torch.float64, device=f2.device) * (2 * math.pi / nx)
+    kx = param_dict["kx"]
+    errs = torch.empty((nx, 8, 2), dtype=torch.float64, device=f2.device)
+    for ny in range(8):
+        for ri, rot in enumerate((True, False)):
+            f = torch.exp(1j * kx * shifts.reshape(-1, 1, 1, 1)) * f2
+            f = shift_reflect(f, ny, param_dict)
+            if rot:
+                f = rot180(f)
+            errs[:, ny, ri] = torch.mean(torch.square(torch.abs(f1 - f)), dim=(-3, -2, -1))
+    flat = errs.reshape(-1).cpu().numpy()
+    k = int(flat.argmin())     # numpy's argmin returns the first minimum
+    i, rem = divmod(k, 16)
+    ny, ri = divmod(rem, 2)
+    best = (float((2 * math.pi / nx) * i), ny, ri == 0)
+    print(f"Best symmetry found is {best}")
+    return best

@@ -145,10 +145,22 @@ int ecs_tdrk4_jvp(ecs_ctx* ctx, double* f, double* df, int batch, const double* 
  * ecs_mgs: one modified-Gram-Schmidt sweep of Arnoldi step k (linalg.py:70-85):
  * for j = 0..k: h[j] = <w, Q[:, j]>, w -= h[j] Q[:, j]; then h[k+1] = |w| and
  * Q[:, k+1] = w / h[k+1].  Q is column-major with leading dimension ldq
- * (Q[:, j] = Q + j*ldq); h: device [k+2]. */
+ * (Q[:, j] = Q + j*ldq); h: device [k+2].
+ * ecs_mgs_reorth: the same sweep followed by `passes` more orthogonalisation
+ * passes against Q[:, 0..k] whose dots are discarded, then the norm and
+ * normalisation: the Arnoldi step of the hookstep solvers (linalg.py:137-153
+ * and 255-268, two passes).  Reductions use one scratch buffer per (device,
+ * stream), so calls on different streams may overlap. */
 int ecs_mgs(double* Q, int ldq, int n, int k, double* w, double* h, void* stream);
+int ecs_mgs_reorth(double* Q, int ldq, int n, int k, double* w, double* h, int passes, void* stream);
 /* out[0] = <a, b> */
 int ecs_dot(const double* a, const double* b, int n, double* out, void* stream);
+
+/* Recurrence diagram of a turbulent time series (generate_turbulence.py:140-151):
+ * dist[i][j] = || |fs_i| - |fs_j| ||_2 with fs complex128 [nsnap][modes]
+ * (each snapshot's spectrum flattened) and dist [nsnap][nsnap], exactly
+ * symmetric. */
+int ecs_recurrence(const double* fs, int nsnap, long long modes, double* dist, void* stream);
 
 /* Measurement hook (no counterpart in the reference, which has no native
  * code): average duration in ms of one launch of a fused RHS kernel over

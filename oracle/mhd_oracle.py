@@ -686,6 +686,90 @@ def apply_symmetry_vjp(f, sx, pd, gbar):
 
 
 # --------------------------------------------------------------------------
+# guess generation  (generate_turbulence.py:86-167, utils.py:60-86, symmetry.py:28-56)
+# --------------------------------------------------------------------------
+def turbulence_series(f, pd, dt, ministeps, steps, transient_steps):
+    """Transient of Lawson-RK6 steps of dt, then `steps` snapshots, each after
+    ministeps steps of h = (dt ministeps) / ministeps (generate_turbulence.py:86-125)."""
+    for _ in range(int(transient_steps)):
+        f = lawson_rk6(f, dt, 1, pd, True)
+    fs = []
+    for _ in range(int(steps)):
+        f = lawson_rk6(f, dt * ministeps, ministeps, pd, True)
+        fs.append(f)
+    return np.stack(fs)
+
+
+def recurrence_diagram(fs):
+    """dist[:, i] = norm(|fs| - |fs_i|) over each snapshot (generate_turbulence.py:140-151)."""
+    S = fs.shape[0]
+    dist = np.zeros((S, S))
+    a = np.abs(fs).reshape(S, -1)
+    for i in range(S):
+        dist[:, i] = np.linalg.norm(a - a[i], axis=1)
+    return dist
+
+
+def find_optimal_symmetry(f1, f2, pd, nx):
+    """Brute-force search of symmetry.py:28-56 (first minimum kept)."""
+    best_g, best_err = (0.0, 0, False), 1e100
+    for i in range(nx):
+        s = (2 * np.pi / nx) * i
+        for ny in range(8):
+            for rot in (True, False):
+                f = shift_x(f2, s, pd)
+                f = shift_reflect(f, ny, pd)
+                if rot:
+                    f = rot180(f)
+                err = np.mean(np.square(np.abs(f1 - f)))
+                if err < best_err:
+                    best_err, best_g = err, (s, ny, rot)
+    return best_g
+
+
+def create_state_from_turb(fs, idx, pd, dt, ministeps, nx=64):
+    """utils.py:60-86: (input_dict, steps, shift_reflect_ny, rot)."""
+    f1, f2 = fs[idx[0] - 1], fs[idx[1] - 1]
+    g = find_optimal_symmetry(f1, f2, pd, nx)
+    T = dt * ministeps * (idx[1] - idx[0])
+    return {"fields": np.fft.irfft2(f1), "T": T, "sx": g[0]}, int(ministeps * (idx[1] - idx[0])), g[1], g[2]
+
+
+def floquet_block_power(inp, pd, tang, maxit, mode="TDRK4"):
+    """Block power iteration of floquet.py:42-99 with the oracle's tangents:
+    J t = irfft2(mask * g dPhi_T rfft2(t)) at fixed T and sx; numpy (LAPACK
+    Householder) QR after each application; R = Q J(Q)^T."""
+    f0 = np.fft.rfft2(inp["fields"])
+    T, sx = float(inp["T"]), float(inp["sx"])
+
+    def jac(t):
+        out = []
+        for b in range(t.shape[0]):
+            df0 = np.fft.rfft2(t[b])
+            if mode in ("RK4", "TDRK4", "Lawson_RK4"):
+                nc, ms = int(pd["num_checkpoints"]), int(pd["ministeps"])
+                step = {"RK4": lambda f, d, a, c: rk4_jvp(f, d, a, c, ms, pd),
+                        "TDRK4": lambda f, d, a, c: tdrk4_jvp(f, d, a, c, ms, pd),
+                        "Lawson_RK4": lambda f, d, a, c: lawson_rk4_jvp(f, d, a, c, ms, pd, True)}[mode]
+                f, df = f0, df0
+                for _ in range(nc):
+                    f, df = step(f, df, T / nc, 0.0)
+            else:
+                raise ValueError(mode)
+            _, dg = apply_symmetry_jvp(f, df, sx, 0.0, pd)
+            out.append(np.fft.irfft2(dg * pd["mask"]))
+        return np.stack(out)
+
+    B = tang.shape[0]
+    for _ in range(maxit):
+        q, _ = np.linalg.qr(jac(tang).reshape(B, -1).T, mode="reduced")
+        tang = q.T.reshape(tang.shape)
+    t = tang.reshape(B, -1)
+    jt = jac(tang).reshape(B, -1)
+    return t @ jt.T, tang
+
+
+# --------------------------------------------------------------------------
 # RPO objective and loss  (loss_functions.py:9-148)
 # --------------------------------------------------------------------------
 def mismatch_RPO(inp, pd, mode):
@@ -876,6 +960,92 @@ def gmres(A, b, inner, outer=1, preconditioner_list=()):
     x = Q[:, :inner] @ y
     rel_res = np.linalg.norm(precond(A(x)) - b) / np.linalg.norm(b)
     return x, float(rel_res), H, Q
+
+
+def _bisect(g, lower, upper, maxit=128):
+    """Bisection of linalg.py:165-177 / :289-301: t = 0 if g(0) > 0, else the
+    last midpoint after maxit halvings (g(t) > 0 moves the upper end)."""
+    if g(0.0) > 0:
+        return 0.0
+    t = lower
+    for _ in range(maxit):
+        t = (lower + upper) / 2
+        if g(t) > 0:
+            upper = t
+        else:
+            lower = t
+    return t
+
+
+def _arnoldi_reorth(A, b, m):
+    """Arnoldi with one recorded MGS pass and two discarded reorthogonalisation
+    passes (linalg.py:137-153, :255-268).  Returns Q (n, m+1), H (m+1, m)."""
+    n = b.size
+    Q = np.zeros((n, m + 1))
+    H = np.zeros((m + 1, m))
+    Q[:, 0] = b / np.linalg.norm(b)
+    for k in range(m):
+        w = A(Q[:, k])
+        for j in range(k + 1):
+            hj = np.dot(w, Q[:, j])
+            w = w - hj * Q[:, j]
+            H[j, k] = hj
+        for _ in range(2):
+            for j in range(k + 1):
+                w = w - np.dot(w, Q[:, j]) * Q[:, j]
+        H[k + 1, k] = np.linalg.norm(w)
+        Q[:, k + 1] = w / H[k + 1, k]
+    return Q, H
+
+
+def hookstep_v2(A, b, m, s, f_vec, x, max_halvings=64):
+    """newton_gmres_hookstep_v2 (linalg.py:217-337) on flat vectors: f_vec(x)
+    is the flattened objective.  Returns (x - dx, s, rel_res, dx).  The
+    trust-region test compares |y| with s*s, and the bisection makes |z| = s,
+    as the reference does (:282, :287)."""
+    Q, H = _arnoldi_reorth(A, b, m)
+    b2 = Q.T @ b
+    U, S, Vh = np.linalg.svd(H, full_matrices=False)
+    y = Vh.T @ ((U.T @ b2) / S)
+    nb = np.linalg.norm(b)
+    for _ in range(max_halvings):
+        if np.linalg.norm(y) > s * s:
+            b4 = Vh @ (H.T @ b2)
+            g = lambda t: s - np.linalg.norm(b4 / (S * S + t))
+            t = _bisect(g, 0.0, np.sqrt(m) * np.linalg.norm(b4) / s)
+            y = Vh.T @ (b4 / (S * S + t))
+        dx = Q[:, :m] @ y
+        if np.linalg.norm(f_vec(x - dx)) < nb:
+            s = s * 2
+            break
+        s = s / 2
+    else:
+        raise RuntimeError("hookstep: no step decreased |f|")
+    rel_res = np.linalg.norm(A(dx) - b) / nb
+    return x - dx, s, float(rel_res), dx
+
+
+def hookstep_v1(AtA, Atb, m, s, f_vec, f0, J, b, x, max_halvings=64):
+    """newton_gmres_hookstep (linalg.py:102-213): Arnoldi on A^T A (square H,
+    no last subdiagonal), eigh of H (lower triangle), bisection for the step of
+    norm s.  Returns (x - step, s, rel_res_1, rel_res_2, step)."""
+    Q, Hf = _arnoldi_reorth(AtA, Atb, m)
+    H = Hf[:m, :m]
+    D, V = np.linalg.eigh(H)
+    b2 = V.T @ (Q[:, :m].T @ Atb)
+    for _ in range(max_halvings):
+        g = lambda t: s - np.linalg.norm(b2 / (D + t))
+        t = 0.0 if g(0.0) > 0 else _bisect(g, 0.0, np.linalg.norm(b) / s)
+        step = Q[:, :m] @ (V @ (b2 / (D + t)))
+        if np.linalg.norm(f_vec(x - step)) < np.linalg.norm(f0):
+            s = s * 2
+            break
+        s = s / 2
+    else:
+        raise RuntimeError("hookstep: no step decreased |f|")
+    rr1 = np.linalg.norm(AtA(step) - Atb) / np.linalg.norm(Atb)
+    rr2 = np.linalg.norm(J(step) + b) / np.linalg.norm(b)
+    return x - step, s, float(rr1), float(rr2), step
 
 
 def line_search(x, step, obj_norm, f_norm, max_iters=20):

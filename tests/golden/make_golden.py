@@ -10,6 +10,9 @@
 2. residuals.json: the relative RPO residual |irfft2(f0 - g Phi_T f0)| / |f0|
    of each solution under the oracle's Lawson RK4 with dt = T / steps
    (ministeps x num_checkpoints as stored).
+3. dictio/Re40_EQ1.npz: one solution file with every entry of the reference's
+   dictionaryIO layout (input_keys, param_keys, forcing_str, ...), re-saved
+   entry by entry, for the load_dicts format test.
 """
 import json
 import os
@@ -46,6 +49,9 @@ def main():
         print(tag, n, r, flush=True)
     with open(os.path.join(HERE, "residuals.json"), "w") as fh:
         json.dump(res, fh, indent=1, sort_keys=True)
+    os.makedirs(os.path.join(HERE, "dictio"), exist_ok=True)
+    d = np.load(os.path.join(REF, "Re40/EQ1.npz"), allow_pickle=False)
+    np.savez(os.path.join(HERE, "dictio", "Re40_EQ1.npz"), **{k: np.asarray(d[k]) for k in d.files})
 
 
 if __name__ == "__main__":

@@ -423,3 +423,79 @@ def test_objective_jvp_fd_other_modes(mode):
     om, _ = O.objective_RPO({k: inp[k] - eps * tan[k] for k in inp}, pd, mode)
     fd = (op["fields"] - om["fields"]) / (2 * eps)
     assert rel(jv["fields"], fd) < 1e-7
+
+
+# ------------------------------------------------------------ hookstep Newton
+def _dense_problem(n=14, seed=3):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((n, n)) + 3 * np.eye(n)
+    c = rng.standard_normal(n)
+    x0 = rng.standard_normal(n)
+    f_vec = lambda x: A @ x - c          # objective with Jacobian A
+    return A, c, x0, f_vec
+
+
+def test_hookstep_v2_unconstrained_is_the_newton_step():
+    """Full Krylov space and a large trust region: dx solves A dx = f(x)."""
+    A, c, x0, f_vec = _dense_problem()
+    b = f_vec(x0)
+    xn, s, rr, dx = O.hookstep_v2(lambda v: A @ v, b, A.shape[0], 1e6, f_vec, x0)
+    assert rel(dx, np.linalg.solve(A, b)) < 1e-10
+    assert np.linalg.norm(f_vec(xn)) < 1e-9 * np.linalg.norm(b)
+    assert s == 2e6 and rr < 1e-10
+
+
+def test_hookstep_v2_trust_region_solution():
+    """When |y| > s^2 the step solves (H^T H + t I) y = H^T Q^T b with |y| = s
+    (the regularised least-squares optimality condition)."""
+    A, c, x0, f_vec = _dense_problem()
+    b = f_vec(x0)
+    m = 8
+    Q, H = O._arnoldi_reorth(lambda v: A @ v, b, m)
+    b2 = Q.T @ b
+    y_unc = np.linalg.lstsq(H, b2, rcond=None)[0]
+    s = 0.5 * np.sqrt(np.linalg.norm(y_unc))          # so |y_unc| > s^2
+    xn, s_out, rr, dx = O.hookstep_v2(lambda v: A @ v, b, m, s, f_vec, x0)
+    y = Q[:, :m].T @ dx
+    assert abs(np.linalg.norm(y) - s) < 1e-9 * s
+    t = (np.dot(H.T @ b2 - H.T @ (H @ y), y)) / np.dot(y, y)
+    assert t > 0
+    assert rel(H.T @ (H @ y) + t * y, H.T @ b2) < 1e-8
+    assert s_out == 2 * s and np.linalg.norm(f_vec(xn)) < np.linalg.norm(b)
+
+
+def test_hookstep_v1_normal_equations():
+    A, c, x0, f_vec = _dense_problem()
+    b = f_vec(x0)
+    AtA = lambda v: A.T @ (A @ v)
+    Atb = A.T @ b
+    n = A.shape[0]
+    xn, s, rr1, rr2, step = O.hookstep_v1(AtA, Atb, n, 1e6, f_vec, b, lambda v: A @ v, -b, x0)
+    assert rel(step, np.linalg.solve(A, b)) < 1e-8 and rr1 < 1e-8
+    # trust region active: the bisection bracket is [0, |b| / s] (linalg.py:188)
+    Q, Hf = O._arnoldi_reorth(AtA, Atb, 6)
+    D, V = np.linalg.eigh(Hf[:6, :6])
+    b2 = V.T @ (Q[:, :6].T @ Atb)
+    for s0 in (2.2, 0.3):               # bracketed, and not
+        upper = np.linalg.norm(b) / s0
+        xn, s2, rr1, rr2, step = O.hookstep_v1(AtA, Atb, 6, s0, f_vec, b, lambda v: A @ v, -b, x0)
+        assert s2 == 2 * s0
+        if s0 - np.linalg.norm(b2 / (D + upper)) > 0:      # root bracketed: |step| = s
+            assert abs(np.linalg.norm(step) - s0) < 1e-9 * s0
+        else:                                              # not bracketed: t runs to the upper end
+            assert rel(step, Q[:, :6] @ (V @ (b2 / (D + upper)))) < 1e-12
+
+
+def test_floquet_block_power_finds_the_dissipative_multiplier():
+    """Zero state, no forcing, b0 = 0: the map is the linear decay exp(L T),
+    exact under Lawson RK4, whose leading Floquet multiplier exp(-nu T) (the
+    |k| = 1 modes, 8 real directions over the two fields) the block finds."""
+    n = 16
+    nu = 0.5
+    pd = params(n, nu=nu, eta=nu, b0=(0.0, 0.0), amp=0.0)
+    pd.update({"num_checkpoints": 2, "ministeps": 4, "shift_reflect_ny": 0, "rot": False})
+    inp = {"fields": np.zeros((2, n, n)), "T": 2.0, "sx": 0.0}
+    rng = np.random.default_rng(1)
+    R, Q = O.floquet_block_power(inp, pd, rng.standard_normal((4, 2, n, n)), 40, "Lawson_RK4")
+    assert np.allclose(np.linalg.eigvals(R), np.exp(-nu * 2.0), rtol=1e-9)
+    assert np.allclose(Q.reshape(4, -1) @ Q.reshape(4, -1).T, np.eye(4), atol=1e-13)
