@@ -111,6 +111,42 @@ def newton_leg(inp, pd):
             "krylov_residual_last": log["krylov_residuals"][-1]}
 
 
+def guess_generation_leg():
+    """The reference's published guess-generation workload (generate_turbulence.py:22-151,
+    timings in benchmarking.m:11-59): from 10 N(0,1) random fields at N=128
+    (b0 = (0, 0.3)), a transient of 8192 Lawson-RK6 steps of dt = 1/256, then 256
+    snapshots 64 steps apart (16384 steps), then the 256 x 256 recurrence diagram."""
+    import torch
+    from ecs_mhd_2d_amd import mhd, turbulence
+    n = 128
+    pd = mhd.make_param_dict(n, 1 / 40, 1 / 40, (0.0, 0.3))
+    ctx = mhd.context(pd)
+    rng = np.random.default_rng(111111)
+    f = torch.as_tensor(10.0 * rng.standard_normal((2, n, n))).cuda()
+    f = (pd["mask"] * torch.fft.rfft2(f)).contiguous()
+    ctx.lawson_rk6(f, 1 / 256, 64, masked=True)      # warm-up (autotuner, workspace)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f = ctx.lawson_rk6(f, 1 / 256, 8192, masked=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    fs = torch.empty((256, 2, n, n // 2 + 1), dtype=torch.complex128, device="cuda")
+    for i in range(256):
+        f = ctx.lawson_rk6(f, (64 / 256) / 64, 64, masked=True)
+        fs[i] = f
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    dist = turbulence.recurrence_diagram(fs)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    assert torch.isfinite(dist).all()
+    return {"workload": "N=128: 8192 Lawson-RK6 transient steps (dt 1/256), 256 snapshots x 64 steps, "
+                        "256 x 256 recurrence diagram",
+            "transient_s": t1 - t0, "transient_steps_per_s": 8192 / (t1 - t0),
+            "turbulence_s": t2 - t1, "turbulence_steps_per_s": 16384 / (t2 - t1), "recurrence_s": t3 - t2,
+            "reference_published_h100_s": {"transient": 3.13, "turbulence": 5.67, "recurrence": 0.669}}
+
+
 def gradient_leg(reps=3):
     """Adjoint-descent gradient: loss_RPO value and gradient at N=256, T=4.3, 1088
     Lawson-RK4 steps (17 x 64), reverse pass with checkpoint/recompute every 64."""
@@ -319,6 +355,7 @@ def main():
     if not args.no_extras:
         extras["newton_iteration"] = newton_leg(inp, pd)
         extras["adjoint_gradient"] = gradient_leg()
+        extras["guess_generation"] = guess_generation_leg()
         extras["batched_candidates"] = batched_leg()
         extras["serving"] = serving_leg()
 

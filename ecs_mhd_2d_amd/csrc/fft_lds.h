@@ -8,9 +8,9 @@
 //   forward  X[k] = sum_n x[n] exp(-2 pi i n k / N)
 //   inverse  x[n] = sum_k X[k] exp(+2 pi i n k / N)
 // Element i of sequence s lives at buf[s * SSTRIDE + pidx(i)].
-// Twiddles come from an LDS copy of tw[j] = exp(-2 pi i j / N) (built on the
-// host from long-double sin/cos); radix-8/16 internal constants are the exact
-// cos/sin of multiples of pi/8 to double precision.
+// Twiddles come from per-pass LDS tables built from tw[j] = exp(-2 pi i j / N)
+// (computed on the host from long-double sin/cos); radix-8/16 internal
+// constants are the exact cos/sin of multiples of pi/8 to double precision.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -152,7 +152,22 @@ template <> struct Plan<64, 8> { static constexpr int R[3] = {4, 4, 4}; };
 template <> struct Plan<128, 8> { static constexpr int R[3] = {8, 4, 4}; };
 template <> struct Plan<256, 8> { static constexpr int R[3] = {8, 8, 4}; };
 
-template <int N, int R, int NS, int NSEQ, int SSTRIDE, int NT, bool INV>
+// Twiddles in LDS, one table per pass with NS > 1: entry (r - 1) * NS + k
+// holds exp(-2 pi i r k / (NS R)) = tw[r k N / (NS R)], the factor of input r
+// of a butterfly whose position in its sub-transform is k.  Lanes with
+// consecutive k then read consecutive words (a table indexed r k N/(NS R)
+// strides through the banks and conflicts 2-4 ways).  Pass 2 uses offset 0,
+// pass 3 offset (R1 - 1) R0; the tables fit in N entries for every plan.
+template <int N, int P>
+struct TwLayout {
+  static constexpr int R0 = Plan<N, P>::R[0], R1 = Plan<N, P>::R[1], R2 = Plan<N, P>::R[2];
+  static constexpr int OFF2 = 0;
+  static constexpr int OFF3 = (R1 > 1 ? (R1 - 1) * R0 : 0);
+  static constexpr int SIZE = OFF3 + (R2 > 1 ? (R2 - 1) * R0 * R1 : 0);
+  static_assert(SIZE <= N, "per-pass twiddle tables must fit the N-entry LDS table");
+};
+
+template <int N, int R, int NS, int NSEQ, int SSTRIDE, int NT, bool INV, int TWOFF>
 __device__ __forceinline__ void stockham_pass(double2* __restrict__ buf, const double2* __restrict__ twl) {
   constexpr int NB = NSEQ * (N / R);
   constexpr int BPT = (NB + NT - 1) / NT;
@@ -169,10 +184,9 @@ __device__ __forceinline__ void stockham_pass(double2* __restrict__ buf, const d
       for (int r = 0; r < R; ++r) v[q][r] = src[pidx(j + r * (N / R))];
       if (NS > 1) {
         const int k = j & (NS - 1);
-        constexpr int TSTEP = N / (NS * R);
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          double2 w = twl[(r * k * TSTEP) & (N - 1)];
+          double2 w = twl[TWOFF + (r - 1) * NS + k];
           if (INV) w.y = -w.y;
           v[q][r] = cmul(v[q][r], w);
         }
@@ -197,19 +211,35 @@ __device__ __forceinline__ void stockham_pass(double2* __restrict__ buf, const d
   __syncthreads();
 }
 
-// twl must hold the twiddle table in LDS (load_twiddles + a barrier).
+// twl must hold the plan's twiddle tables in LDS (load_twiddles<N, NT, P> + a barrier).
 template <int N, int NSEQ, int SSTRIDE, int NT, bool INV, int P = 16>
 __device__ __forceinline__ void fft_lds(double2* __restrict__ buf, const double2* __restrict__ twl) {
-  constexpr int R0 = Plan<N, P>::R[0], R1 = Plan<N, P>::R[1], R2 = Plan<N, P>::R[2];
+  using TL = TwLayout<N, P>;
+  constexpr int R0 = TL::R0, R1 = TL::R1, R2 = TL::R2;
   static_assert(R0 * R1 * R2 == N, "radix plan must factor N");
-  stockham_pass<N, R0, 1, NSEQ, SSTRIDE, NT, INV>(buf, twl);
-  if constexpr (R1 > 1) stockham_pass<N, R1, R0, NSEQ, SSTRIDE, NT, INV>(buf, twl);
-  if constexpr (R2 > 1) stockham_pass<N, R2, R0 * R1, NSEQ, SSTRIDE, NT, INV>(buf, twl);
+  stockham_pass<N, R0, 1, NSEQ, SSTRIDE, NT, INV, 0>(buf, twl);
+  if constexpr (R1 > 1) stockham_pass<N, R1, R0, NSEQ, SSTRIDE, NT, INV, TL::OFF2>(buf, twl);
+  if constexpr (R2 > 1) stockham_pass<N, R2, R0 * R1, NSEQ, SSTRIDE, NT, INV, TL::OFF3>(buf, twl);
 }
 
-template <int N, int NT>
+// Builds the per-pass twiddle tables of plan P from tw[j] = exp(-2 pi i j / N).
+template <int N, int NT, int P = 16>
 __device__ __forceinline__ void load_twiddles(double2* __restrict__ twl, const double2* __restrict__ tw) {
-  for (int i = threadIdx.x; i < N; i += NT) twl[i] = tw[i];
+  using TL = TwLayout<N, P>;
+  if constexpr (TL::R1 > 1) {
+    constexpr int NS = TL::R0, R = TL::R1, TSTEP = N / (NS * R);
+    for (int i = threadIdx.x; i < (R - 1) * NS; i += NT) {
+      const int r = i / NS + 1, k = i - (r - 1) * NS;
+      twl[TL::OFF2 + i] = tw[(r * k * TSTEP) & (N - 1)];
+    }
+  }
+  if constexpr (TL::R2 > 1) {
+    constexpr int NS = TL::R0 * TL::R1, R = TL::R2, TSTEP = N / (NS * R);
+    for (int i = threadIdx.x; i < (R - 1) * NS; i += NT) {
+      const int r = i / NS + 1, k = i - (r - 1) * NS;
+      twl[TL::OFF3 + i] = tw[(r * k * TSTEP) & (N - 1)];
+    }
+  }
 }
 
 }  // namespace ecs

@@ -343,7 +343,7 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   constexpr int SS = Seq<N>::S;
   __shared__ double2 sm[NSEQ * SS];
   __shared__ double2 twl[N];
-  load_twiddles<N, NT>(twl, a.tw);
+  load_twiddles<N, NT, P>(twl, a.tw);
   const int b = blockIdx.z, c = blockIdx.y, ky0 = blockIdx.x * TY, tid = threadIdx.x;
   const Blk bk = prologue<MODE, TAN>(a, b);
   if (!bk.run) return;
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
   constexpr int SS = Seq<N>::S;
   __shared__ double2 sm[TX * NP * SS];
   __shared__ double2 twl[N];
-  load_twiddles<N, NT>(twl, a.tw);
+  load_twiddles<N, NT, P>(twl, a.tw);
   const int b = blockIdx.z, xbase = blockIdx.x * TX, tid = threadIdx.x;
   if (!row_active(a, b)) return;
   const double2* I = a.inter + (size_t)b * ND * N * KC;
