@@ -159,6 +159,25 @@ class Context:
     def tdrk4_jvp(self, f, df, dt, ddt, steps: int):
         return self._fixed_jvp("ecs_tdrk4_jvp", f, df, dt, ddt, steps)
 
+    def _fixed_vjp(self, name, f0, fbar, dt, steps, ckpt, *extra):
+        f0 = _as_spec(f0, self.n)
+        g = _as_spec(fbar, self.n).clone()
+        B = _batch(f0)
+        hbar = (ctypes.c_double * B)()
+        args = [self.handle, f0.data_ptr(), g.data_ptr(), hbar, B, _lib.dptr(_per_batch(dt, B)), int(steps)]
+        _lib.check(getattr(self.lib, name)(*args, *extra, int(ckpt), _stream()), name)
+        return g, list(hbar)
+
+    def rk4_vjp(self, f0, fbar, dt, steps: int, ckpt: int = 64):
+        """(f0bar, hbar[B]) of classical RK4 for the cotangent fbar of the final state."""
+        return self._fixed_vjp("ecs_rk4_vjp", f0, fbar, dt, steps, ckpt)
+
+    def lawson_rk6_vjp(self, f0, fbar, dt, steps: int, masked: bool = True, ckpt: int = 64):
+        return self._fixed_vjp("ecs_lawson_rk6_vjp", f0, fbar, dt, steps, ckpt, int(bool(masked)))
+
+    def tdrk4_vjp(self, f0, fbar, dt, steps: int, ckpt: int = 64):
+        return self._fixed_vjp("ecs_tdrk4_vjp", f0, fbar, dt, steps, ckpt)
+
     # ------------------------------------------------------------ Lawson RK4(3)
     def lawson_rk43(self, x, T, h0: float, atol: float, max_steps_per_checkpoint: int, checkpoints: int):
         """Returns (x_T, infos) with one info dict per candidate (keys of timestepping.py:327-333

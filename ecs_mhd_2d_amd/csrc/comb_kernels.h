@@ -82,3 +82,69 @@ __global__ __launch_bounds__(256) void comb_kernel(Comb cb, int B) {
 }
 
 }  // namespace ecs
+
+namespace ecs {
+
+// Transpose of one stage combination, for the reverse pass:
+//   out_t += c_t h^a_t E^p_t yb          (each target once per call)
+//   hb    += sum_t d(c_t h^a_t E^p_t)/dh <yb, in_t>   (per mode, real inner product)
+// where in_t is the primal operand of term t (null: no d/dh term) and the
+// spectral cotangent convention holds d/dRe, d/dIm in the real, imag parts.
+struct CombAdj {
+  const double2* yb;
+  double2* out[COMB_MAX];
+  const double2* in[COMB_MAX];
+  double coef[COMB_MAX];
+  int hp[COMB_MAX];
+  int ep[COMB_MAX];
+  int nterm;
+  double ediv;
+  int masked;
+  const double* h;
+  double nu[2];
+  double* hb;                    // per-mode accumulator [B][2][R][KC] (null: no d/dh)
+  size_t elems;
+};
+
+template <int N>
+__global__ __launch_bounds__(256) void comb_adj_kernel(CombAdj cb, int B) {
+  using G = Geo<N>;
+  constexpr int KC = G::KC, R = G::R;
+  const size_t tot = (size_t)B * cb.elems;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+    const int b = (int)(i / cb.elems);
+    const size_t e = i - (size_t)b * cb.elems;
+    const int ky = (int)(e % KC);
+    const int r = (int)((e / KC) % R);
+    const int c = (int)(e / ((size_t)KC * R));
+    const int kx = G::row_kx(r);
+    const bool dc = (kx == 0 && ky == 0);
+    const double L = -(double)(kx * kx + ky * ky) * cb.nu[c];
+    const double h = cb.h[b];
+    const double E1 = exp(h / cb.ediv * L);
+    const double2 yb = cb.yb[i];
+    double hacc = 0.0;
+    for (int t = 0; t < cb.nterm; ++t) {
+      double w = cb.coef[t];
+      for (int q = 0; q < cb.hp[t]; ++q) w *= h;
+      double em = 1.0;
+      if (cb.ep[t] > 0) {
+        double Ep = E1;
+        for (int q = 1; q < cb.ep[t]; ++q) Ep *= E1;
+        em = (cb.masked && dc) ? 0.0 : Ep;
+        w *= em;
+      }
+      if (cb.out[t]) cb.out[t][i] = cadd(cb.out[t][i], cscale(w, yb));
+      if (cb.in[t] && cb.hb) {
+        double dw = cb.coef[t] * (double)cb.hp[t];
+        for (int q = 1; q < cb.hp[t]; ++q) dw *= h;
+        if (cb.ep[t] > 0) dw = dw * em + w * ((double)cb.ep[t] * L / cb.ediv);
+        const double2 v = cb.in[t][i];
+        hacc += dw * (yb.x * v.x + yb.y * v.y);
+      }
+    }
+    if (cb.hb) cb.hb[i] += hacc;
+  }
+}
+
+}  // namespace ecs

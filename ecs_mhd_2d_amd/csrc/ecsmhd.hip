@@ -363,6 +363,30 @@ int ecs_tdrk4_jvp(ecs_ctx* c, double* f, double* df, int batch, const double* dt
   return OPS(c->n)->gen_jvp(c, f, df, batch, dt, ddt, steps, 2, 0, (hipStream_t)stream);
 }
 
+int ecs_rk4_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar, int batch, const double* dt, int steps,
+                int ckpt, void* stream) {
+  if (!c || !f0 || !fbar || !hbar || !dt || batch < 1 || steps < 1 || ckpt < 1)
+    return fail(1, "ecs_rk4_vjp: bad arguments");
+  GUARD(c);
+  return OPS(c->n)->gen_vjp(c, f0, fbar, hbar, batch, dt, steps, 0, 0, ckpt, (hipStream_t)stream);
+}
+
+int ecs_lawson_rk6_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar, int batch, const double* dt,
+                       int steps, int masked, int ckpt, void* stream) {
+  if (!c || !f0 || !fbar || !hbar || !dt || batch < 1 || steps < 1 || ckpt < 1)
+    return fail(1, "ecs_lawson_rk6_vjp: bad arguments");
+  GUARD(c);
+  return OPS(c->n)->gen_vjp(c, f0, fbar, hbar, batch, dt, steps, 1, masked ? 1 : 0, ckpt, (hipStream_t)stream);
+}
+
+int ecs_tdrk4_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar, int batch, const double* dt, int steps,
+                  int ckpt, void* stream) {
+  if (!c || !f0 || !fbar || !hbar || !dt || batch < 1 || steps < 1 || ckpt < 1)
+    return fail(1, "ecs_tdrk4_vjp: bad arguments");
+  GUARD(c);
+  return OPS(c->n)->gen_vjp(c, f0, fbar, hbar, batch, dt, steps, 2, 0, ckpt, (hipStream_t)stream);
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------------------ Krylov vectors

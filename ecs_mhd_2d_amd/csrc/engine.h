@@ -944,6 +944,230 @@ static int run_gen(ecs_ctx* c, double* f, double* df, int B, const double* dt, c
   return 0;
 }
 
+// ------------------------------------------------------------------ generic steppers: reverse pass
+template <int N>
+static int comb_adj_launch(const CombAdj& cb, int B, hipStream_t st) {
+  const size_t tot = (size_t)B * cb.elems;
+  const int blocks = (int)std::min<size_t>((tot + 255) / 256, 4096);
+  hipLaunchKernelGGL((comb_adj_kernel<N>), dim3(blocks), dim3(256), 0, st, cb, B);
+  LAUNCHCHK();
+  return 0;
+}
+
+struct CombAdjBuilder {
+  CombAdj cb;
+  CombAdjBuilder(ecs_ctx* c, size_t elems, double ediv, int masked, const double2* yb, double* hb) {
+    memset(&cb, 0, sizeof(cb));
+    cb.ediv = ediv; cb.masked = masked; cb.h = c->scal;
+    cb.nu[0] = c->nu; cb.nu[1] = c->eta;
+    cb.yb = yb; cb.hb = hb; cb.elems = elems;
+  }
+  CombAdjBuilder& term(double2* out, const double2* in, double coef, int hp, int ep) {
+    const int t = cb.nterm++;
+    cb.out[t] = out; cb.in[t] = in; cb.coef[t] = coef; cb.hp[t] = hp; cb.ep[t] = ep;
+    return *this;
+  }
+};
+
+// out = J(x)^T g (with the RHS's dissipation when diss), on compact spectra;
+// hess: out = (J_0(x))^T g with J_0 the Jacobian of the nonlinearity without
+// b0 (the transpose of y -> H[y, x]).
+template <int N>
+static int vjp_pass(ecs_ctx* c, const Args& base, double2* x, const double2* g, double2* out, int B, int diss,
+                    hipStream_t st, bool hess = false) {
+  Args a = base;
+  a.x0 = x; a.gin = g; a.out = out;
+  a.include_diss = hess ? 0 : diss;
+  a.aout = c->AO; a.ain = c->AI; a.prim = c->I;
+  a.adaptive = AD_FIXED;
+  if (hess) { a.b0x = 0.0; a.b0y = 0.0; }
+  TRY((col<N, C_LOAD, false>(a, B, st)));
+  TRY((acol<N, B_G>(a, B, st)));
+  TRY(arow<N>(a, B, st));
+  TRY((acol<N, B_OUT>(a, B, st)));
+  return 0;
+}
+
+// Reverse pass of run_gen: f0 (full spectra, not modified), fbar_inout (on
+// entry the cotangent of the final spectra, on exit that of f0), hbar[b] =
+// d/dh.  The state is checkpointed every ckpt steps; each segment is
+// recomputed storing its step inputs, and each step's stages are recomputed
+// from its input before the stage algebra is transposed:
+//   Butcher form  x' = E^{p_S} x + h sum_j b_j E^{p_S-p_j} v_j,
+//                 y_s = E^{p_s} x + h sum_j A_sj E^{p_s-p_j} v_j,  v_s = f(y_s):
+//     vbar_j = h b_j E^{p_S-p_j} x'bar, then for s = S..1: ybar_s = J(y_s)^T vbar_s
+//     distributed to xbar and vbar_j (j < s) by the transposed combination;
+//   TDRK4 (a_j = h^2 w_j, w_j = J(z_j) v_j): wbar_j gives J(z_j)^T wbar_j to
+//     vbar_j and J_0(v_j)^T wbar_j to zbar_j.
+// d/dh of every weight is accumulated per mode and summed at the end.
+template <int N>
+static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar, int B, const double* dt, int steps,
+                       int scheme, int masked, int ckpt, hipStream_t st) {
+  const bool td = scheme == G_TDRK4;
+  TRY(ensure_adj(c, td ? 2 * B : B));
+  TRY(ensure_cap(c, td ? 2 * B : B));
+  TRY(autotune<N>(c, B, st));
+  if (td) TRY(autotune<N>(c, 2 * B, st));
+  TRY(ensure_adj(c, td ? 2 * B : B));   // the autotuner may have grown the batch workspace
+  const size_t ce = c->compact_elems();
+  const size_t cs = ce * B;
+  const Tableau& tb = tableau(scheme);
+  const int S = td ? 2 : tb.S;
+  const int nseg = (steps + ckpt - 1) / ckpt;
+  // units of cs: x, xb, xbn, yb, tmp, scr(2) | segment inputs [ckpt] | stages
+  const int stage_units = td ? 10 : 3 * S;
+  const int units = 7 + ckpt + stage_units;
+  TRY(grow(&c->gen, &c->gen_cap, (size_t)units * cs));
+  TRY(grow(&c->ckpt, &c->ckpt_cap, (size_t)nseg * cs));
+  TRY(upload_scalars(c, 0, dt, B, st));
+  Args a = base_args(c);
+  double2* g = c->gen;
+  auto slot = [&](int u) { return g + (size_t)u * cs; };
+  double2 *x = slot(0), *xb = slot(1), *xbn = slot(2), *yb = slot(3), *tmp = slot(4), *scr = slot(5);
+  auto xt = [&](int t) { return slot(7 + t); };
+  const int s0 = 7 + ckpt;
+  const int diss = scheme == G_LRK6 ? 0 : 1;
+  const int msk = scheme == G_LRK6 ? masked : 0;
+  const double ediv = tb.ediv;
+  // stage storage
+  auto y = [&](int k) { return slot(s0 + k); };           // RK: y_k (k >= 1); TD: y
+  auto v = [&](int k) { return slot(s0 + S + k); };       // RK: v_k;  TD: v1, v2
+  auto vb = [&](int k) { return slot(s0 + 2 * S + k); };  // RK: vbar_k; TD: vbar1, vbar2
+  // TDRK4 extra: w1, w2, wbar1, wbar2 at s0 + 6 .. s0 + 9
+  auto w = [&](int k) { return slot(s0 + 6 + k); };
+  auto wb = [&](int k) { return slot(s0 + 8 + k); };
+  const size_t bytes = cs * sizeof(double2);
+
+  // one step from xin (not modified) storing the stages; the step's output in xout
+  auto step_fwd = [&](double2* xin, double2* xout) -> int {
+    if (!td) {
+      for (int k = 0; k < tb.S; ++k) {
+        double2* in = xin;
+        if (k > 0) {
+          CombBuilder cbd(c, ce, ediv, msk, y(k), nullptr);
+          cbd.term(xin, nullptr, 1.0, 0, tb.p[k]);
+          for (int j = 0; j < k; ++j)
+            if (tb.A[k][j] != 0.0) cbd.term(v(j), nullptr, tb.A[k][j], 1, tb.p[k] - tb.p[j]);
+          TRY((comb_launch<N, false>(cbd.cb, B, st)));
+          in = y(k);
+        }
+        TRY((rhs_pass<N, false>(a, in, nullptr, v(k), nullptr, B, diss, st)));
+      }
+      if (xout) {
+        CombBuilder cbd(c, ce, ediv, msk, xout, nullptr);
+        cbd.term(xin, nullptr, 1.0, 0, tb.p[tb.S]);
+        for (int j = 0; j < tb.S; ++j)
+          if (tb.b[j] != 0.0) cbd.term(v(j), nullptr, tb.b[j], 1, tb.p[tb.S] - tb.p[j]);
+        TRY((comb_launch<N, false>(cbd.cb, B, st)));
+      }
+      return 0;
+    }
+    TRY((rhs_pass<N, false>(a, xin, nullptr, v(0), nullptr, B, diss, st)));
+    TRY((rhs_pass<N, true>(a, xin, v(0), scr, w(0), B, diss, st)));
+    CombBuilder cy(c, ce, 1.0, 0, y(0), nullptr);
+    cy.term(xin, nullptr, 1.0, 0, 0).term(v(0), nullptr, 0.5, 1, 0).term(w(0), nullptr, 0.125, 2, 0);
+    TRY((comb_launch<N, false>(cy.cb, B, st)));
+    TRY((rhs_pass<N, false>(a, y(0), nullptr, v(1), nullptr, B, diss, st)));
+    TRY((rhs_pass<N, true>(a, y(0), v(1), scr, w(1), B, diss, st)));
+    if (xout) {
+      CombBuilder cx(c, ce, 1.0, 0, xout, nullptr);
+      cx.term(xin, nullptr, 1.0, 0, 0).term(v(0), nullptr, 1.0, 1, 0).term(w(0), nullptr, 1.0 / 6, 2, 0)
+        .term(w(1), nullptr, 1.0 / 3, 2, 0);
+      TRY((comb_launch<N, false>(cx.cb, B, st)));
+    }
+    return 0;
+  };
+
+  // forward pass with checkpoints
+  TRY(pack<N>(f0, x, B, st));
+  for (int s = 0; s < steps; ++s) {
+    if (s % ckpt == 0) HIPCHK(hipMemcpyAsync(c->ckpt + (size_t)(s / ckpt) * cs, x, bytes, hipMemcpyDeviceToDevice, st));
+    TRY(step_fwd(x, tmp));
+    std::swap(x, tmp);
+  }
+  // reverse
+  TRY(pack<N>(fbar, xb, B, st));
+  HIPCHK(hipMemsetAsync(c->hb, 0, cs * sizeof(double), st));
+  for (int jseg = nseg - 1; jseg >= 0; --jseg) {
+    const int t0 = jseg * ckpt;
+    const int L = std::min(ckpt, steps - t0);
+    HIPCHK(hipMemcpyAsync(xt(0), c->ckpt + (size_t)jseg * cs, bytes, hipMemcpyDeviceToDevice, st));
+    for (int t = 0; t + 1 < L; ++t) TRY(step_fwd(xt(t), xt(t + 1)));
+    for (int t = L - 1; t >= 0; --t) {
+      double2* xs = xt(t);
+      TRY(step_fwd(xs, nullptr));
+      HIPCHK(hipMemsetAsync(xbn, 0, bytes, st));
+      if (!td) {
+        for (int k = 0; k < S; ++k) HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));
+        CombAdjBuilder fin(c, ce, ediv, msk, xb, c->hb);
+        fin.term(xbn, xs, 1.0, 0, tb.p[S]);
+        for (int j = 0; j < S; ++j)
+          if (tb.b[j] != 0.0) fin.term(vb(j), v(j), tb.b[j], 1, tb.p[S] - tb.p[j]);
+        TRY(comb_adj_launch<N>(fin.cb, B, st));
+        for (int k = S - 1; k >= 0; --k) {
+          TRY(vjp_pass<N>(c, a, k == 0 ? xs : y(k), vb(k), yb, B, diss, st));
+          CombAdjBuilder st_(c, ce, ediv, msk, yb, c->hb);
+          st_.term(xbn, k == 0 ? nullptr : xs, 1.0, 0, tb.p[k]);
+          for (int j = 0; j < k; ++j)
+            if (tb.A[k][j] != 0.0) st_.term(vb(j), v(j), tb.A[k][j], 1, tb.p[k] - tb.p[j]);
+          TRY(comb_adj_launch<N>(st_.cb, B, st));
+        }
+      } else {
+        for (int k = 0; k < 2; ++k) {
+          HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));
+          HIPCHK(hipMemsetAsync(wb(k), 0, bytes, st));
+        }
+        // x' = x + h v1 + h^2/6 w1 + h^2/3 w2
+        CombAdjBuilder fin(c, ce, 1.0, 0, xb, c->hb);
+        fin.term(xbn, nullptr, 1.0, 0, 0).term(vb(0), v(0), 1.0, 1, 0).term(wb(0), w(0), 1.0 / 6, 2, 0)
+          .term(wb(1), w(1), 1.0 / 3, 2, 0);
+        TRY(comb_adj_launch<N>(fin.cb, B, st));
+        // w2 = J(y) v2, v2 = f(y): vbar2 = J(y)^T wbar2; ybar = J(y)^T vbar2 + J_0(v2)^T wbar2
+        TRY(vjp_pass<N>(c, a, y(0), wb(1), vb(1), B, diss, st));
+        TRY(vjp_pass<N>(c, a, y(0), vb(1), yb, B, diss, st));
+        TRY(vjp_pass<N>(c, a, v(1), wb(1), scr, B, diss, st, true));
+        {
+          CombAdjBuilder acc(c, ce, 1.0, 0, scr, nullptr);
+          acc.term(yb, nullptr, 1.0, 0, 0);
+          TRY(comb_adj_launch<N>(acc.cb, B, st));
+        }
+        // y = x + h/2 v1 + h^2/8 w1
+        CombAdjBuilder cy(c, ce, 1.0, 0, yb, c->hb);
+        cy.term(xbn, nullptr, 1.0, 0, 0).term(vb(0), v(0), 0.5, 1, 0).term(wb(0), w(0), 0.125, 2, 0);
+        TRY(comb_adj_launch<N>(cy.cb, B, st));
+        // w1 = J(x) v1: vbar1 += J(x)^T wbar1, xbar += J_0(v1)^T wbar1
+        TRY(vjp_pass<N>(c, a, xs, wb(0), yb, B, diss, st));
+        {
+          CombAdjBuilder acc(c, ce, 1.0, 0, yb, nullptr);
+          acc.term(vb(0), nullptr, 1.0, 0, 0);
+          TRY(comb_adj_launch<N>(acc.cb, B, st));
+        }
+        TRY(vjp_pass<N>(c, a, v(0), wb(0), yb, B, diss, st, true));
+        {
+          CombAdjBuilder acc(c, ce, 1.0, 0, yb, nullptr);
+          acc.term(xbn, nullptr, 1.0, 0, 0);
+          TRY(comb_adj_launch<N>(acc.cb, B, st));
+        }
+        // v1 = f(x): xbar += J(x)^T vbar1
+        TRY(vjp_pass<N>(c, a, xs, vb(0), yb, B, diss, st));
+        {
+          CombAdjBuilder acc(c, ce, 1.0, 0, yb, nullptr);
+          acc.term(xbn, nullptr, 1.0, 0, 0);
+          TRY(comb_adj_launch<N>(acc.cb, B, st));
+        }
+      }
+      std::swap(xb, xbn);
+    }
+  }
+  TRY(unpack<N>(xb, fbar, B, msk, st));
+  hipLaunchKernelGGL((hb_sum_kernel<N>), dim3(B), dim3(256), 0, st, c->hb, c->red);
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(c->hred, c->red, B * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int b2 = 0; b2 < B; ++b2) hbar[b2] = c->hred[b2];
+  return 0;
+}
+
 template <int N> static int gen_p(ecs_ctx* c, double* f, int B, const double* dt, int steps, int scheme, int masked,
                                   hipStream_t st) {
   return run_gen<N, false>(c, f, nullptr, B, dt, nullptr, steps, scheme, masked, st);
@@ -958,6 +1182,6 @@ template <int N> static int gen_t(ecs_ctx* c, double* f, double* df, int B, cons
     static const GridOps ops = {&run_state_vel<NV>, &rk4_p<NV>,          &rk4_t<NV>,    &run_rk43<NV>,  \
                                 &run_rk43_jvp<NV>,  &run_state_vel_vjp<NV>, &run_rk4_ckpt<NV>,            \
                                 &run_rk4_rev<NV>,   &run_rk4_vjp<NV>,    &run_time_kernel<NV>,            \
-                                &gen_p<NV>,         &gen_t<NV>};                                          \
+                                &gen_p<NV>,         &gen_t<NV>,          &run_gen_vjp<NV>};               \
     return &ops;                                                                                         \
   }

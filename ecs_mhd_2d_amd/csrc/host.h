@@ -208,6 +208,7 @@ struct GridOps {
   int (*time_kernel)(ecs_ctx*, int, int, int, double*, hipStream_t);
   int (*gen)(ecs_ctx*, double*, int, const double*, int, int, int, hipStream_t);
   int (*gen_jvp)(ecs_ctx*, double*, double*, int, const double*, const double*, int, int, int, hipStream_t);
+  int (*gen_vjp)(ecs_ctx*, const double*, double*, double*, int, const double*, int, int, int, int, hipStream_t);
 };
 const GridOps* grid_ops(int n);
 const GridOps* grid_ops_16();

@@ -208,26 +208,50 @@ def _rk4_vjp_from(ctx, f0, fbar, h, steps, pd):
     return fw, g, hb[0]
 
 
+REVERSE_MODES = ("Lawson_RK4", "RK4", "TDRK4", "Lawson_RK6")
+
+
+def _integrate_vjp(ctx, f0, gbar, T, param_dict, mode):
+    """Reverse pass of mismatch_RPO's integration: (fT, f0bar, Tbar) for the
+    cotangent gbar of the integrated spectra (h = T / steps, so Tbar = hbar / steps)."""
+    if mode == "Lawson_RK4":
+        _, _, steps = _rk4_steps(param_dict)
+        fT, f0bar, hbar = _rk4_vjp_from(ctx, f0, gbar, _rk4_h(T, param_dict), steps, param_dict)
+        return fT, f0bar, hbar / steps
+    if mode in ("RK4", "TDRK4"):
+        _, _, steps = _rk4_steps(param_dict)
+        h = _rk4_h(T, param_dict)
+        if mode == "RK4":
+            fT = ctx.rk4(f0, h, steps)
+            f0bar, hb = ctx.rk4_vjp(f0, gbar, h, steps, CKPT)
+        else:
+            fT = ctx.tdrk4(f0, h, steps)
+            f0bar, hb = ctx.tdrk4_vjp(f0, gbar, h, steps, CKPT)
+        return fT, f0bar, hb[0] / steps
+    if mode == "Lawson_RK6":
+        steps = int(param_dict["steps"])
+        h = T / steps
+        fT = ctx.lawson_rk6(f0, h, steps, True)
+        f0bar, hb = ctx.lawson_rk6_vjp(f0, gbar, h, steps, True, CKPT)
+        return fT, f0bar, hb[0] / steps
+    raise NotImplementedError(f"the reverse pass is built for modes {REVERSE_MODES}, not {mode!r}")
+
+
 def objective_RPO_vjp(primal, cot, param_dict, mode="Lawson_RK4"):
-    """J(x)^T c for the cotangent dict c (newton.py:168-169), mode "Lawson_RK4"."""
-    if mode != "Lawson_RK4":
-        raise NotImplementedError("the reverse pass is built for mode 'Lawson_RK4'")
+    """J(x)^T c for the cotangent dict c (newton.py:168-169), modes of REVERSE_MODES."""
     ctx = mhd.context(param_dict)
     fields = _fields(primal)
     n = fields.shape[-1]
     T, sx = _f(primal["T"]), _f(primal["sx"])
-    nc, ms, steps = _rk4_steps(param_dict)
     f0 = _rfft2(fields)
     cbar = irfft2_T(_fields(cot), n)                 # cotangent of (f0 - g)
     # g = sym(fT): the symmetry transpose of -cbar does not need fT; sxbar does
-    fT, f0bar, hbar = _rk4_vjp_from(ctx, f0, _sym_T(-cbar, sx, param_dict), _rk4_h(T, param_dict), steps,
-                                    param_dict)
+    fT, f0bar, Tbar = _integrate_vjp(ctx, f0, _sym_T(-cbar, sx, param_dict), T, param_dict, mode)
     _, sxbar = symmetry.apply_vjp(fT, sx, param_dict, -cbar)
     vt, vx = phase_vectors(f0, param_dict)
     n2 = 2 * n * n
     gfields = rfft2_T(cbar + f0bar, n) + (_f(cot["T"]) * vt + _f(cot["sx"]) * vx) / n2
-    return {"fields": gfields, "T": torch.tensor(hbar / steps, dtype=torch.float64, device=fields.device),
-            "sx": sxbar}
+    return {"fields": gfields, "T": torch.tensor(Tbar, dtype=torch.float64, device=fields.device), "sx": sxbar}
 
 
 def _sym_T(gbar, sx, pd):
@@ -285,14 +309,45 @@ class RPOLoss(torch.autograd.Function):
         return gfields, Tbar, sxbar, None
 
 
+class RPOLossModes(torch.autograd.Function):
+    """loss_RPO in modes "RK4", "TDRK4" and "Lawson_RK6" with the hand-written
+    reverse pass of their integrators (checkpoint every 64 steps, stages
+    recomputed per step) as the backward."""
+
+    @staticmethod
+    def forward(ctx_, fields, T, sx, param_dict, mode):
+        n = fields.shape[-1]
+        f0 = _rfft2(fields.detach().to(torch.float64))
+        Tf, sxf = float(T), float(sx)
+        fT, _ = integrate(f0, Tf, param_dict, mode)
+        g = symmetry.apply(fT, sxf, param_dict)
+        diff = _irfft2(g - f0, n)
+        ctx_.save_for_backward(f0, fT, diff)
+        ctx_.pd, ctx_.mode, ctx_.T, ctx_.sx, ctx_.n = param_dict, mode, Tf, sxf, n
+        return torch.mean(diff * diff)
+
+    @staticmethod
+    def backward(ctx_, gl):
+        f0, fT, diff = ctx_.saved_tensors
+        pd, n = ctx_.pd, ctx_.n
+        dbar = irfft2_T(diff * (gl / (n * n)), n)         # cotangent of g - f0
+        fTbar, sxbar = symmetry.apply_vjp(fT, ctx_.sx, pd, dbar)
+        _, f0bar, Tbar = _integrate_vjp(mhd.context(pd), f0, fTbar.contiguous(), ctx_.T, pd, ctx_.mode)
+        gfields = rfft2_T(f0bar - dbar, n)
+        return gfields, torch.tensor(Tbar, dtype=torch.float64, device=diff.device), sxbar, None, None
+
+
 def loss_RPO_value_and_grad(input_dict, param_dict, mode="Lawson_RK4"):
-    """jax.value_and_grad(loss_RPO, has_aux=True) for mode "Lawson_RK4":
-    returns ((loss, None), grad_dict)  (adjoint_descent.py:90-96)."""
-    if mode != "Lawson_RK4":
-        raise NotImplementedError("the reverse pass is built for mode 'Lawson_RK4'")
+    """jax.value_and_grad(loss_RPO, has_aux=True): returns ((loss, None),
+    grad_dict)  (adjoint_descent.py:90-96), modes of REVERSE_MODES."""
+    if mode not in REVERSE_MODES:
+        raise NotImplementedError(f"the reverse pass is built for modes {REVERSE_MODES}, not {mode!r}")
     fields = _fields(input_dict).requires_grad_(True)
     T = torch.tensor(_f(input_dict["T"]), dtype=torch.float64, device=fields.device, requires_grad=True)
     sx = torch.tensor(_f(input_dict["sx"]), dtype=torch.float64, device=fields.device, requires_grad=True)
-    loss = RPOLoss.apply(fields, T, sx, param_dict)
+    if mode == "Lawson_RK4":
+        loss = RPOLoss.apply(fields, T, sx, param_dict)
+    else:
+        loss = RPOLossModes.apply(fields, T, sx, param_dict, mode)
     gf, gT, gs = torch.autograd.grad(loss, (fields, T, sx))
     return (loss.detach(), None), {"fields": gf, "T": gT, "sx": gs}

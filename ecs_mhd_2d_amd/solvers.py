@@ -34,8 +34,8 @@ def newton_gmres_update(input_dict, param_dict, mode, inner=64, outer=1, do_line
     A = lambda v: utils.flatten(lin.jvp(utils.unflatten(v, n)))
     precond = []
     if use_transpose:
-        if mode != "Lawson_RK4":
-            raise NotImplementedError("use_transpose needs the reverse pass (mode 'Lawson_RK4')")
+        if mode not in loss_functions.REVERSE_MODES:
+            raise NotImplementedError(f"use_transpose needs the reverse pass (modes {loss_functions.REVERSE_MODES})")
         A_T = lambda v: utils.flatten(loss_functions.objective_RPO_vjp(input_dict, utils.unflatten(v, n),
                                                                         param_dict, mode))
         precond = [A_T]
@@ -88,7 +88,7 @@ class AdjointDescent:
     """ADAM on the loss_RPO gradient (adjoint_descent.py:79-123): lr 1e-2,
     betas (0.9, 0.999), eps 1e-6, fields dealiased after every step."""
 
-    def __init__(self, input_dict, param_dict, lr=1e-2, beta1=0.9, beta2=0.999, eps=1e-6):
+    def __init__(self, input_dict, param_dict, lr=1e-2, beta1=0.9, beta2=0.999, eps=1e-6, mode="Lawson_RK4"):
         self.params = {"fields": loss_functions._fields(input_dict),
                        "T": torch.as_tensor(input_dict["T"], dtype=torch.float64, device="cuda").reshape(()),
                        "sx": torch.as_tensor(input_dict["sx"], dtype=torch.float64, device="cuda").reshape(())}
@@ -96,9 +96,10 @@ class AdjointDescent:
         self.m, self.v = adam.init_adam(self.params)
         self.t = 0
         self.hyper = (lr, beta1, beta2, eps)
+        self.mode = mode   # adjoint_descent.py:49 uses "Lawson_RK4"
 
     def step(self):
-        (loss, _), grad = loss_functions.loss_RPO_value_and_grad(self.params, self.pd, "Lawson_RK4")
+        (loss, _), grad = loss_functions.loss_RPO_value_and_grad(self.params, self.pd, self.mode)
         lr, b1, b2, eps = self.hyper
         self.params, self.m, self.v = adam.adam_update(self.params, grad, self.m, self.v, self.t + 1, lr=lr,
                                                        beta1=b1, beta2=b2, eps=eps)

@@ -141,6 +141,20 @@ int ecs_tdrk4(ecs_ctx* ctx, double* f, int batch, const double* dt, int steps, v
 int ecs_tdrk4_jvp(ecs_ctx* ctx, double* f, double* df, int batch, const double* dt, const double* ddt,
                   int steps, void* stream);
 
+/* Reverse-mode derivatives of the three integrators above (what jax.vjp of
+ * mismatch_RPO's integration gives in modes "RK4", "Lawson_RK6", "TDRK4"),
+ * with the conventions of ecs_lawson_rk4_vjp: f0 not modified, fbar_inout
+ * the cotangent of the final spectra on entry and of f0 on exit, hbar host
+ * [batch] = d/dh.  The state is checkpointed every ckpt steps and each step's
+ * stages are recomputed before being transposed; TDRK4's transpose carries
+ * the second derivative of the RHS. */
+int ecs_rk4_vjp(ecs_ctx* ctx, const double* f0, double* fbar_inout, double* hbar, int batch, const double* dt,
+                int steps, int ckpt, void* stream);
+int ecs_lawson_rk6_vjp(ecs_ctx* ctx, const double* f0, double* fbar_inout, double* hbar, int batch,
+                       const double* dt, int steps, int masked, int ckpt, void* stream);
+int ecs_tdrk4_vjp(ecs_ctx* ctx, const double* f0, double* fbar_inout, double* hbar, int batch, const double* dt,
+                  int steps, int ckpt, void* stream);
+
 /* GMRES vector kernels (linalg.py:58-97) on flat FP64 vectors of length n.
  * ecs_mgs: one modified-Gram-Schmidt sweep of Arnoldi step k (linalg.py:70-85):
  * for j = 0..k: h[j] = <w, Q[:, j]>, w -= h[j] Q[:, j]; then h[k+1] = |w| and

@@ -502,6 +502,28 @@ def rk4_jvp(f, df, t, dt_t, steps, pd):
     return f, df
 
 
+def dense_vjp(jvp, shape, cbar):
+    """Transpose of a real-linear tangent map by brute force on tiny inputs:
+    jvp(df, dt) -> dout for spectra of `shape`.  Builds the Jacobian column by
+    column (real and imaginary unit perturbations of every stored entry, then
+    the scalar t) and returns (J^T cbar restricted to the state, and to t)
+    in the real inner product sum Re(conj(a) b) on stored entries."""
+    m = int(np.prod(shape))
+    cols = []
+    zero = np.zeros(shape, dtype=complex)
+    for k in range(m):
+        for unit in (1.0, 1j):
+            d = zero.copy().reshape(-1)
+            d[k] = unit
+            cols.append(jvp(d.reshape(shape), 0.0))
+    tcol = jvp(zero, 1.0)
+    fbar = np.zeros(m, dtype=complex)
+    cr = cbar.reshape(-1)
+    for k in range(m):
+        fbar[k] = rdot(cr, cols[2 * k].reshape(-1)) + 1j * rdot(cr, cols[2 * k + 1].reshape(-1))
+    return fbar.reshape(shape), rdot(cr, tcol.reshape(-1))
+
+
 # Lawson RK6 of timestepping.py:83-125, written as data: stage s (0-based)
 # evaluates h v(f + sum_j RK6_A[s][j] k_j) after the integrating factor
 # e = exp(h/6 L) [* mask] has been applied to f and all k_j whenever

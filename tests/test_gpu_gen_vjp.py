@@ -1,0 +1,114 @@
+"""Reverse pass of classical RK4, Lawson RK6 and two-derivative RK4 on the
+GPU: against the brute-force transpose of the oracle's tangent (N=16), and
+the dot test <c, J d> = <J^T c, d> + hbar dh against the GPU tangent (N=64,
+several checkpoint segments, batch of two with their own steps)."""
+import numpy as np
+import pytest
+import torch
+
+from ecs_mhd_2d_amd import _runtime
+from ecs_mhd_2d_amd.synthetic import spectrum
+from oracle import mhd_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(n):
+    pd = O.make_params(n, 1 / 40, 1 / 40)
+    return pd, _runtime.Context(n, 1 / 40, 1 / 40, (0.0, 0.1), pd["forcing"])
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+ORACLE_JVP = {
+    "rk4": lambda f, d, T, dT, s, pd: O.rk4_jvp(f, d, T, dT, s, pd),
+    "lawson_rk6": lambda f, d, T, dT, s, pd: O.lawson_rk6_jvp(f, d, T, dT, s, pd, True),
+    "tdrk4": lambda f, d, T, dT, s, pd: O.tdrk4_jvp(f, d, T, dT, s, pd),
+}
+
+
+@pytest.mark.parametrize("scheme", ["rk4", "lawson_rk6", "tdrk4"])
+def test_reverse_pass_is_the_exact_transpose(scheme):
+    print(f"{scheme}_vjp dense n=16", flush=True)
+    n, steps, h = 16, 3, 0.01
+    pd, ctx = setup(n)
+    f, c = spectrum(n, 11), spectrum(n, 12)
+    gb, hb = getattr(ctx, scheme + "_vjp")(torch.as_tensor(f).cuda(), torch.as_tensor(c).cuda(), h, steps, ckpt=2)
+    jv = lambda d, dt: ORACLE_JVP[scheme](f, d, h * steps, dt, steps, pd)[1]
+    rf, rt = O.dense_vjp(jv, f.shape, c)
+    assert rel(gb.cpu().numpy(), rf) < 1e-10
+    # hbar = d/dh = steps * d/dT
+    assert abs(hb[0] - rt * steps) < 1e-9 * max(1.0, abs(rt * steps))
+
+
+@pytest.mark.parametrize("scheme", ["rk4", "lawson_rk6", "tdrk4"])
+def test_dot_test_against_gpu_tangent(scheme):
+    print(f"{scheme}_vjp dot n=64", flush=True)
+    n, steps = 64, 5
+    pd, ctx = setup(n)
+    f = torch.as_tensor(np.stack([spectrum(n, 21), spectrum(n, 22)])).cuda()
+    d = torch.as_tensor(np.stack([spectrum(n, 23), spectrum(n, 24)])).cuda()
+    c = torch.as_tensor(np.stack([spectrum(n, 25), spectrum(n, 26)])).cuda()
+    dts, ddts = [0.004, 0.005], [0.3, -0.2]
+    _, jd = getattr(ctx, scheme + "_jvp")(f, d, dts, ddts, steps)
+    gb, hb = getattr(ctx, scheme + "_vjp")(f, c, dts, steps, ckpt=2)
+    for b in range(2):
+        lhs = O.rdot(c[b].cpu().numpy(), jd[b].cpu().numpy())
+        rhs = O.rdot(gb[b].cpu().numpy(), d[b].cpu().numpy()) + hb[b] * ddts[b]
+        assert abs(lhs - rhs) < 1e-10 * abs(lhs), (b, lhs, rhs)
+
+
+@pytest.mark.parametrize("mode", ["RK4", "TDRK4", "Lawson_RK6"])
+def test_objective_vjp_and_gradient_in_other_modes(mode):
+    """objective_RPO_vjp and loss_RPO_value_and_grad in the other modes: dot
+    test against the objective JVP, and the gradient against the oracle's
+    central differences of the loss."""
+    from ecs_mhd_2d_amd import loss_functions as LF, mhd, utils
+    print(f"objective vjp {mode}", flush=True)
+    n = 16
+    extra = {"num_checkpoints": 2, "ministeps": 3, "steps": 6, "shift_reflect_ny": 3, "rot": True}
+    pd = mhd.make_param_dict(n, 1 / 40, 1 / 40)
+    pd.update(extra)
+    opd = O.make_params(n, 1 / 40, 1 / 40)
+    opd.update(extra)
+    rng = np.random.default_rng(3)
+    f = np.fft.irfft2(spectrum(n, 5))
+    inp = {"fields": torch.as_tensor(f).cuda(), "T": 0.4, "sx": 0.2}
+    u = torch.as_tensor(rng.standard_normal(2 * n * n + 2)).cuda()
+    v = torch.as_tensor(rng.standard_normal(2 * n * n + 2)).cuda()
+    Jv = utils.flatten(LF.objective_RPO_jvp(inp, utils.unflatten(v, n), pd, mode))
+    JTu = utils.flatten(LF.objective_RPO_vjp(inp, utils.unflatten(u, n), pd, mode))
+    lhs, rhs = float(torch.dot(u, Jv)), float(torch.dot(JTu, v))
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+    (loss, _), grad = LF.loss_RPO_value_and_grad(inp, pd, mode)
+    oinp = {"fields": f, "T": 0.4, "sx": 0.2}
+    d = {"fields": rng.standard_normal((2, n, n)), "T": 0.3, "sx": -0.2}
+    eps = 1e-6
+    lp = O.loss_RPO({k: oinp[k] + eps * d[k] for k in oinp}, opd, mode)[0]
+    lm = O.loss_RPO({k: oinp[k] - eps * d[k] for k in oinp}, opd, mode)[0]
+    g = utils.flatten(grad).cpu().numpy()
+    an = float(np.dot(g, O.flatten(d)))
+    assert abs(float(loss) - O.loss_RPO(oinp, opd, mode)[0]) < 1e-12 * float(loss)
+    assert abs((lp - lm) / (2 * eps) - an) < 1e-6 * max(1.0, abs(an))
+
+
+def test_newton_with_transpose_in_tdrk4_mode():
+    """The reference's default Newton setting (newton.py:70,84,87: mode TDRK4,
+    use_transpose=True, inner 16) reduces the residual of a perturbed orbit.
+    GMRES on the normal equations A^T A squares the condition number, so 16
+    Krylov vectors buy less than plain GMRES does (about 30% here, against
+    >2x for the plain form in test_gpu_api); the reverse pass itself is pinned
+    by the dot tests above."""
+    from ecs_mhd_2d_amd import loss_functions as LF, solvers
+    from tests.test_gpu_api import load_solution
+    inp, pd = load_solution("Re40_RPO6")
+    gen = torch.Generator(device="cuda").manual_seed(2)
+    pert = 1e-4 * torch.randn(inp["fields"].shape, generator=gen, device="cuda", dtype=torch.float64)
+    x = {"fields": solvers.dealias(inp["fields"] + pert, pd), "T": inp["T"], "sx": inp["sx"]}
+    r0 = float(torch.linalg.vector_norm(LF.objective_RPO(x, pd, "TDRK4")["fields"]))
+    x1, log = solvers.newton_gmres_update(x, pd, "TDRK4", inner=16, use_transpose=True)
+    r1 = float(torch.linalg.vector_norm(LF.objective_RPO(x1, pd, "TDRK4")["fields"]))
+    print("newton TDRK4 transpose", r0, r1, log["gmres_rel_res"], log["damp"], flush=True)
+    assert r1 < 0.8 * r0 and log["gmres_rel_res"] < 0.05

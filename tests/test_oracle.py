@@ -499,3 +499,15 @@ def test_floquet_block_power_finds_the_dissipative_multiplier():
     R, Q = O.floquet_block_power(inp, pd, rng.standard_normal((4, 2, n, n)), 40, "Lawson_RK4")
     assert np.allclose(np.linalg.eigvals(R), np.exp(-nu * 2.0), rtol=1e-9)
     assert np.allclose(Q.reshape(4, -1) @ Q.reshape(4, -1).T, np.eye(4), atol=1e-13)
+
+
+def test_dense_vjp_reproduces_the_lawson_rk4_adjoint():
+    """The brute-force transpose (used to pin the GPU reverse pass of the other
+    integrators) agrees with the oracle's hand-derived Lawson-RK4 adjoint."""
+    n = 8
+    pd = params(n)
+    f, c = spectrum(n, 2), spectrum(n, 3)
+    T, steps = 0.05, 2
+    fb, tb = O.dense_vjp(lambda d, dt: O.lawson_rk4_jvp(f, d, T, dt, steps, pd, True)[1], f.shape, c)
+    rf, rt = O.lawson_rk4_vjp(f, T, steps, pd, c, True)
+    assert rel(fb, rf) < 1e-12 and abs(tb - rt) < 1e-12 * abs(rt)
