@@ -56,6 +56,8 @@ _SIGS = {
                                    c_void_p]),
     "ecs_tdrk4": (c_int, [c_void_p, c_void_p, c_int, P_double, c_int, c_void_p]),
     "ecs_tdrk4_jvp": (c_int, [c_void_p, c_void_p, c_void_p, c_int, P_double, P_double, c_int, c_void_p]),
+    "ecs_lawson_rk43_vjp": (c_int, [c_void_p, c_void_p, c_void_p, P_double, c_int, P_double, P_double, P_int8,
+                                    P_int, c_int, c_int, c_void_p]),
     "ecs_rk4_vjp": (c_int, [c_void_p, c_void_p, c_void_p, P_double, c_int, P_double, c_int, c_int, c_void_p]),
     "ecs_lawson_rk6_vjp": (c_int, [c_void_p, c_void_p, c_void_p, P_double, c_int, P_double, c_int, c_int, c_int,
                                    c_void_p]),

@@ -214,6 +214,23 @@ class Context:
         return x, dx
 
 
+    def lawson_rk43_vjp(self, x0, xbar, T, infos, ckpt: int = 64):
+        """(x0bar, Tbar[B]) along the schedules ``infos`` of a lawson_rk43 call."""
+        x0 = _as_spec(x0, self.n)
+        g = _as_spec(xbar, self.n).clone()
+        B = _batch(x0)
+        budget = len(infos[0]["hs"])
+        hs = np.ascontiguousarray(np.stack([i["hs"] for i in infos]).astype(np.float64))
+        acc = np.ascontiguousarray(np.stack([i["acc"] for i in infos]).astype(np.int8))
+        att = np.ascontiguousarray(np.array([i["attempts"] for i in infos], dtype=np.int32))
+        Tbar = (ctypes.c_double * B)()
+        _lib.check(self.lib.ecs_lawson_rk43_vjp(self.handle, x0.data_ptr(), g.data_ptr(), Tbar, B,
+                                                _lib.dptr(_per_batch(T, B)), hs.ctypes.data_as(_lib.P_double),
+                                                acc.ctypes.data_as(_lib.P_int8), att.ctypes.data_as(_lib.P_int),
+                                                budget, int(ckpt), _stream()), "ecs_lawson_rk43_vjp")
+        return g, list(Tbar)
+
+
 _CACHE: dict = {}
 
 

@@ -103,6 +103,7 @@ struct CombAdj {
   const double* h;
   double nu[2];
   double* hb;                    // per-mode accumulator [B][2][R][KC] (null: no d/dh)
+  const double* hscale;          // [B] multiplier of the d/dh terms (null: 1), e.g. dh/dT
   size_t elems;
 };
 
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(256) void comb_adj_kernel(CombAdj cb, int B) {
         hacc += dw * (yb.x * v.x + yb.y * v.y);
       }
     }
-    if (cb.hb) cb.hb[i] += hacc;
+    if (cb.hb) cb.hb[i] += cb.hscale ? cb.hscale[b] * hacc : hacc;
   }
 }
 

@@ -250,7 +250,7 @@ int ecs_ctx_destroy(ecs_ctx* c) {
   void* ps[] = {c->tw, c->fhat, c->I, c->O, c->x0, c->acc, c->v1, c->v4, c->v5, c->xf, c->out, c->dx0, c->dacc,
                 c->dv1, c->dv4, c->dxf, c->dout, c->scal, c->hist, c->s_h, c->s_rs, c->s_rc, c->s_rh, c->s_nacc,
                 c->ckpt, c->xtape, c->itape, c->red, c->AO, c->AI, c->xb, c->k1b, c->k2b, c->k3b, c->hb,
-                c->gen};
+                c->gen, c->tab};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   drop_graphs(c);
@@ -385,6 +385,17 @@ int ecs_tdrk4_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar, int 
     return fail(1, "ecs_tdrk4_vjp: bad arguments");
   GUARD(c);
   return OPS(c->n)->gen_vjp(c, f0, fbar, hbar, batch, dt, steps, 2, 0, ckpt, (hipStream_t)stream);
+}
+
+int ecs_lawson_rk43_vjp(ecs_ctx* c, const double* x0, double* xbar, double* Tbar, int batch, const double* T,
+                        const double* hs, const int8_t* acc, const int* attempts, int budget, int ckpt,
+                        void* stream) {
+  if (!c || !x0 || !xbar || !Tbar || !T || !hs || !acc || !attempts || batch < 1 || budget < 1 || ckpt < 1)
+    return fail(1, "ecs_lawson_rk43_vjp: bad arguments");
+  for (int b = 0; b < batch; ++b)
+    if (attempts[b] < 0 || attempts[b] > budget) return fail(1, "ecs_lawson_rk43_vjp: attempts out of range");
+  GUARD(c);
+  return OPS(c->n)->rk43_vjp(c, x0, xbar, Tbar, batch, T, hs, acc, attempts, budget, ckpt, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -757,7 +757,7 @@ static int run_time_kernel(ecs_ctx* c, int B, int which, int reps, double* ms, h
 // Classical RK4 (timestepping.py:17-44), Lawson RK6 (:83-125) and two-derivative
 // RK4 (:127-149), run as RHS passes (column load, row pass, column output)
 // with the stage algebra in comb_kernel between them.
-enum GenScheme : int { G_RK4 = 0, G_LRK6 = 1, G_TDRK4 = 2 };
+enum GenScheme : int { G_RK4 = 0, G_LRK6 = 1, G_TDRK4 = 2, G_LRK4 = 3 };
 
 // Butcher form of a (Lawson) scheme: stage s reads
 //   x0 E^{p_s} + h sum_j A[s][j] E^{p_s - p_j} v_j,   v_j = RHS(stage j input),
@@ -790,7 +790,13 @@ static const Tableau& tableau(int scheme) {
                                {13.0 / 200, 0, 4.0 / 25, 11.0 / 40, 0, 11.0 / 40, 4.0 / 25, 13.0 / 200},
                                {0, 1, 1, 2, 3, 4, 5, 6, 6},
                                6.0};
-  return scheme == G_LRK6 ? lrk6 : rk4;
+  // Lawson RK4 (timestepping.py:46-81): e = exp(h/2 L) after stages 1 and 3
+  static const Tableau lrk4 = {4,
+                               {{0}, {0.5}, {0, 0.5}, {0, 0, 1.0}},
+                               {1.0 / 6, 1.0 / 3, 1.0 / 3, 1.0 / 6},
+                               {0, 1, 1, 2, 2},
+                               2.0};
+  return scheme == G_LRK6 ? lrk6 : (scheme == G_LRK4 ? lrk4 : rk4);
 }
 
 template <int N, bool TAN>
@@ -1168,6 +1174,188 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   return 0;
 }
 
+// Reverse pass of the adaptive Lawson RK4(3) along a recorded schedule.  h is
+// stop-gradient (timestepping.py:308), so the trajectory is a sequence of
+// Lawson-RK4 steps of dt_m = T h_m with unmasked e = exp(dt_m/2 L), except that
+// the first stage's v1 = v(x) is multiplied per mode by P_m = exp(T s_m L),
+// s_m the sum of the steps of the rejections before accepted step m (each
+// rejection turns the carried k1 into e e k1, timestepping.py:267,276,297).
+// Candidates with fewer accepted steps take zero-length steps (dt = 0: the
+// identity, with zero derivative).  d/dT is accumulated per mode with the
+// weights dt_m/dT = h_m and d(T s_m)/dT = s_m.  Returns Tbar[b] on the host.
+template <int N>
+static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tbar, int B, const double* T,
+                        const double* hs, const int8_t* accf, const int* attempts, int budget, int ckpt,
+                        hipStream_t st) {
+  TRY(ensure_adj(c, B));
+  TRY(autotune<N>(c, B, st));
+  TRY(ensure_adj(c, B));
+  // accepted-step tables [m][b]: dt, T s, h, s
+  std::vector<int> M(B, 0);
+  std::vector<std::vector<double>> hh(B), ss(B);
+  int Mmax = 0;
+  for (int b = 0; b < B; ++b) {
+    double sig = 0.0;
+    for (int i = 0; i < attempts[b]; ++i) {
+      const double h = hs[(size_t)b * budget + i];
+      if (accf[(size_t)b * budget + i]) {
+        hh[b].push_back(h);
+        ss[b].push_back(sig);
+        sig = 0.0;
+      } else {
+        sig += h;
+      }
+    }
+    M[b] = (int)hh[b].size();
+    Mmax = std::max(Mmax, M[b]);
+  }
+  const int steps = std::max(Mmax, 1);
+  std::vector<double> tab((size_t)4 * steps * B, 0.0);
+  for (int m = 0; m < steps; ++m)
+    for (int b = 0; b < B; ++b) {
+      const bool on = m < M[b];
+      tab[((size_t)0 * steps + m) * B + b] = on ? T[b] * hh[b][m] : 0.0;
+      tab[((size_t)1 * steps + m) * B + b] = on ? T[b] * ss[b][m] : 0.0;
+      tab[((size_t)2 * steps + m) * B + b] = on ? hh[b][m] : 0.0;
+      tab[((size_t)3 * steps + m) * B + b] = on ? ss[b][m] : 0.0;
+    }
+  {
+    std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
+    size_t cap = c->tab_cap;
+    if (tab.size() > cap) {
+      if (c->tab) HIPCHK(hipFree(c->tab));
+      c->tab = nullptr;
+      HIPCHK(hipMalloc((void**)&c->tab, tab.size() * sizeof(double)));
+      c->tab_cap = tab.size();
+    }
+  }
+  HIPCHK(hipMemcpyAsync(c->tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));   // the host table goes out of scope
+  auto DT = [&](int m) { return (const double*)c->tab + ((size_t)0 * steps + m) * B; };
+  auto PDT = [&](int m) { return (const double*)c->tab + ((size_t)1 * steps + m) * B; };
+  auto HSC = [&](int m) { return (const double*)c->tab + ((size_t)2 * steps + m) * B; };
+  auto SIG = [&](int m) { return (const double*)c->tab + ((size_t)3 * steps + m) * B; };
+
+  const size_t ce = c->compact_elems();
+  const size_t cs = ce * B;
+  const Tableau& tb = tableau(G_LRK4);
+  const int S = tb.S;
+  const int nseg = (steps + ckpt - 1) / ckpt;
+  // units: x, xb, xbn, yb, tmp, vraw, vbraw | segment inputs [ckpt] | y[S], v[S], vbar[S]
+  const int units = 7 + ckpt + 3 * S;
+  TRY(grow(&c->gen, &c->gen_cap, (size_t)units * cs));
+  TRY(grow(&c->ckpt, &c->ckpt_cap, (size_t)nseg * cs));
+  Args a = base_args(c);
+  double2* g = c->gen;
+  auto slot = [&](int u) { return g + (size_t)u * cs; };
+  double2 *x = slot(0), *xb = slot(1), *xbn = slot(2), *yb = slot(3), *tmp = slot(4), *vraw = slot(5);
+  double2* vbraw = slot(6);
+  auto xt = [&](int t) { return slot(7 + t); };
+  const int s0 = 7 + ckpt;
+  auto y = [&](int k) { return slot(s0 + k); };
+  auto v = [&](int k) { return slot(s0 + S + k); };
+  auto vb = [&](int k) { return slot(s0 + 2 * S + k); };
+  const size_t bytes = cs * sizeof(double2);
+  const double ediv = tb.ediv;
+
+  auto step_fwd = [&](int m, double2* xin, double2* xout) -> int {
+    Args am = a;
+    am.dt = DT(m);
+    for (int k = 0; k < S; ++k) {
+      double2* in = xin;
+      if (k > 0) {
+        CombBuilder cbd(c, ce, ediv, 0, y(k), nullptr);
+        cbd.cb.h = DT(m);
+        cbd.term(xin, nullptr, 1.0, 0, tb.p[k]);
+        for (int j = 0; j < k; ++j)
+          if (tb.A[k][j] != 0.0) cbd.term(v(j), nullptr, tb.A[k][j], 1, tb.p[k] - tb.p[j]);
+        TRY((comb_launch<N, false>(cbd.cb, B, st)));
+        in = y(k);
+      }
+      if (k == 0) {
+        TRY((rhs_pass<N, false>(am, in, nullptr, vraw, nullptr, B, 0, st)));
+        CombBuilder pc(c, ce, 1.0, 0, v(0), nullptr);   // v1 = P vraw, P = exp(T s L)
+        pc.cb.h = PDT(m);
+        pc.term(vraw, nullptr, 1.0, 0, 1);
+        TRY((comb_launch<N, false>(pc.cb, B, st)));
+      } else {
+        TRY((rhs_pass<N, false>(am, in, nullptr, v(k), nullptr, B, 0, st)));
+      }
+    }
+    if (xout) {
+      CombBuilder cbd(c, ce, ediv, 0, xout, nullptr);
+      cbd.cb.h = DT(m);
+      cbd.term(xin, nullptr, 1.0, 0, tb.p[S]);
+      for (int j = 0; j < S; ++j)
+        if (tb.b[j] != 0.0) cbd.term(v(j), nullptr, tb.b[j], 1, tb.p[S] - tb.p[j]);
+      TRY((comb_launch<N, false>(cbd.cb, B, st)));
+    }
+    return 0;
+  };
+
+  TRY(pack<N>(x0f, x, B, st));
+  for (int m = 0; m < steps; ++m) {
+    if (m % ckpt == 0) HIPCHK(hipMemcpyAsync(c->ckpt + (size_t)(m / ckpt) * cs, x, bytes, hipMemcpyDeviceToDevice, st));
+    TRY(step_fwd(m, x, tmp));
+    std::swap(x, tmp);
+  }
+  TRY(pack<N>(xbar, xb, B, st));
+  HIPCHK(hipMemsetAsync(c->hb, 0, cs * sizeof(double), st));
+  for (int jseg = nseg - 1; jseg >= 0; --jseg) {
+    const int t0 = jseg * ckpt;
+    const int L = std::min(ckpt, steps - t0);
+    HIPCHK(hipMemcpyAsync(xt(0), c->ckpt + (size_t)jseg * cs, bytes, hipMemcpyDeviceToDevice, st));
+    for (int t = 0; t + 1 < L; ++t) TRY(step_fwd(t0 + t, xt(t), xt(t + 1)));
+    for (int t = L - 1; t >= 0; --t) {
+      const int m = t0 + t;
+      double2* xs = xt(t);
+      TRY(step_fwd(m, xs, nullptr));
+      HIPCHK(hipMemsetAsync(xbn, 0, bytes, st));
+      for (int k = 0; k < S; ++k) HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));
+      CombAdjBuilder fin(c, ce, ediv, 0, xb, c->hb);
+      fin.cb.h = DT(m);
+      fin.cb.hscale = HSC(m);
+      fin.term(xbn, xs, 1.0, 0, tb.p[S]);
+      for (int j = 0; j < S; ++j)
+        if (tb.b[j] != 0.0) fin.term(vb(j), v(j), tb.b[j], 1, tb.p[S] - tb.p[j]);
+      TRY(comb_adj_launch<N>(fin.cb, B, st));
+      for (int k = S - 1; k >= 0; --k) {
+        if (k == 0) {
+          // v1 = P vraw: vbraw = P vbar1, d/dT through T s
+          HIPCHK(hipMemsetAsync(vbraw, 0, bytes, st));
+          CombAdjBuilder pc(c, ce, 1.0, 0, vb(0), c->hb);
+          pc.cb.h = PDT(m);
+          pc.cb.hscale = SIG(m);
+          pc.term(vbraw, vraw, 1.0, 0, 1);
+          TRY(comb_adj_launch<N>(pc.cb, B, st));
+          Args am = a;
+          am.dt = DT(m);
+          TRY(vjp_pass<N>(c, am, xs, vbraw, yb, B, 0, st));
+        } else {
+          Args am = a;
+          am.dt = DT(m);
+          TRY(vjp_pass<N>(c, am, y(k), vb(k), yb, B, 0, st));
+        }
+        CombAdjBuilder sa(c, ce, ediv, 0, yb, c->hb);
+        sa.cb.h = DT(m);
+        sa.cb.hscale = HSC(m);
+        sa.term(xbn, k == 0 ? nullptr : xs, 1.0, 0, tb.p[k]);
+        for (int j = 0; j < k; ++j)
+          if (tb.A[k][j] != 0.0) sa.term(vb(j), v(j), tb.A[k][j], 1, tb.p[k] - tb.p[j]);
+        TRY(comb_adj_launch<N>(sa.cb, B, st));
+      }
+      std::swap(xb, xbn);
+    }
+  }
+  TRY(unpack<N>(xb, xbar, B, 0, st));
+  hipLaunchKernelGGL((hb_sum_kernel<N>), dim3(B), dim3(256), 0, st, c->hb, c->red);
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(c->hred, c->red, B * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int b2 = 0; b2 < B; ++b2) Tbar[b2] = c->hred[b2];
+  return 0;
+}
+
 template <int N> static int gen_p(ecs_ctx* c, double* f, int B, const double* dt, int steps, int scheme, int masked,
                                   hipStream_t st) {
   return run_gen<N, false>(c, f, nullptr, B, dt, nullptr, steps, scheme, masked, st);
@@ -1182,6 +1370,7 @@ template <int N> static int gen_t(ecs_ctx* c, double* f, double* df, int B, cons
     static const GridOps ops = {&run_state_vel<NV>, &rk4_p<NV>,          &rk4_t<NV>,    &run_rk43<NV>,  \
                                 &run_rk43_jvp<NV>,  &run_state_vel_vjp<NV>, &run_rk4_ckpt<NV>,            \
                                 &run_rk4_rev<NV>,   &run_rk4_vjp<NV>,    &run_time_kernel<NV>,            \
-                                &gen_p<NV>,         &gen_t<NV>,          &run_gen_vjp<NV>};               \
+                                &gen_p<NV>,         &gen_t<NV>,          &run_gen_vjp<NV>,                \
+                                &run_rk43_vjp<NV>};                                                   \
     return &ops;                                                                                         \
   }

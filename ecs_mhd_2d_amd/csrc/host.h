@@ -96,6 +96,8 @@ struct ecs_ctx {
   size_t it_cap = 0;
   double2* gen = nullptr;      // stage arrays of the generic steppers
   size_t gen_cap = 0;
+  double* tab = nullptr;       // per-step tables of the RK4(3) reverse pass
+  size_t tab_cap = 0;
   double2 *AO = nullptr, *AI = nullptr, *xb = nullptr, *k1b = nullptr, *k2b = nullptr, *k3b = nullptr;
   double* hb = nullptr;
   int adj_cap = 0;
@@ -209,6 +211,8 @@ struct GridOps {
   int (*gen)(ecs_ctx*, double*, int, const double*, int, int, int, hipStream_t);
   int (*gen_jvp)(ecs_ctx*, double*, double*, int, const double*, const double*, int, int, int, hipStream_t);
   int (*gen_vjp)(ecs_ctx*, const double*, double*, double*, int, const double*, int, int, int, int, hipStream_t);
+  int (*rk43_vjp)(ecs_ctx*, const double*, double*, double*, int, const double*, const double*, const int8_t*,
+                  const int*, int, int, hipStream_t);
 };
 const GridOps* grid_ops(int n);
 const GridOps* grid_ops_16();

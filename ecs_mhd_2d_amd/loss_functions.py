@@ -208,7 +208,7 @@ def _rk4_vjp_from(ctx, f0, fbar, h, steps, pd):
     return fw, g, hb[0]
 
 
-REVERSE_MODES = ("Lawson_RK4", "RK4", "TDRK4", "Lawson_RK6")
+REVERSE_MODES = ("Lawson_RK4", "RK4", "TDRK4", "Lawson_RK6", "Lawson_RK43")
 
 
 def _integrate_vjp(ctx, f0, gbar, T, param_dict, mode):
@@ -234,6 +234,10 @@ def _integrate_vjp(ctx, f0, gbar, T, param_dict, mode):
         fT = ctx.lawson_rk6(f0, h, steps, True)
         f0bar, hb = ctx.lawson_rk6_vjp(f0, gbar, h, steps, True, CKPT)
         return fT, f0bar, hb[0] / steps
+    if mode == "Lawson_RK43":
+        fT, info = integrate(f0, T, param_dict, mode)
+        f0bar, Tb = ctx.lawson_rk43_vjp(f0, gbar, T, [info], CKPT)
+        return fT, f0bar, Tb[0]
     raise NotImplementedError(f"the reverse pass is built for modes {REVERSE_MODES}, not {mode!r}")
 
 
@@ -310,9 +314,10 @@ class RPOLoss(torch.autograd.Function):
 
 
 class RPOLossModes(torch.autograd.Function):
-    """loss_RPO in modes "RK4", "TDRK4" and "Lawson_RK6" with the hand-written
-    reverse pass of their integrators (checkpoint every 64 steps, stages
-    recomputed per step) as the backward."""
+    """loss_RPO in modes "RK4", "TDRK4", "Lawson_RK6" and "Lawson_RK43" with
+    the hand-written reverse pass of their integrators (checkpoint every 64
+    steps, stages recomputed per step; RK4(3) along the primal's schedule) as
+    the backward."""
 
     @staticmethod
     def forward(ctx_, fields, T, sx, param_dict, mode):

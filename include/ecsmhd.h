@@ -117,6 +117,16 @@ int ecs_lawson_rk43_jvp(ecs_ctx* ctx, double* x, double* dx, int batch, const do
                         const double* dT, const double* hs, const int8_t* acc, const int* attempts,
                         int budget, void* stream);
 
+/* Reverse-mode derivative of lawson_rk43 in (x, T) along the schedule a
+ * previous ecs_lawson_rk43 call recorded (what jax.vjp of mode "Lawson_RK43"
+ * gives: h is stop-gradient, timestepping.py:308).  x0: initial spectra (not
+ * modified); xbar_inout: cotangent of the final spectra on entry, of x0 on
+ * exit; Tbar: host [batch] output d/dT; hs, acc, attempts, budget as for
+ * ecs_lawson_rk43_jvp.  Accepted steps are checkpointed every ckpt. */
+int ecs_lawson_rk43_vjp(ecs_ctx* ctx, const double* x0, double* xbar_inout, double* Tbar, int batch,
+                        const double* T, const double* hs, const int8_t* acc, const int* attempts, int budget,
+                        int ckpt, void* stream);
+
 /* The other fixed-step integrators of mismatch_RPO, with their forward-mode
  * derivatives (f, df full spectra updated in place; dt, ddt host [batch], as
  * for ecs_lawson_rk4).  They run as RHS passes with separate stage

@@ -112,3 +112,58 @@ def test_newton_with_transpose_in_tdrk4_mode():
     r1 = float(torch.linalg.vector_norm(LF.objective_RPO(x1, pd, "TDRK4")["fields"]))
     print("newton TDRK4 transpose", r0, r1, log["gmres_rel_res"], log["damp"], flush=True)
     assert r1 < 0.8 * r0 and log["gmres_rel_res"] < 0.05
+
+
+def test_rk43_reverse_pass_is_the_exact_transpose():
+    """Reverse pass of the adaptive stepper along its recorded schedule
+    (rejections first: the carried k1 quirk enters the first accepted step):
+    the brute-force transpose of the oracle's tangent along the same schedule."""
+    print("lawson_rk43_vjp dense n=16", flush=True)
+    n = 16
+    pd, ctx = setup(n)
+    x0, c = spectrum(n, 41), spectrum(n, 13)
+    T = 0.15
+    _, infos = ctx.lawson_rk43(torch.as_tensor(x0).cuda(), T, 0.7, 1e-6, 32, 8)
+    info = infos[0]
+    sched = [(info["hs"][i], bool(info["acc"][i])) for i in range(info["attempts"])]
+    assert info["completed"] and info["rejected"] >= 2
+    gb, Tb = ctx.lawson_rk43_vjp(torch.as_tensor(x0).cuda(), torch.as_tensor(c).cuda(), T, infos, ckpt=8)
+    rf, rt = O.dense_vjp(lambda d, dt: O.lawson_rk43_jvp(x0, d, pd, T, dt, sched)[1], x0.shape, c)
+    assert rel(gb.cpu().numpy(), rf) < 1e-10
+    assert abs(Tb[0] - rt) < 1e-9 * max(1.0, abs(rt))
+
+
+def test_rk43_reverse_pass_dot_test_batched():
+    """Batch of two candidates with different schedules (the shorter one takes
+    zero-length steps at the end): dot test against the GPU tangent."""
+    print("lawson_rk43_vjp dot n=32", flush=True)
+    n = 32
+    pd, ctx = setup(n)
+    x0 = torch.as_tensor(np.stack([spectrum(n, 51), 1.5 * spectrum(n, 52)])).cuda()
+    d = torch.as_tensor(np.stack([spectrum(n, 53), spectrum(n, 54)])).cuda()
+    c = torch.as_tensor(np.stack([spectrum(n, 55), spectrum(n, 56)])).cuda()
+    T, dT = [0.5, 0.7], [0.3, -0.4]
+    _, infos = ctx.lawson_rk43(x0, T, 0.5, 1e-7, 64, 8)
+    assert infos[0]["accepted"] != infos[1]["accepted"]
+    _, jd = ctx.lawson_rk43_jvp(x0, d, T, dT, infos)
+    gb, Tb = ctx.lawson_rk43_vjp(x0, c, T, infos, ckpt=16)
+    for b in range(2):
+        lhs = O.rdot(c[b].cpu().numpy(), jd[b].cpu().numpy())
+        rhs = O.rdot(gb[b].cpu().numpy(), d[b].cpu().numpy()) + Tb[b] * dT[b]
+        assert abs(lhs - rhs) < 1e-10 * abs(lhs), (b, lhs, rhs)
+
+
+def test_objective_vjp_dot_test_in_rk43_mode():
+    from ecs_mhd_2d_amd import loss_functions as LF, mhd, utils
+    n = 16
+    pd = mhd.make_param_dict(n, 1 / 40, 1 / 40)
+    pd.update({"shift_reflect_ny": 2, "rot": True,
+               "adaptive_dict": {"atol": 1e-7, "checkpoints": 8, "max_steps_per_checkpoint": 32}})
+    rng = np.random.default_rng(4)
+    inp = {"fields": torch.as_tensor(np.fft.irfft2(spectrum(n, 6))).cuda(), "T": 0.3, "sx": 0.1}
+    u = torch.as_tensor(rng.standard_normal(2 * n * n + 2)).cuda()
+    v = torch.as_tensor(rng.standard_normal(2 * n * n + 2)).cuda()
+    Jv = utils.flatten(LF.objective_RPO_jvp(inp, utils.unflatten(v, n), pd, "Lawson_RK43"))
+    JTu = utils.flatten(LF.objective_RPO_vjp(inp, utils.unflatten(u, n), pd, "Lawson_RK43"))
+    lhs, rhs = float(torch.dot(u, Jv)), float(torch.dot(JTu, v))
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
