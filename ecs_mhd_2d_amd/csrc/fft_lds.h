@@ -14,6 +14,14 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// No multiply-add contraction in the device code of the library (this header
+// comes first in every kernel translation unit): whether the compiler fuses
+// a product into an FMA depends on the surrounding schedule, which differs
+// between template instances (tile sizes), and would make a candidate's
+// rounding depend on the batch size it runs at.  With contraction off every
+// instance computes bit-identical results.
+#pragma clang fp contract(off)
+
 namespace ecs {
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
@@ -218,6 +226,52 @@ __device__ __forceinline__ void fft_lds(double2* __restrict__ buf, const double2
   constexpr int R0 = TL::R0, R1 = TL::R1, R2 = TL::R2;
   static_assert(R0 * R1 * R2 == N, "radix plan must factor N");
   stockham_pass<N, R0, 1, NSEQ, SSTRIDE, NT, INV, 0>(buf, twl);
+  if constexpr (R1 > 1) stockham_pass<N, R1, R0, NSEQ, SSTRIDE, NT, INV, TL::OFF2>(buf, twl);
+  if constexpr (R2 > 1) stockham_pass<N, R2, R0 * R1, NSEQ, SSTRIDE, NT, INV, TL::OFF3>(buf, twl);
+}
+
+// The same transform with the first pass reading its inputs through
+// gather(s, i) (element i of sequence s, typically computed from global
+// memory) instead of from LDS: the caller skips staging the inputs, and the
+// first pass (NS = 1, no twiddles) writes the first LDS copy.  The twiddle
+// tables must have been written before the call (the barrier after the first
+// pass orders them for the second).  SEQFAST maps neighbouring lanes to
+// neighbouring sequences at one butterfly (coalesced when the sequences are
+// neighbouring columns of one array) instead of neighbouring butterflies of
+// one sequence; the arithmetic is the same either way.
+template <int N, int NSEQ, int SSTRIDE, int NT, bool INV, int P, bool SEQFAST = false, class F>
+__device__ __forceinline__ void fft_lds_gather(double2* __restrict__ buf, const double2* __restrict__ twl,
+                                               F&& gather) {
+  using TL = TwLayout<N, P>;
+  constexpr int R0 = TL::R0, R1 = TL::R1, R2 = TL::R2;
+  static_assert(R0 * R1 * R2 == N, "radix plan must factor N");
+  constexpr int NB = NSEQ * (N / R0);
+  constexpr int BPT = (NB + NT - 1) / NT;
+  const int tid = threadIdx.x;
+  double2 v[BPT][R0];
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int bfly = tid + q * NT;
+    if (bfly < NB) {
+      const int s = SEQFAST ? bfly % NSEQ : bfly / (N / R0);
+      const int j = SEQFAST ? bfly / NSEQ : bfly - s * (N / R0);
+#pragma unroll
+      for (int r = 0; r < R0; ++r) v[q][r] = gather(s, j + r * (N / R0));
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int bfly = tid + q * NT;
+    if (bfly < NB) {
+      const int s = SEQFAST ? bfly % NSEQ : bfly / (N / R0);
+      const int j = SEQFAST ? bfly / NSEQ : bfly - s * (N / R0);
+      Dft<R0, INV>::run(v[q]);
+      double2* dst = buf + s * SSTRIDE;
+#pragma unroll
+      for (int r = 0; r < R0; ++r) dst[pidx(j * R0 + r)] = v[q][r];
+    }
+  }
+  __syncthreads();
   if constexpr (R1 > 1) stockham_pass<N, R1, R0, NSEQ, SSTRIDE, NT, INV, TL::OFF2>(buf, twl);
   if constexpr (R2 > 1) stockham_pass<N, R2, R0 * R1, NSEQ, SSTRIDE, NT, INV, TL::OFF3>(buf, twl);
 }

@@ -363,18 +363,12 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
 
   if (FWD) {
     const double2* O = a.rowout + (size_t)b * (2 * NF) * N * KC;
-    for (int idx = tid; idx < NF * N * TY; idx += NT) {
-      const int cy = idx % TY;
-      const int t = idx / TY;
-      const int x = t % N;
-      const int f = t / N;
-      const int ky = ky0 + cy;
-      double2 v = make_double2(0.0, 0.0);
-      if (ky < KC) v = O[((size_t)(f * 2 + c) * N + x) * KC + ky];
-      sm[(f * TY + cy) * SS + pidx(x)] = v;
-    }
-    __syncthreads();
-    fft_lds<N, NF * TY, SS, NT, false, P>(sm, twl);
+    // the first x-FFT pass reads the row output directly; with several
+    // columns per block neighbouring lanes take neighbouring ky of one row
+    fft_lds_gather<N, NF * TY, SS, NT, false, P, (TY > 1)>(sm, twl, [&](int s, int x) -> double2 {
+      const int f = s / TY, ky = ky0 + s % TY;
+      return ky < KC ? O[((size_t)(f * 2 + c) * N + x) * KC + ky] : make_double2(0.0, 0.0);
+    });
   }
 
   const bool inv = HASINV && (MODE == C_S5A ? bk.next : bk.stage) && !a.skip_inv;
@@ -451,29 +445,24 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
   const int b = blockIdx.z, xbase = blockIdx.x * TX, tid = threadIdx.x;
   if (!row_active(a, b)) return;
   const double2* I = a.inter + (size_t)b * ND * N * KC;
-  // Hermitian extension of the packed pairs (A + iB); the imaginary parts of
-  // the ky = 0 and ky = N/2 bins are dropped, as numpy's c2r does.
-  for (int idx = tid; idx < TX * NP * NHALF; idx += NT) {
-    const int k = idx % NHALF;
-    const int t = idx / NHALF;
-    const int p = t % NP;
-    const int xl = t / NP;
-    const int x = xbase + xl;
-    double2 A = make_double2(0.0, 0.0), B = A;
-    if (k < KC) {
-      A = I[((size_t)(2 * p) * N + x) * KC + k];
-      B = I[((size_t)(2 * p + 1) * N + x) * KC + k];
+  // Hermitian extension of the packed pairs (A + iB), formed while the first
+  // inverse-FFT pass loads its inputs from I; the imaginary parts of the ky = 0
+  // bin are dropped, as numpy's c2r does (ky = N/2 lies in the dealiased gap).
+  fft_lds_gather<N, TX * NP, SS, NT, true, P>(sm, twl, [&](int s, int m) -> double2 {
+    const int p = s % NP;
+    const int x = xbase + s / NP;
+    const double2* Ia = I + ((size_t)(2 * p) * N + x) * KC;
+    const double2* Ib = I + ((size_t)(2 * p + 1) * N + x) * KC;
+    if (m <= G::KMAX) {
+      const double2 A = Ia[m], B = Ib[m];
+      return m == 0 ? make_double2(A.x, B.x) : make_double2(A.x - B.y, A.y + B.x);
     }
-    double2* z = sm + (xl * NP + p) * SS;
-    if (k == 0 || k == N / 2) {
-      z[pidx(k)] = make_double2(A.x, B.x);
-    } else {
-      z[pidx(k)] = make_double2(A.x - B.y, A.y + B.x);
-      z[pidx(N - k)] = make_double2(A.x + B.y, B.x - A.y);
+    if (m >= N - G::KMAX) {
+      const double2 A = Ia[N - m], B = Ib[N - m];
+      return make_double2(A.x + B.y, B.x - A.y);
     }
-  }
-  __syncthreads();
-  fft_lds<N, TX * NP, SS, NT, true, P>(sm, twl);
+    return make_double2(0.0, 0.0);
+  });
 
   if (RMODE == R_ERR) {
     unsigned long long mx = 0ull;
