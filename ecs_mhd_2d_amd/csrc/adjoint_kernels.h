@@ -128,18 +128,11 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
   if (a.nstep && a.step >= a.nstep[b]) return;  // candidate past its own step count
   if (FWD) {
     const double2* AI = a.ain + (size_t)b * 8 * N * KC;
-    for (int idx = tid; idx < 4 * N * TY; idx += NT) {
-      const int cy = idx % TY;
-      const int t = idx / TY;
-      const int x = t % N;
-      const int d = t / N;
-      const int ky = ky0 + cy;
-      double2 v = make_double2(0.0, 0.0);
-      if (ky < KC) v = AI[((size_t)(4 * c + d) * N + x) * KC + ky];
-      sm[(d * TY + cy) * SS + pidx(x)] = v;
-    }
-    __syncthreads();
-    fft_lds<N, 4 * TY, SS, NT, false>(sm, twl);
+    // first x-FFT pass straight from the row-adjoint output
+    fft_lds_gather<N, 4 * TY, SS, NT, false, 16, (TY > 1)>(sm, twl, [&](int s, int x) -> double2 {
+      const int d = s / TY, ky = ky0 + s % TY;
+      return ky < KC ? AI[((size_t)(4 * c + d) * N + x) * KC + ky] : make_double2(0.0, 0.0);
+    });
   }
   constexpr double invN2 = 1.0 / ((double)N * (double)N);
   for (int idx = tid; idx < R * TY; idx += NT) {
@@ -208,34 +201,27 @@ __global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
   if (a.nstep && a.step >= a.nstep[b]) return;  // candidate past its own step count
   const double2* P = a.prim + (size_t)b * 8 * N * KC;
   const double2* AO = a.aout + (size_t)b * 2 * N * KC;
-  for (int idx = tid; idx < TX * NP * NHALF; idx += NT) {
-    const int k = idx % NHALF;
-    const int t = idx / NHALF;
-    const int p = t % NP;
-    const int xl = t / NP;
-    const int x = xbase + xl;
-    double2 A = make_double2(0.0, 0.0), B = A;
-    if (k < KC) {
-      if (p < 4) {
-        A = P[((size_t)(2 * p) * N + x) * KC + k];
-        B = P[((size_t)(2 * p + 1) * N + x) * KC + k];
-      } else {
-        // r2c^T: pbar[y] = sum_k Re(Pbar_k e^{i theta}) = c2r of (Pbar_0, Pbar_k / 2)
-        const double w = (k == 0) ? 1.0 : 0.5;
-        A = cscale(w, AO[((size_t)0 * N + x) * KC + k]);
-        B = cscale(w, AO[((size_t)1 * N + x) * KC + k]);
-      }
-    }
-    double2* z = sm + (xl * NP + p) * SS;
-    if (k == 0 || k == N / 2) {
-      z[pidx(k)] = make_double2(A.x, B.x);
+  // Hermitian extension of the packed pairs, formed while the first inverse
+  // pass loads its inputs: 4 primal pairs, then the adjoint pair with the
+  // r2c^T weights (pbar[y] = c2r of (Pbar_0, Pbar_k / 2))
+  fft_lds_gather<N, TX * NP, SS, NT, true, 16>(sm, twl, [&](int s, int m) -> double2 {
+    const int p = s % NP;
+    const int x = xbase + s / NP;
+    const bool lo = m <= G::KMAX;
+    if (!lo && m < N - G::KMAX) return make_double2(0.0, 0.0);
+    const int k = lo ? m : N - m;
+    double2 A, B;
+    if (p < 4) {
+      A = P[((size_t)(2 * p) * N + x) * KC + k];
+      B = P[((size_t)(2 * p + 1) * N + x) * KC + k];
     } else {
-      z[pidx(k)] = make_double2(A.x - B.y, A.y + B.x);
-      z[pidx(N - k)] = make_double2(A.x + B.y, B.x - A.y);
+      const double w = (k == 0) ? 1.0 : 0.5;
+      A = cscale(w, AO[((size_t)0 * N + x) * KC + k]);
+      B = cscale(w, AO[((size_t)1 * N + x) * KC + k]);
     }
-  }
-  __syncthreads();
-  fft_lds<N, TX * NP, SS, NT, true>(sm, twl);
+    if (m == 0) return make_double2(A.x, B.x);
+    return lo ? make_double2(A.x - B.y, A.y + B.x) : make_double2(A.x + B.y, B.x - A.y);
+  });
   constexpr int EPT = (TX * N + NT - 1) / NT;
   double2 q[EPT][4];
   const double b0x = a.b0x, b0y = a.b0y;
