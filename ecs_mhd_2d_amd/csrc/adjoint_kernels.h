@@ -123,16 +123,22 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
   constexpr int SS = Seq<N>::S;
   __shared__ double2 sm[NSEQ * SS];
   __shared__ double2 twl[N];
-  load_twiddles<N, NT>(twl, a.tw);
+  TwFetch<N, NT> twf;
+  twf.fetch(a.tw);
   const int b = blockIdx.z, c = blockIdx.y, ky0 = blockIdx.x * TY, tid = threadIdx.x;
   if (a.nstep && a.step >= a.nstep[b]) return;  // candidate past its own step count
   if (FWD) {
     const double2* AI = a.ain + (size_t)b * 8 * N * KC;
     // first x-FFT pass straight from the row-adjoint output
-    fft_lds_gather<N, 4 * TY, SS, NT, false, 16, (TY > 1)>(sm, twl, [&](int s, int x) -> double2 {
-      const int d = s / TY, ky = ky0 + s % TY;
-      return ky < KC ? AI[((size_t)(4 * c + d) * N + x) * KC + ky] : make_double2(0.0, 0.0);
-    });
+    fft_lds_gather<N, 4 * TY, SS, NT, false, 16, (TY > 1)>(
+        sm, twl,
+        [&](int s, int x) -> double2 {
+          const int d = s / TY, ky = ky0 + s % TY;
+          return ky < KC ? AI[((size_t)(4 * c + d) * N + x) * KC + ky] : make_double2(0.0, 0.0);
+        },
+        [&] { twf.store(twl); });
+  } else {
+    twf.store(twl);  // ordered by the barrier before the inverse FFT
   }
   constexpr double invN2 = 1.0 / ((double)N * (double)N);
   for (int idx = tid; idx < R * TY; idx += NT) {
@@ -196,32 +202,35 @@ __global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
   constexpr int SS = Seq<N>::S;
   __shared__ double2 sm[TX * NP * SS];
   __shared__ double2 twl[N];
-  load_twiddles<N, NT>(twl, a.tw);
+  TwFetch<N, NT> twf;
+  twf.fetch(a.tw);
   const int b = blockIdx.z, xbase = blockIdx.x * TX, tid = threadIdx.x;
   if (a.nstep && a.step >= a.nstep[b]) return;  // candidate past its own step count
   const double2* P = a.prim + (size_t)b * 8 * N * KC;
   const double2* AO = a.aout + (size_t)b * 2 * N * KC;
   // Hermitian extension of the packed pairs, formed while the first inverse
   // pass loads its inputs: 4 primal pairs, then the adjoint pair with the
-  // r2c^T weights (pbar[y] = c2r of (Pbar_0, Pbar_k / 2))
-  fft_lds_gather<N, TX * NP, SS, NT, true, 16>(sm, twl, [&](int s, int m) -> double2 {
-    const int p = s % NP;
-    const int x = xbase + s / NP;
-    const bool lo = m <= G::KMAX;
-    if (!lo && m < N - G::KMAX) return make_double2(0.0, 0.0);
-    const int k = lo ? m : N - m;
-    double2 A, B;
-    if (p < 4) {
-      A = P[((size_t)(2 * p) * N + x) * KC + k];
-      B = P[((size_t)(2 * p + 1) * N + x) * KC + k];
-    } else {
-      const double w = (k == 0) ? 1.0 : 0.5;
-      A = cscale(w, AO[((size_t)0 * N + x) * KC + k]);
-      B = cscale(w, AO[((size_t)1 * N + x) * KC + k]);
-    }
-    if (m == 0) return make_double2(A.x, B.x);
-    return lo ? make_double2(A.x - B.y, A.y + B.x) : make_double2(A.x + B.y, B.x - A.y);
-  });
+  // r2c^T weights (pbar[y] = c2r of (Pbar_0, Pbar_k / 2)).  Every lane loads
+  // (gap lanes re-read bin 0 and discard it), so all loads of the pass are in
+  // flight before the first wait.
+  fft_lds_gather<N, TX * NP, SS, NT, true, 16>(
+      sm, twl,
+      [&](int s, int m) -> double2 {
+        const int p = s % NP;
+        const int x = xbase + s / NP;
+        const bool lo = m <= G::KMAX, hi = m >= N - G::KMAX;
+        const int k = lo ? m : (hi ? N - m : 0);
+        const double2* src = (p < 4) ? P + (size_t)(2 * p) * N * KC : AO;
+        double2 A = src[((size_t)x) * KC + k];
+        double2 B = src[((size_t)N + x) * KC + k];
+        const double w = (p < 4 || k == 0) ? 1.0 : 0.5;
+        A = cscale(w, A);
+        B = cscale(w, B);
+        if (m == 0) return make_double2(A.x, B.x);
+        const double2 vlo = make_double2(A.x - B.y, A.y + B.x), vhi = make_double2(A.x + B.y, B.x - A.y);
+        return lo ? vlo : (hi ? vhi : make_double2(0.0, 0.0));
+      },
+      [&] { twf.store(twl); });
   constexpr int EPT = (TX * N + NT - 1) / NT;
   double2 q[EPT][4];
   const double b0x = a.b0x, b0y = a.b0y;

@@ -17,50 +17,101 @@ template <> struct DefTile<64> { static constexpr int COL = 2, ROW = 1; };
 template <> struct DefTile<128> { static constexpr int COL = 1, ROW = 0; };
 template <> struct DefTile<256> { static constexpr int COL = 1, ROW = 0; };
 
-template <int N, int TY, int MODE, bool TAN, int P>
+template <int N, int TY, int MODE, bool TAN, int P, int AK>
 static int col_v(const Args& a, int B, hipStream_t st) {
   dim3 grid((Geo<N>::KC + TY - 1) / TY, 2, B);
-  hipLaunchKernelGGL((col_kernel<N, TY, NT, MODE, TAN, P>), grid, dim3(NT), 0, st, a);
+  hipLaunchKernelGGL((col_kernel<N, TY, NT, MODE, TAN, P, AK>), grid, dim3(NT), 0, st, a);
   LAUNCHCHK();
   return 0;
 }
-template <int N, int TX, int RMODE, bool TAN, int P>
+template <int N, int TX, int RMODE, bool TAN, int P, int AK>
 static int row_v(const Args& a, int B, hipStream_t st) {
   dim3 grid(N / TX, 1, B);
-  hipLaunchKernelGGL((row_kernel<N, TX, NT, RMODE, TAN, P>), grid, dim3(NT), 0, st, a);
+  hipLaunchKernelGGL((row_kernel<N, TX, NT, RMODE, TAN, P, AK>), grid, dim3(NT), 0, st, a);
   LAUNCHCHK();
   return 0;
 }
 // variant v: tile 1 << (v % 3), plan wide (v < 3) or narrow (v >= 3)
-template <int N, int MODE, bool TAN>
+template <int N, int MODE, bool TAN, int AK>
 static int col_var(int v, const Args& a, int B, hipStream_t st) {
   switch (v) {
-    case 0: return col_v<N, 1, MODE, TAN, 16>(a, B, st);
-    case 1: return col_v<N, 2, MODE, TAN, 16>(a, B, st);
-    case 2: return col_v<N, 4, MODE, TAN, 16>(a, B, st);
-    case 3: return col_v<N, 1, MODE, TAN, 8>(a, B, st);
-    case 4: return col_v<N, 2, MODE, TAN, 8>(a, B, st);
-    default: return col_v<N, 4, MODE, TAN, 8>(a, B, st);
+    case 0: return col_v<N, 1, MODE, TAN, 16, AK>(a, B, st);
+    case 1: return col_v<N, 2, MODE, TAN, 16, AK>(a, B, st);
+    case 2: return col_v<N, 4, MODE, TAN, 16, AK>(a, B, st);
+    case 3: return col_v<N, 1, MODE, TAN, 8, AK>(a, B, st);
+    case 4: return col_v<N, 2, MODE, TAN, 8, AK>(a, B, st);
+    default: return col_v<N, 4, MODE, TAN, 8, AK>(a, B, st);
   }
 }
-template <int N, int RMODE, bool TAN>
+template <int N, int RMODE, bool TAN, int AK>
 static int row_var(int v, const Args& a, int B, hipStream_t st) {
   switch (v) {
-    case 0: return row_v<N, 1, RMODE, TAN, 16>(a, B, st);
-    case 1: return row_v<N, 2, RMODE, TAN, 16>(a, B, st);
-    case 2: return row_v<N, 4, RMODE, TAN, 16>(a, B, st);
-    case 3: return row_v<N, 1, RMODE, TAN, 8>(a, B, st);
-    case 4: return row_v<N, 2, RMODE, TAN, 8>(a, B, st);
-    default: return row_v<N, 4, RMODE, TAN, 8>(a, B, st);
+    case 0: return row_v<N, 1, RMODE, TAN, 16, AK>(a, B, st);
+    case 1: return row_v<N, 2, RMODE, TAN, 16, AK>(a, B, st);
+    case 2: return row_v<N, 4, RMODE, TAN, 16, AK>(a, B, st);
+    case 3: return row_v<N, 1, RMODE, TAN, 8, AK>(a, B, st);
+    case 4: return row_v<N, 2, RMODE, TAN, 8, AK>(a, B, st);
+    default: return row_v<N, 4, RMODE, TAN, 8, AK>(a, B, st);
   }
+}
+
+// Stepper kinds each kernel mode runs under: fixed-step passes (RHS, classical
+// and Lawson RK4 stages), the adaptive primal (no tangent) and the tangent
+// replay of an adaptive schedule.  Only these instances are compiled.
+template <int MODE, bool TAN, int AK>
+constexpr bool col_kind_ok() {
+  if (AK == AD_FIXED)
+    return MODE == C_LOAD || MODE == C_RHS || MODE == C_S1F || MODE == C_S2 || MODE == C_S3 || MODE == C_S4;
+  if (AK == AD_PRIMAL)
+    return !TAN && (MODE == C_S2 || MODE == C_S3 || MODE == C_S4 || MODE == C_S5 || MODE == C_A0 ||
+                    MODE == C_A || MODE == C_FIN);
+  return TAN && (MODE == C_S2 || MODE == C_S3 || MODE == C_S4 || MODE == C_S5 || MODE == C_S5A ||
+                 MODE == C_A0 || MODE == C_A);
+}
+template <int RMODE, bool TAN, int AK>
+constexpr bool row_kind_ok() {
+  if (RMODE == R_ERR) return AK == AD_PRIMAL && !TAN;
+  if (AK == AD_PRIMAL) return !TAN;
+  if (AK == AD_TANGENT) return TAN;
+  return true;
 }
 template <int N, int MODE, bool TAN>
 static int col(const Args& a, int B, hipStream_t st) {
-  return col_var<N, MODE, TAN>(tuned(N, TAN ? K_COLT : K_COL, B, DefTile<N>::COL), a, B, st);
+  const int v = tuned(N, TAN ? K_COLT : K_COL, B, DefTile<N>::COL);
+  switch (a.adaptive) {
+    case AD_FIXED:
+      if constexpr (col_kind_ok<MODE, TAN, AD_FIXED>()) return col_var<N, MODE, TAN, AD_FIXED>(v, a, B, st);
+      break;
+    case AD_PRIMAL:
+      if constexpr (col_kind_ok<MODE, TAN, AD_PRIMAL>()) return col_var<N, MODE, TAN, AD_PRIMAL>(v, a, B, st);
+      break;
+    case AD_TANGENT:
+      if constexpr (col_kind_ok<MODE, TAN, AD_TANGENT>()) return col_var<N, MODE, TAN, AD_TANGENT>(v, a, B, st);
+      break;
+    default:
+      break;
+  }
+  return fail(3, "column kernel mode " + std::to_string(MODE) + " has no instance for stepper kind " +
+                     std::to_string(a.adaptive));
 }
 template <int N, int RMODE, bool TAN>
 static int row(const Args& a, int B, hipStream_t st) {
-  return row_var<N, RMODE, TAN>(tuned(N, TAN ? K_ROWT : K_ROW, B, DefTile<N>::ROW), a, B, st);
+  const int v = tuned(N, TAN ? K_ROWT : K_ROW, B, DefTile<N>::ROW);
+  switch (a.adaptive) {
+    case AD_FIXED:
+      if constexpr (row_kind_ok<RMODE, TAN, AD_FIXED>()) return row_var<N, RMODE, TAN, AD_FIXED>(v, a, B, st);
+      break;
+    case AD_PRIMAL:
+      if constexpr (row_kind_ok<RMODE, TAN, AD_PRIMAL>()) return row_var<N, RMODE, TAN, AD_PRIMAL>(v, a, B, st);
+      break;
+    case AD_TANGENT:
+      if constexpr (row_kind_ok<RMODE, TAN, AD_TANGENT>()) return row_var<N, RMODE, TAN, AD_TANGENT>(v, a, B, st);
+      break;
+    default:
+      break;
+  }
+  return fail(3, "row kernel mode " + std::to_string(RMODE) + " has no instance for stepper kind " +
+                     std::to_string(a.adaptive));
 }
 
 template <int N>
@@ -95,8 +146,8 @@ static int time_variants(ecs_ctx* c, int B, int kind, int v0, int v1, float* t, 
   HIPCHK(hipMemsetAsync(c->dx0, 0, ce, st));
   HIPCHK(hipMemsetAsync(c->acc, 0, ce, st));
   HIPCHK(hipMemsetAsync(c->dacc, 0, ce, st));
-  TRY((col_var<N, C_LOAD, true>(1, a, B, st)));
-  TRY((row_var<N, R_RHS, true>(1, a, B, st)));
+  TRY((col_var<N, C_LOAD, true, AD_FIXED>(1, a, B, st)));
+  TRY((row_var<N, R_RHS, true, AD_FIXED>(1, a, B, st)));
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -110,10 +161,10 @@ static int time_variants(ecs_ctx* c, int B, int kind, int v0, int v1, float* t, 
     for (int v = v0; v < v1 && !rc; ++v) {
       auto launch = [&]() -> int {
         switch (kind) {
-          case K_COL: return col_var<N, C_S2, false>(v, a, B, st);
-          case K_COLT: return col_var<N, C_S2, true>(v, a, B, st);
-          case K_ROW: return row_var<N, R_RHS, false>(v, a, B, st);
-          default: return row_var<N, R_RHS, true>(v, a, B, st);
+          case K_COL: return col_var<N, C_S2, false, AD_FIXED>(v, a, B, st);
+          case K_COLT: return col_var<N, C_S2, true, AD_FIXED>(v, a, B, st);
+          case K_ROW: return row_var<N, R_RHS, false, AD_FIXED>(v, a, B, st);
+          default: return row_var<N, R_RHS, true, AD_FIXED>(v, a, B, st);
         }
       };
       rc = launch();

@@ -219,7 +219,7 @@ __device__ __forceinline__ void stockham_pass(double2* __restrict__ buf, const d
   __syncthreads();
 }
 
-// twl must hold the plan's twiddle tables in LDS (load_twiddles<N, NT, P> + a barrier).
+// twl must hold the plan's twiddle tables in LDS (TwFetch<N, NT, P>::store + a barrier).
 template <int N, int NSEQ, int SSTRIDE, int NT, bool INV, int P = 16>
 __device__ __forceinline__ void fft_lds(double2* __restrict__ buf, const double2* __restrict__ twl) {
   using TL = TwLayout<N, P>;
@@ -234,14 +234,15 @@ __device__ __forceinline__ void fft_lds(double2* __restrict__ buf, const double2
 // gather(s, i) (element i of sequence s, typically computed from global
 // memory) instead of from LDS: the caller skips staging the inputs, and the
 // first pass (NS = 1, no twiddles) writes the first LDS copy.  The twiddle
-// tables must have been written before the call (the barrier after the first
-// pass orders them for the second).  SEQFAST maps neighbouring lanes to
+// tables must be in LDS by the barrier after the first pass: written before
+// the call, or by before_barrier(), which runs once the first pass's inputs
+// have arrived.  SEQFAST maps neighbouring lanes to
 // neighbouring sequences at one butterfly (coalesced when the sequences are
 // neighbouring columns of one array) instead of neighbouring butterflies of
 // one sequence; the arithmetic is the same either way.
-template <int N, int NSEQ, int SSTRIDE, int NT, bool INV, int P, bool SEQFAST = false, class F>
+template <int N, int NSEQ, int SSTRIDE, int NT, bool INV, int P, bool SEQFAST = false, class F, class H>
 __device__ __forceinline__ void fft_lds_gather(double2* __restrict__ buf, const double2* __restrict__ twl,
-                                               F&& gather) {
+                                               F&& gather, H&& before_barrier) {
   using TL = TwLayout<N, P>;
   constexpr int R0 = TL::R0, R1 = TL::R1, R2 = TL::R2;
   static_assert(R0 * R1 * R2 == N, "radix plan must factor N");
@@ -271,29 +272,57 @@ __device__ __forceinline__ void fft_lds_gather(double2* __restrict__ buf, const 
       for (int r = 0; r < R0; ++r) dst[pidx(j * R0 + r)] = v[q][r];
     }
   }
+  before_barrier();
   __syncthreads();
   if constexpr (R1 > 1) stockham_pass<N, R1, R0, NSEQ, SSTRIDE, NT, INV, TL::OFF2>(buf, twl);
   if constexpr (R2 > 1) stockham_pass<N, R2, R0 * R1, NSEQ, SSTRIDE, NT, INV, TL::OFF3>(buf, twl);
 }
 
-// Builds the per-pass twiddle tables of plan P from tw[j] = exp(-2 pi i j / N).
-template <int N, int NT, int P = 16>
-__device__ __forceinline__ void load_twiddles(double2* __restrict__ twl, const double2* __restrict__ tw) {
-  using TL = TwLayout<N, P>;
-  if constexpr (TL::R1 > 1) {
-    constexpr int NS = TL::R0, R = TL::R1, TSTEP = N / (NS * R);
-    for (int i = threadIdx.x; i < (R - 1) * NS; i += NT) {
-      const int r = i / NS + 1, k = i - (r - 1) * NS;
-      twl[TL::OFF2 + i] = tw[(r * k * TSTEP) & (N - 1)];
-    }
-  }
-  if constexpr (TL::R2 > 1) {
-    constexpr int NS = TL::R0 * TL::R1, R = TL::R2, TSTEP = N / (NS * R);
-    for (int i = threadIdx.x; i < (R - 1) * NS; i += NT) {
-      const int r = i / NS + 1, k = i - (r - 1) * NS;
-      twl[TL::OFF3 + i] = tw[(r * k * TSTEP) & (N - 1)];
-    }
-  }
+template <int N, int NSEQ, int SSTRIDE, int NT, bool INV, int P, bool SEQFAST = false, class F>
+__device__ __forceinline__ void fft_lds_gather(double2* __restrict__ buf, const double2* __restrict__ twl,
+                                               F&& gather) {
+  fft_lds_gather<N, NSEQ, SSTRIDE, NT, INV, P, SEQFAST>(buf, twl, gather, [] {});
 }
+
+// The per-pass twiddle tables of plan P, fetched into registers by fetch()
+// (at most one load per thread and table, issued before anything else a
+// kernel waits on) and written to LDS by store() once the first pass has consumed its
+// inputs, so the table's memory latency overlaps the kernel's other loads
+// instead of preceding them.  The barrier that follows the first pass orders
+// the writes for the second.
+template <int N, int NT, int P = 16>
+struct TwFetch {
+  using TL = TwLayout<N, P>;
+  static constexpr int N2 = TL::R1 > 1 ? (TL::R1 - 1) * TL::R0 : 0;
+  static constexpr int N3 = TL::R2 > 1 ? (TL::R2 - 1) * TL::R0 * TL::R1 : 0;
+  static_assert(N2 <= NT && N3 <= NT, "one twiddle entry per thread and table");
+  // plain doubles: a double2 member is copied with memcpy, which keeps the
+  // object in scratch memory instead of registers
+  double w2x, w2y, w3x, w3y;
+
+  __device__ __forceinline__ void fetch(const double2* __restrict__ tw) {
+    const int i = threadIdx.x;
+    if (N2 > 0 && i < N2) {
+      constexpr int NS = TL::R0, TSTEP = N / (NS * TL::R1);
+      const int r = i / NS + 1, k = i - (r - 1) * NS;
+      const double2 t = tw[(r * k * TSTEP) & (N - 1)];
+      w2x = t.x;
+      w2y = t.y;
+    }
+    if (N3 > 0 && i < N3) {
+      constexpr int NS = TL::R0 * TL::R1, TSTEP = N / (NS * TL::R2);
+      const int r = i / NS + 1, k = i - (r - 1) * NS;
+      const double2 t = tw[(r * k * TSTEP) & (N - 1)];
+      w3x = t.x;
+      w3y = t.y;
+    }
+  }
+
+  __device__ __forceinline__ void store(double2* __restrict__ twl) const {
+    const int i = threadIdx.x;
+    if (N2 > 0 && i < N2) twl[TL::OFF2 + i] = make_double2(w2x, w2y);
+    if (N3 > 0 && i < N3) twl[TL::OFF3 + i] = make_double2(w3x, w3y);
+  }
+};
 
 }  // namespace ecs

@@ -16,6 +16,12 @@ struct Blk {
   double ddt;    // tangent of dt per unit tangent
   bool next;     // C_S5A: the next accepted step exists (its stage-1 glue runs)
   double h2, dt2, ddt2;  // C_S5A: its step, scale and tangent
+  double dT;     // tangent replay: dT[b]
+  // tangent replay: rejections preceding the attempt whose stage 1 runs here
+  // (unsigned: widening it needs no sign extension at the load, which would
+  // make the prologue wait for it)
+  unsigned rs;
+  int rc;
 };
 
 __device__ __forceinline__ int attempt_of(const Args& a) { return a.cnt ? a.cnt[a.cslot] : a.attempt; }
@@ -23,13 +29,17 @@ __device__ __forceinline__ int attempt_of(const Args& a) { return a.cnt ? a.cnt[
 __device__ __forceinline__ double relu_nan(double x) { return (x != x) ? x : (x > 0.0 ? x : 0.0); }
 
 // Controller decision of timestepping.py:288-313, from the record of attempt i-1.
-template <int MODE, bool TAN>
+// AK (an AdaptKind) is a template argument, so each kernel instance has one
+// straight-line prologue and its loads are not forced to complete where the
+// control paths of the other kinds would merge.
+template <int MODE, bool TAN, int AK>
 __device__ __forceinline__ Blk prologue(const Args& a, int b) {
   Blk bk;
   bk.run = true; bk.stage = true; bk.pend = false; bk.pacc = false;
   bk.hprev = 0.0; bk.T = 0.0; bk.h = 0.0; bk.dt = 0.0; bk.ddt = 0.0;
   bk.next = false; bk.h2 = 0.0; bk.dt2 = 0.0; bk.ddt2 = 0.0;
-  if (a.adaptive == AD_FIXED) {
+  bk.dT = 0.0; bk.rs = 0u; bk.rc = 0;
+  if (AK == AD_FIXED) {
     if (a.nstep && a.step >= a.nstep[b]) {
       bk.run = bk.stage = false;
       return bk;
@@ -43,18 +53,28 @@ __device__ __forceinline__ Blk prologue(const Args& a, int b) {
   if ((MODE == C_A || MODE == C_A0) && a.cnt && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 &&
       threadIdx.x == 0)
     a.cnt[a.cslot ^ 1] = att + 1;
-  if (a.adaptive == AD_TANGENT) {
+  if (AK == AD_TANGENT) {
+    // only the run test waits here: the schedule entries below are loaded
+    // now and used after the column gathers (finish() forms the scales)
     const int m = att;
-    bk.run = bk.stage = m < a.s_nacc[b];
+    const int nacc = a.s_nacc[b];
+    bk.dT = a.dTb[b];
+    bk.run = bk.stage = m < nacc;
     if (!bk.run) return bk;
-    bk.h = a.s_h[(size_t)b * a.sstride + m];
-    bk.dt = bk.h * bk.T;
-    bk.ddt = bk.h * a.dTb[b];
-    if (MODE == C_S5A && m + 1 < a.s_nacc[b]) {
-      bk.next = true;
-      bk.h2 = a.s_h[(size_t)b * a.sstride + m + 1];
-      bk.dt2 = bk.h2 * bk.T;
-      bk.ddt2 = bk.h2 * a.dTb[b];
+    const size_t so = (size_t)b * a.sstride + m;
+    bk.h = a.s_h[so];
+    if (MODE == C_S5A) {
+      // loaded unconditionally (from entry m when there is no next step, and
+      // then unused) so no branch has to wait for them
+      bk.next = m + 1 < nacc;
+      const size_t sn = bk.next ? so + 1 : so;
+      bk.h2 = a.s_h[sn];
+      bk.rs = (unsigned)a.s_rs[sn];
+      bk.rc = a.s_rc[sn];
+    }
+    if (MODE == C_A || MODE == C_A0) {
+      bk.rs = (unsigned)a.s_rs[so];
+      bk.rc = a.s_rc[so];
     }
     return bk;
   }
@@ -103,11 +123,23 @@ __device__ __forceinline__ Blk prologue(const Args& a, int b) {
   return bk;
 }
 
+// Scales of the tangent replay from the schedule entries the prologue loaded.
+template <int AK>
+__device__ __forceinline__ void finish(Blk& bk) {
+  if (AK != AD_TANGENT) return;
+  bk.dt = bk.h * bk.T;
+  bk.ddt = bk.h * bk.dT;
+  if (bk.next) {
+    bk.dt2 = bk.h2 * bk.T;
+    bk.ddt2 = bk.h2 * bk.dT;
+  }
+}
+
+template <int AK>
 __device__ __forceinline__ bool row_active(const Args& a, int b) {
-  if (a.adaptive == AD_FIXED) return !(a.nstep && a.step >= a.nstep[b]);
-  if (a.adaptive == AD_PRIMAL) return a.hist[(size_t)b * a.hstride + attempt_of(a)].active != 0;
-  if (a.adaptive == AD_TANGENT) return attempt_of(a) < a.s_nacc[b];
-  return true;
+  if (AK == AD_FIXED) return !(a.nstep && a.step >= a.nstep[b]);
+  if (AK == AD_PRIMAL) return a.hist[(size_t)b * a.hstride + attempt_of(a)].active != 0;
+  return attempt_of(a) < a.s_nacc[b];
 }
 
 __device__ __forceinline__ double2 cdivs(double2 a, double s) { return make_double2(a.x / s, a.y / s); }
@@ -117,9 +149,11 @@ struct Pre {
   double2 F;              // masked forcing spectrum (omega only)
   double2 x0, dx0;        // stage-1 state (modes that read it before writing)
   double2 acc, dacc;      // RK accumulator
+  double2 xf, dxf;        // tangent replay C_S5A: the accepted state
+  double2 v1, dv1;        // tangent replay C_A: the carried k1
 };
 
-template <int MODE, bool TAN>
+template <int MODE, bool TAN, int AK>
 __device__ __forceinline__ Pre prefetch(const Args& a, size_t ci, size_t fi, int c) {
   Pre p;
   const double2 z = make_double2(0.0, 0.0);
@@ -131,6 +165,11 @@ __device__ __forceinline__ Pre prefetch(const Args& a, size_t ci, size_t fi, int
   p.dx0 = (X0 && TAN) ? a.dx0[ci] : z;
   p.acc = ACC ? a.acc[ci] : z;
   p.dacc = (ACC && TAN) ? a.dacc[ci] : z;
+  constexpr bool rep = TAN && AK == AD_TANGENT;
+  p.xf = (MODE == C_S5A && rep) ? a.xf[ci] : z;
+  p.dxf = (MODE == C_S5A && rep) ? a.dxf[ci] : z;
+  p.v1 = (MODE == C_A && rep) ? a.v1[ci] : z;
+  p.dv1 = (MODE == C_A && rep) ? a.dv1[ci] : z;
   return p;
 }
 
@@ -155,11 +194,8 @@ __device__ __forceinline__ void stage1(const Args& a, size_t ci, double e, doubl
 
 // Rejections preceding accepted step m of the tangent replay: the carried k1
 // becomes e (e k1) per rejected step h_j (timestepping.py:267,276,297).
-__device__ __forceinline__ void replay_rejections(const Args& a, int bb, int m, double TL, double L, double2& v1,
-                                                  double2& dv1) {
-  const size_t so = (size_t)bb * a.sstride + m;
-  const int rs = a.s_rs[so], rc = a.s_rc[so];
-  const double dT = a.dTb[bb];
+__device__ __forceinline__ void replay_rejections(const Args& a, int bb, unsigned rs, int rc, double dT, double TL,
+                                                  double L, double2& v1, double2& dv1) {
   for (int q = 0; q < rc; ++q) {
     const double hj = a.s_rh[(size_t)bb * a.rstride + rs + q];
     const double ep = exp((TL * hj) * 0.5);
@@ -173,7 +209,7 @@ __device__ __forceinline__ void replay_rejections(const Args& a, int bb, int m, 
 // Lawson RK4 in the form of timestepping.py:70-80 and :263-282:
 //   y2 = e x0 + e k1/2, y3 = e x0 + k2/2, y4 = e e x0 + e k3,
 //   xf = e e x0 + e e k1/6 + e k2/3 + e k3/3 + k4/6,   k_s = dt v(y_s).
-template <int N, int MODE, bool TAN>
+template <int N, int MODE, bool TAN, int AK>
 __device__ __forceinline__ void glue(const Args& a, const Blk& bk, const Pre& p, int bb, size_t ci, int r, int c,
                                      int kx, int ky, double2 rh, double2 drh, double2& y, double2& dy) {
   const double k2 = (double)(kx * kx + ky * ky);
@@ -204,11 +240,11 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, const Pre& p,
   const double TL = bk.T * L;
   if (MODE == C_S5A) {
     // accept this attempt: k1 <- k5, x0 <- xf; then the next step's C_A glue
-    const double2 x0 = a.xf[ci], dx0 = a.dxf[ci];
+    const double2 x0 = p.xf, dx0 = p.dxf;
     a.x0[ci] = x0;
     a.dx0[ci] = dx0;
     double2 v1 = vh, dv1 = dvh;
-    if (bk.next) replay_rejections(a, bb, attempt_of(a) + 1, TL, L, v1, dv1);
+    if (bk.next) replay_rejections(a, bb, bk.rs, bk.rc, bk.dT, TL, L, v1, dv1);
     a.v1[ci] = v1;
     a.dv1[ci] = dv1;
     if (!bk.next) return;
@@ -224,7 +260,7 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, const Pre& p,
     dv1 = dvh;
   }
   if (MODE == C_A0 || MODE == C_A || MODE == C_FIN) {
-    if (a.adaptive == AD_PRIMAL) {
+    if (AK == AD_PRIMAL) {
       if (MODE == C_A0) {
         v1 = vh;
         a.v1[ci] = v1;
@@ -247,10 +283,10 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, const Pre& p,
         v1 = vh;
         dv1 = dvh;
       } else {
-        v1 = a.v1[ci];
-        dv1 = a.dv1[ci];
+        v1 = p.v1;
+        dv1 = p.dv1;
       }
-      replay_rejections(a, bb, attempt_of(a), TL, L, v1, dv1);
+      replay_rejections(a, bb, bk.rs, bk.rc, bk.dT, TL, L, v1, dv1);
       a.v1[ci] = v1;
       a.dv1[ci] = dv1;
     }
@@ -258,7 +294,7 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, const Pre& p,
   }
   // Lawson factor
   double e;
-  if (a.adaptive == AD_FIXED) {
+  if (AK == AD_FIXED) {
     e = exp((bk.dt * 0.5) * L);
     if (a.masked_e && dc) e = 0.0;
   } else {
@@ -303,7 +339,7 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, const Pre& p,
       dxf = cadd(p.dacc, cdivs(dks, 6.0));
       dy = dxf;
     }
-    if (a.adaptive == AD_FIXED) {
+    if (AK == AD_FIXED) {
       a.x0[ci] = xf;
       if (TAN) a.dx0[ci] = dxf;
     } else {
@@ -328,7 +364,7 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, const Pre& p,
   }
 }
 
-template <int N, int TY, int NT, int MODE, bool TAN, int P = 16>
+template <int N, int TY, int NT, int MODE, bool TAN, int P, int AK>
 __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   using G = Geo<N>;
   constexpr int KC = G::KC, R = G::R, KMAX = G::KMAX;
@@ -343,9 +379,10 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   constexpr int SS = Seq<N>::S;
   __shared__ double2 sm[NSEQ * SS];
   __shared__ double2 twl[N];
-  load_twiddles<N, NT, P>(twl, a.tw);
+  TwFetch<N, NT, P> twf;
+  twf.fetch(a.tw);
   const int b = blockIdx.z, c = blockIdx.y, ky0 = blockIdx.x * TY, tid = threadIdx.x;
-  const Blk bk = prologue<MODE, TAN>(a, b);
+  Blk bk = prologue<MODE, TAN, AK>(a, b);
   if (!bk.run) return;
   // issue the glue operand loads first: their latency overlaps the x-FFT
   Pre pre[EPT];
@@ -357,7 +394,7 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
     const int ky = ky0 + cy;
     if (idx < R * TY && ky < KC) {
       const size_t ci = (((size_t)b * 2 + c) * R + r) * KC + ky;
-      pre[q] = prefetch<MODE, TAN>(a, ci, (size_t)r * KC + ky, c);
+      pre[q] = prefetch<MODE, TAN, AK>(a, ci, (size_t)r * KC + ky, c);
     }
   }
 
@@ -365,11 +402,17 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
     const double2* O = a.rowout + (size_t)b * (2 * NF) * N * KC;
     // the first x-FFT pass reads the row output directly; with several
     // columns per block neighbouring lanes take neighbouring ky of one row
-    fft_lds_gather<N, NF * TY, SS, NT, false, P, (TY > 1)>(sm, twl, [&](int s, int x) -> double2 {
-      const int f = s / TY, ky = ky0 + s % TY;
-      return ky < KC ? O[((size_t)(f * 2 + c) * N + x) * KC + ky] : make_double2(0.0, 0.0);
-    });
+    fft_lds_gather<N, NF * TY, SS, NT, false, P, (TY > 1)>(
+        sm, twl,
+        [&](int s, int x) -> double2 {
+          const int f = s / TY, ky = ky0 + s % TY;
+          return ky < KC ? O[((size_t)(f * 2 + c) * N + x) * KC + ky] : make_double2(0.0, 0.0);
+        },
+        [&] { twf.store(twl); });
+  } else {
+    twf.store(twl);  // ordered by the barrier before the inverse FFT
   }
+  finish<AK>(bk);
 
   const bool inv = HASINV && (MODE == C_S5A ? bk.next : bk.stage) && !a.skip_inv;
   constexpr double invN2 = 1.0 / ((double)N * (double)N);
@@ -389,7 +432,7 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
       if (TAN) drh = sm[(1 * TY + cy) * SS + pidx(xr)];
     }
     double2 y, dy;
-    glue<N, MODE, TAN>(a, bk, pre[q], b, ci, r, c, kx, ky, rh, drh, y, dy);
+    glue<N, MODE, TAN, AK>(a, bk, pre[q], b, ci, r, c, kx, ky, rh, drh, y, dy);
     if (HASINV && inv) {
       if (NDO == 1) {
         sm[(0 * TY + cy) * SS + pidx(xr)] = cscale(invN2, y);
@@ -430,7 +473,7 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   }
 }
 
-template <int N, int TX, int NT, int RMODE, bool TAN, int P = 16>
+template <int N, int TX, int NT, int RMODE, bool TAN, int P, int AK>
 __global__ __launch_bounds__(NT) void row_kernel(Args a) {
   using G = Geo<N>;
   constexpr int KC = G::KC;
@@ -441,28 +484,30 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
   constexpr int SS = Seq<N>::S;
   __shared__ double2 sm[TX * NP * SS];
   __shared__ double2 twl[N];
-  load_twiddles<N, NT, P>(twl, a.tw);
+  TwFetch<N, NT, P> twf;
+  twf.fetch(a.tw);
   const int b = blockIdx.z, xbase = blockIdx.x * TX, tid = threadIdx.x;
-  if (!row_active(a, b)) return;
+  if (!row_active<AK>(a, b)) return;
   const double2* I = a.inter + (size_t)b * ND * N * KC;
   // Hermitian extension of the packed pairs (A + iB), formed while the first
   // inverse-FFT pass loads its inputs from I; the imaginary parts of the ky = 0
   // bin are dropped, as numpy's c2r does (ky = N/2 lies in the dealiased gap).
-  fft_lds_gather<N, TX * NP, SS, NT, true, P>(sm, twl, [&](int s, int m) -> double2 {
-    const int p = s % NP;
-    const int x = xbase + s / NP;
-    const double2* Ia = I + ((size_t)(2 * p) * N + x) * KC;
-    const double2* Ib = I + ((size_t)(2 * p + 1) * N + x) * KC;
-    if (m <= G::KMAX) {
-      const double2 A = Ia[m], B = Ib[m];
-      return m == 0 ? make_double2(A.x, B.x) : make_double2(A.x - B.y, A.y + B.x);
-    }
-    if (m >= N - G::KMAX) {
-      const double2 A = Ia[N - m], B = Ib[N - m];
-      return make_double2(A.x + B.y, B.x - A.y);
-    }
-    return make_double2(0.0, 0.0);
-  });
+  // Every lane loads (gap lanes re-read bin 0 and discard it), so all loads of
+  // the pass are in flight before the first wait.
+  fft_lds_gather<N, TX * NP, SS, NT, true, P>(
+      sm, twl,
+      [&](int s, int m) -> double2 {
+        const int p = s % NP;
+        const int x = xbase + s / NP;
+        const bool lo = m <= G::KMAX, hi = m >= N - G::KMAX;
+        const int k = lo ? m : (hi ? N - m : 0);
+        const double2 A = I[((size_t)(2 * p) * N + x) * KC + k];
+        const double2 B = I[((size_t)(2 * p + 1) * N + x) * KC + k];
+        const double2 vlo = m == 0 ? make_double2(A.x, B.x) : make_double2(A.x - B.y, A.y + B.x);
+        const double2 vhi = make_double2(A.x + B.y, B.x - A.y);
+        return lo ? vlo : (hi ? vhi : make_double2(0.0, 0.0));
+      },
+      [&] { twf.store(twl); });
 
   if (RMODE == R_ERR) {
     unsigned long long mx = 0ull;
