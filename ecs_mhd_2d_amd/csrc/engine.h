@@ -533,9 +533,13 @@ static int run_rk43_jvp(ecs_ctx* c, double* x, double* dx, int B, const double* 
       }));
     }
   } else {
+    int minacc = maxacc;
+    for (int b = 0; b < B; ++b) minacc = std::min(minacc, snacc[b]);
     for (int m = 0; m < maxacc; ++m) {
       Args am = a;
       am.attempt = m;
+      am.run_all = m < minacc;
+      am.next_all = m + 1 < minacc;
       TRY(step(am, st, m == 0, m == 0, m + 1 < maxacc));
     }
   }

@@ -139,6 +139,11 @@ struct Args {
   int adaptive;            // AdaptKind
   int masked_e;            // fixed step: e = exp(dt/2 L) * mask
   int skip_inv;            // 1: do not produce the inverse pass (last stage of a run)
+  // tangent replay with the index as an argument: 1 when every candidate runs
+  // accepted step `attempt` (resp. has a step after it), so the kernels need
+  // not read s_nacc before their gathers
+  int run_all;
+  int next_all;
 };
 
 }  // namespace ecs

@@ -24,7 +24,15 @@ struct Blk {
   int rc;
 };
 
-__device__ __forceinline__ int attempt_of(const Args& a) { return a.cnt ? a.cnt[a.cslot] : a.attempt; }
+// The index comes from the argument unless a device counter is set.  The
+// empty asm makes the argument an opaque register value: otherwise the
+// compiler turns the choice into one load through a selected pointer, and the
+// argument path would wait on a memory load too.
+__device__ __forceinline__ int attempt_of(const Args& a) {
+  int att = a.attempt;
+  asm volatile("" : "+s"(att));
+  return a.cnt ? a.cnt[a.cslot] : att;
+}
 
 __device__ __forceinline__ double relu_nan(double x) { return (x != x) ? x : (x > 0.0 ? x : 0.0); }
 
@@ -57,16 +65,17 @@ __device__ __forceinline__ Blk prologue(const Args& a, int b) {
     // only the run test waits here: the schedule entries below are loaded
     // now and used after the column gathers (finish() forms the scales)
     const int m = att;
-    const int nacc = a.s_nacc[b];
+    const bool known = a.run_all && (MODE != C_S5A || a.next_all);
+    const int nacc = known ? 0 : a.s_nacc[b];
     bk.dT = a.dTb[b];
-    bk.run = bk.stage = m < nacc;
+    bk.run = bk.stage = a.run_all || m < nacc;
     if (!bk.run) return bk;
     const size_t so = (size_t)b * a.sstride + m;
     bk.h = a.s_h[so];
     if (MODE == C_S5A) {
       // loaded unconditionally (from entry m when there is no next step, and
       // then unused) so no branch has to wait for them
-      bk.next = m + 1 < nacc;
+      bk.next = a.next_all || m + 1 < nacc;
       const size_t sn = bk.next ? so + 1 : so;
       bk.h2 = a.s_h[sn];
       bk.rs = (unsigned)a.s_rs[sn];
@@ -139,7 +148,7 @@ template <int AK>
 __device__ __forceinline__ bool row_active(const Args& a, int b) {
   if (AK == AD_FIXED) return !(a.nstep && a.step >= a.nstep[b]);
   if (AK == AD_PRIMAL) return a.hist[(size_t)b * a.hstride + attempt_of(a)].active != 0;
-  return attempt_of(a) < a.s_nacc[b];
+  return a.run_all || attempt_of(a) < a.s_nacc[b];
 }
 
 __device__ __forceinline__ double2 cdivs(double2 a, double s) { return make_double2(a.x / s, a.y / s); }
