@@ -347,7 +347,10 @@ def main():
         lib.ecs_tuned_tiles(nn, bb, arr)
         plans = (ctypes.c_int * 4)()
         lib.ecs_tuned_plans(nn, plans)
+        splits = (ctypes.c_int * 2)()
+        lib.ecs_tuned_splits(nn, bb, splits)
         tiles[f"n{nn}_b{bb}"] = {"col_ty": arr[0], "col_tan_ty": arr[1], "row_tx": arr[2], "row_tan_tx": arr[3],
+                                 "col_split": splits[0], "col_tan_split": splits[1],
                                  "fft_plan_col": plans[0], "fft_plan_col_tan": plans[1], "fft_plan_row": plans[2],
                                  "fft_plan_row_tan": plans[3]}
 

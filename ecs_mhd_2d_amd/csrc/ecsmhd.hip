@@ -580,7 +580,17 @@ extern "C" int ecs_tuned_tiles(int n, int batch, int* out) {
   for (int kind = 0; kind < 4; ++kind) {
     auto it = g_tune.find(tune_key(n, kind, batch));
     const int v = it == g_tune.end() ? -1 : it->second;
-    out[kind] = v < 0 ? -1 : (1 << (v % 3));
+    out[kind] = v < 0 ? -1 : var_tile(v);
+  }
+  return 0;
+}
+
+extern "C" int ecs_tuned_splits(int n, int batch, int* out) {
+  if (!out || batch < 1) return fail(1, "ecs_tuned_splits: bad arguments");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (int kind = 0; kind < 2; ++kind) {
+    auto it = g_tune.find(tune_key(n, kind, batch));
+    out[kind] = it == g_tune.end() ? -1 : var_split(it->second);
   }
   return 0;
 }

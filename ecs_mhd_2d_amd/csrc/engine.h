@@ -17,10 +17,12 @@ template <> struct DefTile<64> { static constexpr int COL = 2, ROW = 1; };
 template <> struct DefTile<128> { static constexpr int COL = 1, ROW = 0; };
 template <> struct DefTile<256> { static constexpr int COL = 1, ROW = 0; };
 
-template <int N, int TY, int MODE, bool TAN, int P, int AK>
+template <int N, int TY, int MODE, bool TAN, int P, int AK, int SPL = 1>
 static int col_v(const Args& a, int B, hipStream_t st) {
-  dim3 grid((Geo<N>::KC + TY - 1) / TY, 2, B);
-  hipLaunchKernelGGL((col_kernel<N, TY, NT, MODE, TAN, P, AK>), grid, dim3(NT), 0, st, a);
+  // modes that cannot split run their unsplit instance under a split variant
+  constexpr int SP = col_splittable<MODE>() ? SPL : 1;
+  dim3 grid((Geo<N>::KC + TY - 1) / TY, 2 * SP, B);
+  hipLaunchKernelGGL((col_kernel<N, TY, NT, MODE, TAN, P, AK, SP>), grid, dim3(NT), 0, st, a);
   LAUNCHCHK();
   return 0;
 }
@@ -31,7 +33,10 @@ static int row_v(const Args& a, int B, hipStream_t st) {
   LAUNCHCHK();
   return 0;
 }
-// variant v: tile 1 << (v % 3), plan wide (v < 3) or narrow (v >= 3)
+// Variants (var_* in host.h): v < 6: tile 1 << (v % 3), plan wide (v < 3) or
+// narrow; column kernels also v = 6..9: one column per block, its output
+// sequences split over 2 (even v) or 4 (odd v) blocks, plan wide (6, 7) or
+// narrow (8, 9).
 template <int N, int MODE, bool TAN, int AK>
 static int col_var(int v, const Args& a, int B, hipStream_t st) {
   switch (v) {
@@ -40,7 +45,11 @@ static int col_var(int v, const Args& a, int B, hipStream_t st) {
     case 2: return col_v<N, 4, MODE, TAN, 16, AK>(a, B, st);
     case 3: return col_v<N, 1, MODE, TAN, 8, AK>(a, B, st);
     case 4: return col_v<N, 2, MODE, TAN, 8, AK>(a, B, st);
-    default: return col_v<N, 4, MODE, TAN, 8, AK>(a, B, st);
+    case 5: return col_v<N, 4, MODE, TAN, 8, AK>(a, B, st);
+    case 6: return col_v<N, 1, MODE, TAN, 16, AK, 2>(a, B, st);
+    case 7: return col_v<N, 1, MODE, TAN, 16, AK, 4>(a, B, st);
+    case 8: return col_v<N, 1, MODE, TAN, 8, AK, 2>(a, B, st);
+    default: return col_v<N, 1, MODE, TAN, 8, AK, 4>(a, B, st);
   }
 }
 template <int N, int RMODE, bool TAN, int AK>
@@ -132,11 +141,11 @@ static int unpack(const double2* comp, double* full, int B, int zero_outside, hi
 }
 
 // ------------------------------------------------------------------ autotuner
-// Average time (ms) of variants [v0, v1) of one kernel family at batch B,
+// Average time (ms) of variants vs[0..nv) of one kernel family at batch B,
 // timed with HIP events on `st` over back-to-back launches on zeroed
 // workspace (the workspace must hold B candidates).
 template <int N>
-static int time_variants(ecs_ctx* c, int B, int kind, int v0, int v1, float* t, hipStream_t st) {
+static int time_variants(ecs_ctx* c, int B, int kind, const int* vs, int nv, float* t, hipStream_t st) {
   const size_t ce = c->compact_elems() * B * sizeof(double2);
   std::vector<double> dt(B, 1e-3);
   TRY(upload_scalars(c, 0, dt.data(), B, st));
@@ -156,9 +165,10 @@ static int time_variants(ecs_ctx* c, int B, int kind, int v0, int v1, float* t, 
   // cannot decide the choice
   const int reps = std::max(8, 256 / B);
   int rc = 0;
-  for (int v = v0; v < v1; ++v) t[v - v0] = 1e30f;
+  for (int i = 0; i < nv; ++i) t[i] = 1e30f;
   for (int round = 0; round < 3 && !rc; ++round) {
-    for (int v = v0; v < v1 && !rc; ++v) {
+    for (int iv = 0; iv < nv && !rc; ++iv) {
+      const int v = vs[iv];
       auto launch = [&]() -> int {
         switch (kind) {
           case K_COL: return col_var<N, C_S2, false, AD_FIXED>(v, a, B, st);
@@ -176,7 +186,7 @@ static int time_variants(ecs_ctx* c, int B, int kind, int v0, int v1, float* t, 
       if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = fail(2, "hipEventSynchronize");
       float ms = 0.f;
       if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(2, "hipEventElapsedTime");
-      t[v - v0] = std::min(t[v - v0], ms / reps);
+      t[iv] = std::min(t[iv], ms / reps);
     }
   }
   (void)hipEventDestroy(e0);
@@ -196,10 +206,40 @@ static int time_variants(ecs_ctx* c, int B, int kind, int v0, int v1, float* t, 
 // start of an entry point, before any input is packed (it overwrites the
 // workspace and may grow it to 16 candidates).
 // ECSMHD_NO_TUNE=1 keeps the default tiles and the wide plan;
-// ECSMHD_PLAN=wide|narrow fixes the plan and tunes the tiles only.
+// ECSMHD_PLAN=wide|narrow fixes the plan and tunes the tiles only;
+// ECSMHD_VARIANTS="N:c,ct,r,rt[;N:...]" fixes the variants (var_* in host.h)
+// of the four families for grid N at every batch, so a profiled run can use
+// the configuration an unprofiled run tuned.
+static bool forced_variants(int n, int v[4]) {
+  const char* s = getenv("ECSMHD_VARIANTS");
+  while (s && *s) {
+    int nn, a, b, c, d;
+    if (sscanf(s, "%d:%d,%d,%d,%d", &nn, &a, &b, &c, &d) == 5 && nn == n) {
+      const int t[4] = {a, b, c, d};
+      for (int k = 0; k < 4; ++k) {
+        if (t[k] < 0 || t[k] >= var_count(k)) return false;
+        v[k] = t[k];
+      }
+      return true;
+    }
+    s = strchr(s, ';');
+    if (s) ++s;
+  }
+  return false;
+}
+
 template <int N>
 static int autotune(ecs_ctx* c, int B, hipStream_t st) {
   if (getenv("ECSMHD_NO_TUNE")) return 0;
+  int fv[4];
+  if (forced_variants(N, fv)) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (int kind = 0; kind < 4; ++kind) {
+      g_plan[plan_key(N, kind)] = var_plan(fv[kind]);
+      g_tune[tune_key(N, kind, B)] = fv[kind];
+    }
+    return 0;
+  }
   static std::mutex run_mu;  // one tuning pass per grid size at a time
   std::lock_guard<std::mutex> rl(run_mu);
   const char* force = getenv("ECSMHD_PLAN");
@@ -216,9 +256,13 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
   if (plan[0] < 0 || plan[1] < 0 || plan[2] < 0 || plan[3] < 0) {
     constexpr int PB[2] = {1, 16};
     TRY(ensure_cap(c, PB[1]));
-    float t[2][4][6];
+    float t[2][4][10];
     for (int p = 0; p < 2; ++p)
-      for (int kind = 0; kind < 4; ++kind) TRY(time_variants<N>(c, PB[p], kind, 0, 6, t[p][kind], st));
+      for (int kind = 0; kind < 4; ++kind) {
+        int vs[10];
+        for (int v = 0; v < var_count(kind); ++v) vs[v] = v;
+        TRY(time_variants<N>(c, PB[p], kind, vs, var_count(kind), t[p][kind], st));
+      }
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (int kind = 0; kind < 4; ++kind) {
       int pl = plan[kind];
@@ -226,8 +270,8 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
       if (pl < 0) {
         float worst[2] = {0.f, 0.f};
         for (int p = 0; p < 2; ++p) {
-          float bp[2];
-          for (int q = 0; q < 2; ++q) bp[q] = std::min(t[p][kind][3 * q], std::min(t[p][kind][3 * q + 1], t[p][kind][3 * q + 2]));
+          float bp[2] = {1e30f, 1e30f};
+          for (int v = 0; v < var_count(kind); ++v) bp[var_plan(v)] = std::min(bp[var_plan(v)], t[p][kind][v]);
           for (int q = 0; q < 2; ++q) worst[q] = std::max(worst[q], bp[q] / std::min(bp[0], bp[1]));
         }
         pl = worst[1] < worst[0] ? 1 : 0;
@@ -235,9 +279,9 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
       plan[kind] = pl;
       g_plan[plan_key(N, kind)] = pl;
       for (int p = 0; p < 2; ++p) {
-        int v = 3 * pl;
-        for (int q = 3 * pl; q < 3 * pl + 3; ++q)
-          if (t[p][kind][q] < t[p][kind][v]) v = q;
+        int v = -1;
+        for (int q = 0; q < var_count(kind); ++q)
+          if (var_plan(q) == pl && (v < 0 || t[p][kind][q] < t[p][kind][v])) v = q;
         if (!g_tune.count(tune_key(N, kind, PB[p]))) g_tune[tune_key(N, kind, PB[p])] = v;
       }
     }
@@ -245,9 +289,14 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
   }
   int pick[4];
   for (int kind = 0; kind < 4; ++kind) {
-    float t[3];
-    TRY(time_variants<N>(c, B, kind, 3 * plan[kind], 3 * plan[kind] + 3, t, st));
-    pick[kind] = 3 * plan[kind] + (t[1] < t[0] ? (t[2] < t[1] ? 2 : 1) : (t[2] < t[0] ? 2 : 0));
+    int vs[10];
+    float t[10];
+    const int nv = var_list(kind, plan[kind], vs);
+    TRY(time_variants<N>(c, B, kind, vs, nv, t, st));
+    int best = 0;
+    for (int i = 1; i < nv; ++i)
+      if (t[i] < t[best]) best = i;
+    pick[kind] = vs[best];
   }
   std::lock_guard<std::mutex> lk(g_tune_mu);
   for (int kind = 0; kind < 4; ++kind) g_tune[tune_key(N, kind, B)] = pick[kind];

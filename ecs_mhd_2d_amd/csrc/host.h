@@ -136,12 +136,26 @@ int upload_ints(ecs_ctx* c, const int* vals, int B, hipStream_t st);
 // ------------------------------------------------------------------ autotuner tables
 // Tile variants: column kernels take TY in {1, 2, 4} columns per block, row
 // kernels TX in {1, 2, 4} rows per block; both come in a wide and a narrow
-// FFT radix plan.  Variant v: tile index v % 3 (TY or TX = 1 << (v % 3)),
-// plan v / 3 (0: wide, radix 16/8; 1: narrow, radix 8/4/2).  The autotuner
-// picks the plan once per (N, kernel family), at the first batch it sees, so
-// every batch size computes with the same arithmetic, and the tile per
-// (N, kernel family, batch bucket).
+// FFT radix plan.  Variant v < 6: tile 1 << (v % 3), plan v / 3 (0: wide,
+// radix 16/8; 1: narrow, radix 8/4/2).  Column families also have v = 6..9:
+// one column per block with the column's output sequences split over 2 (even
+// v) or 4 (odd v) blocks, plan (v - 6) / 2; the split spreads a small batch
+// over more CUs and does not change the arithmetic.  The autotuner picks the
+// plan once per (N, kernel family), so every batch size computes with the
+// same arithmetic, and the variant within that plan per (N, kernel family,
+// batch bucket).
 enum TuneKind : int { K_COL = 0, K_COLT = 1, K_ROW = 2, K_ROWT = 3 };
+inline int var_count(int kind) { return (kind == K_COL || kind == K_COLT) ? 10 : 6; }
+inline int var_plan(int v) { return v < 6 ? v / 3 : (v - 6) / 2; }
+inline int var_tile(int v) { return v < 6 ? 1 << (v % 3) : 1; }
+inline int var_split(int v) { return v < 6 ? 1 : (v % 2 == 0 ? 2 : 4); }
+// the variants of family `kind` in plan p; returns their number
+inline int var_list(int kind, int p, int* out) {
+  int n = 0;
+  for (int v = 0; v < var_count(kind); ++v)
+    if (var_plan(v) == p) out[n++] = v;
+  return n;
+}
 extern std::mutex g_tune_mu;
 extern std::map<long long, int> g_tune;  // (n, kind, bucket) -> variant
 extern std::map<long long, int> g_plan;  // (n, kind) -> plan index

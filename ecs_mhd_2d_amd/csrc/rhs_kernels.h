@@ -185,18 +185,19 @@ __device__ __forceinline__ Pre prefetch(const Args& a, size_t ci, size_t fi, int
 // Stage-1 glue of one Lawson step (timestepping.py:70-74, :263-270):
 // acc = e e x0 + e e k1/6, y2 = e x0 + e k1/2 with k1 = dt v1, and tangents.
 template <bool TAN>
-__device__ __forceinline__ void stage1(const Args& a, size_t ci, double e, double de, double dt, double ddt,
+__device__ __forceinline__ void stage1(const Args& a, bool wr, size_t ci, double e, double de, double dt, double ddt,
                                        double2 x0, double2 dx0, double2 v1, double2 dv1, double2& y, double2& dy) {
   const double de2 = 2.0 * e * de;
   const double2 k1 = cscale(dt, v1);
   const double2 ek1 = cscale(e, k1);
   const double2 ex0 = cscale(e, x0);
-  a.acc[ci] = cadd(cscale(e, ex0), cdivs(cscale(e, ek1), 6.0));
+  if (wr) a.acc[ci] = cadd(cscale(e, ex0), cdivs(cscale(e, ek1), 6.0));
   y = cadd(ex0, cdivs(ek1, 2.0));
   if (TAN) {
     const double2 dk1 = cadd(cscale(ddt, v1), cscale(dt, dv1));
-    a.dacc[ci] = cadd(cadd(cscale(de2, x0), cscale(e * e, dx0)),
-                      cdivs(cadd(cscale(de2, k1), cscale(e * e, dk1)), 6.0));
+    if (wr)
+      a.dacc[ci] = cadd(cadd(cscale(de2, x0), cscale(e * e, dx0)),
+                        cdivs(cadd(cscale(de2, k1), cscale(e * e, dk1)), 6.0));
     dy = cadd(cadd(cscale(de, x0), cscale(e, dx0)), cdivs(cadd(cscale(de, k1), cscale(e, dk1)), 2.0));
   }
 }
@@ -215,12 +216,14 @@ __device__ __forceinline__ void replay_rejections(const Args& a, int bb, unsigne
 
 // Stage glue for one kept mode.  rh/drh: row-pass spectrum after the x-FFT
 // (nonlinear term before masking).  Produces the next stage input y (dy).
+// Global writes happen only when wr is set (one of the blocks that share a
+// column when its output sequences are split over several blocks).
 // Lawson RK4 in the form of timestepping.py:70-80 and :263-282:
 //   y2 = e x0 + e k1/2, y3 = e x0 + k2/2, y4 = e e x0 + e k3,
 //   xf = e e x0 + e e k1/6 + e k2/3 + e k3/3 + k4/6,   k_s = dt v(y_s).
 template <int N, int MODE, bool TAN, int AK>
-__device__ __forceinline__ void glue(const Args& a, const Blk& bk, const Pre& p, int bb, size_t ci, int r, int c,
-                                     int kx, int ky, double2 rh, double2 drh, double2& y, double2& dy) {
+__device__ __forceinline__ void glue(const Args& a, bool wr, const Blk& bk, const Pre& p, int bb, size_t ci, int r,
+                                     int c, int kx, int ky, double2 rh, double2 drh, double2& y, double2& dy) {
   const double k2 = (double)(kx * kx + ky * ky);
   const bool dc = (kx == 0 && ky == 0);
   // mask (omega) and k^2 mask (j) of mhd_jax.py:160,164; forcing added spectrally
@@ -238,11 +241,11 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, const Pre& p,
   if (MODE == C_RHS) {
     double2 o = vh;
     if (a.include_diss) o = cadd(o, cscale(L, p.x0));
-    a.out[ci] = o;
+    if (wr) a.out[ci] = o;
     if (TAN) {
       double2 d = dvh;
       if (a.include_diss) d = cadd(d, cscale(L, p.dx0));
-      a.dout[ci] = d;
+      if (wr) a.dout[ci] = d;
     }
     return;
   }
@@ -250,16 +253,20 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, const Pre& p,
   if (MODE == C_S5A) {
     // accept this attempt: k1 <- k5, x0 <- xf; then the next step's C_A glue
     const double2 x0 = p.xf, dx0 = p.dxf;
-    a.x0[ci] = x0;
-    a.dx0[ci] = dx0;
+    if (wr) {
+      a.x0[ci] = x0;
+      a.dx0[ci] = dx0;
+    }
     double2 v1 = vh, dv1 = dvh;
     if (bk.next) replay_rejections(a, bb, bk.rs, bk.rc, bk.dT, TL, L, v1, dv1);
-    a.v1[ci] = v1;
-    a.dv1[ci] = dv1;
+    if (wr) {
+      a.v1[ci] = v1;
+      a.dv1[ci] = dv1;
+    }
     if (!bk.next) return;
     const double e = exp((TL * bk.h2) * 0.5);
     const double de = (bk.ddt2 * 0.5) * L * e;
-    stage1<TAN>(a, ci, e, de, bk.dt2, bk.ddt2, x0, dx0, v1, dv1, y, dy);
+    stage1<TAN>(a, wr, ci, e, de, bk.dt2, bk.ddt2, x0, dx0, v1, dv1, y, dy);
     return;
   }
   // v1 of the stage-1 glue
@@ -272,19 +279,19 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, const Pre& p,
     if (AK == AD_PRIMAL) {
       if (MODE == C_A0) {
         v1 = vh;
-        a.v1[ci] = v1;
+        if (wr) a.v1[ci] = v1;
       } else {
         v1 = a.v1[ci];
         if (bk.pend) {
           if (bk.pacc) {
-            a.x0[ci] = a.xf[ci];
+            if (wr) a.x0[ci] = a.xf[ci];
             v1 = a.v5[ci];
           } else {
             // rejected: carried k1 is e (e k1) (timestepping.py:267,276,297)
             const double ep = exp((TL * bk.hprev) * 0.5);
             v1 = cscale(ep, cscale(ep, v1));
           }
-          a.v1[ci] = v1;
+          if (wr) a.v1[ci] = v1;
         }
       }
     } else {  // tangent replay: rejections preceding this accepted attempt
@@ -296,8 +303,10 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, const Pre& p,
         dv1 = p.dv1;
       }
       replay_rejections(a, bb, bk.rs, bk.rc, bk.dT, TL, L, v1, dv1);
-      a.v1[ci] = v1;
-      a.dv1[ci] = dv1;
+      if (wr) {
+        a.v1[ci] = v1;
+        a.dv1[ci] = dv1;
+      }
     }
     if (!bk.stage) return;
   }
@@ -314,28 +323,28 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, const Pre& p,
   // C_A may have just replaced x0 (accepted previous attempt): read it back
   const double2 x0 = (MODE == C_A) ? a.x0[ci] : p.x0;
   const double2 dx0 = (MODE == C_A && TAN) ? a.dx0[ci] : p.dx0;
-  if (MODE == C_S1F && a.xtape) a.xtape[ci] = x0;
-  if ((MODE == C_S1F || MODE == C_S2 || MODE == C_S3 || MODE == C_S4) && a.vtape) a.vtape[ci] = vh;
+  if (MODE == C_S1F && a.xtape && wr) a.xtape[ci] = x0;
+  if ((MODE == C_S1F || MODE == C_S2 || MODE == C_S3 || MODE == C_S4) && a.vtape && wr) a.vtape[ci] = vh;
   if (MODE == C_S1F || MODE == C_A0 || MODE == C_A) {
-    stage1<TAN>(a, ci, e, de, bk.dt, bk.ddt, x0, dx0, v1, dv1, y, dy);
+    stage1<TAN>(a, wr, ci, e, de, bk.dt, bk.ddt, x0, dx0, v1, dv1, y, dy);
     return;
   }
   const double2 ks = cscale(bk.dt, vh);
   const double2 dks = TAN ? cadd(cscale(bk.ddt, vh), cscale(bk.dt, dvh)) : make_double2(0.0, 0.0);
   if (MODE == C_S2) {
-    a.acc[ci] = cadd(p.acc, cdivs(cscale(e, ks), 3.0));
+    if (wr) a.acc[ci] = cadd(p.acc, cdivs(cscale(e, ks), 3.0));
     y = cadd(cscale(e, x0), cdivs(ks, 2.0));
     if (TAN) {
-      a.dacc[ci] = cadd(p.dacc, cdivs(cadd(cscale(de, ks), cscale(e, dks)), 3.0));
+      if (wr) a.dacc[ci] = cadd(p.dacc, cdivs(cadd(cscale(de, ks), cscale(e, dks)), 3.0));
       dy = cadd(cadd(cscale(de, x0), cscale(e, dx0)), cdivs(dks, 2.0));
     }
     return;
   }
   if (MODE == C_S3) {
-    a.acc[ci] = cadd(p.acc, cdivs(cscale(e, ks), 3.0));
+    if (wr) a.acc[ci] = cadd(p.acc, cdivs(cscale(e, ks), 3.0));
     y = cadd(cscale(e, cscale(e, x0)), cscale(e, ks));
     if (TAN) {
-      a.dacc[ci] = cadd(p.dacc, cdivs(cadd(cscale(de, ks), cscale(e, dks)), 3.0));
+      if (wr) a.dacc[ci] = cadd(p.dacc, cdivs(cadd(cscale(de, ks), cscale(e, dks)), 3.0));
       dy = cadd(cadd(cscale(de2, x0), cscale(e * e, dx0)), cadd(cscale(de, ks), cscale(e, dks)));
     }
     return;
@@ -348,6 +357,7 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, const Pre& p,
       dxf = cadd(p.dacc, cdivs(dks, 6.0));
       dy = dxf;
     }
+    if (!wr) return;
     if (AK == AD_FIXED) {
       a.x0[ci] = xf;
       if (TAN) a.dx0[ci] = dxf;
@@ -373,7 +383,18 @@ __device__ __forceinline__ void glue(const Args& a, const Blk& bk, const Pre& p,
   }
 }
 
-template <int N, int TY, int NT, int MODE, bool TAN, int P, int AK>
+// Modes whose column work may be split over several blocks: each block of a
+// column runs the forward FFT and the glue, and produces NDO / SPL of the
+// derived output sequences; only split 0 writes the glue's global state.  A
+// mode qualifies when no block reads a value another block of its column
+// writes in the same launch (C_A reads back the state it replaces).
+template <int MODE>
+constexpr bool col_splittable() {
+  return MODE == C_LOAD || MODE == C_S1F || MODE == C_S2 || MODE == C_S3 || MODE == C_S4 || MODE == C_A0 ||
+         MODE == C_S5A;
+}
+
+template <int N, int TY, int NT, int MODE, bool TAN, int P, int AK, int SPL>
 __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   using G = Geo<N>;
   constexpr int KC = G::KC, R = G::R, KMAX = G::KMAX;
@@ -382,15 +403,21 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   constexpr bool FWD = MODE == C_RHS || MODE == C_S1F || MODE == C_S2 || MODE == C_S3 || MODE == C_S4 ||
                        MODE == C_S5 || MODE == C_A0 || MODE == C_S5A;
   constexpr bool HASINV = !(MODE == C_RHS || MODE == C_FIN || (MODE == C_S5 && TAN));
+  static_assert(SPL == 1 || (col_splittable<MODE>() && NDO % SPL == 0), "column split not allowed for this mode");
+  constexpr int DPS = NDO / SPL;               // derived outputs per block
   constexpr int EPT = (R * TY + NT - 1) / NT;  // glue elements per thread
-  constexpr int NSEQ = HASINV ? NDO * NF * TY : NF * TY;
+  constexpr int NSEQ = HASINV ? DPS * NF * TY : NF * TY;
+  constexpr int NSM = NSEQ > NF * TY ? NSEQ : NF * TY;
   constexpr int ND = (NDO == 1) ? 2 : 8 * NF;
   constexpr int SS = Seq<N>::S;
-  __shared__ double2 sm[NSEQ * SS];
+  __shared__ double2 sm[NSM * SS];
   __shared__ double2 twl[N];
   TwFetch<N, NT, P> twf;
   twf.fetch(a.tw);
-  const int b = blockIdx.z, c = blockIdx.y, ky0 = blockIdx.x * TY, tid = threadIdx.x;
+  const int b = blockIdx.z, ky0 = blockIdx.x * TY, tid = threadIdx.x;
+  const int c = SPL > 1 ? (int)blockIdx.y % 2 : (int)blockIdx.y;
+  const int sp = SPL > 1 ? (int)blockIdx.y / 2 : 0;
+  const bool wr = sp == 0;
   Blk bk = prologue<MODE, TAN, AK>(a, b);
   if (!bk.run) return;
   // issue the glue operand loads first: their latency overlaps the x-FFT
@@ -441,21 +468,20 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
       if (TAN) drh = sm[(1 * TY + cy) * SS + pidx(xr)];
     }
     double2 y, dy;
-    glue<N, MODE, TAN, AK>(a, bk, pre[q], b, ci, r, c, kx, ky, rh, drh, y, dy);
+    glue<N, MODE, TAN, AK>(a, wr, bk, pre[q], b, ci, r, c, kx, ky, rh, drh, y, dy);
     if (HASINV && inv) {
       if (NDO == 1) {
         sm[(0 * TY + cy) * SS + pidx(xr)] = cscale(invN2, y);
       } else {
         const bool dc = (kx == 0 && ky == 0);
         const double k2s = dc ? 1.0 : (double)(kx * kx + ky * ky);
-        const double mult[4] = {(double)kx, (double)ky, (double)ky / k2s, -(double)kx / k2s};
-        const double2 iy = cmuli(y);
+        const double2 iy = cmuli(y), idy = cmuli(dy);
 #pragma unroll
-        for (int d = 0; d < 4; ++d) sm[((d * NF + 0) * TY + cy) * SS + pidx(xr)] = cscale(mult[d] * invN2, iy);
-        if (TAN) {
-          const double2 idy = cmuli(dy);
-#pragma unroll
-          for (int d = 0; d < 4; ++d) sm[((d * NF + 1) * TY + cy) * SS + pidx(xr)] = cscale(mult[d] * invN2, idy);
+        for (int dl = 0; dl < DPS; ++dl) {
+          const int d = sp * DPS + dl;
+          const double md = d == 0 ? (double)kx : d == 1 ? (double)ky : d == 2 ? (double)ky / k2s : -(double)kx / k2s;
+          sm[((dl * NF + 0) * TY + cy) * SS + pidx(xr)] = cscale(md * invN2, iy);
+          if (TAN) sm[((dl * NF + 1) * TY + cy) * SS + pidx(xr)] = cscale(md * invN2, idy);
         }
       }
     }
@@ -476,7 +502,7 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
     const int s2 = t / N;
     const int ky = ky0 + cy;
     if (ky >= KC) continue;
-    const int d = s2 / NF, f = s2 % NF;
+    const int d = sp * DPS + s2 / NF, f = s2 % NF;
     const int g = (NDO == 1) ? c : (f * 8 + 4 * c + d);
     a.inter[(((size_t)b * ND + g) * N + x) * KC + ky] = sm[(s2 * TY + cy) * SS + pidx(x)];
   }

@@ -205,6 +205,13 @@ int ecs_tuned_tiles(int n, int batch, int* out);
  * (radix 8/4/2: 128 = 8x4x4, 256 = 8x8x4); -1 where not tuned yet. */
 int ecs_tuned_plans(int n, int* out);
 
+/* Column split of the autotuner for grid size n and this batch bucket (no
+ * reference counterpart): out[0] for the column kernel, out[1] for the
+ * tangent column kernel, the number of blocks that share one column's output
+ * sequences (1, 2 or 4; a split variant has one column per block); -1 where
+ * not tuned yet. */
+int ecs_tuned_splits(int n, int batch, int* out);
+
 #ifdef __cplusplus
 }
 #endif
