@@ -28,6 +28,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the two roofline kernels, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE passes over `bench.py --probe-roofline[-batched]`
+# (KiB per dispatch scaled by a 64 MiB device copy: FETCH x 2047.3, WRITE x
+# 1024); the runs are summarised in profiles/pmc_traffic_tangent_row_kernel_n128_b1.md
+# and profiles/pmc_batched_row_kernel_n256_b64.md.
+TRAFFIC_ROW_T_N128_B1 = 851.0 * 2047.3 + 352.0 * 1024
+TRAFFIC_ROW_N256_B64 = 90194.5 * 2047.3 + 44032.0 * 1024
 METRIC = "FP64 RK steps/s at N=256 + GMRES matvecs/s; p50/p99 RPO solve latency"
 
 
@@ -349,8 +356,11 @@ def main():
         lib.ecs_tuned_plans(nn, plans)
         splits = (ctypes.c_int * 2)()
         lib.ecs_tuned_splits(nn, bb, splits)
+        variants = (ctypes.c_int * 4)()
+        lib.ecs_tuned_variants(nn, bb, variants)
         tiles[f"n{nn}_b{bb}"] = {"col_ty": arr[0], "col_tan_ty": arr[1], "row_tx": arr[2], "row_tan_tx": arr[3],
                                  "col_split": splits[0], "col_tan_split": splits[1],
+                                 "variants": list(variants),
                                  "fft_plan_col": plans[0], "fft_plan_col_tan": plans[1], "fft_plan_row": plans[2],
                                  "fft_plan_row_tan": plans[3]}
 
@@ -376,12 +386,12 @@ def main():
                    "rejected": info["rejected"], "parallelism": "single GPU"},
         "rk4_steps_per_s_n256_b1": rk4[1], "rk4_steps_per_s_n256_b64": rk4[64],
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC_ROW_T_N128_B1,
                      "kernel": f"row_kernel<128,TX={tiles['n128_b1']['row_tan_tx']},tangent> (batch 1)",
                      "avg_ms": row_t_ms,
                      "algorithmic_bytes": bytes_row_t},
         "roofline_batched": {"bound": "hbm", "achieved": ach_b, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": ach_b / HBM_PEAK_GBS,
+                             "frac": ach_b / HBM_PEAK_GBS, "traffic": TRAFFIC_ROW_N256_B64,
                              "kernel": f"row_kernel<256,TX={tiles['n256_b64']['row_tx']}> (batch 64)",
                              "avg_ms": row_b_ms, "algorithmic_bytes": 64 * 10 * zbytes(256)},
         "cpu_baseline": cpu,

@@ -585,6 +585,16 @@ extern "C" int ecs_tuned_tiles(int n, int batch, int* out) {
   return 0;
 }
 
+extern "C" int ecs_tuned_variants(int n, int batch, int* out) {
+  if (!out || batch < 1) return fail(1, "ecs_tuned_variants: bad arguments");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (int kind = 0; kind < 4; ++kind) {
+    auto it = g_tune.find(tune_key(n, kind, batch));
+    out[kind] = it == g_tune.end() ? -1 : it->second;
+  }
+  return 0;
+}
+
 extern "C" int ecs_tuned_splits(int n, int batch, int* out) {
   if (!out || batch < 1) return fail(1, "ecs_tuned_splits: bad arguments");
   std::lock_guard<std::mutex> lk(g_tune_mu);
