@@ -196,6 +196,9 @@ def newton_job(fields, T, sx, n, adaptive, inner, iters):
         for k in range(inner):
             w = yield Op("jvp", n, lin=lin, tangent=_unflat(Q[k], n))
             w = _flat(w).contiguous().clone()
+            if not bool(torch.isfinite(w).all()):
+                raise FloatingPointError(f"non-finite matvec in Newton iteration {it}, Krylov column {k} "
+                                         f"(|b| = {float(nb):.3e}, T = {float(x['T']):.6g})")
             linalg._mgs(Q, k, w, Hd[k])
         H = np.zeros((inner + 1, inner))
         Hh = Hd.cpu().numpy()
@@ -228,6 +231,7 @@ class Job:
         self.id, self.kind, self.arrival, self.gen, self.mem, self.n = jid, kind, arrival, gen, mem_bytes, n
         self.pending = None
         self.result = None
+        self.error = None
         self.start = None
         self.finish = None
 
@@ -299,6 +303,11 @@ class Server:
                     done.append(j)
                     used -= j.mem
                     continue
+                except Exception as e:
+                    j.error, j.finish = repr(e), now
+                    done.append(j)
+                    used -= j.mem
+                    continue
                 active.append(j)
             if not active:
                 if waiting:
@@ -364,11 +373,19 @@ class Server:
                         used -= j.mem
                         if log is not None:
                             log(f"job {j.id} ({j.kind}, N={j.n}) done at {now:.1f} s, latency {now - j.arrival:.1f} s")
+                    except Exception as e:  # the job's own failure ends that job, not the server
+                        j.error, j.finish = repr(e), now
+                        done.append(j)
+                        used -= j.mem
+                        if log is not None:
+                            log(f"job {j.id} ({j.kind}, N={j.n}) FAILED at {now:.1f} s: {j.error}")
             active = still
         wall = time.perf_counter() - t0
         self.max_concurrent = getattr(self, "max_concurrent", 0)
-        lat = np.array([j.finish - j.arrival for j in done])
-        return done, {"jobs": len(done), "wall_s": wall, "throughput_jobs_per_s": len(done) / wall,
+        ok = [j for j in done if j.error is None]
+        lat = np.array([j.finish - j.arrival for j in ok]) if ok else np.array([np.nan])
+        return done, {"jobs": len(done), "failed_jobs": len(done) - len(ok), "wall_s": wall,
+                      "throughput_jobs_per_s": len(ok) / wall,
                       "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
                       "ticks": self.ticks, "mean_batch": self.batched_ops / max(1, self.ticks),
                       "peak_resident_bytes": peak, "hbm_budget_bytes": self.budget}

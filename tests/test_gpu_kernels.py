@@ -203,3 +203,29 @@ def test_lawson_rk4_vjp(n, steps, ckpt):
         lhs = O.rdot(fbar[b], dx[b])
         rhs = O.rdot(gb[b], df[b]) + hb[b] * ddt[b]
         assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+@pytest.mark.parametrize("n", [32, 128])
+def test_batched_uneven_rk43_jvp_equals_solo_and_oracle(n):
+    """Tangent replay of candidates with different periods (so different
+    numbers of accepted steps, some with rejections) in one batch: every
+    candidate bit for bit equal to its own one-candidate replay, and to the
+    oracle's replay of its schedule."""
+    print(f"n={n}", flush=True)
+    pd, ctx = setup(n)
+    Ts, dTs, atol = [0.35, 0.9, 0.6], [0.4, -0.3, 0.1], 1e-7
+    x0 = np.stack([spectrum(n, 60 + b) for b in range(3)])
+    dx0 = np.stack([spectrum(n, 70 + b) for b in range(3)])
+    x, infos = ctx.lawson_rk43(dev(x0), np.array(Ts), 1e-2, atol, 64, 8)
+    assert len({i["accepted"] for i in infos}) == 3
+    xj, dxj = ctx.lawson_rk43_jvp(dev(x0), dev(dx0), np.array(Ts), np.array(dTs), infos)
+    assert torch.equal(xj, x) or rel(host(xj), host(x)) < 1e-12
+    for b in range(3):
+        xs, inf_s = ctx.lawson_rk43(dev(x0[b:b + 1]), Ts[b], 1e-2, atol, 64, 8)
+        assert inf_s[0]["accepted"] == infos[b]["accepted"] and inf_s[0]["rejected"] == infos[b]["rejected"]
+        _, dxs = ctx.lawson_rk43_jvp(dev(x0[b:b + 1]), dev(dx0[b:b + 1]), Ts[b], dTs[b], inf_s)
+        assert torch.equal(dxs[0], dxj[b]), b
+        if n <= 32:
+            sched = [(infos[b]["hs"][i], bool(infos[b]["acc"][i])) for i in range(infos[b]["attempts"])]
+            _, rdx = O.lawson_rk43_jvp(x0[b], dx0[b], pd, Ts[b], dTs[b], sched)
+            assert rel(host(dxj[b]), rdx) < 1e-10, b

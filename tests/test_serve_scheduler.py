@@ -83,3 +83,25 @@ def test_max_batch_splits_groups_into_lanes():
     sizes = sorted(c[1] for c in ex.calls)
     assert sizes == [1, 2, 2]
     assert sorted(c[2] for c in ex.calls) == [0, 1, 2]   # concurrent tasks on one N get distinct lanes
+
+
+def failing_job(n, fail_at):
+    for k in range(3):
+        r = yield serve.Op("grad", n, steps=64, x=1.0)
+        if k == fail_at:
+            raise FloatingPointError("non-finite matvec")
+    return r
+
+
+def test_a_failing_job_ends_alone():
+    """A job whose own code raises is recorded as failed; the others finish
+    and the latency statistics cover the completed jobs only."""
+    ex = Exec()
+    arr = arrivals(4) + [(0.0, "newton", (lambda: failing_job(32, 1)), 10.0, 32)]
+    done, stats = serve.Server(executor=ex).run(arr)
+    assert stats["jobs"] == 5 and stats["failed_jobs"] == 1
+    bad = [j for j in done if j.error is not None]
+    assert len(bad) == 1 and "non-finite" in bad[0].error and bad[0].result is None
+    res = {j.id: j.result for j in done if j.error is None}
+    assert sorted(res.values()) == [0.0, 6.0, 12.0, 18.0]
+    assert np.isfinite(stats["p50_latency_s"])
