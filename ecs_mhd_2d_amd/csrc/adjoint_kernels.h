@@ -125,7 +125,9 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
   __shared__ double2 twl[N];
   TwFetch<N, NT> twf;
   twf.fetch(a.tw);
-  const int b = blockIdx.z, c = blockIdx.y, ky0 = blockIdx.x * TY, tid = threadIdx.x;
+  int lx, ly, lz;
+  xcd_tile(lx, ly, lz);
+  const int b = lz, c = ly, ky0 = lx * TY, tid = threadIdx.x;
   if (a.nstep && a.step >= a.nstep[b]) return;  // candidate past its own step count
   if (FWD) {
     const double2* AI = a.ain + (size_t)b * 8 * N * KC;
@@ -204,7 +206,9 @@ __global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
   __shared__ double2 twl[N];
   TwFetch<N, NT> twf;
   twf.fetch(a.tw);
-  const int b = blockIdx.z, xbase = blockIdx.x * TX, tid = threadIdx.x;
+  int lx, ly, lz;
+  xcd_tile(lx, ly, lz);
+  const int b = lz, xbase = lx * TX, tid = threadIdx.x;
   if (a.nstep && a.step >= a.nstep[b]) return;  // candidate past its own step count
   const double2* P = a.prim + (size_t)b * 8 * N * KC;
   const double2* AO = a.aout + (size_t)b * 2 * N * KC;

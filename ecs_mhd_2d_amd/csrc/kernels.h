@@ -33,6 +33,26 @@ struct Geo {
   __device__ __host__ static constexpr int row_x(int r) { return r <= KMAX ? r : r - R + N; }
 };
 
+// Logical block coordinates of the launch grid, remapped for the 8 XCDs of
+// an MI355X: the dispatcher deals workgroups to the XCDs round-robin by linear
+// id (x fastest), so linear id L runs on XCD L % 8.  The logical tiles are
+// dealt to the XCDs in contiguous chunks instead, so tiles with consecutive
+// logical x (neighbouring columns or rows, which share cache lines) run on one
+// XCD at about the same time and its L2 serves the shared lines once.  A
+// bijection of the grid onto itself: every tile runs exactly once.
+__device__ __forceinline__ void xcd_tile(int& lx, int& ly, int& lz) {
+  const int gx = (int)gridDim.x, gy = (int)gridDim.y;
+  const int total = gx * gy * (int)gridDim.z;
+  const int L = (int)blockIdx.x + gx * ((int)blockIdx.y + gy * (int)blockIdx.z);
+  const int q = total / 8, r = total % 8;
+  const int x = L % 8, loc = L / 8;
+  const int T = x < r ? x * (q + 1) + loc : r * (q + 1) + (x - r) * q + loc;
+  lx = T % gx;
+  const int t = T / gx;
+  ly = t % gy;
+  lz = t / gy;
+}
+
 // Per-attempt record of the adaptive controller, one per candidate and attempt.
 // Entry i describes attempt i; entry i+1 also records whether attempt i was
 // accepted (prev_acc), because that decision is taken by the first kernel of

@@ -39,9 +39,10 @@ __device__ __forceinline__ double relu_nan(double x) { return (x != x) ? x : (x 
 // Controller decision of timestepping.py:288-313, from the record of attempt i-1.
 // AK (an AdaptKind) is a template argument, so each kernel instance has one
 // straight-line prologue and its loads are not forced to complete where the
-// control paths of the other kinds would merge.
+// control paths of the other kinds would merge.  `lead` marks one block per
+// candidate, the one that writes the candidate's controller record.
 template <int MODE, bool TAN, int AK>
-__device__ __forceinline__ Blk prologue(const Args& a, int b) {
+__device__ __forceinline__ Blk prologue(const Args& a, int b, bool lead) {
   Blk bk;
   bk.run = true; bk.stage = true; bk.pend = false; bk.pacc = false;
   bk.hprev = 0.0; bk.T = 0.0; bk.h = 0.0; bk.dt = 0.0; bk.ddt = 0.0;
@@ -58,7 +59,7 @@ __device__ __forceinline__ Blk prologue(const Args& a, int b) {
   }
   bk.T = a.Tb[b];
   const int att = attempt_of(a);
-  if ((MODE == C_A || MODE == C_A0) && a.cnt && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 &&
+  if ((MODE == C_A || MODE == C_A0) && a.cnt && lead && b == 0 &&
       threadIdx.x == 0)
     a.cnt[a.cslot ^ 1] = att + 1;
   if (AK == AD_TANGENT) {
@@ -93,7 +94,7 @@ __device__ __forceinline__ Blk prologue(const Args& a, int b) {
   if (MODE == C_A0) {
     bk.h = a.h0;
     bk.dt = bk.h * bk.T;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    if (lead && threadIdx.x == 0) {
       Hist w; w.s = 0.0; w.h = a.h0; w.err = 0ull; w.active = 1; w.prev_acc = 0; w.nacc = 0; w.nrej = 0;
       H[0] = w;
     }
@@ -118,7 +119,7 @@ __device__ __forceinline__ Blk prologue(const Args& a, int b) {
     bk.run = bk.pend || act;
     bk.h = hn;
     bk.dt = hn * bk.T;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    if (lead && threadIdx.x == 0) {
       Hist w; w.s = s; w.h = hn; w.err = 0ull; w.active = act ? 1 : 0; w.prev_acc = bk.pacc ? 1 : 0;
       w.nacc = nacc; w.nrej = nrej;
       H[i] = w;
@@ -414,11 +415,13 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   __shared__ double2 twl[N];
   TwFetch<N, NT, P> twf;
   twf.fetch(a.tw);
-  const int b = blockIdx.z, ky0 = blockIdx.x * TY, tid = threadIdx.x;
-  const int c = SPL > 1 ? (int)blockIdx.y % 2 : (int)blockIdx.y;
-  const int sp = SPL > 1 ? (int)blockIdx.y / 2 : 0;
+  int lx, ly, lz;
+  xcd_tile(lx, ly, lz);
+  const int b = lz, ky0 = lx * TY, tid = threadIdx.x;
+  const int c = SPL > 1 ? ly % 2 : ly;
+  const int sp = SPL > 1 ? ly / 2 : 0;
   const bool wr = sp == 0;
-  Blk bk = prologue<MODE, TAN, AK>(a, b);
+  Blk bk = prologue<MODE, TAN, AK>(a, b, lx == 0 && ly == 0);
   if (!bk.run) return;
   // issue the glue operand loads first: their latency overlaps the x-FFT
   Pre pre[EPT];
@@ -521,7 +524,9 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
   __shared__ double2 twl[N];
   TwFetch<N, NT, P> twf;
   twf.fetch(a.tw);
-  const int b = blockIdx.z, xbase = blockIdx.x * TX, tid = threadIdx.x;
+  int lx, ly, lz;
+  xcd_tile(lx, ly, lz);
+  const int b = lz, xbase = lx * TX, tid = threadIdx.x;
   if (!row_active<AK>(a, b)) return;
   const double2* I = a.inter + (size_t)b * ND * N * KC;
   // Hermitian extension of the packed pairs (A + iB), formed while the first
