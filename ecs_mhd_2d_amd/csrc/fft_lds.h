@@ -38,10 +38,14 @@ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 
 // LDS sequences are padded by one element per 16 (element i of a sequence
 // lives at i + i/16), which keeps the strided butterfly writes of the
-// radix-8/16 passes free of bank conflicts.  SEQ<N> is the padded length.
+// radix-8/16 passes free of bank conflicts.  Seq<N>::S, the stride between
+// sequences, is one element more than the padded length: an odd number of
+// 16-byte elements, so lanes that touch the same element of neighbouring
+// sequences (the column kernel's tiles of several columns) fall in
+// different banks instead of all in the same ones.
 template <int N>
 struct Seq {
-  static constexpr int S = N + N / 16;
+  static constexpr int S = N + N / 16 + 1;
 };
 __device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
 
