@@ -407,7 +407,15 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   static_assert(SPL == 1 || (col_splittable<MODE>() && NDO % SPL == 0), "column split not allowed for this mode");
   constexpr int DPS = NDO / SPL;               // derived outputs per block
   constexpr int EPT = (R * TY + NT - 1) / NT;  // glue elements per thread
-  constexpr int NSEQ = HASINV ? DPS * NF * TY : NF * TY;
+  // A block's derived outputs are transformed in TCH chunks of at most 8
+  // sequences that reuse one LDS region (the glue results wait in registers),
+  // so tiles of 2-4 columns keep the LDS of a block small enough for several
+  // blocks per CU.
+  constexpr int NSF = DPS * NF * TY;
+  constexpr int TCH = (HASINV && NDO == 4 && NSF > 8) ? (NSF / 8 < DPS ? NSF / 8 : DPS) : 1;
+  static_assert(DPS % TCH == 0, "chunks must divide the derived outputs");
+  constexpr int DPC = DPS / TCH;  // derived outputs per chunk
+  constexpr int NSEQ = HASINV ? DPC * NF * TY : NF * TY;
   constexpr int NSM = NSEQ > NF * TY ? NSEQ : NF * TY;
   constexpr int ND = (NDO == 1) ? 2 : 8 * NF;
   constexpr int SS = Seq<N>::S;
@@ -455,6 +463,14 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
 
   const bool inv = HASINV && (MODE == C_S5A ? bk.next : bk.stage) && !a.skip_inv;
   constexpr double invN2 = 1.0 / ((double)N * (double)N);
+  // derived output d of stage input y at mode (kx, ky): i m_d y / N^2
+  auto derived = [&](int d, int kx, int ky, double2 y) -> double2 {
+    const bool dc = (kx == 0 && ky == 0);
+    const double k2s = dc ? 1.0 : (double)(kx * kx + ky * ky);
+    const double md = d == 0 ? (double)kx : d == 1 ? (double)ky : d == 2 ? (double)ky / k2s : -(double)kx / k2s;
+    return cscale(md * invN2, cmuli(y));
+  };
+  double2 yv[TCH > 1 ? EPT : 1], dyv[TCH > 1 ? EPT : 1];
 #pragma unroll
   for (int q = 0; q < EPT; ++q) {
     const int idx = tid + q * NT;
@@ -472,42 +488,63 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
     }
     double2 y, dy;
     glue<N, MODE, TAN, AK>(a, wr, bk, pre[q], b, ci, r, c, kx, ky, rh, drh, y, dy);
+    if (TCH > 1) {
+      yv[q] = y;
+      dyv[q] = dy;
+    }
     if (HASINV && inv) {
       if (NDO == 1) {
         sm[(0 * TY + cy) * SS + pidx(xr)] = cscale(invN2, y);
       } else {
-        const bool dc = (kx == 0 && ky == 0);
-        const double k2s = dc ? 1.0 : (double)(kx * kx + ky * ky);
-        const double2 iy = cmuli(y), idy = cmuli(dy);
 #pragma unroll
-        for (int dl = 0; dl < DPS; ++dl) {
+        for (int dl = 0; dl < DPC; ++dl) {
           const int d = sp * DPS + dl;
-          const double md = d == 0 ? (double)kx : d == 1 ? (double)ky : d == 2 ? (double)ky / k2s : -(double)kx / k2s;
-          sm[((dl * NF + 0) * TY + cy) * SS + pidx(xr)] = cscale(md * invN2, iy);
-          if (TAN) sm[((dl * NF + 1) * TY + cy) * SS + pidx(xr)] = cscale(md * invN2, idy);
+          sm[((dl * NF + 0) * TY + cy) * SS + pidx(xr)] = derived(d, kx, ky, y);
+          if (TAN) sm[((dl * NF + 1) * TY + cy) * SS + pidx(xr)] = derived(d, kx, ky, dy);
         }
       }
     }
   }
   if (!HASINV || !inv) return;
   constexpr int GAP = N - R;
-  for (int idx = tid; idx < NSEQ * GAP; idx += NT) {
-    const int s = idx / GAP;
-    const int g = idx - s * GAP;
-    sm[s * SS + pidx(KMAX + 1 + g)] = make_double2(0.0, 0.0);
-  }
-  __syncthreads();
-  fft_lds<N, NSEQ, SS, NT, true, P>(sm, twl);
-  for (int idx = tid; idx < NSEQ * N; idx += NT) {
-    const int cy = idx % TY;
-    const int t = idx / TY;
-    const int x = t % N;
-    const int s2 = t / N;
-    const int ky = ky0 + cy;
-    if (ky >= KC) continue;
-    const int d = sp * DPS + s2 / NF, f = s2 % NF;
-    const int g = (NDO == 1) ? c : (f * 8 + 4 * c + d);
-    a.inter[(((size_t)b * ND + g) * N + x) * KC + ky] = sm[(s2 * TY + cy) * SS + pidx(x)];
+  for (int ch = 0; ch < TCH; ++ch) {
+    if (ch > 0) {
+      __syncthreads();  // the previous chunk's sequences have been stored
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+        const int idx = tid + q * NT;
+        const int cy = idx % TY;
+        const int r = idx / TY;
+        const int ky = ky0 + cy;
+        if (idx >= R * TY || ky >= KC) continue;
+        const int kx = G::row_kx(r);
+        const int xr = G::row_x(r);
+#pragma unroll
+        for (int dl = 0; dl < DPC; ++dl) {
+          const int d = sp * DPS + ch * DPC + dl;
+          sm[((dl * NF + 0) * TY + cy) * SS + pidx(xr)] = derived(d, kx, ky, yv[q]);
+          if (TAN) sm[((dl * NF + 1) * TY + cy) * SS + pidx(xr)] = derived(d, kx, ky, dyv[q]);
+        }
+      }
+    }
+    for (int idx = tid; idx < NSEQ * GAP; idx += NT) {
+      const int s = idx / GAP;
+      const int g = idx - s * GAP;
+      sm[s * SS + pidx(KMAX + 1 + g)] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_lds<N, NSEQ, SS, NT, true, P>(sm, twl);
+    for (int idx = tid; idx < NSEQ * N; idx += NT) {
+      const int cy = idx % TY;
+      const int t = idx / TY;
+      const int x = t % N;
+      const int s2 = t / N;
+      const int ky = ky0 + cy;
+      if (ky >= KC) continue;
+      const int d = sp * DPS + ch * DPC + s2 / NF, f = s2 % NF;
+      const int g = (NDO == 1) ? c : (f * 8 + 4 * c + d);
+      a.inter[(((size_t)b * ND + g) * N + x) * KC + ky] = sm[(s2 * TY + cy) * SS + pidx(x)];
+    }
   }
 }
 
