@@ -7,7 +7,8 @@ atol=1e-8, h0=1e-2 (budget 64 x 64 attempts).  One "step" is one GMRES
 matvec: the JVP of objective_RPO through the whole adaptive trajectory, on a
 random unit tangent.  value = matvecs/s.
 
-Also reported: Lawson RK4 steps/s at N=256 (batch 1 and batch 64), the
+Also reported: Lawson RK4 steps/s at N=128 on four seeds (batch 4 and 1, with
+the oracle's rate beside it) and at N=256 (batch 1 and batch 64), the
 roofline of the dominant kernel of the matvec (tangent row kernel), measured
 live with HIP events, and the CPU oracle's matvec rate on a bounded sample.
 
@@ -101,6 +102,47 @@ def probe():
     lin.jvp(utils.unflatten(v, n))
     torch.cuda.synchronize()
     print("probe rk43 jvp", lin.info["accepted"], lin.info["rejected"], flush=True)
+
+
+def config0_leg(cpu=True):
+    """Lawson RK4 at N=128 on G(128, s), s = 0..3, h = 0.005, 1024 steps, as
+    one batch of 4 and as one trajectory; with the oracle's rate on a bounded
+    sample of one trajectory (1 thread) beside it."""
+    import torch
+    from ecs_mhd_2d_amd import mhd
+    from ecs_mhd_2d_amd.synthetic import spectrum
+    n, h, steps = 128, 0.005, 1024
+    pd = mhd.make_param_dict(n, 1 / 40, 1 / 40, (0.0, 0.1))
+    ctx = mhd.context(pd)
+    f4 = torch.as_tensor(np.stack([spectrum(n, s) for s in range(4)])).cuda()
+    ctx.lawson_rk4(f4, h, 8)
+    ctx.lawson_rk4(f4[:1], h, 8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.lawson_rk4(f4, h, steps)
+    torch.cuda.synchronize()
+    t4 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ctx.lawson_rk4(f4[:1], h, steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter() - t0
+    out = {"workload": "N=128, G(128, s) s=0..3, Lawson RK4 h=0.005, 1024 steps (T=5.12)",
+           "batch4_steps_per_s": 4 * steps / t4, "batch4_rhs_per_s": 16 * steps / t4,
+           "batch1_steps_per_s": steps / t1, "batch1_rhs_per_s": 4 * steps / t1}
+    if cpu:
+        from oracle import mhd_oracle as O
+        opd = O.make_params(n, 1 / 40, 1 / 40)
+        f = spectrum(n, 0)
+        O.lawson_rk4(f, 2 * h, 2, opd)  # warm-up
+        k, t = 0, 0.0
+        while t < 8.0 and k < steps:
+            t0 = time.perf_counter()
+            f = O.lawson_rk4(f, 16 * h, 16, opd)
+            t += time.perf_counter() - t0
+            k += 16
+        out["cpu_oracle_steps_per_s"] = k / t
+        out["cpu_sample"] = f"oracle Lawson RK4 on G(128, 0), the first {k} of 1024 steps ({t:.1f} s), 1 thread"
+    return out
 
 
 def newton_leg(inp, pd):
@@ -364,7 +406,7 @@ def main():
                                  "fft_plan_col": plans[0], "fft_plan_col_tan": plans[1], "fft_plan_row": plans[2],
                                  "fft_plan_row_tan": plans[3]}
 
-    extras = {}
+    extras = {"config0_lawson_rk4_n128": config0_leg(cpu=not args.no_cpu)}
     if not args.no_extras:
         extras["newton_iteration"] = newton_leg(inp, pd)
         extras["adjoint_gradient"] = gradient_leg()

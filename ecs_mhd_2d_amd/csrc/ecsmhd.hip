@@ -206,18 +206,9 @@ const GridOps* grid_ops(int n) {
 // contexts exist only for supported sizes (ecs_ctx_create checks), so the table is never null here
 #define OPS(n) grid_ops(n)
 
-extern "C" {
-
-const char* ecs_last_error(void) { return g_err.c_str(); }
-
-int ecs_ctx_create(ecs_ctx** out, int n, double nu, double eta, double b0x, double b0y, const double* fhat_full,
-                   int device) {
-  if (!out) return fail(1, "null ctx pointer");
-  if (n != 16 && n != 32 && n != 64 && n != 128 && n != 256) return fail(1, "unsupported grid size n");
-  std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
-  HIPCHK(hipSetDevice(device));
-  ecs_ctx* c = new ecs_ctx();
-  c->n = n; c->dev = device; c->nu = nu; c->eta = eta; c->b0x = b0x; c->b0y = b0y;
+// Twiddles and masked forcing spectrum of a new context (device memory).
+static int ctx_tables(ecs_ctx* c, const double* fhat_full) {
+  const int n = c->n;
   std::vector<double2> tw(n);
   const long double pi = 3.141592653589793238462643383279502884L;
   for (int j = 0; j < n; ++j) {
@@ -239,14 +230,38 @@ int ecs_ctx_create(ecs_ctx** out, int n, double nu, double eta, double b0x, doub
   }
   HIPCHK(hipMalloc((void**)&c->fhat, fc.size() * sizeof(double2)));
   HIPCHK(hipMemcpy(c->fhat, fc.data(), fc.size() * sizeof(double2), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" {
+
+const char* ecs_last_error(void) { return g_err.c_str(); }
+
+int ecs_ctx_create(ecs_ctx** out, int n, double nu, double eta, double b0x, double b0y, const double* fhat_full,
+                   int device) {
+  if (!out) return fail(1, "null ctx pointer");
+  if (n != 16 && n != 32 && n != 64 && n != 128 && n != 256) return fail(1, "unsupported grid size n");
+  std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
+  DevGuard dg(device);
+  if (!dg.ok) return fail(2, "ecs_ctx_create: cannot make the device current");
+  ecs_ctx* c = new ecs_ctx();
+  c->n = n; c->dev = device; c->nu = nu; c->eta = eta; c->b0x = b0x; c->b0y = b0y;
+  const int rc = ctx_tables(c, fhat_full);
+  if (rc) {
+    const std::string msg = g_err;
+    ecs_ctx_destroy(c);  // frees what was allocated before the failure
+    g_err = msg;
+    return rc;
+  }
   *out = c;
   return 0;
 }
 
+
 int ecs_ctx_destroy(ecs_ctx* c) {
   if (!c) return 0;
   std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
-  (void)hipSetDevice(c->dev);
+  DevGuard dg(c->dev);
   void* ps[] = {c->tw, c->fhat, c->I, c->O, c->x0, c->acc, c->v1, c->v4, c->v5, c->xf, c->out, c->dx0, c->dacc,
                 c->dv1, c->dv4, c->dxf, c->dout, c->scal, c->hist, c->s_h, c->s_rs, c->s_rc, c->s_rh, c->s_nacc,
                 c->ckpt, c->xtape, c->itape, c->red, c->AO, c->AI, c->xb, c->k1b, c->k2b, c->k3b, c->hb,

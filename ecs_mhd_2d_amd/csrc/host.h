@@ -45,9 +45,29 @@ struct CtxGuard {
     if (ok) flag->store(0);
   }
 };
+// Makes `dev` the calling thread's current device for the scope of a call
+// and restores the previous one afterwards: a context's workspace grows, and
+// its kernels launch, on the context's own device whatever the caller's
+// current device is.
+struct DevGuard {
+  int prev = -1;
+  bool ok = true;
+  explicit DevGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) {
+      ok = false;
+      return;
+    }
+    if (prev != dev) ok = hipSetDevice(dev) == hipSuccess;
+  }
+  ~DevGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
 #define GUARD(c)                                                                              \
   CtxGuard guard_(&(c)->busy);                                                                \
-  if (!guard_.ok) return fail(4, "context is in use by another call (use one context per thread)")
+  if (!guard_.ok) return fail(4, "context is in use by another call (use one context per thread)"); \
+  DevGuard devguard_((c)->dev);                                                               \
+  if (!devguard_.ok) return fail(2, "cannot make the context's device current")
 
 constexpr int NT = 256;
 
