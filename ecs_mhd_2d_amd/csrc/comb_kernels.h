@@ -15,24 +15,6 @@
 
 namespace ecs {
 
-constexpr int COMB_MAX = 10;
-
-struct Comb {
-  const double2* in[COMB_MAX];   // null: the term has no primal part (tangent-only)
-  const double2* din[COMB_MAX];  // tangent inputs (null: zero)
-  double coef[COMB_MAX];
-  int hp[COMB_MAX];              // power of h (0, 1 or 2)
-  int ep[COMB_MAX];              // power of E (0..6)
-  int nterm;
-  double ediv;
-  int masked;
-  const double* h;               // [B]
-  const double* dh;              // [B] (tangent)
-  double nu[2];
-  double2* out;
-  double2* dout;
-  size_t elems;                  // compact elements per candidate
-};
 
 template <int N, bool TAN>
 __global__ __launch_bounds__(256) void comb_kernel(Comb cb, int B) {

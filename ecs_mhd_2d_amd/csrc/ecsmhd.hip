@@ -265,7 +265,7 @@ int ecs_ctx_destroy(ecs_ctx* c) {
   void* ps[] = {c->tw, c->fhat, c->I, c->O, c->x0, c->acc, c->v1, c->v4, c->v5, c->xf, c->out, c->dx0, c->dacc,
                 c->dv1, c->dv4, c->dxf, c->dout, c->scal, c->hist, c->s_h, c->s_rs, c->s_rc, c->s_rh, c->s_nacc,
                 c->ckpt, c->xtape, c->itape, c->red, c->AO, c->AI, c->xb, c->k1b, c->k2b, c->k3b, c->hb,
-                c->gen, c->tab};
+                c->gen, c->tab, c->gcomb};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   drop_graphs(c);

@@ -70,7 +70,8 @@ static int row_var(int v, const Args& a, int B, hipStream_t st) {
 template <int MODE, bool TAN, int AK>
 constexpr bool col_kind_ok() {
   if (AK == AD_FIXED)
-    return MODE == C_LOAD || MODE == C_RHS || MODE == C_S1F || MODE == C_S2 || MODE == C_S3 || MODE == C_S4;
+    return MODE == C_LOAD || MODE == C_RHS || MODE == C_S1F || MODE == C_S2 || MODE == C_S3 || MODE == C_S4 ||
+           (MODE == C_GEN && !TAN);
   if (AK == AD_PRIMAL)
     return !TAN && (MODE == C_S2 || MODE == C_S3 || MODE == C_S4 || MODE == C_S5 || MODE == C_A0 ||
                     MODE == C_A || MODE == C_FIN);
@@ -977,6 +978,52 @@ static int run_gen(ecs_ctx* c, double* f, double* df, int B, const double* dt, c
   const int msk = scheme == G_LRK6 ? masked : 0;
   double2* x = slot(0);
   TRY(pack<N>(f, x, B, st));
+  if (!td && !TAN) {
+    // Fused form: each stage is a row pass and one C_GEN column kernel, which
+    // stores the stage's RHS v_k and forms the next stage input (or the step's
+    // result) from the tableau row in registers.  The combinations are the
+    // comb_kernel's, term by term, so the result is the same.
+    double2* y = slot(1);
+    auto v = [&](int j) { return slot(2 + j); };
+    std::vector<Comb> tabs(tb.S);
+    for (int k = 0; k < tb.S; ++k) {
+      const bool last = k == tb.S - 1;
+      CombBuilder cb(c, ce, tb.ediv, msk, last ? x : y, nullptr);
+      const int pk = last ? tb.p[tb.S] : tb.p[k + 1];
+      cb.term(x, nullptr, 1.0, 0, pk);
+      for (int j = 0; j <= k; ++j) {
+        const double cj = last ? tb.b[j] : tb.A[k + 1][j];
+        if (cj != 0.0) cb.term(v(j), nullptr, cj, 1, pk - tb.p[j]);
+      }
+      tabs[k] = cb.cb;
+    }
+    TRY(grow(&c->gcomb, &c->gcomb_cap, (tabs.size() * sizeof(Comb) + sizeof(double2) - 1) / sizeof(double2)));
+    Comb* dtabs = (Comb*)c->gcomb;
+    HIPCHK(hipMemcpyAsync(dtabs, tabs.data(), tabs.size() * sizeof(Comb), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));  // the host table goes out of scope
+    if (steps > 0) {
+      Args a0 = a;
+      a0.x0 = x;
+      a0.adaptive = AD_FIXED;
+      TRY((col<N, C_LOAD, false>(a0, B, st)));
+    }
+    for (int s = 0; s < steps; ++s) {
+      for (int k = 0; k < tb.S; ++k) {
+        Args ak = a;
+        ak.adaptive = AD_FIXED;
+        ak.include_diss = diss;
+        TRY((row<N, R_RHS, false>(ak, B, st)));
+        ak.x0 = k == 0 ? x : y;
+        ak.out = v(k);
+        ak.xf = k == tb.S - 1 ? x : y;
+        ak.gc = dtabs + k;
+        ak.skip_inv = (s == steps - 1 && k == tb.S - 1) ? 1 : 0;
+        TRY((col<N, C_GEN, false>(ak, B, st)));
+      }
+    }
+    TRY(unpack<N>(x, f, B, msk, st));
+    return 0;
+  }
   if (!td) {
     double2* y = slot(1);
     double2* dx = TAN ? slot(2) : nullptr;

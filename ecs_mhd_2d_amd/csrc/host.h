@@ -116,6 +116,8 @@ struct ecs_ctx {
   size_t it_cap = 0;
   double2* gen = nullptr;      // stage arrays of the generic steppers
   size_t gen_cap = 0;
+  double2* gcomb = nullptr;    // stage-combination tables of the fused generic steppers (Comb records)
+  size_t gcomb_cap = 0;
   double* tab = nullptr;       // per-step tables of the RK4(3) reverse pass
   size_t tab_cap = 0;
   double2 *AO = nullptr, *AI = nullptr, *xb = nullptr, *k1b = nullptr, *k2b = nullptr, *k3b = nullptr;

@@ -66,6 +66,29 @@ struct Hist {
   int nacc, nrej;           // counters before this attempt
 };
 
+// One stage combination out = sum_t coef_t h^hp_t E^ep_t in_t (with its
+// tangent), E = exp(h L / ediv); run by comb_kernel, or read from device
+// memory by the column kernel's C_GEN glue (which takes the term whose input
+// is its own output pointer from registers).
+constexpr int COMB_MAX = 10;
+
+struct Comb {
+  const double2* in[COMB_MAX];   // null: the term has no primal part (tangent-only)
+  const double2* din[COMB_MAX];  // tangent inputs (null: zero)
+  double coef[COMB_MAX];
+  int hp[COMB_MAX];              // power of h (0, 1 or 2)
+  int ep[COMB_MAX];              // power of E (0..6)
+  int nterm;
+  double ediv;
+  int masked;
+  const double* h;               // [B]
+  const double* dh;              // [B] (tangent)
+  double nu[2];
+  double2* out;
+  double2* dout;
+  size_t elems;                  // compact elements per candidate
+};
+
 // Column-kernel modes.
 enum ColMode : int {
   C_LOAD = 0,   // y = x0
@@ -79,6 +102,7 @@ enum ColMode : int {
   C_A = 8,      // adaptive: apply previous decision, new h, stage-1 glue from stored v1
   C_FIN = 9,    // adaptive: apply the last decision only
   C_S5A = 10,   // tangent replay: accept this attempt (C_S5) and run the next step's C_A in the same kernel
+  C_GEN = 11,   // generic explicit stepper: out = v(x0) [+ L x0], then the stage combination *gc -> xf -> y
 };
 
 // Row-kernel modes.
@@ -101,7 +125,8 @@ struct Args {
   double2* v4;
   double2* v5;
   double2* xf;
-  double2* out;            // C_RHS output
+  double2* out;            // C_RHS / C_GEN output
+  const Comb* gc;          // C_GEN: stage combination giving the next stage input (device memory)
   // tangents
   double2* dx0;
   double2* dacc;

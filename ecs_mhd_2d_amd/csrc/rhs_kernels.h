@@ -169,7 +169,7 @@ __device__ __forceinline__ Pre prefetch(const Args& a, size_t ci, size_t fi, int
   const double2 z = make_double2(0.0, 0.0);
   p.F = (c == 0 && MODE != C_LOAD) ? a.fhat[fi] : z;
   constexpr bool X0 = MODE == C_LOAD || MODE == C_RHS || MODE == C_S1F || MODE == C_S2 || MODE == C_S3 ||
-                      MODE == C_A0;
+                      MODE == C_A0 || MODE == C_GEN;
   constexpr bool ACC = MODE == C_S2 || MODE == C_S3 || MODE == C_S4;
   p.x0 = X0 ? a.x0[ci] : z;
   p.dx0 = (X0 && TAN) ? a.dx0[ci] : z;
@@ -237,6 +237,30 @@ __device__ __forceinline__ void glue(const Args& a, bool wr, const Blk& bk, cons
   if (MODE == C_LOAD) {
     y = p.x0;
     if (TAN) dy = p.dx0;
+    return;
+  }
+  if (MODE == C_GEN) {
+    // this stage's RHS, then the next stage input from the combination
+    double2 o = vh;
+    if (a.include_diss) o = cadd(o, cscale(L, p.x0));
+    a.out[ci] = o;
+    const Comb* cm = a.gc;
+    const double h = cm->h[bb];
+    const double E1 = exp(h / cm->ediv * L);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int t = 0; t < cm->nterm; ++t) {
+      double w = cm->coef[t];
+      for (int q = 0; q < cm->hp[t]; ++q) w *= h;
+      if (cm->ep[t] > 0) {
+        double Ep = E1;
+        for (int q = 1; q < cm->ep[t]; ++q) Ep *= E1;
+        w *= (cm->masked && dc) ? 0.0 : Ep;
+      }
+      const double2 v = cm->in[t] == a.out ? o : cm->in[t][ci];
+      acc = cadd(acc, cscale(w, v));
+    }
+    a.xf[ci] = acc;
+    y = acc;
     return;
   }
   if (MODE == C_RHS) {
@@ -402,7 +426,7 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   constexpr int NF = TAN ? 2 : 1;
   constexpr int NDO = (MODE == C_S5 && !TAN) ? 1 : 4;
   constexpr bool FWD = MODE == C_RHS || MODE == C_S1F || MODE == C_S2 || MODE == C_S3 || MODE == C_S4 ||
-                       MODE == C_S5 || MODE == C_A0 || MODE == C_S5A;
+                       MODE == C_S5 || MODE == C_A0 || MODE == C_S5A || MODE == C_GEN;
   constexpr bool HASINV = !(MODE == C_RHS || MODE == C_FIN || (MODE == C_S5 && TAN));
   static_assert(SPL == 1 || (col_splittable<MODE>() && NDO % SPL == 0), "column split not allowed for this mode");
   constexpr int DPS = NDO / SPL;               // derived outputs per block
