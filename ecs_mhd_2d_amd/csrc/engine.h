@@ -71,7 +71,7 @@ template <int MODE, bool TAN, int AK>
 constexpr bool col_kind_ok() {
   if (AK == AD_FIXED)
     return MODE == C_LOAD || MODE == C_RHS || MODE == C_S1F || MODE == C_S2 || MODE == C_S3 || MODE == C_S4 ||
-           (MODE == C_GEN && !TAN);
+           MODE == C_GEN;
   if (AK == AD_PRIMAL)
     return !TAN && (MODE == C_S2 || MODE == C_S3 || MODE == C_S4 || MODE == C_S5 || MODE == C_A0 ||
                     MODE == C_A || MODE == C_FIN);
@@ -978,22 +978,28 @@ static int run_gen(ecs_ctx* c, double* f, double* df, int B, const double* dt, c
   const int msk = scheme == G_LRK6 ? masked : 0;
   double2* x = slot(0);
   TRY(pack<N>(f, x, B, st));
-  if (!td && !TAN) {
+  if (!td) {
     // Fused form: each stage is a row pass and one C_GEN column kernel, which
-    // stores the stage's RHS v_k and forms the next stage input (or the step's
-    // result) from the tableau row in registers.  The combinations are the
-    // comb_kernel's, term by term, so the result is the same.
+    // stores the stage's RHS v_k (and tangent) and forms the next stage input
+    // (or the step's result) from the tableau row in registers.  The
+    // combinations are the comb_kernel's, term by term, so the result is the
+    // same.
     double2* y = slot(1);
-    auto v = [&](int j) { return slot(2 + j); };
+    double2* dx = TAN ? slot(2) : nullptr;
+    double2* dy = TAN ? slot(3) : nullptr;
+    const int v0 = TAN ? 4 : 2;
+    auto v = [&](int j) { return slot(v0 + j); };
+    auto dv = [&](int j) { return TAN ? slot(v0 + tb.S + j) : nullptr; };
+    if (TAN) TRY(pack<N>(df, dx, B, st));
     std::vector<Comb> tabs(tb.S);
     for (int k = 0; k < tb.S; ++k) {
       const bool last = k == tb.S - 1;
-      CombBuilder cb(c, ce, tb.ediv, msk, last ? x : y, nullptr);
+      CombBuilder cb(c, ce, tb.ediv, msk, last ? x : y, last ? dx : dy);
       const int pk = last ? tb.p[tb.S] : tb.p[k + 1];
-      cb.term(x, nullptr, 1.0, 0, pk);
+      cb.term(x, dx, 1.0, 0, pk);
       for (int j = 0; j <= k; ++j) {
         const double cj = last ? tb.b[j] : tb.A[k + 1][j];
-        if (cj != 0.0) cb.term(v(j), nullptr, cj, 1, pk - tb.p[j]);
+        if (cj != 0.0) cb.term(v(j), dv(j), cj, 1, pk - tb.p[j]);
       }
       tabs[k] = cb.cb;
     }
@@ -1004,54 +1010,27 @@ static int run_gen(ecs_ctx* c, double* f, double* df, int B, const double* dt, c
     if (steps > 0) {
       Args a0 = a;
       a0.x0 = x;
+      a0.dx0 = dx;
       a0.adaptive = AD_FIXED;
-      TRY((col<N, C_LOAD, false>(a0, B, st)));
+      TRY((col<N, C_LOAD, TAN>(a0, B, st)));
     }
     for (int s = 0; s < steps; ++s) {
       for (int k = 0; k < tb.S; ++k) {
         Args ak = a;
         ak.adaptive = AD_FIXED;
         ak.include_diss = diss;
-        TRY((row<N, R_RHS, false>(ak, B, st)));
+        TRY((row<N, R_RHS, TAN>(ak, B, st)));
+        const bool last = k == tb.S - 1;
         ak.x0 = k == 0 ? x : y;
+        ak.dx0 = k == 0 ? dx : dy;
         ak.out = v(k);
-        ak.xf = k == tb.S - 1 ? x : y;
+        ak.dout = dv(k);
+        ak.xf = last ? x : y;
+        ak.dxf = last ? dx : dy;
         ak.gc = dtabs + k;
-        ak.skip_inv = (s == steps - 1 && k == tb.S - 1) ? 1 : 0;
-        TRY((col<N, C_GEN, false>(ak, B, st)));
+        ak.skip_inv = (s == steps - 1 && last) ? 1 : 0;
+        TRY((col<N, C_GEN, TAN>(ak, B, st)));
       }
-    }
-    TRY(unpack<N>(x, f, B, msk, st));
-    return 0;
-  }
-  if (!td) {
-    double2* y = slot(1);
-    double2* dx = TAN ? slot(2) : nullptr;
-    double2* dy = TAN ? slot(3) : nullptr;
-    const int v0 = TAN ? 4 : 2;
-    auto v = [&](int j) { return slot(v0 + j); };
-    auto dv = [&](int j) { return TAN ? slot(v0 + tb.S + j) : nullptr; };
-    if (TAN) TRY(pack<N>(df, dx, B, st));
-    for (int s = 0; s < steps; ++s) {
-      for (int k = 0; k < tb.S; ++k) {
-        double2* in = x;
-        double2* din = dx;
-        if (k > 0) {
-          CombBuilder cb(c, ce, tb.ediv, msk, y, dy);
-          cb.term(x, dx, 1.0, 0, tb.p[k]);
-          for (int j = 0; j < k; ++j)
-            if (tb.A[k][j] != 0.0) cb.term(v(j), dv(j), tb.A[k][j], 1, tb.p[k] - tb.p[j]);
-          TRY((comb_launch<N, TAN>(cb.cb, B, st)));
-          in = y;
-          din = dy;
-        }
-        TRY((rhs_pass<N, TAN>(a, in, din, v(k), dv(k), B, diss, st)));
-      }
-      CombBuilder cb(c, ce, tb.ediv, msk, x, dx);
-      cb.term(x, dx, 1.0, 0, tb.p[tb.S]);
-      for (int j = 0; j < tb.S; ++j)
-        if (tb.b[j] != 0.0) cb.term(v(j), dv(j), tb.b[j], 1, tb.p[tb.S] - tb.p[j]);
-      TRY((comb_launch<N, TAN>(cb.cb, B, st)));
     }
     TRY(unpack<N>(x, f, B, msk, st));
     if (TAN) TRY(unpack<N>(dx, df, B, msk, st));

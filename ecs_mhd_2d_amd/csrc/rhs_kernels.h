@@ -240,14 +240,20 @@ __device__ __forceinline__ void glue(const Args& a, bool wr, const Blk& bk, cons
     return;
   }
   if (MODE == C_GEN) {
-    // this stage's RHS, then the next stage input from the combination
-    double2 o = vh;
-    if (a.include_diss) o = cadd(o, cscale(L, p.x0));
+    // this stage's RHS (and tangent), then the next stage input from the
+    // combination, term by term as comb_kernel evaluates it
+    double2 o = vh, od = dvh;
+    if (a.include_diss) {
+      o = cadd(o, cscale(L, p.x0));
+      if (TAN) od = cadd(od, cscale(L, p.dx0));
+    }
     a.out[ci] = o;
+    if (TAN) a.dout[ci] = od;
     const Comb* cm = a.gc;
     const double h = cm->h[bb];
+    const double dh = TAN ? cm->dh[bb] : 0.0;
     const double E1 = exp(h / cm->ediv * L);
-    double2 acc = make_double2(0.0, 0.0);
+    double2 acc = make_double2(0.0, 0.0), dacc = acc;
     for (int t = 0; t < cm->nterm; ++t) {
       double w = cm->coef[t];
       for (int q = 0; q < cm->hp[t]; ++q) w *= h;
@@ -258,9 +264,25 @@ __device__ __forceinline__ void glue(const Args& a, bool wr, const Blk& bk, cons
       }
       const double2 v = cm->in[t] == a.out ? o : cm->in[t][ci];
       acc = cadd(acc, cscale(w, v));
+      if (TAN) {
+        double dw = cm->coef[t] * (double)cm->hp[t];
+        for (int q = 1; q < cm->hp[t]; ++q) dw *= h;
+        if (cm->ep[t] > 0) {
+          double Ep = E1;
+          for (int q = 1; q < cm->ep[t]; ++q) Ep *= E1;
+          const double em = (cm->masked && dc) ? 0.0 : Ep;
+          dw = dw * em + w * ((double)cm->ep[t] * L / cm->ediv);
+        }
+        dacc = cadd(dacc, cscale(dh * dw, v));
+        if (cm->din[t]) dacc = cadd(dacc, cscale(w, cm->din[t] == a.dout ? od : cm->din[t][ci]));
+      }
     }
     a.xf[ci] = acc;
     y = acc;
+    if (TAN) {
+      a.dxf[ci] = dacc;
+      dy = dacc;
+    }
     return;
   }
   if (MODE == C_RHS) {
