@@ -71,7 +71,7 @@ template <int MODE, bool TAN, int AK>
 constexpr bool col_kind_ok() {
   if (AK == AD_FIXED)
     return MODE == C_LOAD || MODE == C_RHS || MODE == C_S1F || MODE == C_S2 || MODE == C_S3 || MODE == C_S4 ||
-           MODE == C_GEN;
+           MODE == C_GEN || (MODE == C_TD1 && !TAN) || (MODE == C_TD2 && TAN);
   if (AK == AD_PRIMAL)
     return !TAN && (MODE == C_S2 || MODE == C_S3 || MODE == C_S4 || MODE == C_S5 || MODE == C_A0 ||
                     MODE == C_A || MODE == C_FIN);
@@ -1041,19 +1041,49 @@ static int run_gen(ecs_ctx* c, double* f, double* df, int B, const double* dt, c
   // (a_j = h J k_j = h^2 w_j).  Tangents use J for the first-order terms and
   // dw = H[dz, v] + J(z) dv; J(z) v and J(z) dv share one pass of batch 2B.
   if (!TAN) {
-    double2 *y = slot(1), *v1 = slot(2), *w1 = slot(3), *v2 = slot(4), *w2 = slot(5), *scr = slot(6);
+    // Fused: per half step a row pass, C_TD1 (v = RHS(z), then the pair
+    // (z, v) transformed as a tangent pass's input), the tangent row pass and
+    // C_TD2 (w = J(z) v, then the combination giving the next point), so 8
+    // launches per step; the combinations are comb_kernel's term by term.
+    double2 *y = slot(1), *v1 = slot(2), *w1 = slot(3), *v2 = slot(4), *w2 = slot(5);
+    CombBuilder cy(c, ce, 1.0, 0, y, nullptr);
+    cy.term(x, nullptr, 1.0, 0, 0).term(v1, nullptr, 0.5, 1, 0).term(w1, nullptr, 0.125, 2, 0);
+    CombBuilder cx(c, ce, 1.0, 0, x, nullptr);
+    cx.term(x, nullptr, 1.0, 0, 0).term(v1, nullptr, 1.0, 1, 0).term(w1, nullptr, 1.0 / 6, 2, 0)
+      .term(w2, nullptr, 1.0 / 3, 2, 0);
+    const Comb tabs[2] = {cy.cb, cx.cb};
+    TRY(grow(&c->gcomb, &c->gcomb_cap, (sizeof(tabs) + sizeof(double2) - 1) / sizeof(double2)));
+    Comb* dtabs = (Comb*)c->gcomb;
+    HIPCHK(hipMemcpyAsync(dtabs, tabs, sizeof(tabs), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));  // the host table goes out of scope
+    Args a0 = a;
+    a0.adaptive = AD_FIXED;
+    a0.include_diss = diss;
+    if (steps > 0) {
+      Args al = a0;
+      al.x0 = x;
+      TRY((col<N, C_LOAD, false>(al, B, st)));
+    }
     for (int s = 0; s < steps; ++s) {
-      TRY((rhs_pass<N, false>(a, x, nullptr, v1, nullptr, B, diss, st)));
-      TRY((rhs_pass<N, true>(a, x, v1, scr, w1, B, diss, st)));
-      CombBuilder cy(c, ce, 1.0, 0, y, nullptr);
-      cy.term(x, nullptr, 1.0, 0, 0).term(v1, nullptr, 0.5, 1, 0).term(w1, nullptr, 0.125, 2, 0);
-      TRY((comb_launch<N, false>(cy.cb, B, st)));
-      TRY((rhs_pass<N, false>(a, y, nullptr, v2, nullptr, B, diss, st)));
-      TRY((rhs_pass<N, true>(a, y, v2, scr, w2, B, diss, st)));
-      CombBuilder cx(c, ce, 1.0, 0, x, nullptr);
-      cx.term(x, nullptr, 1.0, 0, 0).term(v1, nullptr, 1.0, 1, 0).term(w1, nullptr, 1.0 / 6, 2, 0)
-        .term(w2, nullptr, 1.0 / 3, 2, 0);
-      TRY((comb_launch<N, false>(cx.cb, B, st)));
+      for (int half = 0; half < 2; ++half) {
+        double2* z = half == 0 ? x : y;
+        double2* v = half == 0 ? v1 : v2;
+        double2* w = half == 0 ? w1 : w2;
+        TRY((row<N, R_RHS, false>(a0, B, st)));
+        Args a1 = a0;
+        a1.x0 = z;
+        a1.out = v;
+        TRY((col<N, C_TD1, false>(a1, B, st)));
+        TRY((row<N, R_RHS, true>(a0, B, st)));
+        Args a2 = a0;
+        a2.x0 = z;
+        a2.dx0 = v;
+        a2.dout = w;
+        a2.gc = dtabs + half;
+        a2.xf = half == 0 ? y : x;
+        a2.skip_inv = (s == steps - 1 && half == 1) ? 1 : 0;
+        TRY((col<N, C_TD2, true>(a2, B, st)));
+      }
     }
     TRY(unpack<N>(x, f, B, 0, st));
     return 0;

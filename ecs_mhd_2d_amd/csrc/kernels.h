@@ -103,6 +103,11 @@ enum ColMode : int {
   C_FIN = 9,    // adaptive: apply the last decision only
   C_S5A = 10,   // tangent replay: accept this attempt (C_S5) and run the next step's C_A in the same kernel
   C_GEN = 11,   // generic explicit stepper: out = v(x0) [+ L x0], then the stage combination *gc -> xf -> y
+  // two-derivative RK4 (primal): C_TD1 (no tangent input) stores out = v(x0) and transforms the
+  // pair (x0, v) as a tangent pass's input; C_TD2 (tangent input) stores dout = J v (the tangent
+  // output) and forms the next stage input (or the step's result) from *gc
+  C_TD1 = 12,
+  C_TD2 = 13,
 };
 
 // Row-kernel modes.

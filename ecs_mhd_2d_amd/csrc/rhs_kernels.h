@@ -169,7 +169,7 @@ __device__ __forceinline__ Pre prefetch(const Args& a, size_t ci, size_t fi, int
   const double2 z = make_double2(0.0, 0.0);
   p.F = (c == 0 && MODE != C_LOAD) ? a.fhat[fi] : z;
   constexpr bool X0 = MODE == C_LOAD || MODE == C_RHS || MODE == C_S1F || MODE == C_S2 || MODE == C_S3 ||
-                      MODE == C_A0 || MODE == C_GEN;
+                      MODE == C_A0 || MODE == C_GEN || MODE == C_TD1 || MODE == C_TD2;
   constexpr bool ACC = MODE == C_S2 || MODE == C_S3 || MODE == C_S4;
   p.x0 = X0 ? a.x0[ci] : z;
   p.dx0 = (X0 && TAN) ? a.dx0[ci] : z;
@@ -237,6 +237,34 @@ __device__ __forceinline__ void glue(const Args& a, bool wr, const Blk& bk, cons
   if (MODE == C_LOAD) {
     y = p.x0;
     if (TAN) dy = p.dx0;
+    return;
+  }
+  if (MODE == C_TD1) {
+    // v = RHS(x0); the next pass is the tangent pass at x0 along v
+    double2 o = vh;
+    if (a.include_diss) o = cadd(o, cscale(L, p.x0));
+    a.out[ci] = o;
+    y = p.x0;
+    dy = o;
+    return;
+  }
+  if (MODE == C_TD2) {
+    // w = J(x0) dx0 (the tangent output), then the combination of *gc, whose
+    // term reading dout takes w from registers
+    double2 od = dvh;
+    if (a.include_diss) od = cadd(od, cscale(L, p.dx0));
+    a.dout[ci] = od;
+    const Comb* cm = a.gc;
+    const double h = cm->h[bb];
+    double2 acc = make_double2(0.0, 0.0);
+    for (int t = 0; t < cm->nterm; ++t) {
+      double w = cm->coef[t];
+      for (int q = 0; q < cm->hp[t]; ++q) w *= h;
+      const double2 v = cm->in[t] == a.dout ? od : cm->in[t][ci];
+      acc = cadd(acc, cscale(w, v));
+    }
+    a.xf[ci] = acc;
+    y = acc;
     return;
   }
   if (MODE == C_GEN) {
@@ -445,10 +473,16 @@ template <int N, int TY, int NT, int MODE, bool TAN, int P, int AK, int SPL>
 __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   using G = Geo<N>;
   constexpr int KC = G::KC, R = G::R, KMAX = G::KMAX;
+  // fields of the row output read (NFI) and of the derived output written
+  // (NFO): primal only, or primal and tangent; they differ only in the
+  // two-derivative RK4 modes
   constexpr int NF = TAN ? 2 : 1;
+  constexpr int NFI = NF;
+  constexpr int NFO = MODE == C_TD1 ? 2 : (MODE == C_TD2 ? 1 : NF);
   constexpr int NDO = (MODE == C_S5 && !TAN) ? 1 : 4;
   constexpr bool FWD = MODE == C_RHS || MODE == C_S1F || MODE == C_S2 || MODE == C_S3 || MODE == C_S4 ||
-                       MODE == C_S5 || MODE == C_A0 || MODE == C_S5A || MODE == C_GEN;
+                       MODE == C_S5 || MODE == C_A0 || MODE == C_S5A || MODE == C_GEN || MODE == C_TD1 ||
+                       MODE == C_TD2;
   constexpr bool HASINV = !(MODE == C_RHS || MODE == C_FIN || (MODE == C_S5 && TAN));
   static_assert(SPL == 1 || (col_splittable<MODE>() && NDO % SPL == 0), "column split not allowed for this mode");
   constexpr int DPS = NDO / SPL;               // derived outputs per block
@@ -457,13 +491,13 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   // sequences that reuse one LDS region (the glue results wait in registers),
   // so tiles of 2-4 columns keep the LDS of a block small enough for several
   // blocks per CU.
-  constexpr int NSF = DPS * NF * TY;
+  constexpr int NSF = DPS * NFO * TY;
   constexpr int TCH = (HASINV && NDO == 4 && NSF > 8) ? (NSF / 8 < DPS ? NSF / 8 : DPS) : 1;
   static_assert(DPS % TCH == 0, "chunks must divide the derived outputs");
   constexpr int DPC = DPS / TCH;  // derived outputs per chunk
-  constexpr int NSEQ = HASINV ? DPC * NF * TY : NF * TY;
-  constexpr int NSM = NSEQ > NF * TY ? NSEQ : NF * TY;
-  constexpr int ND = (NDO == 1) ? 2 : 8 * NF;
+  constexpr int NSEQ = HASINV ? DPC * NFO * TY : NFI * TY;
+  constexpr int NSM = NSEQ > NFI * TY ? NSEQ : NFI * TY;
+  constexpr int ND = (NDO == 1) ? 2 : 8 * NFO;
   constexpr int SS = Seq<N>::S;
   __shared__ double2 sm[NSM * SS];
   __shared__ double2 twl[N];
@@ -492,10 +526,10 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   }
 
   if (FWD) {
-    const double2* O = a.rowout + (size_t)b * (2 * NF) * N * KC;
+    const double2* O = a.rowout + (size_t)b * (2 * NFI) * N * KC;
     // the first x-FFT pass reads the row output directly; with several
     // columns per block neighbouring lanes take neighbouring ky of one row
-    fft_lds_gather<N, NF * TY, SS, NT, false, P, (TY > 1)>(
+    fft_lds_gather<N, NFI * TY, SS, NT, false, P, (TY > 1)>(
         sm, twl,
         [&](int s, int x) -> double2 {
           const int f = s / TY, ky = ky0 + s % TY;
@@ -530,7 +564,7 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
     double2 rh = make_double2(0.0, 0.0), drh = rh;
     if (FWD) {
       rh = sm[(0 * TY + cy) * SS + pidx(xr)];
-      if (TAN) drh = sm[(1 * TY + cy) * SS + pidx(xr)];
+      if (NFI == 2) drh = sm[(1 * TY + cy) * SS + pidx(xr)];
     }
     double2 y, dy;
     glue<N, MODE, TAN, AK>(a, wr, bk, pre[q], b, ci, r, c, kx, ky, rh, drh, y, dy);
@@ -545,8 +579,8 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
 #pragma unroll
         for (int dl = 0; dl < DPC; ++dl) {
           const int d = sp * DPS + dl;
-          sm[((dl * NF + 0) * TY + cy) * SS + pidx(xr)] = derived(d, kx, ky, y);
-          if (TAN) sm[((dl * NF + 1) * TY + cy) * SS + pidx(xr)] = derived(d, kx, ky, dy);
+          sm[((dl * NFO + 0) * TY + cy) * SS + pidx(xr)] = derived(d, kx, ky, y);
+          if (NFO == 2) sm[((dl * NFO + 1) * TY + cy) * SS + pidx(xr)] = derived(d, kx, ky, dy);
         }
       }
     }
@@ -568,8 +602,8 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
 #pragma unroll
         for (int dl = 0; dl < DPC; ++dl) {
           const int d = sp * DPS + ch * DPC + dl;
-          sm[((dl * NF + 0) * TY + cy) * SS + pidx(xr)] = derived(d, kx, ky, yv[q]);
-          if (TAN) sm[((dl * NF + 1) * TY + cy) * SS + pidx(xr)] = derived(d, kx, ky, dyv[q]);
+          sm[((dl * NFO + 0) * TY + cy) * SS + pidx(xr)] = derived(d, kx, ky, yv[q]);
+          if (NFO == 2) sm[((dl * NFO + 1) * TY + cy) * SS + pidx(xr)] = derived(d, kx, ky, dyv[q]);
         }
       }
     }
@@ -587,7 +621,7 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
       const int s2 = t / N;
       const int ky = ky0 + cy;
       if (ky >= KC) continue;
-      const int d = sp * DPS + ch * DPC + s2 / NF, f = s2 % NF;
+      const int d = sp * DPS + ch * DPC + s2 / NFO, f = s2 % NFO;
       const int g = (NDO == 1) ? c : (f * 8 + 4 * c + d);
       a.inter[(((size_t)b * ND + g) * N + x) * KC + ky] = sm[(s2 * TY + cy) * SS + pidx(x)];
     }
