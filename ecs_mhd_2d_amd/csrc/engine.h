@@ -6,6 +6,7 @@
 #include "rhs_kernels.h"
 #include "adjoint_kernels.h"
 #include "comb_kernels.h"
+#include "td_kernels.h"
 #include "host.h"
 
 // ------------------------------------------------------------------ launches
@@ -957,13 +958,29 @@ struct CombBuilder {
 // (tangent df along ddt[b]).  The RHS includes dissipation for RK4 and TDRK4
 // (loss_functions.py:53,81) and excludes it for Lawson RK6 (:68).  Unmasked
 // schemes keep the modes outside the 2/3 region (their RHS is zero there).
+// The two column kernels of the two-derivative RK4 tangent step (which 0: A,
+// 1: B), one column per block, in FFT plan `plan` (0 wide, 1 narrow).
+template <int N>
+static int td_launch(int which, int plan, const Args& a, int B, hipStream_t st) {
+  dim3 grid(Geo<N>::KC, 2, B);
+  if (which == 0) {
+    if (plan == 0) hipLaunchKernelGGL((td_col_a<N, NT, 16>), grid, dim3(NT), 0, st, a, B);
+    else hipLaunchKernelGGL((td_col_a<N, NT, 8>), grid, dim3(NT), 0, st, a, B);
+  } else {
+    if (plan == 0) hipLaunchKernelGGL((td_col_b<N, NT, 16>), grid, dim3(NT), 0, st, a, B);
+    else hipLaunchKernelGGL((td_col_b<N, NT, 8>), grid, dim3(NT), 0, st, a, B);
+  }
+  LAUNCHCHK();
+  return 0;
+}
+
 template <int N, bool TAN>
 static int run_gen(ecs_ctx* c, double* f, double* df, int B, const double* dt, const double* ddt, int steps,
                    int scheme, int masked, hipStream_t st) {
   const bool td = scheme == G_TDRK4;
-  TRY(ensure_cap(c, (td && TAN) ? 2 * B : B));
+  TRY(ensure_cap(c, (td && TAN) ? 3 * B : B));
   TRY(autotune<N>(c, B, st));
-  if (td && TAN) TRY(autotune<N>(c, 2 * B, st));
+  if (td && TAN) TRY(autotune<N>(c, 3 * B, st));
   const size_t ce = c->compact_elems();
   const size_t cs = ce * B;
   const Tableau& tb = tableau(scheme);
@@ -1088,30 +1105,61 @@ static int run_gen(ecs_ctx* c, double* f, double* df, int B, const double* dt, c
     TRY(unpack<N>(x, f, B, 0, st));
     return 0;
   }
-  // slots (units of B): XX = [x | x] 0-1, DV = [v1 | dv1] 2-3, W = [w1 | J dv1] 4-5,
-  // SCR 6-7, H1 8, dx 9, YY 10-11, DV2 12-13, W2 14-15, H2 16, dy 17
-  double2 *XX = slot(0), *DV = slot(2), *W = slot(4), *SCR = slot(6), *H1 = slot(8), *dx = slot(9);
-  double2 *YY = slot(10), *DV2 = slot(12), *W2 = slot(14), *H2 = slot(16), *dy = slot(17);
+  // Tangent: per half step a tangent row pass at (z, dz), td_col_a (v, dv and
+  // the inputs of the passes for J v, J dv and H[dz, v], batched 3B), one
+  // tangent row pass over the 3B candidates (b0 = 0 for the H part) and
+  // td_col_b (w = J v, J dv, H, then the combination and the next point's
+  // inverse transforms): 8 launches per step.
+  //   y = z + h/2 v + h^2/8 w,  dy = dz + h/2 dv + h^2/8 (J dv + H) + dh d/dh(...)
+  // slots (units of B): x 0, dx 1, y 2, dy 3, v1 4, dv1 5, w1 6, Jdv1 7, H1 8,
+  // v2 9, dv2 10, w2 11, Jdv2 12, H2 13
+  double2 *dx = slot(1), *y = slot(2), *dy = slot(3);
+  double2 *v1 = slot(4), *dv1 = slot(5), *w1 = slot(6), *jdv1 = slot(7), *h1 = slot(8);
+  double2 *v2 = slot(9), *dv2 = slot(10), *w2 = slot(11), *jdv2 = slot(12), *h2 = slot(13);
   TRY(pack<N>(df, dx, B, st));
-  for (int s = 0; s < steps; ++s) {
-    HIPCHK(hipMemcpyAsync(XX + cs, XX, cs * sizeof(double2), hipMemcpyDeviceToDevice, st));
-    TRY((rhs_pass<N, true>(a, XX, dx, DV, DV + cs, B, diss, st)));          // v1, dv1 = J(x) dx
-    TRY((rhs_pass<N, true>(a, XX, DV, SCR, W, 2 * B, diss, st)));           // w1 = J(x) v1, J(x) dv1
-    TRY((rhs_pass<N, true>(a, dx, DV, SCR, H1, B, diss, st, true)));        // H[dx, v1]
-    CombBuilder cy(c, ce, 1.0, 0, YY, dy);
-    cy.term(XX, dx, 1.0, 0, 0).term(DV, DV + cs, 0.5, 1, 0).term(W, W + cs, 0.125, 2, 0)
-      .term(nullptr, H1, 0.125, 2, 0);
-    TRY((comb_launch<N, true>(cy.cb, B, st)));
-    HIPCHK(hipMemcpyAsync(YY + cs, YY, cs * sizeof(double2), hipMemcpyDeviceToDevice, st));
-    TRY((rhs_pass<N, true>(a, YY, dy, DV2, DV2 + cs, B, diss, st)));
-    TRY((rhs_pass<N, true>(a, YY, DV2, SCR, W2, 2 * B, diss, st)));
-    TRY((rhs_pass<N, true>(a, dy, DV2, SCR, H2, B, diss, st, true)));
-    CombBuilder cx(c, ce, 1.0, 0, XX, dx);
-    cx.term(XX, dx, 1.0, 0, 0).term(DV, DV + cs, 1.0, 1, 0).term(W, W + cs, 1.0 / 6, 2, 0)
-      .term(nullptr, H1, 1.0 / 6, 2, 0).term(W2, W2 + cs, 1.0 / 3, 2, 0).term(nullptr, H2, 1.0 / 3, 2, 0);
-    TRY((comb_launch<N, true>(cx.cb, B, st)));
+  CombBuilder cy(c, ce, 1.0, 0, y, dy);
+  cy.term(x, dx, 1.0, 0, 0).term(v1, dv1, 0.5, 1, 0).term(w1, jdv1, 0.125, 2, 0).term(nullptr, h1, 0.125, 2, 0);
+  CombBuilder cx(c, ce, 1.0, 0, x, dx);
+  cx.term(x, dx, 1.0, 0, 0).term(v1, dv1, 1.0, 1, 0).term(w1, jdv1, 1.0 / 6, 2, 0)
+    .term(nullptr, h1, 1.0 / 6, 2, 0).term(w2, jdv2, 1.0 / 3, 2, 0).term(nullptr, h2, 1.0 / 3, 2, 0);
+  const Comb tabs[2] = {cy.cb, cx.cb};
+  TRY(grow(&c->gcomb, &c->gcomb_cap, (sizeof(tabs) + sizeof(double2) - 1) / sizeof(double2)));
+  Comb* dtabs = (Comb*)c->gcomb;
+  HIPCHK(hipMemcpyAsync(dtabs, tabs, sizeof(tabs), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // the host table goes out of scope
+  Args a0 = a;
+  a0.adaptive = AD_FIXED;
+  a0.include_diss = diss;
+  if (steps > 0) {
+    Args al = a0;
+    al.x0 = x;
+    al.dx0 = dx;
+    TRY((col<N, C_LOAD, true>(al, B, st)));
   }
-  TRY(unpack<N>(XX, f, B, 0, st));
+  const int plan = var_plan(tuned(N, K_COLT, B, DefTile<N>::COL));
+  for (int s = 0; s < steps; ++s) {
+    for (int half = 0; half < 2; ++half) {
+      Args ah = a0;
+      ah.x0 = half == 0 ? x : y;
+      ah.dx0 = half == 0 ? dx : dy;
+      ah.out = half == 0 ? v1 : v2;
+      ah.dout = half == 0 ? dv1 : dv2;
+      ah.tdw = half == 0 ? w1 : w2;
+      ah.tdjdv = half == 0 ? jdv1 : jdv2;
+      ah.tdh = half == 0 ? h1 : h2;
+      ah.gc = dtabs + half;
+      ah.xf = half == 0 ? y : x;
+      ah.dxf = half == 0 ? dy : dx;
+      ah.skip_inv = (s == steps - 1 && half == 1) ? 1 : 0;
+      TRY((row<N, R_RHS, true>(a0, B, st)));
+      TRY(td_launch<N>(0, plan, ah, B, st));
+      Args a3 = a0;
+      a3.b0_from = 2 * B;
+      TRY((row<N, R_RHS, true>(a3, 3 * B, st)));
+      TRY(td_launch<N>(1, plan, ah, B, st));
+    }
+  }
+  TRY(unpack<N>(x, f, B, 0, st));
   TRY(unpack<N>(dx, df, B, 0, st));
   return 0;
 }

@@ -194,6 +194,13 @@ struct Args {
   // not read s_nacc before their gathers
   int run_all;
   int next_all;
+  // row passes: candidates b >= b0_from see b0 = 0 (the second-derivative
+  // pass of the two-derivative RK4 tangent, batched with the other passes)
+  int b0_from;
+  // two-derivative RK4 tangent: J(z) v, J(z) dv and H[dz, v] of the half step
+  double2* tdw;
+  double2* tdjdv;
+  double2* tdh;
 };
 
 }  // namespace ecs

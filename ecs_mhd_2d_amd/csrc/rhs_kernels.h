@@ -691,7 +691,8 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
 
   constexpr int EPT = (TX * N + NT - 1) / NT;
   double2 q[EPT], dq[EPT];
-  const double b0x = a.b0x, b0y = a.b0y;
+  const bool nob0 = a.b0_from > 0 && b >= a.b0_from;
+  const double b0x = nob0 ? 0.0 : a.b0x, b0y = nob0 ? 0.0 : a.b0y;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const int idx = tid + e * NT;
