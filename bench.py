@@ -145,6 +145,32 @@ def config0_leg(cpu=True):
     return out
 
 
+def steppers_leg():
+    """The other integrators of mismatch_RPO (classical RK4, Lawson RK6,
+    two-derivative RK4), primal and JVP, at N=128 and N=256, batch 1, 256
+    steps of h = 1/256 on G(N, 0)."""
+    import torch
+    from ecs_mhd_2d_amd import mhd
+    from ecs_mhd_2d_amd.synthetic import spectrum
+    out = {"workload": "batch 1, 256 steps of h=1/256 on G(N, 0); JVP along G(N, 1), dT=1e-3"}
+    for n in (128, 256):
+        ctx = mhd.context(mhd.make_param_dict(n, 1 / 40, 1 / 40, (0.0, 0.1)))
+        f = torch.as_tensor(spectrum(n, 0))[None].cuda()
+        df = torch.as_tensor(spectrum(n, 1))[None].cuda()
+        runs = {"rk4": lambda s: ctx.rk4(f, 1 / 256, s), "rk4_jvp": lambda s: ctx.rk4_jvp(f, df, 1 / 256, 1e-3, s),
+                "lawson_rk6": lambda s: ctx.lawson_rk6(f, 1 / 256, s),
+                "lawson_rk6_jvp": lambda s: ctx.lawson_rk6_jvp(f, df, 1 / 256, 1e-3, s),
+                "tdrk4": lambda s: ctx.tdrk4(f, 1 / 256, s), "tdrk4_jvp": lambda s: ctx.tdrk4_jvp(f, df, 1 / 256, 1e-3, s)}
+        for name, fn in runs.items():
+            fn(4)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn(256)
+            torch.cuda.synchronize()
+            out[f"{name}_steps_per_s_n{n}"] = 256 / (time.perf_counter() - t0)
+    return out
+
+
 def newton_leg(inp, pd):
     """One full Newton-GMRES iteration on the headline guess (Krylov dimension 64:
     objective, linearisation, 65 matvecs, line search, dealias)."""
@@ -409,6 +435,7 @@ def main():
     extras = {"config0_lawson_rk4_n128": config0_leg(cpu=not args.no_cpu)}
     if not args.no_extras:
         extras["newton_iteration"] = newton_leg(inp, pd)
+        extras["other_steppers"] = steppers_leg()
         extras["adjoint_gradient"] = gradient_leg()
         extras["guess_generation"] = guess_generation_leg()
         extras["batched_candidates"] = batched_leg()
