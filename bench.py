@@ -34,8 +34,8 @@ HBM_PEAK_GBS = 8000.0
 # (KiB per dispatch scaled by a 64 MiB device copy: FETCH x 2047.3, WRITE x
 # 1024); the runs are summarised in profiles/pmc_traffic_tangent_row_kernel_n128_b1.md
 # and profiles/pmc_batched_row_kernel_n256_b64.md.
-TRAFFIC_ROW_T_N128_B1 = 851.0 * 2047.3 + 352.0 * 1024
-TRAFFIC_ROW_N256_B64 = 90194.5 * 2047.3 + 44032.0 * 1024
+TRAFFIC_ROW_T_N128_B1 = 767.5 * 2047.3 + 344.0 * 1024
+TRAFFIC_ROW_N256_B64 = 88149.6 * 2047.3 + 44032.0 * 1024
 METRIC = "FP64 RK steps/s at N=256 + GMRES matvecs/s; p50/p99 RPO solve latency"
 
 
@@ -302,15 +302,17 @@ def serving_full_leg(jobs):
     return stats
 
 
-def probe_roofline(batched=False):
+def probe_roofline(batched=False, kernel=None):
     """The matvec's dominant kernel (tangent row kernel, N=128, batch 1) or,
     with ``batched``, the batched roofline kernel (row kernel, N=256, batch
-    64): 50 launches through ecs_time_kernel; then a plain device copy of 64
-    MiB (read 64 MiB, write 64 MiB), whose counter readings scale FETCH_SIZE /
-    WRITE_SIZE to bytes."""
+    64), or ``kernel`` = (N, batch, which) with which as in ecs_time_kernel
+    (0 column, 1 row, 2 tangent column, 3 tangent row kernel): 50 launches
+    through ecs_time_kernel; then a plain device copy of 64 MiB (read 64 MiB,
+    write 64 MiB), whose counter readings scale FETCH_SIZE / WRITE_SIZE to
+    bytes."""
     import torch
     from ecs_mhd_2d_amd import _lib, _runtime, mhd
-    n, B, which = (256, 64, 1) if batched else (128, 1, 3)
+    n, B, which = kernel if kernel else ((256, 64, 1) if batched else (128, 1, 3))
     pd = mhd.make_param_dict(n, 1 / 40, 1 / 40)
     ctx = mhd.context(pd)
     ms = ctypes.c_double()
@@ -339,6 +341,8 @@ def main():
                     help="launch only the matvec's dominant kernel (50x) and a 64 MiB copy, for rocprofv3 --pmc")
     ap.add_argument("--probe-roofline-batched", action="store_true",
                     help="the same for the batched row kernel (N=256, batch 64)")
+    ap.add_argument("--probe-kernel", default="", metavar="N,BATCH,WHICH",
+                    help="the same for one kernel family (WHICH: 0 column, 1 row, 2 tangent column, 3 tangent row)")
     ap.add_argument("--probe", action="store_true",
                     help="run a short fixed kernel workload for rocprofv3 and exit")
     args = ap.parse_args()
@@ -353,8 +357,9 @@ def main():
     if args.probe:
         probe()
         return
-    if args.probe_roofline or args.probe_roofline_batched:
-        probe_roofline(batched=args.probe_roofline_batched)
+    if args.probe_roofline or args.probe_roofline_batched or args.probe_kernel:
+        kernel = tuple(int(t) for t in args.probe_kernel.split(",")) if args.probe_kernel else None
+        probe_roofline(batched=args.probe_roofline_batched, kernel=kernel)
         return
     if args.serving_full:
         stats = serving_full_leg(args.serving_full)
