@@ -130,13 +130,13 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
   const int b = lz, c = ly, ky0 = lx * TY, tid = threadIdx.x;
   if (a.nstep && a.step >= a.nstep[b]) return;  // candidate past its own step count
   if (FWD) {
-    const double2* AI = a.ain + (size_t)b * 8 * N * KC;
+    const double2* AI = a.ain + (size_t)b * 8 * N * G::KP;
     // first x-FFT pass straight from the row-adjoint output
     fft_lds_gather<N, 4 * TY, SS, NT, false, 16, (TY > 1)>(
         sm, twl,
         [&](int s, int x) -> double2 {
           const int d = s / TY, ky = ky0 + s % TY;
-          return ky < KC ? AI[((size_t)(4 * c + d) * N + x) * KC + ky] : make_double2(0.0, 0.0);
+          return ky < KC ? AI[((size_t)(4 * c + d) * N + x) * G::KP + ky] : make_double2(0.0, 0.0);
         },
         [&] { twf.store(twl); });
   } else {
@@ -183,13 +183,13 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
   }
   __syncthreads();
   fft_lds<N, TY, SS, NT, true>(sm, twl);
-  double2* AO = a.aout + (size_t)b * 2 * N * KC;
+  double2* AO = a.aout + (size_t)b * 2 * N * G::KP;
   for (int idx = tid; idx < N * TY; idx += NT) {
     const int cy = idx % TY;
     const int x = idx / TY;
     const int ky = ky0 + cy;
     if (ky >= KC) continue;
-    AO[((size_t)c * N + x) * KC + ky] = sm[cy * SS + pidx(x)];
+    AO[((size_t)c * N + x) * G::KP + ky] = sm[cy * SS + pidx(x)];
   }
 }
 
@@ -210,8 +210,8 @@ __global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
   xcd_tile(lx, ly, lz);
   const int b = lz, xbase = lx * TX, tid = threadIdx.x;
   if (a.nstep && a.step >= a.nstep[b]) return;  // candidate past its own step count
-  const double2* P = a.prim + (size_t)b * 8 * N * KC;
-  const double2* AO = a.aout + (size_t)b * 2 * N * KC;
+  const double2* P = a.prim + (size_t)b * 8 * N * G::KP;
+  const double2* AO = a.aout + (size_t)b * 2 * N * G::KP;
   // Hermitian extension of the packed pairs, formed while the first inverse
   // pass loads its inputs: 4 primal pairs, then the adjoint pair with the
   // r2c^T weights (pbar[y] = c2r of (Pbar_0, Pbar_k / 2)).  Every lane loads
@@ -224,9 +224,9 @@ __global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
         const int x = xbase + s / NP;
         const bool lo = m <= G::KMAX, hi = m >= N - G::KMAX;
         const int k = lo ? m : (hi ? N - m : 0);
-        const double2* src = (p < 4) ? P + (size_t)(2 * p) * N * KC : AO;
-        double2 A = src[((size_t)x) * KC + k];
-        double2 B = src[((size_t)N + x) * KC + k];
+        const double2* src = (p < 4) ? P + (size_t)(2 * p) * N * G::KP : AO;
+        double2 A = src[((size_t)x) * G::KP + k];
+        double2 B = src[((size_t)N + x) * G::KP + k];
         const double w = (p < 4 || k == 0) ? 1.0 : 0.5;
         A = cscale(w, A);
         B = cscale(w, B);
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
   }
   __syncthreads();
   fft_lds<N, TX * 4, SS, NT, false>(sm, twl);
-  double2* AI = a.ain + (size_t)b * 8 * N * KC;
+  double2* AI = a.ain + (size_t)b * 8 * N * G::KP;
   for (int idx = tid; idx < TX * 4 * KC; idx += NT) {
     const int k = idx % KC;
     const int t = idx / KC;
@@ -280,8 +280,8 @@ __global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
     const double w = (k == 0) ? 0.5 : 1.0;
     const double2 da = make_double2(w * (z.x + zm.x), w * (z.y - zm.y));
     const double2 db = make_double2(w * (z.y + zm.y), -w * (z.x - zm.x));
-    AI[((size_t)(2 * p) * N + x) * KC + k] = da;
-    AI[((size_t)(2 * p + 1) * N + x) * KC + k] = db;
+    AI[((size_t)(2 * p) * N + x) * G::KP + k] = da;
+    AI[((size_t)(2 * p + 1) * N + x) * G::KP + k] = db;
   }
 }
 

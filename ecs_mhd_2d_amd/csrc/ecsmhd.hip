@@ -24,10 +24,9 @@ int ensure_cap(ecs_ctx* c, int B) {
   std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
   drop_graphs(c);
   int nc = std::max(B, c->cap * 2);
-  const int kmax = (c->n - 1) / 3, KC = kmax + 1;
   const size_t comp = c->compact_elems();
-  const size_t inter = (size_t)16 * c->n * KC;
-  const size_t rowo = (size_t)4 * c->n * KC;
+  const size_t inter = (size_t)16 * c->n * kp_of(c->n);
+  const size_t rowo = (size_t)4 * c->n * kp_of(c->n);
   double2** arrs[] = {&c->x0, &c->acc, &c->v1, &c->v4, &c->v5, &c->xf, &c->out,
                       &c->dx0, &c->dacc, &c->dv1, &c->dv4, &c->dxf, &c->dout};
   for (auto p : arrs) {
@@ -169,7 +168,6 @@ int ensure_adj(ecs_ctx* c, int B) {
   if (B <= c->adj_cap) return 0;
   std::lock_guard<std::recursive_mutex> lk(g_alloc_mu);
   const int nc = c->cap;
-  const int KC = (c->n - 1) / 3 + 1;
   const size_t comp = c->compact_elems();
   double2** arrs[] = {&c->xb, &c->k1b, &c->k2b, &c->k3b};
   for (auto p : arrs) {
@@ -180,8 +178,8 @@ int ensure_adj(ecs_ctx* c, int B) {
   HIPCHK(hipMalloc((void**)&c->hb, comp * nc * sizeof(double)));
   if (c->AO) HIPCHK(hipFree(c->AO));
   if (c->AI) HIPCHK(hipFree(c->AI));
-  HIPCHK(hipMalloc((void**)&c->AO, (size_t)2 * c->n * KC * nc * sizeof(double2)));
-  HIPCHK(hipMalloc((void**)&c->AI, (size_t)8 * c->n * KC * nc * sizeof(double2)));
+  HIPCHK(hipMalloc((void**)&c->AO, (size_t)2 * c->n * kp_of(c->n) * nc * sizeof(double2)));
+  HIPCHK(hipMalloc((void**)&c->AI, (size_t)8 * c->n * kp_of(c->n) * nc * sizeof(double2)));
   c->adj_cap = nc;
   return 0;
 }

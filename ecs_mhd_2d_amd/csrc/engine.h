@@ -37,7 +37,8 @@ static int row_v(const Args& a, int B, hipStream_t st) {
 // Variants (var_* in host.h): v < 6: tile 1 << (v % 3), plan wide (v < 3) or
 // narrow; column kernels also v = 6..9: one column per block, its output
 // sequences split over 2 (even v) or 4 (odd v) blocks, plan wide (6, 7) or
-// narrow (8, 9).
+// narrow (8, 9); and for N >= 128 v = 10, 11: tiles of 8 columns, plan wide
+// (10) or narrow (11).
 template <int N, int MODE, bool TAN, int AK>
 static int col_var(int v, const Args& a, int B, hipStream_t st) {
   switch (v) {
@@ -50,7 +51,13 @@ static int col_var(int v, const Args& a, int B, hipStream_t st) {
     case 6: return col_v<N, 1, MODE, TAN, 16, AK, 2>(a, B, st);
     case 7: return col_v<N, 1, MODE, TAN, 16, AK, 4>(a, B, st);
     case 8: return col_v<N, 1, MODE, TAN, 8, AK, 2>(a, B, st);
-    default: return col_v<N, 1, MODE, TAN, 8, AK, 4>(a, B, st);
+    case 9: return col_v<N, 1, MODE, TAN, 8, AK, 4>(a, B, st);
+    default:
+      if constexpr (N >= 128) {
+        if (v == 10) return col_v<N, 8, MODE, TAN, 16, AK>(a, B, st);
+        return col_v<N, 8, MODE, TAN, 8, AK>(a, B, st);
+      }
+      return fail(1, "column variant " + std::to_string(v) + " does not exist for N = " + std::to_string(N));
   }
 }
 template <int N, int RMODE, bool TAN, int AK>
@@ -219,7 +226,7 @@ static bool forced_variants(int n, int v[4]) {
     if (sscanf(s, "%d:%d,%d,%d,%d", &nn, &a, &b, &c, &d) == 5 && nn == n) {
       const int t[4] = {a, b, c, d};
       for (int k = 0; k < 4; ++k) {
-        if (t[k] < 0 || t[k] >= var_count(k)) return false;
+        if (t[k] < 0 || t[k] >= var_count(k, n)) return false;
         v[k] = t[k];
       }
       return true;
@@ -258,12 +265,12 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
   if (plan[0] < 0 || plan[1] < 0 || plan[2] < 0 || plan[3] < 0) {
     constexpr int PB[2] = {1, 16};
     TRY(ensure_cap(c, PB[1]));
-    float t[2][4][10];
+    float t[2][4][VMAX];
     for (int p = 0; p < 2; ++p)
       for (int kind = 0; kind < 4; ++kind) {
-        int vs[10];
-        for (int v = 0; v < var_count(kind); ++v) vs[v] = v;
-        TRY(time_variants<N>(c, PB[p], kind, vs, var_count(kind), t[p][kind], st));
+        int vs[VMAX];
+        for (int v = 0; v < var_count(kind, N); ++v) vs[v] = v;
+        TRY(time_variants<N>(c, PB[p], kind, vs, var_count(kind, N), t[p][kind], st));
       }
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (int kind = 0; kind < 4; ++kind) {
@@ -273,7 +280,7 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
         float worst[2] = {0.f, 0.f};
         for (int p = 0; p < 2; ++p) {
           float bp[2] = {1e30f, 1e30f};
-          for (int v = 0; v < var_count(kind); ++v) bp[var_plan(v)] = std::min(bp[var_plan(v)], t[p][kind][v]);
+          for (int v = 0; v < var_count(kind, N); ++v) bp[var_plan(v)] = std::min(bp[var_plan(v)], t[p][kind][v]);
           for (int q = 0; q < 2; ++q) worst[q] = std::max(worst[q], bp[q] / std::min(bp[0], bp[1]));
         }
         pl = worst[1] < worst[0] ? 1 : 0;
@@ -282,7 +289,7 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
       g_plan[plan_key(N, kind)] = pl;
       for (int p = 0; p < 2; ++p) {
         int v = -1;
-        for (int q = 0; q < var_count(kind); ++q)
+        for (int q = 0; q < var_count(kind, N); ++q)
           if (var_plan(q) == pl && (v < 0 || t[p][kind][q] < t[p][kind][v])) v = q;
         if (!g_tune.count(tune_key(N, kind, PB[p]))) g_tune[tune_key(N, kind, PB[p])] = v;
       }
@@ -291,9 +298,9 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
   }
   int pick[4];
   for (int kind = 0; kind < 4; ++kind) {
-    int vs[10];
-    float t[10];
-    const int nv = var_list(kind, plan[kind], vs);
+    int vs[VMAX];
+    float t[VMAX];
+    const int nv = var_list(kind, N, plan[kind], vs);
     TRY(time_variants<N>(c, B, kind, vs, nv, t, st));
     int best = 0;
     for (int i = 1; i < nv; ++i)
@@ -653,7 +660,7 @@ static int adj_buffers(ecs_ctx* c, int B, int ckpt) {
   TRY(ensure_adj(c, B));
   using G = Geo<N>;
   const size_t cs = (size_t)B * 2 * G::R * G::KC;
-  const size_t is = (size_t)B * 8 * N * G::KC;
+  const size_t is = (size_t)B * 8 * N * G::KP;
   const size_t need = (size_t)ckpt * 5 * cs;
   if (need > c->vt_cap) {
     if (c->xtape) HIPCHK(hipFree(c->xtape));
@@ -715,7 +722,7 @@ static int run_rk4_rev(ecs_ctx* c, const double2* ck, double* fbar, double* hbar
   TRY(autotune<N>(c, B, st));
   using G = Geo<N>;
   const size_t cs = (size_t)B * 2 * G::R * G::KC;
-  const size_t is = (size_t)B * 8 * N * G::KC;
+  const size_t is = (size_t)B * 8 * N * G::KP;
   const int nseg = (steps + ckpt - 1) / ckpt;
   TRY(upload_scalars(c, 0, dt, B, st));
   Args a = base_args(c);

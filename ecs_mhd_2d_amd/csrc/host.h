@@ -112,7 +112,7 @@ struct ecs_ctx {
   double2* xtape = nullptr;    // [ckpt][B][2][R][KC]
   double2* vtape = nullptr;    // [ckpt][4][B][2][R][KC]
   size_t vt_cap = 0;           // elements of xtape + vtape
-  double2* itape = nullptr;    // primal column outputs [ckpt][4][B][8][N][KC]
+  double2* itape = nullptr;    // primal column outputs [ckpt][4][B][8][N][KP]
   size_t it_cap = 0;
   double2* gen = nullptr;      // stage arrays of the generic steppers
   size_t gen_cap = 0;
@@ -162,21 +162,29 @@ int upload_ints(ecs_ctx* c, const int* vals, int B, hipStream_t st);
 // radix 16/8; 1: narrow, radix 8/4/2).  Column families also have v = 6..9:
 // one column per block with the column's output sequences split over 2 (even
 // v) or 4 (odd v) blocks, plan (v - 6) / 2; the split spreads a small batch
-// over more CUs and does not change the arithmetic.  The autotuner picks the
+// over more CUs and does not change the arithmetic.  For N >= 128 column
+// families also have v = 10, 11: tiles of 8 columns (whole 128-byte lines of
+// the padded x-space arrays), plan v - 10; they pay at large batches, where
+// wider tiles move the same bytes in fewer, fuller transactions.  The autotuner picks the
 // plan once per (N, kernel family), so every batch size computes with the
 // same arithmetic, and the variant within that plan per (N, kernel family,
 // batch bucket).
 enum TuneKind : int { K_COL = 0, K_COLT = 1, K_ROW = 2, K_ROWT = 3 };
-inline int var_count(int kind) { return (kind == K_COL || kind == K_COLT) ? 10 : 6; }
-inline int var_plan(int v) { return v < 6 ? v / 3 : (v - 6) / 2; }
-inline int var_tile(int v) { return v < 6 ? 1 << (v % 3) : 1; }
-inline int var_split(int v) { return v < 6 ? 1 : (v % 2 == 0 ? 2 : 4); }
-// the variants of family `kind` in plan p; returns their number
-inline int var_list(int kind, int p, int* out) {
-  int n = 0;
-  for (int v = 0; v < var_count(kind); ++v)
-    if (var_plan(v) == p) out[n++] = v;
-  return n;
+constexpr int VMAX = 12;  // most variants of one family
+inline int var_count(int kind, int n) {
+  return (kind == K_COL || kind == K_COLT) ? (n >= 128 ? 12 : 10) : 6;
+}
+inline int var_plan(int v) { return v < 6 ? v / 3 : (v < 10 ? (v - 6) / 2 : v - 10); }
+inline int var_tile(int v) { return v < 6 ? 1 << (v % 3) : (v < 10 ? 1 : 8); }
+inline int var_split(int v) { return (v < 6 || v >= 10) ? 1 : (v % 2 == 0 ? 2 : 4); }
+// padded row stride of the x-space arrays of grid n (Geo<N>::KP)
+inline int kp_of(int n) { return (((n - 1) / 3 + 1) + 7) & ~7; }
+// the variants of family `kind` of grid n in plan p; returns their number
+inline int var_list(int kind, int n, int p, int* out) {
+  int k = 0;
+  for (int v = 0; v < var_count(kind, n); ++v)
+    if (var_plan(v) == p) out[k++] = v;
+  return k;
 }
 extern std::mutex g_tune_mu;
 extern std::map<long long, int> g_tune;  // (n, kind, bucket) -> variant

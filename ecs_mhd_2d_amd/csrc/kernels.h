@@ -29,6 +29,10 @@ struct Geo {
   static constexpr int KC = KMAX + 1;
   static constexpr int R = 2 * KMAX + 1;
   static constexpr int NH = N / 2 + 1;
+  // row stride of the x-space arrays [..][N][KP] the column and row passes
+  // exchange: KC padded to a multiple of 8, so every row starts on a 128-byte
+  // line and a tile of 8 columns writes whole lines
+  static constexpr int KP = (KC + 7) & ~7;
   __device__ __host__ static constexpr int row_kx(int r) { return r <= KMAX ? r : r - R; }
   __device__ __host__ static constexpr int row_x(int r) { return r <= KMAX ? r : r - R + N; }
 };

@@ -472,7 +472,7 @@ constexpr bool col_splittable() {
 template <int N, int TY, int NT, int MODE, bool TAN, int P, int AK, int SPL>
 __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   using G = Geo<N>;
-  constexpr int KC = G::KC, R = G::R, KMAX = G::KMAX;
+  constexpr int KC = G::KC, KP = G::KP, R = G::R, KMAX = G::KMAX;
   // fields of the row output read (NFI) and of the derived output written
   // (NFO): primal only, or primal and tangent; they differ only in the
   // two-derivative RK4 modes
@@ -526,14 +526,14 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   }
 
   if (FWD) {
-    const double2* O = a.rowout + (size_t)b * (2 * NFI) * N * KC;
+    const double2* O = a.rowout + (size_t)b * (2 * NFI) * N * KP;
     // the first x-FFT pass reads the row output directly; with several
     // columns per block neighbouring lanes take neighbouring ky of one row
     fft_lds_gather<N, NFI * TY, SS, NT, false, P, (TY > 1)>(
         sm, twl,
         [&](int s, int x) -> double2 {
           const int f = s / TY, ky = ky0 + s % TY;
-          return ky < KC ? O[((size_t)(f * 2 + c) * N + x) * KC + ky] : make_double2(0.0, 0.0);
+          return ky < KC ? O[((size_t)(f * 2 + c) * N + x) * KP + ky] : make_double2(0.0, 0.0);
         },
         [&] { twf.store(twl); });
   } else {
@@ -623,7 +623,7 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
       if (ky >= KC) continue;
       const int d = sp * DPS + ch * DPC + s2 / NFO, f = s2 % NFO;
       const int g = (NDO == 1) ? c : (f * 8 + 4 * c + d);
-      a.inter[(((size_t)b * ND + g) * N + x) * KC + ky] = sm[(s2 * TY + cy) * SS + pidx(x)];
+      a.inter[(((size_t)b * ND + g) * N + x) * KP + ky] = sm[(s2 * TY + cy) * SS + pidx(x)];
     }
   }
 }
@@ -631,7 +631,7 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
 template <int N, int TX, int NT, int RMODE, bool TAN, int P, int AK>
 __global__ __launch_bounds__(NT) void row_kernel(Args a) {
   using G = Geo<N>;
-  constexpr int KC = G::KC;
+  constexpr int KC = G::KC, KP = G::KP;
   constexpr int NP = (RMODE == R_ERR) ? 1 : (TAN ? 8 : 4);
   constexpr int NF = TAN ? 2 : 1;
   constexpr int ND = (RMODE == R_ERR) ? 2 : 8 * NF;
@@ -645,7 +645,7 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
   xcd_tile(lx, ly, lz);
   const int b = lz, xbase = lx * TX, tid = threadIdx.x;
   if (!row_active<AK>(a, b)) return;
-  const double2* I = a.inter + (size_t)b * ND * N * KC;
+  const double2* I = a.inter + (size_t)b * ND * N * KP;
   // Hermitian extension of the packed pairs (A + iB), formed while the first
   // inverse-FFT pass loads its inputs from I; the imaginary parts of the ky = 0
   // bin are dropped, as numpy's c2r does (ky = N/2 lies in the dealiased gap).
@@ -658,8 +658,8 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
         const int x = xbase + s / NP;
         const bool lo = m <= G::KMAX, hi = m >= N - G::KMAX;
         const int k = lo ? m : (hi ? N - m : 0);
-        const double2 A = I[((size_t)(2 * p) * N + x) * KC + k];
-        const double2 B = I[((size_t)(2 * p + 1) * N + x) * KC + k];
+        const double2 A = I[((size_t)(2 * p) * N + x) * KP + k];
+        const double2 B = I[((size_t)(2 * p + 1) * N + x) * KP + k];
         const double2 vlo = m == 0 ? make_double2(A.x, B.x) : make_double2(A.x - B.y, A.y + B.x);
         const double2 vhi = make_double2(A.x + B.y, B.x - A.y);
         return lo ? vlo : (hi ? vhi : make_double2(0.0, 0.0));
@@ -724,7 +724,7 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
   }
   __syncthreads();
   fft_lds<N, TX * NF, SS, NT, false, P>(sm, twl);
-  double2* O = a.rowout + (size_t)b * (2 * NF) * N * KC;
+  double2* O = a.rowout + (size_t)b * (2 * NF) * N * KP;
   for (int idx = tid; idx < TX * NF * KC; idx += NT) {
     const int k = idx % KC;
     const int t = idx / KC;
@@ -735,8 +735,8 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
     const double2 z = Z[pidx(k)], zm = Z[pidx((N - k) & (N - 1))];
     const double2 pw = make_double2(0.5 * (z.x + zm.x), 0.5 * (z.y - zm.y));
     const double2 pj = make_double2(0.5 * (z.y + zm.y), -0.5 * (z.x - zm.x));
-    O[((size_t)(f * 2 + 0) * N + x) * KC + k] = pw;
-    O[((size_t)(f * 2 + 1) * N + x) * KC + k] = pj;
+    O[((size_t)(f * 2 + 0) * N + x) * KP + k] = pw;
+    O[((size_t)(f * 2 + 1) * N + x) * KP + k] = pj;
   }
 }
 

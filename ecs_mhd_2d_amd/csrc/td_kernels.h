@@ -59,7 +59,7 @@ __device__ __forceinline__ void td_emit_pair(const Args& a, double2* sm, const d
     const int x = idx % N, s2 = idx / N;
     const int d = s2 / 2, f = s2 % 2;
     const int g = f * 8 + 4 * c + d;
-    a.inter[(((size_t)cand * 16 + g) * N + x) * KC + ky] = sm[s2 * SS + pidx(x)];
+    a.inter[(((size_t)cand * 16 + g) * N + x) * G::KP + ky] = sm[s2 * SS + pidx(x)];
   }
 }
 
@@ -87,9 +87,9 @@ __global__ __launch_bounds__(NT) void td_col_a(Args a, int B) {
       if (c == 0) F[q] = a.fhat[(size_t)r * KC + ky];
     }
   }
-  const double2* O = a.rowout + (size_t)b * 4 * N * KC;
+  const double2* O = a.rowout + (size_t)b * 4 * N * G::KP;
   fft_lds_gather<N, 2, SS, NT, false, P>(
-      sm, twl, [&](int s, int x) -> double2 { return O[((size_t)(s * 2 + c) * N + x) * KC + ky]; },
+      sm, twl, [&](int s, int x) -> double2 { return O[((size_t)(s * 2 + c) * N + x) * G::KP + ky]; },
       [&] { twf.store(twl); });
   // v = f(z), dv = J(z) dz (mask and forcing as in the column kernel's glue)
   double2 v[EPT], dv[EPT];
@@ -148,8 +148,8 @@ __global__ __launch_bounds__(NT) void td_col_b(Args a, int B) {
   fft_lds_gather<N, 3, SS, NT, false, P>(
       sm, twl,
       [&](int s, int x) -> double2 {
-        const double2* O = a.rowout + (size_t)(s * B + b) * 4 * N * KC;
-        return O[((size_t)(1 * 2 + c) * N + x) * KC + ky];
+        const double2* O = a.rowout + (size_t)(s * B + b) * 4 * N * G::KP;
+        return O[((size_t)(1 * 2 + c) * N + x) * G::KP + ky];
       },
       [&] { twf.store(twl); });
   double2 y[EPT], dy[EPT];
