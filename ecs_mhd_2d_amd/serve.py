@@ -241,10 +241,11 @@ def job_memory(kind, n, inner=64, steps=0, budget=4096):
     vec = (2 * n * n + 2) * 8
     kmax = (n - 1) // 3
     comp = 2 * (2 * kmax + 1) * (kmax + 1) * 16
+    kp = (kmax + 1 + 7) // 8 * 8  # padded row stride of the x-space tape
     if kind == "newton":
         return (inner + 1) * vec + 8 * comp + budget * 64
     nseg = math.ceil(steps / 64)
-    return nseg * comp + 64 * 5 * comp + 64 * 4 * 8 * n * (kmax + 1) * 16 + 6 * vec
+    return nseg * comp + 64 * 5 * comp + 64 * 4 * 8 * n * kp * 16 + 6 * vec
 
 
 # ------------------------------------------------------------------ scheduler
