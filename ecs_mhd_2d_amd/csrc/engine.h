@@ -20,10 +20,12 @@ template <> struct DefTile<256> { static constexpr int COL = 1, ROW = 0; };
 
 template <int N, int TY, int MODE, bool TAN, int P, int AK, int SPL = 1>
 static int col_v(const Args& a, int B, hipStream_t st) {
-  // modes that cannot split run their unsplit instance under a split variant
+  // modes that cannot split run unsplit under a split variant, one column per
+  // block: the split variants serve small batches, which want the most blocks
   constexpr int SP = col_splittable<MODE>() ? SPL : 1;
-  dim3 grid((Geo<N>::KC + TY - 1) / TY, 2 * SP, B);
-  hipLaunchKernelGGL((col_kernel<N, TY, NT, MODE, TAN, P, AK, SP>), grid, dim3(NT), 0, st, a);
+  constexpr int T = (SPL > 1 && SP == 1) ? 1 : TY;
+  dim3 grid((Geo<N>::KC + T - 1) / T, 2 * SP, B);
+  hipLaunchKernelGGL((col_kernel<N, T, NT, MODE, TAN, P, AK, SP>), grid, dim3(NT), 0, st, a);
   LAUNCHCHK();
   return 0;
 }
@@ -37,8 +39,9 @@ static int row_v(const Args& a, int B, hipStream_t st) {
 // Variants (var_* in host.h): v < 6: tile 1 << (v % 3), plan wide (v < 3) or
 // narrow; column kernels also v = 6..9: one column per block, its output
 // sequences split over 2 (even v) or 4 (odd v) blocks, plan wide (6, 7) or
-// narrow (8, 9); and for N >= 128 v = 10, 11: tiles of 8 columns, plan wide
-// (10) or narrow (11).
+// narrow (8, 9); for N >= 128 v = 10, 11: tiles of 8 columns, plan wide (10)
+// or narrow (11); v = 12..15: tiles of 2 columns split over 2 (12, 14) or 4
+// (13, 15) blocks, plan wide (12, 13) or narrow (14, 15).
 template <int N, int MODE, bool TAN, int AK>
 static int col_var(int v, const Args& a, int B, hipStream_t st) {
   switch (v) {
@@ -55,7 +58,14 @@ static int col_var(int v, const Args& a, int B, hipStream_t st) {
     default:
       if constexpr (N >= 128) {
         if (v == 10) return col_v<N, 8, MODE, TAN, 16, AK>(a, B, st);
-        return col_v<N, 8, MODE, TAN, 8, AK>(a, B, st);
+        if (v == 11) return col_v<N, 8, MODE, TAN, 8, AK>(a, B, st);
+      }
+      switch (v) {
+        case 12: return col_v<N, 2, MODE, TAN, 16, AK, 2>(a, B, st);
+        case 13: return col_v<N, 2, MODE, TAN, 16, AK, 4>(a, B, st);
+        case 14: return col_v<N, 2, MODE, TAN, 8, AK, 2>(a, B, st);
+        case 15: return col_v<N, 2, MODE, TAN, 8, AK, 4>(a, B, st);
+        default: break;
       }
       return fail(1, "column variant " + std::to_string(v) + " does not exist for N = " + std::to_string(N));
   }
@@ -226,7 +236,7 @@ static bool forced_variants(int n, int v[4]) {
     if (sscanf(s, "%d:%d,%d,%d,%d", &nn, &a, &b, &c, &d) == 5 && nn == n) {
       const int t[4] = {a, b, c, d};
       for (int k = 0; k < 4; ++k) {
-        if (t[k] < 0 || t[k] >= var_count(k, n)) return false;
+        if (!var_ok(k, n, t[k])) return false;
         v[k] = t[k];
       }
       return true;
@@ -269,8 +279,11 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
     for (int p = 0; p < 2; ++p)
       for (int kind = 0; kind < 4; ++kind) {
         int vs[VMAX];
-        for (int v = 0; v < var_count(kind, N); ++v) vs[v] = v;
-        TRY(time_variants<N>(c, PB[p], kind, vs, var_count(kind, N), t[p][kind], st));
+        float tl[VMAX];
+        const int nv = var_list(kind, N, -1, vs);
+        TRY(time_variants<N>(c, PB[p], kind, vs, nv, tl, st));
+        for (int v = 0; v < VMAX; ++v) t[p][kind][v] = 1e30f;
+        for (int i = 0; i < nv; ++i) t[p][kind][vs[i]] = tl[i];
       }
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (int kind = 0; kind < 4; ++kind) {
@@ -280,7 +293,8 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
         float worst[2] = {0.f, 0.f};
         for (int p = 0; p < 2; ++p) {
           float bp[2] = {1e30f, 1e30f};
-          for (int v = 0; v < var_count(kind, N); ++v) bp[var_plan(v)] = std::min(bp[var_plan(v)], t[p][kind][v]);
+          for (int v = 0; v < VMAX; ++v)
+            if (var_ok(kind, N, v)) bp[var_plan(v)] = std::min(bp[var_plan(v)], t[p][kind][v]);
           for (int q = 0; q < 2; ++q) worst[q] = std::max(worst[q], bp[q] / std::min(bp[0], bp[1]));
         }
         pl = worst[1] < worst[0] ? 1 : 0;
@@ -289,8 +303,8 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
       g_plan[plan_key(N, kind)] = pl;
       for (int p = 0; p < 2; ++p) {
         int v = -1;
-        for (int q = 0; q < var_count(kind, N); ++q)
-          if (var_plan(q) == pl && (v < 0 || t[p][kind][q] < t[p][kind][v])) v = q;
+        for (int q = 0; q < VMAX; ++q)
+          if (var_ok(kind, N, q) && var_plan(q) == pl && (v < 0 || t[p][kind][q] < t[p][kind][v])) v = q;
         if (!g_tune.count(tune_key(N, kind, PB[p]))) g_tune[tune_key(N, kind, PB[p])] = v;
       }
     }

@@ -165,25 +165,30 @@ int upload_ints(ecs_ctx* c, const int* vals, int B, hipStream_t st);
 // over more CUs and does not change the arithmetic.  For N >= 128 column
 // families also have v = 10, 11: tiles of 8 columns (whole 128-byte lines of
 // the padded x-space arrays), plan v - 10; they pay at large batches, where
-// wider tiles move the same bytes in fewer, fuller transactions.  The autotuner picks the
+// wider tiles move the same bytes in fewer, fuller transactions.  And v =
+// 12..15: tiles of 2 columns split over 2 (even v) or 4 (odd v) blocks, plan
+// (v - 12) / 2, for small batches (32-byte runs instead of 16-byte ones).  The autotuner picks the
 // plan once per (N, kernel family), so every batch size computes with the
 // same arithmetic, and the variant within that plan per (N, kernel family,
 // batch bucket).
 enum TuneKind : int { K_COL = 0, K_COLT = 1, K_ROW = 2, K_ROWT = 3 };
-constexpr int VMAX = 12;  // most variants of one family
-inline int var_count(int kind, int n) {
-  return (kind == K_COL || kind == K_COLT) ? (n >= 128 ? 12 : 10) : 6;
+constexpr int VMAX = 16;  // variant indices of one family lie in [0, VMAX)
+inline bool var_ok(int kind, int n, int v) {
+  if (v < 0 || v >= VMAX) return false;
+  if (kind == K_ROW || kind == K_ROWT) return v < 6;
+  return v < 10 || v >= 12 || n >= 128;
 }
-inline int var_plan(int v) { return v < 6 ? v / 3 : (v < 10 ? (v - 6) / 2 : v - 10); }
-inline int var_tile(int v) { return v < 6 ? 1 << (v % 3) : (v < 10 ? 1 : 8); }
-inline int var_split(int v) { return (v < 6 || v >= 10) ? 1 : (v % 2 == 0 ? 2 : 4); }
+inline int var_plan(int v) { return v < 6 ? v / 3 : v < 10 ? (v - 6) / 2 : v < 12 ? v - 10 : (v - 12) / 2; }
+inline int var_tile(int v) { return v < 6 ? 1 << (v % 3) : v < 10 ? 1 : v < 12 ? 8 : 2; }
+inline int var_split(int v) { return (v < 6 || v == 10 || v == 11) ? 1 : (v % 2 == 0 ? 2 : 4); }
 // padded row stride of the x-space arrays of grid n (Geo<N>::KP)
 inline int kp_of(int n) { return (((n - 1) / 3 + 1) + 7) & ~7; }
-// the variants of family `kind` of grid n in plan p; returns their number
+// the variants of family `kind` of grid n in plan p (every plan for p < 0);
+// returns their number
 inline int var_list(int kind, int n, int p, int* out) {
   int k = 0;
-  for (int v = 0; v < var_count(kind, n); ++v)
-    if (var_plan(v) == p) out[k++] = v;
+  for (int v = 0; v < VMAX; ++v)
+    if (var_ok(kind, n, v) && (p < 0 || var_plan(v) == p)) out[k++] = v;
   return k;
 }
 extern std::mutex g_tune_mu;

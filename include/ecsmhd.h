@@ -217,7 +217,9 @@ int ecs_tuned_splits(int n, int batch, int* out);
  * ecs_tuned_tiles; -1 where not tuned yet.  v < 6: tile 1 << (v % 3), wide
  * plan for v < 3, narrow for 3..5; column families 6..9: one column per block
  * split over 2 (6, 8) or 4 (7, 9) blocks, wide plan for 6, 7; for n >= 128
- * also 10 and 11: tiles of 8 columns, wide and narrow plan.  The same
+ * also 10 and 11: tiles of 8 columns, wide and narrow plan; 12..15: tiles of
+ * 2 columns split over 2 (12, 14) or 4 (13, 15) blocks, wide plan for 12,
+ * 13.  The same
  * indices are accepted by ECSMHD_VARIANTS="N:c,ct,r,rt". */
 int ecs_tuned_variants(int n, int batch, int* out);
 
