@@ -113,7 +113,7 @@ __device__ __forceinline__ void adj_glue(const Args& a, int b, size_t ci, int r,
 
 // Column-adjoint kernel: [x-FFT of the row-adjoint output -> ybar] -> glue ->
 // [m g -> inverse x-FFT -> AO].
-template <int N, int TY, int NT, int MODE>
+template <int N, int TY, int NT, int MODE, int PL>
 __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
   using G = Geo<N>;
   constexpr int KC = G::KC, R = G::R, KMAX = G::KMAX;
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
   constexpr int SS = Seq<N>::S;
   __shared__ double2 sm[NSEQ * SS];
   __shared__ double2 twl[N];
-  TwFetch<N, NT> twf;
+  TwFetch<N, NT, PL> twf;
   twf.fetch(a.tw);
   int lx, ly, lz;
   xcd_tile(lx, ly, lz);
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
   if (FWD) {
     const double2* AI = a.ain + (size_t)b * 8 * N * G::KP;
     // first x-FFT pass straight from the row-adjoint output
-    fft_lds_gather<N, 4 * TY, SS, NT, false, 16, (TY > 1)>(
+    fft_lds_gather<N, 4 * TY, SS, NT, false, PL, (TY > 1)>(
         sm, twl,
         [&](int s, int x) -> double2 {
           const int d = s / TY, ky = ky0 + s % TY;
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
     sm[s * SS + pidx(KMAX + 1 + gg)] = make_double2(0.0, 0.0);
   }
   __syncthreads();
-  fft_lds<N, TY, SS, NT, true>(sm, twl);
+  fft_lds<N, TY, SS, NT, true, PL>(sm, twl);
   double2* AO = a.aout + (size_t)b * 2 * N * G::KP;
   for (int idx = tid; idx < N * TY; idx += NT) {
     const int cy = idx % TY;
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
 
 // Row-adjoint kernel: weighted c2r of (Pbar_omega, Pbar_j), transposed
 // products with the primal real fields, weighted r2c of the 8 cotangents.
-template <int N, int TX, int NT>
+template <int N, int TX, int NT, int PL>
 __global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
   using G = Geo<N>;
   constexpr int KC = G::KC;
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
   constexpr int SS = Seq<N>::S;
   __shared__ double2 sm[TX * NP * SS];
   __shared__ double2 twl[N];
-  TwFetch<N, NT> twf;
+  TwFetch<N, NT, PL> twf;
   twf.fetch(a.tw);
   int lx, ly, lz;
   xcd_tile(lx, ly, lz);
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
   // r2c^T weights (pbar[y] = c2r of (Pbar_0, Pbar_k / 2)).  Every lane loads
   // (gap lanes re-read bin 0 and discard it), so all loads of the pass are in
   // flight before the first wait.
-  fft_lds_gather<N, TX * NP, SS, NT, true, 16>(
+  fft_lds_gather<N, TX * NP, SS, NT, true, PL>(
       sm, twl,
       [&](int s, int m) -> double2 {
         const int p = s % NP;
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(NT) void adj_row_kernel(Args a) {
     }
   }
   __syncthreads();
-  fft_lds<N, TX * 4, SS, NT, false>(sm, twl);
+  fft_lds<N, TX * 4, SS, NT, false, PL>(sm, twl);
   double2* AI = a.ain + (size_t)b * 8 * N * G::KP;
   for (int idx = tid; idx < TX * 4 * KC; idx += NT) {
     const int k = idx % KC;

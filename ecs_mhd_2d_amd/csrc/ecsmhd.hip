@@ -97,6 +97,12 @@ int tuned(int n, int kind, int B, int dflt) {
   return ip == g_plan.end() ? dflt : 3 * ip->second + dflt % 3;
 }
 
+int plan_of(int n, int kind) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto ip = g_plan.find(plan_key(n, kind));
+  return ip == g_plan.end() ? 0 : ip->second;
+}
+
 int upload_scalars(ecs_ctx* c, int slot, const double* vals, int B, hipStream_t st) {
   // the previous copy out of this pinned slot may still be queued behind
   // earlier work: wait for it before overwriting the staging memory

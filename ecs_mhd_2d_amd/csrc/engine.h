@@ -631,11 +631,16 @@ template <int N> static int rk4_t(ecs_ctx* c, double* f, double* df, int B, cons
 
 
 // ------------------------------------------------------------------ adjoint
+// The adjoint kernels transform with the FFT plan the autotuner fixed for the
+// forward kernels of the same shape (column or row family), so they take the
+// radix that suits the grid size; the plan is fixed per grid size, so the
+// reverse pass, too, computes the same bits at every batch.
 template <int N, int MODE>
 static int acol(const Args& a, int B, hipStream_t st) {
   constexpr int TY = Tiles<N>::TY;
   dim3 grid((Geo<N>::KC + TY - 1) / TY, 2, B);
-  hipLaunchKernelGGL((adj_col_kernel<N, TY, NT, MODE>), grid, dim3(NT), 0, st, a);
+  if (plan_of(N, K_COL) == 1) hipLaunchKernelGGL((adj_col_kernel<N, TY, NT, MODE, 8>), grid, dim3(NT), 0, st, a);
+  else hipLaunchKernelGGL((adj_col_kernel<N, TY, NT, MODE, 16>), grid, dim3(NT), 0, st, a);
   LAUNCHCHK();
   return 0;
 }
@@ -643,7 +648,8 @@ template <int N>
 static int arow(const Args& a, int B, hipStream_t st) {
   constexpr int TX = Tiles<N>::TX;
   dim3 grid(N / TX, 1, B);
-  hipLaunchKernelGGL((adj_row_kernel<N, TX, NT>), grid, dim3(NT), 0, st, a);
+  if (plan_of(N, K_ROW) == 1) hipLaunchKernelGGL((adj_row_kernel<N, TX, NT, 8>), grid, dim3(NT), 0, st, a);
+  else hipLaunchKernelGGL((adj_row_kernel<N, TX, NT, 16>), grid, dim3(NT), 0, st, a);
   LAUNCHCHK();
   return 0;
 }

@@ -197,6 +197,8 @@ extern std::map<long long, int> g_plan;  // (n, kind) -> plan index
 long long tune_key(int n, int kind, int B);
 long long plan_key(int n, int kind);
 int tuned(int n, int kind, int B, int dflt);
+// the FFT plan fixed for (n, kind): 0 wide, 1 narrow; 0 before tuning
+int plan_of(int n, int kind);
 
 // ------------------------------------------------------------------ hipGraph chunks
 // Captures the launches `record` makes on the context's own stream once per
