@@ -635,23 +635,31 @@ template <int N> static int rk4_t(ecs_ctx* c, double* f, double* df, int B, cons
 // forward kernels of the same shape (column or row family), so they take the
 // radix that suits the grid size; the plan is fixed per grid size, so the
 // reverse pass, too, computes the same bits at every batch.
-template <int N, int MODE>
-static int acol(const Args& a, int B, hipStream_t st) {
-  constexpr int TY = Tiles<N>::TY;
+// Below batch 16 they take one column / row per block (the most blocks for a
+// small batch); tiles do not change the arithmetic.
+template <int N, int MODE, int TY>
+static int acol_t(const Args& a, int B, hipStream_t st) {
   dim3 grid((Geo<N>::KC + TY - 1) / TY, 2, B);
   if (plan_of(N, K_COL) == 1) hipLaunchKernelGGL((adj_col_kernel<N, TY, NT, MODE, 8>), grid, dim3(NT), 0, st, a);
   else hipLaunchKernelGGL((adj_col_kernel<N, TY, NT, MODE, 16>), grid, dim3(NT), 0, st, a);
   LAUNCHCHK();
   return 0;
 }
-template <int N>
-static int arow(const Args& a, int B, hipStream_t st) {
-  constexpr int TX = Tiles<N>::TX;
+template <int N, int MODE>
+static int acol(const Args& a, int B, hipStream_t st) {
+  return B < 16 ? acol_t<N, MODE, 1>(a, B, st) : acol_t<N, MODE, Tiles<N>::TY>(a, B, st);
+}
+template <int N, int TX>
+static int arow_t(const Args& a, int B, hipStream_t st) {
   dim3 grid(N / TX, 1, B);
   if (plan_of(N, K_ROW) == 1) hipLaunchKernelGGL((adj_row_kernel<N, TX, NT, 8>), grid, dim3(NT), 0, st, a);
   else hipLaunchKernelGGL((adj_row_kernel<N, TX, NT, 16>), grid, dim3(NT), 0, st, a);
   LAUNCHCHK();
   return 0;
+}
+template <int N>
+static int arow(const Args& a, int B, hipStream_t st) {
+  return B < 16 ? arow_t<N, 1>(a, B, st) : arow_t<N, Tiles<N>::TX>(a, B, st);
 }
 
 template <int N>
