@@ -9,6 +9,8 @@
 #include "td_kernels.h"
 #include "host.h"
 
+#include <functional>
+
 // ------------------------------------------------------------------ launches
 // default variant indices (column: TY = 1 << v, row: TX = 1 << v; wide plan)
 template <int N> struct DefTile;
@@ -1269,7 +1271,7 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   const Tableau& tb = tableau(scheme);
   const int S = td ? 2 : tb.S;
   const int nseg = (steps + ckpt - 1) / ckpt;
-  // units of cs: x, xb, xbn, yb, tmp, scr(2) | segment inputs [ckpt] | stages
+  // units of cs: xw (state of the fused steps), xb, xbn, yb, spare, scr(2) | segment inputs [ckpt] | stages
   const int stage_units = td ? 10 : 3 * S;
   const int units = 7 + ckpt + stage_units;
   TRY(grow(&c->gen, &c->gen_cap, (size_t)units * cs));
@@ -1278,7 +1280,7 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   Args a = base_args(c);
   double2* g = c->gen;
   auto slot = [&](int u) { return g + (size_t)u * cs; };
-  double2 *x = slot(0), *xb = slot(1), *xbn = slot(2), *yb = slot(3), *tmp = slot(4), *scr = slot(5);
+  double2 *xw = slot(0), *xb = slot(1), *xbn = slot(2), *yb = slot(3), *scr = slot(5);
   auto xt = [&](int t) { return slot(7 + t); };
   const int s0 = 7 + ckpt;
   const int diss = scheme == G_LRK6 ? 0 : 1;
@@ -1293,64 +1295,113 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   auto wb = [&](int k) { return slot(s0 + 8 + k); };
   const size_t bytes = cs * sizeof(double2);
 
-  // one step from xin (not modified) storing the stages; the step's output in xout
-  auto step_fwd = [&](double2* xin, double2* xout) -> int {
-    if (!td) {
-      for (int k = 0; k < tb.S; ++k) {
-        double2* in = xin;
-        if (k > 0) {
-          CombBuilder cbd(c, ce, ediv, msk, y(k), nullptr);
-          cbd.term(xin, nullptr, 1.0, 0, tb.p[k]);
-          for (int j = 0; j < k; ++j)
-            if (tb.A[k][j] != 0.0) cbd.term(v(j), nullptr, tb.A[k][j], 1, tb.p[k] - tb.p[j]);
-          TRY((comb_launch<N, false>(cbd.cb, B, st)));
-          in = y(k);
-        }
-        TRY((rhs_pass<N, false>(a, in, nullptr, v(k), nullptr, B, diss, st)));
+  // Forward steps with the fused stage kernels of run_gen (a row pass and one
+  // C_GEN column kernel per stage; TDRK4: C_TD1 / C_TD2 per half step), in
+  // place on the scratch slot xw: each step stores its stage inputs y(k)
+  // (TDRK4: y) and stage values v(k) (TDRK4: v1, v2, w1, w2) where the
+  // transposed stage algebra below reads them, and leaves its result in xw.
+  // The stage kernels evaluate rhs_pass and comb_kernel's arithmetic term by
+  // term, so the stored stages are the ones the separate passes would give.
+  std::vector<Comb> tabs;
+  if (!td) {
+    for (int k = 0; k < tb.S; ++k) {
+      const bool last = k == tb.S - 1;
+      CombBuilder cb(c, ce, ediv, msk, last ? xw : y(k + 1), nullptr);
+      const int pk = last ? tb.p[tb.S] : tb.p[k + 1];
+      cb.term(xw, nullptr, 1.0, 0, pk);
+      for (int j = 0; j <= k; ++j) {
+        const double cj = last ? tb.b[j] : tb.A[k + 1][j];
+        if (cj != 0.0) cb.term(v(j), nullptr, cj, 1, pk - tb.p[j]);
       }
-      if (xout) {
-        CombBuilder cbd(c, ce, ediv, msk, xout, nullptr);
-        cbd.term(xin, nullptr, 1.0, 0, tb.p[tb.S]);
-        for (int j = 0; j < tb.S; ++j)
-          if (tb.b[j] != 0.0) cbd.term(v(j), nullptr, tb.b[j], 1, tb.p[tb.S] - tb.p[j]);
-        TRY((comb_launch<N, false>(cbd.cb, B, st)));
-      }
-      return 0;
+      tabs.push_back(cb.cb);
     }
-    TRY((rhs_pass<N, false>(a, xin, nullptr, v(0), nullptr, B, diss, st)));
-    TRY((rhs_pass<N, true>(a, xin, v(0), scr, w(0), B, diss, st)));
+  } else {
+    // y = x + h/2 v1 + h^2/8 w1,  x' = x + h v1 + h^2/6 w1 + h^2/3 w2
     CombBuilder cy(c, ce, 1.0, 0, y(0), nullptr);
-    cy.term(xin, nullptr, 1.0, 0, 0).term(v(0), nullptr, 0.5, 1, 0).term(w(0), nullptr, 0.125, 2, 0);
-    TRY((comb_launch<N, false>(cy.cb, B, st)));
-    TRY((rhs_pass<N, false>(a, y(0), nullptr, v(1), nullptr, B, diss, st)));
-    TRY((rhs_pass<N, true>(a, y(0), v(1), scr, w(1), B, diss, st)));
-    if (xout) {
-      CombBuilder cx(c, ce, 1.0, 0, xout, nullptr);
-      cx.term(xin, nullptr, 1.0, 0, 0).term(v(0), nullptr, 1.0, 1, 0).term(w(0), nullptr, 1.0 / 6, 2, 0)
-        .term(w(1), nullptr, 1.0 / 3, 2, 0);
-      TRY((comb_launch<N, false>(cx.cb, B, st)));
+    cy.term(xw, nullptr, 1.0, 0, 0).term(v(0), nullptr, 0.5, 1, 0).term(w(0), nullptr, 0.125, 2, 0);
+    CombBuilder cx(c, ce, 1.0, 0, xw, nullptr);
+    cx.term(xw, nullptr, 1.0, 0, 0).term(v(0), nullptr, 1.0, 1, 0).term(w(0), nullptr, 1.0 / 6, 2, 0)
+      .term(w(1), nullptr, 1.0 / 3, 2, 0);
+    tabs.push_back(cy.cb);
+    tabs.push_back(cx.cb);
+  }
+  TRY(grow(&c->gcomb, &c->gcomb_cap, (tabs.size() * sizeof(Comb) + sizeof(double2) - 1) / sizeof(double2)));
+  Comb* dtabs = (Comb*)c->gcomb;
+  HIPCHK(hipMemcpyAsync(dtabs, tabs.data(), tabs.size() * sizeof(Comb), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // the host table goes out of scope
+  Args af = a;
+  af.adaptive = AD_FIXED;
+  af.include_diss = diss;
+  // n >= 1 steps from the state in xw; after(s), when given, runs once step s
+  // has left its result in xw
+  auto fsteps = [&](int n, const std::function<int(int)>& after) -> int {
+    Args al = af;
+    al.x0 = xw;
+    TRY((col<N, C_LOAD, false>(al, B, st)));
+    for (int s = 0; s < n; ++s) {
+      if (!td) {
+        for (int k = 0; k < tb.S; ++k) {
+          const bool last = k == tb.S - 1;
+          TRY((row<N, R_RHS, false>(af, B, st)));
+          Args ak = af;
+          ak.x0 = k == 0 ? xw : y(k);
+          ak.out = v(k);
+          ak.xf = last ? xw : y(k + 1);
+          ak.gc = dtabs + k;
+          ak.skip_inv = (s == n - 1 && last) ? 1 : 0;
+          TRY((col<N, C_GEN, false>(ak, B, st)));
+        }
+      } else {
+        for (int half = 0; half < 2; ++half) {
+          double2* z = half == 0 ? xw : y(0);
+          TRY((row<N, R_RHS, false>(af, B, st)));
+          Args a1 = af;
+          a1.x0 = z;
+          a1.out = v(half);
+          TRY((col<N, C_TD1, false>(a1, B, st)));
+          TRY((row<N, R_RHS, true>(af, B, st)));
+          Args a2 = af;
+          a2.x0 = z;
+          a2.dx0 = v(half);
+          a2.dout = w(half);
+          a2.gc = dtabs + half;
+          a2.xf = half == 0 ? y(0) : xw;
+          a2.skip_inv = (s == n - 1 && half == 1) ? 1 : 0;
+          TRY((col<N, C_TD2, true>(a2, B, st)));
+        }
+      }
+      if (after) TRY(after(s));
     }
     return 0;
   };
+  auto copy = [&](double2* dst, const double2* src) -> int {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+    return 0;
+  };
 
-  // forward pass with checkpoints
-  TRY(pack<N>(f0, x, B, st));
-  for (int s = 0; s < steps; ++s) {
-    if (s % ckpt == 0) HIPCHK(hipMemcpyAsync(c->ckpt + (size_t)(s / ckpt) * cs, x, bytes, hipMemcpyDeviceToDevice, st));
-    TRY(step_fwd(x, tmp));
-    std::swap(x, tmp);
-  }
+  // forward pass to the last segment's start, checkpointing every ckpt steps
+  TRY(pack<N>(f0, xw, B, st));
+  TRY(copy(c->ckpt, xw));
+  if (nseg > 1)
+    TRY(fsteps((nseg - 1) * ckpt, [&](int s) -> int {
+      return (s + 1) % ckpt == 0 ? copy(c->ckpt + (size_t)((s + 1) / ckpt) * cs, xw) : 0;
+    }));
   // reverse
   TRY(pack<N>(fbar, xb, B, st));
   HIPCHK(hipMemsetAsync(c->hb, 0, cs * sizeof(double), st));
   for (int jseg = nseg - 1; jseg >= 0; --jseg) {
     const int t0 = jseg * ckpt;
     const int L = std::min(ckpt, steps - t0);
-    HIPCHK(hipMemcpyAsync(xt(0), c->ckpt + (size_t)jseg * cs, bytes, hipMemcpyDeviceToDevice, st));
-    for (int t = 0; t + 1 < L; ++t) TRY(step_fwd(xt(t), xt(t + 1)));
+    // the segment's step inputs, then per step (last first) its stages
+    TRY(copy(xt(0), c->ckpt + (size_t)jseg * cs));
+    if (L > 1) {
+      TRY(copy(xw, xt(0)));
+      TRY(fsteps(L - 1, [&](int t) -> int { return copy(xt(t + 1), xw); }));
+    }
     for (int t = L - 1; t >= 0; --t) {
       double2* xs = xt(t);
-      TRY(step_fwd(xs, nullptr));
+      TRY(copy(xw, xs));
+      TRY(fsteps(1, nullptr));
       HIPCHK(hipMemsetAsync(xbn, 0, bytes, st));
       if (!td) {
         for (int k = 0; k < S; ++k) HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));
