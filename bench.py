@@ -147,20 +147,26 @@ def config0_leg(cpu=True):
 
 def steppers_leg():
     """The other integrators of mismatch_RPO (classical RK4, Lawson RK6,
-    two-derivative RK4), primal and JVP, at N=128 and N=256, batch 1, 256
-    steps of h = 1/256 on G(N, 0)."""
+    two-derivative RK4), primal, JVP and VJP (reverse pass with checkpoints
+    every 64 steps), at N=128 and N=256, batch 1, 256 steps of h = 1/256 on
+    G(N, 0)."""
     import torch
     from ecs_mhd_2d_amd import mhd
     from ecs_mhd_2d_amd.synthetic import spectrum
-    out = {"workload": "batch 1, 256 steps of h=1/256 on G(N, 0); JVP along G(N, 1), dT=1e-3"}
+    out = {"workload": "batch 1, 256 steps of h=1/256 on G(N, 0); JVP along G(N, 1), dT=1e-3; "
+                       "VJP of the final state along G(N, 2), checkpoint every 64"}
     for n in (128, 256):
         ctx = mhd.context(mhd.make_param_dict(n, 1 / 40, 1 / 40, (0.0, 0.1)))
         f = torch.as_tensor(spectrum(n, 0))[None].cuda()
         df = torch.as_tensor(spectrum(n, 1))[None].cuda()
+        fb = torch.as_tensor(spectrum(n, 2))[None].cuda()
         runs = {"rk4": lambda s: ctx.rk4(f, 1 / 256, s), "rk4_jvp": lambda s: ctx.rk4_jvp(f, df, 1 / 256, 1e-3, s),
                 "lawson_rk6": lambda s: ctx.lawson_rk6(f, 1 / 256, s),
                 "lawson_rk6_jvp": lambda s: ctx.lawson_rk6_jvp(f, df, 1 / 256, 1e-3, s),
-                "tdrk4": lambda s: ctx.tdrk4(f, 1 / 256, s), "tdrk4_jvp": lambda s: ctx.tdrk4_jvp(f, df, 1 / 256, 1e-3, s)}
+                "tdrk4": lambda s: ctx.tdrk4(f, 1 / 256, s), "tdrk4_jvp": lambda s: ctx.tdrk4_jvp(f, df, 1 / 256, 1e-3, s),
+                "rk4_vjp": lambda s: ctx.rk4_vjp(f, fb, 1 / 256, s, 64),
+                "lawson_rk6_vjp": lambda s: ctx.lawson_rk6_vjp(f, fb, 1 / 256, s, True, 64),
+                "tdrk4_vjp": lambda s: ctx.tdrk4_vjp(f, fb, 1 / 256, s, 64)}
         for name, fn in runs.items():
             fn(4)
             torch.cuda.synchronize()
