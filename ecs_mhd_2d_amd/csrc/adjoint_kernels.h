@@ -43,7 +43,8 @@ __device__ __forceinline__ void adj_glue(const Args& a, int b, size_t ci, int r,
   if (MODE == B_OUT) {
     double2 o = yb;
     if (a.include_diss) o = cadd(o, cscale(L, a.gin[ci]));
-    a.out[ci] = o;
+    // added as the combination kernel adds a term of weight 1
+    a.out[ci] = a.out_acc ? cadd(a.out[ci], cscale(1.0, o)) : o;
     return;
   }
   const double dt = a.dt[b];

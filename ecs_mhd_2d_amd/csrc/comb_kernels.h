@@ -68,7 +68,9 @@ __global__ __launch_bounds__(256) void comb_kernel(Comb cb, int B) {
 namespace ecs {
 
 // Transpose of one stage combination, for the reverse pass:
-//   out_t += c_t h^a_t E^p_t yb          (each target once per call)
+//   out_t += c_t h^a_t E^p_t yb          (each target once per call; = for the
+//                                          terms marked in `assign`, the first
+//                                          contribution a target gets)
 //   hb    += sum_t d(c_t h^a_t E^p_t)/dh <yb, in_t>   (per mode, real inner product)
 // where in_t is the primal operand of term t (null: no d/dh term) and the
 // spectral cotangent convention holds d/dRe, d/dIm in the real, imag parts.
@@ -80,6 +82,7 @@ struct CombAdj {
   int hp[COMB_MAX];
   int ep[COMB_MAX];
   int nterm;
+  unsigned assign;               // bit t: term t overwrites out_t instead of adding
   double ediv;
   int masked;
   const double* h;
@@ -117,7 +120,7 @@ __global__ __launch_bounds__(256) void comb_adj_kernel(CombAdj cb, int B) {
         em = (cb.masked && dc) ? 0.0 : Ep;
         w *= em;
       }
-      if (cb.out[t]) cb.out[t][i] = cadd(cb.out[t][i], cscale(w, yb));
+      if (cb.out[t]) cb.out[t][i] = ((cb.assign >> t) & 1u) ? cscale(w, yb) : cadd(cb.out[t][i], cscale(w, yb));
       if (cb.in[t] && cb.hb) {
         double dw = cb.coef[t] * (double)cb.hp[t];
         for (int q = 1; q < cb.hp[t]; ++q) dw *= h;

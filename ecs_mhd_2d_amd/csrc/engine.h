@@ -1219,26 +1219,30 @@ struct CombAdjBuilder {
     cb.nu[0] = c->nu; cb.nu[1] = c->eta;
     cb.yb = yb; cb.hb = hb; cb.elems = elems;
   }
-  CombAdjBuilder& term(double2* out, const double2* in, double coef, int hp, int ep) {
+  // assign: this is the first contribution to `out` (it overwrites instead of adding)
+  CombAdjBuilder& term(double2* out, const double2* in, double coef, int hp, int ep, bool assign = false) {
     const int t = cb.nterm++;
     cb.out[t] = out; cb.in[t] = in; cb.coef[t] = coef; cb.hp[t] = hp; cb.ep[t] = ep;
+    if (assign) cb.assign |= 1u << t;
     return *this;
   }
 };
 
 // out = J(x)^T g (with the RHS's dissipation when diss), on compact spectra;
 // hess: out = (J_0(x))^T g with J_0 the Jacobian of the nonlinearity without
-// b0 (the transpose of y -> H[y, x]).
+// b0 (the transpose of y -> H[y, x]).  prim: the column pass's output at x
+// (the 8 derived fields in x-space), when a forward pass left it; otherwise
+// it is recomputed.  acc: out += instead of out =.
 template <int N>
 static int vjp_pass(ecs_ctx* c, const Args& base, double2* x, const double2* g, double2* out, int B, int diss,
-                    hipStream_t st, bool hess = false) {
+                    hipStream_t st, bool hess = false, const double2* prim = nullptr, bool acc = false) {
   Args a = base;
-  a.x0 = x; a.gin = g; a.out = out;
+  a.x0 = x; a.gin = g; a.out = out; a.out_acc = acc ? 1 : 0;
   a.include_diss = hess ? 0 : diss;
-  a.aout = c->AO; a.ain = c->AI; a.prim = c->I;
+  a.aout = c->AO; a.ain = c->AI; a.prim = prim ? prim : c->I;
   a.adaptive = AD_FIXED;
   if (hess) { a.b0x = 0.0; a.b0y = 0.0; }
-  TRY((col<N, C_LOAD, false>(a, B, st)));
+  if (!prim) TRY((col<N, C_LOAD, false>(a, B, st)));
   TRY((acol<N, B_G>(a, B, st)));
   TRY(arow<N>(a, B, st));
   TRY((acol<N, B_OUT>(a, B, st)));
@@ -1271,7 +1275,7 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   const Tableau& tb = tableau(scheme);
   const int S = td ? 2 : tb.S;
   const int nseg = (steps + ckpt - 1) / ckpt;
-  // units of cs: xw (state of the fused steps), xb, xbn, yb, spare, scr(2) | segment inputs [ckpt] | stages
+  // units of cs: xw (state of the fused steps), xb, xbn, yb, 3 spare | segment inputs [ckpt] | stages
   const int stage_units = td ? 10 : 3 * S;
   const int units = 7 + ckpt + stage_units;
   TRY(grow(&c->gen, &c->gen_cap, (size_t)units * cs));
@@ -1280,7 +1284,7 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   Args a = base_args(c);
   double2* g = c->gen;
   auto slot = [&](int u) { return g + (size_t)u * cs; };
-  double2 *xw = slot(0), *xb = slot(1), *xbn = slot(2), *yb = slot(3), *scr = slot(5);
+  double2 *xw = slot(0), *xb = slot(1), *xbn = slot(2), *yb = slot(3);
   auto xt = [&](int t) { return slot(7 + t); };
   const int s0 = 7 + ckpt;
   const int diss = scheme == G_LRK6 ? 0 : 1;
@@ -1332,29 +1336,42 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   Args af = a;
   af.adaptive = AD_FIXED;
   af.include_diss = diss;
+  // The derived fields (column pass output) of the stage inputs of the step
+  // being transposed, kept for its RHS-VJP passes: slot k for stage input k
+  // (TDRK4: 0 the step input, 1 the midpoint y).
+  const size_t is = (size_t)B * 8 * N * Geo<N>::KP;
+  TRY(grow(&c->itape, &c->it_cap, (size_t)(td ? 2 : S + 1) * is));
+  auto ti = [&](int k) { return c->itape + (size_t)k * is; };
   // n >= 1 steps from the state in xw; after(s), when given, runs once step s
-  // has left its result in xw
-  auto fsteps = [&](int n, const std::function<int(int)>& after) -> int {
+  // has left its result in xw; tape (one step): the stage inputs' derived
+  // fields go to the slots ti(k) instead of the shared column output
+  auto fsteps = [&](int n, const std::function<int(int)>& after, bool tape) -> int {
     Args al = af;
     al.x0 = xw;
+    if (tape) al.inter = ti(0);
     TRY((col<N, C_LOAD, false>(al, B, st)));
     for (int s = 0; s < n; ++s) {
       if (!td) {
         for (int k = 0; k < tb.S; ++k) {
           const bool last = k == tb.S - 1;
-          TRY((row<N, R_RHS, false>(af, B, st)));
+          Args ar = af;
+          if (tape) ar.inter = ti(k);
+          TRY((row<N, R_RHS, false>(ar, B, st)));
           Args ak = af;
           ak.x0 = k == 0 ? xw : y(k);
           ak.out = v(k);
           ak.xf = last ? xw : y(k + 1);
           ak.gc = dtabs + k;
           ak.skip_inv = (s == n - 1 && last) ? 1 : 0;
+          if (tape) ak.inter = ti(k + 1);
           TRY((col<N, C_GEN, false>(ak, B, st)));
         }
       } else {
         for (int half = 0; half < 2; ++half) {
           double2* z = half == 0 ? xw : y(0);
-          TRY((row<N, R_RHS, false>(af, B, st)));
+          Args ar = af;
+          if (tape) ar.inter = ti(half);
+          TRY((row<N, R_RHS, false>(ar, B, st)));
           Args a1 = af;
           a1.x0 = z;
           a1.out = v(half);
@@ -1367,6 +1384,7 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
           a2.gc = dtabs + half;
           a2.xf = half == 0 ? y(0) : xw;
           a2.skip_inv = (s == n - 1 && half == 1) ? 1 : 0;
+          if (tape && half == 0) a2.inter = ti(1);
           TRY((col<N, C_TD2, true>(a2, B, st)));
         }
       }
@@ -1385,7 +1403,7 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   if (nseg > 1)
     TRY(fsteps((nseg - 1) * ckpt, [&](int s) -> int {
       return (s + 1) % ckpt == 0 ? copy(c->ckpt + (size_t)((s + 1) / ckpt) * cs, xw) : 0;
-    }));
+    }, false));
   // reverse
   TRY(pack<N>(fbar, xb, B, st));
   HIPCHK(hipMemsetAsync(c->hb, 0, cs * sizeof(double), st));
@@ -1396,71 +1414,55 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
     TRY(copy(xt(0), c->ckpt + (size_t)jseg * cs));
     if (L > 1) {
       TRY(copy(xw, xt(0)));
-      TRY(fsteps(L - 1, [&](int t) -> int { return copy(xt(t + 1), xw); }));
+      TRY(fsteps(L - 1, [&](int t) -> int { return copy(xt(t + 1), xw); }, false));
     }
     for (int t = L - 1; t >= 0; --t) {
       double2* xs = xt(t);
       TRY(copy(xw, xs));
-      TRY(fsteps(1, nullptr));
-      HIPCHK(hipMemsetAsync(xbn, 0, bytes, st));
+      TRY(fsteps(1, nullptr, true));
+      // the step's cotangent accumulators start with their first contribution
+      // (assign), so they need no clearing
       if (!td) {
-        for (int k = 0; k < S; ++k) HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));
+        std::vector<char> touched(S, 0);
         CombAdjBuilder fin(c, ce, ediv, msk, xb, c->hb);
-        fin.term(xbn, xs, 1.0, 0, tb.p[S]);
+        fin.term(xbn, xs, 1.0, 0, tb.p[S], true);
         for (int j = 0; j < S; ++j)
-          if (tb.b[j] != 0.0) fin.term(vb(j), v(j), tb.b[j], 1, tb.p[S] - tb.p[j]);
+          if (tb.b[j] != 0.0) {
+            fin.term(vb(j), v(j), tb.b[j], 1, tb.p[S] - tb.p[j], true);
+            touched[j] = 1;
+          }
         TRY(comb_adj_launch<N>(fin.cb, B, st));
         for (int k = S - 1; k >= 0; --k) {
-          TRY(vjp_pass<N>(c, a, k == 0 ? xs : y(k), vb(k), yb, B, diss, st));
+          if (!touched[k]) HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));  // a stage no combination reads
+          TRY(vjp_pass<N>(c, a, k == 0 ? xs : y(k), vb(k), yb, B, diss, st, false, ti(k)));
           CombAdjBuilder st_(c, ce, ediv, msk, yb, c->hb);
           st_.term(xbn, k == 0 ? nullptr : xs, 1.0, 0, tb.p[k]);
           for (int j = 0; j < k; ++j)
-            if (tb.A[k][j] != 0.0) st_.term(vb(j), v(j), tb.A[k][j], 1, tb.p[k] - tb.p[j]);
+            if (tb.A[k][j] != 0.0) {
+              st_.term(vb(j), v(j), tb.A[k][j], 1, tb.p[k] - tb.p[j], !touched[j]);
+              touched[j] = 1;
+            }
           TRY(comb_adj_launch<N>(st_.cb, B, st));
         }
       } else {
-        for (int k = 0; k < 2; ++k) {
-          HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));
-          HIPCHK(hipMemsetAsync(wb(k), 0, bytes, st));
-        }
-        // x' = x + h v1 + h^2/6 w1 + h^2/3 w2
+        // x' = x + h v1 + h^2/6 w1 + h^2/3 w2 (vbar2 is written whole by its VJP pass below)
         CombAdjBuilder fin(c, ce, 1.0, 0, xb, c->hb);
-        fin.term(xbn, nullptr, 1.0, 0, 0).term(vb(0), v(0), 1.0, 1, 0).term(wb(0), w(0), 1.0 / 6, 2, 0)
-          .term(wb(1), w(1), 1.0 / 3, 2, 0);
+        fin.term(xbn, nullptr, 1.0, 0, 0, true).term(vb(0), v(0), 1.0, 1, 0, true)
+          .term(wb(0), w(0), 1.0 / 6, 2, 0, true).term(wb(1), w(1), 1.0 / 3, 2, 0, true);
         TRY(comb_adj_launch<N>(fin.cb, B, st));
         // w2 = J(y) v2, v2 = f(y): vbar2 = J(y)^T wbar2; ybar = J(y)^T vbar2 + J_0(v2)^T wbar2
-        TRY(vjp_pass<N>(c, a, y(0), wb(1), vb(1), B, diss, st));
-        TRY(vjp_pass<N>(c, a, y(0), vb(1), yb, B, diss, st));
-        TRY(vjp_pass<N>(c, a, v(1), wb(1), scr, B, diss, st, true));
-        {
-          CombAdjBuilder acc(c, ce, 1.0, 0, scr, nullptr);
-          acc.term(yb, nullptr, 1.0, 0, 0);
-          TRY(comb_adj_launch<N>(acc.cb, B, st));
-        }
+        TRY(vjp_pass<N>(c, a, y(0), wb(1), vb(1), B, diss, st, false, ti(1)));
+        TRY(vjp_pass<N>(c, a, y(0), vb(1), yb, B, diss, st, false, ti(1)));
+        TRY(vjp_pass<N>(c, a, v(1), wb(1), yb, B, diss, st, true, nullptr, true));
         // y = x + h/2 v1 + h^2/8 w1
         CombAdjBuilder cy(c, ce, 1.0, 0, yb, c->hb);
         cy.term(xbn, nullptr, 1.0, 0, 0).term(vb(0), v(0), 0.5, 1, 0).term(wb(0), w(0), 0.125, 2, 0);
         TRY(comb_adj_launch<N>(cy.cb, B, st));
         // w1 = J(x) v1: vbar1 += J(x)^T wbar1, xbar += J_0(v1)^T wbar1
-        TRY(vjp_pass<N>(c, a, xs, wb(0), yb, B, diss, st));
-        {
-          CombAdjBuilder acc(c, ce, 1.0, 0, yb, nullptr);
-          acc.term(vb(0), nullptr, 1.0, 0, 0);
-          TRY(comb_adj_launch<N>(acc.cb, B, st));
-        }
-        TRY(vjp_pass<N>(c, a, v(0), wb(0), yb, B, diss, st, true));
-        {
-          CombAdjBuilder acc(c, ce, 1.0, 0, yb, nullptr);
-          acc.term(xbn, nullptr, 1.0, 0, 0);
-          TRY(comb_adj_launch<N>(acc.cb, B, st));
-        }
+        TRY(vjp_pass<N>(c, a, xs, wb(0), vb(0), B, diss, st, false, ti(0), true));
+        TRY(vjp_pass<N>(c, a, v(0), wb(0), xbn, B, diss, st, true, nullptr, true));
         // v1 = f(x): xbar += J(x)^T vbar1
-        TRY(vjp_pass<N>(c, a, xs, vb(0), yb, B, diss, st));
-        {
-          CombAdjBuilder acc(c, ce, 1.0, 0, yb, nullptr);
-          acc.term(xbn, nullptr, 1.0, 0, 0);
-          TRY(comb_adj_launch<N>(acc.cb, B, st));
-        }
+        TRY(vjp_pass<N>(c, a, xs, vb(0), xbn, B, diss, st, false, ti(0), true));
       }
       std::swap(xb, xbn);
     }

@@ -181,10 +181,11 @@ struct Args {
   const double2* txp;      // tape of the previous step (fused entry)
   const double2* tvp;
   size_t cstride;          // elements of one compact batch array
-  double2* aout;           // adjoint column output AO [B][2][N][KC]
-  double2* ain;            // adjoint row output AI [B][8][N][KC]
-  const double2* prim;     // primal column output for the row-adjoint [B][8][N][KC]
+  double2* aout;           // adjoint column output AO [B][2][N][KP]
+  double2* ain;            // adjoint row output AI [B][8][N][KP]
+  const double2* prim;     // primal column output for the row-adjoint [B][8][N][KP]
   const double2* gin;      // cotangent input of a standalone RHS adjoint
+  int out_acc;             // standalone RHS adjoint: add its result to out instead of storing it
   // fixed step with per-candidate step counts: candidate b takes part in the
   // launches of global step `step` only while step < nstep[b] (null: all do)
   const int* nstep;
