@@ -1,5 +1,8 @@
 // Stage combinations of the generic explicit steppers (classical RK4,
-// Lawson RK6, two-derivative RK4) over compact spectra [B][2][R][KC].
+// Lawson RK6, two-derivative RK4, and the Lawson RK4 steps of the RK4(3)
+// reverse pass) over compact spectra [B][2][R][KC].  The forward combination
+// is evaluated in the column kernels' glue (C_GEN, C_TD2, td_col_b) from a
+// table in device memory; this file holds its transpose for the reverse pass.
 //
 // out = sum_t coef_t * h^hp_t * E^ep_t * in_t,   E = exp(h L / ediv)
 // with L = -(kx^2 + ky^2) [nu; eta] and h the candidate's step.  A Lawson
@@ -12,58 +15,6 @@
 //   dout = sum_t w_t din_t + dh * sum_t (hp_t / h + ep_t L / ediv) w_t in_t.
 #pragma once
 #include "kernels.h"
-
-namespace ecs {
-
-
-template <int N, bool TAN>
-__global__ __launch_bounds__(256) void comb_kernel(Comb cb, int B) {
-  using G = Geo<N>;
-  constexpr int KC = G::KC, R = G::R;
-  const size_t tot = (size_t)B * cb.elems;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
-    const int b = (int)(i / cb.elems);
-    const size_t e = i - (size_t)b * cb.elems;
-    const int ky = (int)(e % KC);
-    const int r = (int)((e / KC) % R);
-    const int c = (int)(e / ((size_t)KC * R));
-    const int kx = G::row_kx(r);
-    const bool dc = (kx == 0 && ky == 0);
-    const double L = -(double)(kx * kx + ky * ky) * cb.nu[c];
-    const double h = cb.h[b];
-    const double dh = TAN ? cb.dh[b] : 0.0;
-    const double E1 = exp(h / cb.ediv * L);
-    double2 acc = make_double2(0.0, 0.0), dacc = acc;
-    for (int t = 0; t < cb.nterm; ++t) {
-      double w = cb.coef[t];
-      for (int q = 0; q < cb.hp[t]; ++q) w *= h;
-      if (cb.ep[t] > 0) {
-        double Ep = E1;
-        for (int q = 1; q < cb.ep[t]; ++q) Ep *= E1;
-        w *= (cb.masked && dc) ? 0.0 : Ep;
-      }
-      const double2 v = cb.in[t] ? cb.in[t][i] : make_double2(0.0, 0.0);
-      acc = cadd(acc, cscale(w, v));
-      if (TAN) {
-        // d w / dh = w (hp / h + ep L / ediv); hp / h written without dividing by h
-        double dw = cb.coef[t] * (double)cb.hp[t];
-        for (int q = 1; q < cb.hp[t]; ++q) dw *= h;
-        if (cb.ep[t] > 0) {
-          double Ep = E1;
-          for (int q = 1; q < cb.ep[t]; ++q) Ep *= E1;
-          const double em = (cb.masked && dc) ? 0.0 : Ep;
-          dw = dw * em + w * ((double)cb.ep[t] * L / cb.ediv);
-        }
-        dacc = cadd(dacc, cscale(dh * dw, v));
-        if (cb.din[t]) dacc = cadd(dacc, cscale(w, cb.din[t][i]));
-      }
-    }
-    cb.out[i] = acc;
-    if (TAN) cb.dout[i] = dacc;
-  }
-}
-
-}  // namespace ecs
 
 namespace ecs {
 

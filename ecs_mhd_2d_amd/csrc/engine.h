@@ -904,8 +904,9 @@ static int run_time_kernel(ecs_ctx* c, int B, int which, int reps, double* ms, h
 
 // ------------------------------------------------------------------ generic explicit steppers
 // Classical RK4 (timestepping.py:17-44), Lawson RK6 (:83-125) and two-derivative
-// RK4 (:127-149), run as RHS passes (column load, row pass, column output)
-// with the stage algebra in comb_kernel between them.
+// RK4 (:127-149): each stage a row pass and a column kernel whose glue stores
+// the stage RHS and forms the next stage input from a combination table
+// (comb_kernels.h) in device memory.
 enum GenScheme : int { G_RK4 = 0, G_LRK6 = 1, G_TDRK4 = 2, G_LRK4 = 3 };
 
 // Butcher form of a (Lawson) scheme: stage s reads
@@ -946,33 +947,6 @@ static const Tableau& tableau(int scheme) {
                                {0, 1, 1, 2, 2},
                                2.0};
   return scheme == G_LRK6 ? lrk6 : (scheme == G_LRK4 ? lrk4 : rk4);
-}
-
-template <int N, bool TAN>
-static int comb_launch(const Comb& cb, int B, hipStream_t st) {
-  const size_t tot = (size_t)B * cb.elems;
-  const int blocks = (int)std::min<size_t>((tot + 255) / 256, 4096);
-  hipLaunchKernelGGL((comb_kernel<N, TAN>), dim3(blocks), dim3(256), 0, st, cb, B);
-  LAUNCHCHK();
-  return 0;
-}
-
-// RHS of compact spectra x into v, with the tangent J(x) dx into dv.  hess:
-// the nonlinear term's second derivative H[x, dx] (= J(x) dx with b0 = 0 and
-// no dissipation; the MHD nonlinearity is a symmetric bilinear form) into dv,
-// v being scratch.
-template <int N, bool TAN>
-static int rhs_pass(const Args& base, double2* x, double2* dx, double2* v, double2* dv, int B, int diss,
-                    hipStream_t st, bool hess = false) {
-  Args a = base;
-  a.x0 = x; a.dx0 = dx; a.out = v; a.dout = dv;
-  a.include_diss = hess ? 0 : diss;
-  a.adaptive = AD_FIXED;
-  if (hess) { a.b0x = 0.0; a.b0y = 0.0; }
-  TRY((col<N, C_LOAD, TAN>(a, B, st)));
-  TRY((row<N, R_RHS, TAN>(a, B, st)));
-  TRY((col<N, C_RHS, TAN>(a, B, st)));
-  return 0;
 }
 
 struct CombBuilder {
@@ -1036,8 +1010,7 @@ static int run_gen(ecs_ctx* c, double* f, double* df, int B, const double* dt, c
     // Fused form: each stage is a row pass and one C_GEN column kernel, which
     // stores the stage's RHS v_k (and tangent) and forms the next stage input
     // (or the step's result) from the tableau row in registers.  The
-    // combinations are the comb_kernel's, term by term, so the result is the
-    // same.
+    // combinations are evaluated term by term in table order.
     double2* y = slot(1);
     double2* dx = TAN ? slot(2) : nullptr;
     double2* dy = TAN ? slot(3) : nullptr;
@@ -1098,7 +1071,7 @@ static int run_gen(ecs_ctx* c, double* f, double* df, int B, const double* dt, c
     // Fused: per half step a row pass, C_TD1 (v = RHS(z), then the pair
     // (z, v) transformed as a tangent pass's input), the tangent row pass and
     // C_TD2 (w = J(z) v, then the combination giving the next point), so 8
-    // launches per step; the combinations are comb_kernel's term by term.
+    // launches per step; the combinations are evaluated term by term in table order.
     double2 *y = slot(1), *v1 = slot(2), *w1 = slot(3), *v2 = slot(4), *w2 = slot(5);
     CombBuilder cy(c, ce, 1.0, 0, y, nullptr);
     cy.term(x, nullptr, 1.0, 0, 0).term(v1, nullptr, 0.5, 1, 0).term(w1, nullptr, 0.125, 2, 0);
@@ -1303,9 +1276,8 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   // C_GEN column kernel per stage; TDRK4: C_TD1 / C_TD2 per half step), in
   // place on the scratch slot xw: each step stores its stage inputs y(k)
   // (TDRK4: y) and stage values v(k) (TDRK4: v1, v2, w1, w2) where the
-  // transposed stage algebra below reads them, and leaves its result in xw.
-  // The stage kernels evaluate rhs_pass and comb_kernel's arithmetic term by
-  // term, so the stored stages are the ones the separate passes would give.
+  // transposed stage algebra below reads them, and leaves its result in xw
+  // (the same stages and result as the forward entry points compute).
   std::vector<Comb> tabs;
   if (!td) {
     for (int k = 0; k < tb.S; ++k) {
@@ -1543,14 +1515,15 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
   const Tableau& tb = tableau(G_LRK4);
   const int S = tb.S;
   const int nseg = (steps + ckpt - 1) / ckpt;
-  // units: x, xb, xbn, yb, tmp, vraw, vbraw | segment inputs [ckpt] | y[S], v[S], vbar[S]
+  // units: xw (state of the fused steps), xb, xbn, yb, spare, vraw, vbraw | segment inputs [ckpt] |
+  // y[S], v[S], vbar[S]
   const int units = 7 + ckpt + 3 * S;
   TRY(grow(&c->gen, &c->gen_cap, (size_t)units * cs));
   TRY(grow(&c->ckpt, &c->ckpt_cap, (size_t)nseg * cs));
   Args a = base_args(c);
   double2* g = c->gen;
   auto slot = [&](int u) { return g + (size_t)u * cs; };
-  double2 *x = slot(0), *xb = slot(1), *xbn = slot(2), *yb = slot(3), *tmp = slot(4), *vraw = slot(5);
+  double2 *xw = slot(0), *xb = slot(1), *xbn = slot(2), *yb = slot(3), *vraw = slot(5);
   double2* vbraw = slot(6);
   auto xt = [&](int t) { return slot(7 + t); };
   const int s0 = 7 + ckpt;
@@ -1560,90 +1533,131 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
   const size_t bytes = cs * sizeof(double2);
   const double ediv = tb.ediv;
 
-  auto step_fwd = [&](int m, double2* xin, double2* xout) -> int {
-    Args am = a;
-    am.dt = DT(m);
-    for (int k = 0; k < S; ++k) {
-      double2* in = xin;
-      if (k > 0) {
-        CombBuilder cbd(c, ce, ediv, 0, y(k), nullptr);
-        cbd.cb.h = DT(m);
-        cbd.term(xin, nullptr, 1.0, 0, tb.p[k]);
-        for (int j = 0; j < k; ++j)
-          if (tb.A[k][j] != 0.0) cbd.term(v(j), nullptr, tb.A[k][j], 1, tb.p[k] - tb.p[j]);
-        TRY((comb_launch<N, false>(cbd.cb, B, st)));
-        in = y(k);
-      }
-      if (k == 0) {
-        TRY((rhs_pass<N, false>(am, in, nullptr, vraw, nullptr, B, 0, st)));
-        CombBuilder pc(c, ce, 1.0, 0, v(0), nullptr);   // v1 = P vraw, P = exp(T s L)
-        pc.cb.h = PDT(m);
-        pc.term(vraw, nullptr, 1.0, 0, 1);
-        TRY((comb_launch<N, false>(pc.cb, B, st)));
-      } else {
-        TRY((rhs_pass<N, false>(am, in, nullptr, v(k), nullptr, B, 0, st)));
-      }
+  // Forward steps with the fused stage kernels (a row pass and one C_GEN
+  // column kernel per stage; C_GEN takes step m's dt from the launch and, on
+  // stage 1, stores the raw RHS at vraw and scales it by P = exp(T s L)), in
+  // place on xw; tables without a step array serve every step.  Each step
+  // stores its stage inputs y(k), stage values v(k) and vraw.
+  std::vector<Comb> tabs;
+  for (int k = 0; k < S; ++k) {
+    const bool last = k == S - 1;
+    CombBuilder cb(c, ce, ediv, 0, last ? xw : y(k + 1), nullptr);
+    cb.cb.h = nullptr;
+    const int pk = last ? tb.p[S] : tb.p[k + 1];
+    cb.term(xw, nullptr, 1.0, 0, pk);
+    for (int j = 0; j <= k; ++j) {
+      const double cj = last ? tb.b[j] : tb.A[k + 1][j];
+      if (cj != 0.0) cb.term(v(j), nullptr, cj, 1, pk - tb.p[j]);
     }
-    if (xout) {
-      CombBuilder cbd(c, ce, ediv, 0, xout, nullptr);
-      cbd.cb.h = DT(m);
-      cbd.term(xin, nullptr, 1.0, 0, tb.p[S]);
-      for (int j = 0; j < S; ++j)
-        if (tb.b[j] != 0.0) cbd.term(v(j), nullptr, tb.b[j], 1, tb.p[S] - tb.p[j]);
-      TRY((comb_launch<N, false>(cbd.cb, B, st)));
+    tabs.push_back(cb.cb);
+  }
+  TRY(grow(&c->gcomb, &c->gcomb_cap, (tabs.size() * sizeof(Comb) + sizeof(double2) - 1) / sizeof(double2)));
+  Comb* dtabs = (Comb*)c->gcomb;
+  HIPCHK(hipMemcpyAsync(dtabs, tabs.data(), tabs.size() * sizeof(Comb), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // the host table goes out of scope
+  // derived fields of the stage inputs of the step being transposed
+  const size_t is = (size_t)B * 8 * N * Geo<N>::KP;
+  TRY(grow(&c->itape, &c->it_cap, (size_t)(S + 1) * is));
+  auto ti = [&](int k) { return c->itape + (size_t)k * is; };
+  Args af = a;
+  af.adaptive = AD_FIXED;
+  af.include_diss = 0;  // v_fn of the Lawson schemes has no dissipation
+  // steps m0 .. m0 + n - 1 (n >= 1) from the state in xw; after(s) runs once
+  // step m0 + s has left its result in xw; tape (one step): the stage inputs'
+  // derived fields go to ti(k)
+  auto fsteps = [&](int m0, int n, const std::function<int(int)>& after, bool tape) -> int {
+    Args al = af;
+    al.x0 = xw;
+    if (tape) al.inter = ti(0);
+    TRY((col<N, C_LOAD, false>(al, B, st)));
+    for (int s = 0; s < n; ++s) {
+      const int m = m0 + s;
+      for (int k = 0; k < S; ++k) {
+        const bool last = k == S - 1;
+        Args ar = af;
+        if (tape) ar.inter = ti(k);
+        TRY((row<N, R_RHS, false>(ar, B, st)));
+        Args ak = af;
+        ak.dt = DT(m);
+        ak.x0 = k == 0 ? xw : y(k);
+        ak.out = v(k);
+        ak.xf = last ? xw : y(k + 1);
+        ak.gc = dtabs + k;
+        if (k == 0) {
+          ak.pdt = PDT(m);
+          ak.vraw = vraw;
+        }
+        ak.skip_inv = (s == n - 1 && last) ? 1 : 0;
+        if (tape) ak.inter = ti(k + 1);
+        TRY((col<N, C_GEN, false>(ak, B, st)));
+      }
+      if (after) TRY(after(s));
     }
     return 0;
   };
+  auto copy = [&](double2* dst, const double2* src) -> int {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+    return 0;
+  };
 
-  TRY(pack<N>(x0f, x, B, st));
-  for (int m = 0; m < steps; ++m) {
-    if (m % ckpt == 0) HIPCHK(hipMemcpyAsync(c->ckpt + (size_t)(m / ckpt) * cs, x, bytes, hipMemcpyDeviceToDevice, st));
-    TRY(step_fwd(m, x, tmp));
-    std::swap(x, tmp);
-  }
+  // forward pass to the last segment's start, checkpointing every ckpt steps
+  TRY(pack<N>(x0f, xw, B, st));
+  TRY(copy(c->ckpt, xw));
+  if (nseg > 1)
+    TRY(fsteps(0, (nseg - 1) * ckpt, [&](int s) -> int {
+      return (s + 1) % ckpt == 0 ? copy(c->ckpt + (size_t)((s + 1) / ckpt) * cs, xw) : 0;
+    }, false));
   TRY(pack<N>(xbar, xb, B, st));
   HIPCHK(hipMemsetAsync(c->hb, 0, cs * sizeof(double), st));
   for (int jseg = nseg - 1; jseg >= 0; --jseg) {
     const int t0 = jseg * ckpt;
     const int L = std::min(ckpt, steps - t0);
-    HIPCHK(hipMemcpyAsync(xt(0), c->ckpt + (size_t)jseg * cs, bytes, hipMemcpyDeviceToDevice, st));
-    for (int t = 0; t + 1 < L; ++t) TRY(step_fwd(t0 + t, xt(t), xt(t + 1)));
+    TRY(copy(xt(0), c->ckpt + (size_t)jseg * cs));
+    if (L > 1) {
+      TRY(copy(xw, xt(0)));
+      TRY(fsteps(t0, L - 1, [&](int t) -> int { return copy(xt(t + 1), xw); }, false));
+    }
     for (int t = L - 1; t >= 0; --t) {
       const int m = t0 + t;
       double2* xs = xt(t);
-      TRY(step_fwd(m, xs, nullptr));
-      HIPCHK(hipMemsetAsync(xbn, 0, bytes, st));
-      for (int k = 0; k < S; ++k) HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));
+      TRY(copy(xw, xs));
+      TRY(fsteps(m, 1, nullptr, true));
+      // the step's cotangent accumulators start with their first contribution
+      std::vector<char> touched(S, 0);
       CombAdjBuilder fin(c, ce, ediv, 0, xb, c->hb);
       fin.cb.h = DT(m);
       fin.cb.hscale = HSC(m);
-      fin.term(xbn, xs, 1.0, 0, tb.p[S]);
+      fin.term(xbn, xs, 1.0, 0, tb.p[S], true);
       for (int j = 0; j < S; ++j)
-        if (tb.b[j] != 0.0) fin.term(vb(j), v(j), tb.b[j], 1, tb.p[S] - tb.p[j]);
+        if (tb.b[j] != 0.0) {
+          fin.term(vb(j), v(j), tb.b[j], 1, tb.p[S] - tb.p[j], true);
+          touched[j] = 1;
+        }
       TRY(comb_adj_launch<N>(fin.cb, B, st));
       for (int k = S - 1; k >= 0; --k) {
+        if (!touched[k]) HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));  // a stage no combination reads
+        Args am = a;
+        am.dt = DT(m);
         if (k == 0) {
           // v1 = P vraw: vbraw = P vbar1, d/dT through T s
-          HIPCHK(hipMemsetAsync(vbraw, 0, bytes, st));
           CombAdjBuilder pc(c, ce, 1.0, 0, vb(0), c->hb);
           pc.cb.h = PDT(m);
           pc.cb.hscale = SIG(m);
-          pc.term(vbraw, vraw, 1.0, 0, 1);
+          pc.term(vbraw, vraw, 1.0, 0, 1, true);
           TRY(comb_adj_launch<N>(pc.cb, B, st));
-          Args am = a;
-          am.dt = DT(m);
-          TRY(vjp_pass<N>(c, am, xs, vbraw, yb, B, 0, st));
+          TRY(vjp_pass<N>(c, am, xs, vbraw, yb, B, 0, st, false, ti(0)));
         } else {
-          Args am = a;
-          am.dt = DT(m);
-          TRY(vjp_pass<N>(c, am, y(k), vb(k), yb, B, 0, st));
+          TRY(vjp_pass<N>(c, am, y(k), vb(k), yb, B, 0, st, false, ti(k)));
         }
         CombAdjBuilder sa(c, ce, ediv, 0, yb, c->hb);
         sa.cb.h = DT(m);
         sa.cb.hscale = HSC(m);
         sa.term(xbn, k == 0 ? nullptr : xs, 1.0, 0, tb.p[k]);
         for (int j = 0; j < k; ++j)
-          if (tb.A[k][j] != 0.0) sa.term(vb(j), v(j), tb.A[k][j], 1, tb.p[k] - tb.p[j]);
+          if (tb.A[k][j] != 0.0) {
+            sa.term(vb(j), v(j), tb.A[k][j], 1, tb.p[k] - tb.p[j], !touched[j]);
+            touched[j] = 1;
+          }
         TRY(comb_adj_launch<N>(sa.cb, B, st));
       }
       std::swap(xb, xbn);

@@ -71,9 +71,9 @@ struct Hist {
 };
 
 // One stage combination out = sum_t coef_t h^hp_t E^ep_t in_t (with its
-// tangent), E = exp(h L / ediv); run by comb_kernel, or read from device
-// memory by the column kernel's C_GEN glue (which takes the term whose input
-// is its own output pointer from registers).
+// tangent), E = exp(h L / ediv); read from device memory by the column
+// kernels' glue (C_GEN, C_TD2, td_col_b; each takes the term whose input is its
+// own output pointer from registers).  C_GEN reads a null h as the launch's Args::dt.
 constexpr int COMB_MAX = 10;
 
 struct Comb {
@@ -186,6 +186,11 @@ struct Args {
   const double2* prim;     // primal column output for the row-adjoint [B][8][N][KP]
   const double2* gin;      // cotangent input of a standalone RHS adjoint
   int out_acc;             // standalone RHS adjoint: add its result to out instead of storing it
+  // C_GEN, primal: when pdt is set, the stage RHS is stored raw at vraw and
+  // scaled by exp(pdt[b] L) before it is stored at out and combined (the
+  // RK4(3) reverse pass's carried-k1 factor on stage 1)
+  const double* pdt;
+  double2* vraw;
   // fixed step with per-candidate step counts: candidate b takes part in the
   // launches of global step `step` only while step < nstep[b] (null: all do)
   const int* nstep;

@@ -269,16 +269,22 @@ __device__ __forceinline__ void glue(const Args& a, bool wr, const Blk& bk, cons
   }
   if (MODE == C_GEN) {
     // this stage's RHS (and tangent), then the next stage input from the
-    // combination, term by term as comb_kernel evaluates it
+    // combination, term by term in table order
     double2 o = vh, od = dvh;
     if (a.include_diss) {
       o = cadd(o, cscale(L, p.x0));
       if (TAN) od = cadd(od, cscale(L, p.dx0));
     }
+    if (!TAN && a.pdt) {
+      // the combination kernel's weight of a one-term table with ediv 1
+      a.vraw[ci] = o;
+      o = cscale(exp(a.pdt[bb] / 1.0 * L), o);
+    }
     a.out[ci] = o;
     if (TAN) a.dout[ci] = od;
     const Comb* cm = a.gc;
-    const double h = cm->h[bb];
+    // a table without its own step array takes the launch's (per-step schedules)
+    const double h = cm->h ? cm->h[bb] : a.dt[bb];
     const double dh = TAN ? cm->dh[bb] : 0.0;
     const double E1 = exp(h / cm->ediv * L);
     double2 acc = make_double2(0.0, 0.0), dacc = acc;

@@ -178,7 +178,7 @@ __global__ __launch_bounds__(NT) void td_col_b(Args a, int B) {
     a.tdw[ci] = w;
     a.tdjdv[ci] = jdv;
     a.tdh[ci] = hh;
-    // the combination, term by term as comb_kernel evaluates it (no E factors)
+    // the combination, term by term in table order (no E factors)
     double2 acc = make_double2(0.0, 0.0), dacc = acc;
     for (int t = 0; t < cm->nterm; ++t) {
       double wt = cm->coef[t];
