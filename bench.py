@@ -177,6 +177,30 @@ def steppers_leg():
     return out
 
 
+def transpose_leg(inp, pd, cots):
+    """The transpose matvec of newton.py with use_transpose=True (newton.py:167-170):
+    the VJP of objective_RPO through the headline's RK4(3) trajectory, as the
+    vjp function of one primal applied per Krylov step (Linearization.vjp:
+    the checkpointed reverse pass along the stored schedule), and as a
+    standalone objective_RPO_vjp call (which solves the primal first)."""
+    import torch
+    from ecs_mhd_2d_amd import loss_functions as LF
+    lin = LF.Linearization(inp, pd, "Lawson_RK43")
+    lin.vjp(cots[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in cots:
+        lin.vjp(c)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / len(cots)
+    t0 = time.perf_counter()
+    LF.objective_RPO_vjp(inp, cots[0], pd, "Lawson_RK43")
+    torch.cuda.synchronize()
+    standalone = time.perf_counter() - t0
+    return {"workload": "VJP of objective_RPO (mode Lawson_RK43) through the headline trajectory, checkpoint every 64",
+            "transpose_matvecs_per_s": 1.0 / dt, "seconds_per_vjp": dt, "standalone_objective_vjp_s": standalone}
+
+
 def newton_leg(inp, pd):
     """One full Newton-GMRES iteration on the headline guess (Krylov dimension 64:
     objective, linearisation, 65 matvecs, line search, dealias)."""
@@ -445,6 +469,7 @@ def main():
 
     extras = {"config0_lawson_rk4_n128": config0_leg(cpu=not args.no_cpu)}
     if not args.no_extras:
+        extras["transpose_matvec"] = transpose_leg(inp, pd, [tans[i] for i in range(3)])
         extras["newton_iteration"] = newton_leg(inp, pd)
         extras["other_steppers"] = steppers_leg()
         extras["adjoint_gradient"] = gradient_leg()

@@ -171,8 +171,9 @@ class Linearization:
         self.n = self.fields.shape[-1]
         self.f0 = _rfft2(self.fields)
         self.info = None
+        self.fT = None
         if mode == "Lawson_RK43":
-            _, self.info = integrate(self.f0, self.T, param_dict, mode)
+            self.fT, self.info = integrate(self.f0, self.T, param_dict, mode)
         self.vt, self.vx = phase_vectors(self.f0, param_dict)
 
     def jvp(self, tangent):
@@ -185,6 +186,14 @@ class Linearization:
         return {"fields": _irfft2(df0 - dg, self.n),
                 "T": torch.mean(self.vt * dfields),
                 "sx": torch.mean(self.vx * dfields)}
+
+
+    def vjp(self, cot):
+        """J(x)^T c for the cotangent dict c: the function jax.vjp(obj, x)
+        returns, applied to c (newton.py:168-169); the adaptive schedule and
+        final state of the primal are the ones computed once for this input."""
+        return _objective_vjp(self.f0, self.T, self.sx, self.n, cot, self.pd, self.mode, self.vt, self.vx,
+                              self.info, self.fT)
 
 
 def objective_RPO_jvp(primal, tangent, param_dict, mode):
@@ -211,9 +220,10 @@ def _rk4_vjp_from(ctx, f0, fbar, h, steps, pd):
 REVERSE_MODES = ("Lawson_RK4", "RK4", "TDRK4", "Lawson_RK6", "Lawson_RK43")
 
 
-def _integrate_vjp(ctx, f0, gbar, T, param_dict, mode):
+def _integrate_vjp(ctx, f0, gbar, T, param_dict, mode, info=None, fT=None):
     """Reverse pass of mismatch_RPO's integration: (fT, f0bar, Tbar) for the
-    cotangent gbar of the integrated spectra (h = T / steps, so Tbar = hbar / steps)."""
+    cotangent gbar of the integrated spectra (h = T / steps, so Tbar = hbar / steps).
+    Lawson_RK43: info, fT of the primal solve when already known."""
     if mode == "Lawson_RK4":
         _, _, steps = _rk4_steps(param_dict)
         fT, f0bar, hbar = _rk4_vjp_from(ctx, f0, gbar, _rk4_h(T, param_dict), steps, param_dict)
@@ -235,27 +245,34 @@ def _integrate_vjp(ctx, f0, gbar, T, param_dict, mode):
         f0bar, hb = ctx.lawson_rk6_vjp(f0, gbar, h, steps, True, CKPT)
         return fT, f0bar, hb[0] / steps
     if mode == "Lawson_RK43":
-        fT, info = integrate(f0, T, param_dict, mode)
+        if info is None:
+            fT, info = integrate(f0, T, param_dict, mode)
         f0bar, Tb = ctx.lawson_rk43_vjp(f0, gbar, T, [info], CKPT)
         return fT, f0bar, Tb[0]
     raise NotImplementedError(f"the reverse pass is built for modes {REVERSE_MODES}, not {mode!r}")
 
 
 def objective_RPO_vjp(primal, cot, param_dict, mode="Lawson_RK4"):
-    """J(x)^T c for the cotangent dict c (newton.py:168-169), modes of REVERSE_MODES."""
-    ctx = mhd.context(param_dict)
+    """J(x)^T c for the cotangent dict c (newton.py:168-169), modes of REVERSE_MODES.
+    For many cotangents at one primal, Linearization(primal, ...).vjp reuses the
+    primal's schedule."""
+    if mode not in MODES:
+        raise ValueError(f"You selected mode = {mode}, which does not exist.")
     fields = _fields(primal)
-    n = fields.shape[-1]
-    T, sx = _f(primal["T"]), _f(primal["sx"])
     f0 = _rfft2(fields)
+    vt, vx = phase_vectors(f0, param_dict)
+    return _objective_vjp(f0, _f(primal["T"]), _f(primal["sx"]), fields.shape[-1], cot, param_dict, mode, vt, vx)
+
+
+def _objective_vjp(f0, T, sx, n, cot, param_dict, mode, vt, vx, info=None, fT=None):
+    ctx = mhd.context(param_dict)
     cbar = irfft2_T(_fields(cot), n)                 # cotangent of (f0 - g)
     # g = sym(fT): the symmetry transpose of -cbar does not need fT; sxbar does
-    fT, f0bar, Tbar = _integrate_vjp(ctx, f0, _sym_T(-cbar, sx, param_dict), T, param_dict, mode)
+    fT, f0bar, Tbar = _integrate_vjp(ctx, f0, _sym_T(-cbar, sx, param_dict), T, param_dict, mode, info, fT)
     _, sxbar = symmetry.apply_vjp(fT, sx, param_dict, -cbar)
-    vt, vx = phase_vectors(f0, param_dict)
     n2 = 2 * n * n
     gfields = rfft2_T(cbar + f0bar, n) + (_f(cot["T"]) * vt + _f(cot["sx"]) * vx) / n2
-    return {"fields": gfields, "T": torch.tensor(Tbar, dtype=torch.float64, device=fields.device), "sx": sxbar}
+    return {"fields": gfields, "T": torch.tensor(Tbar, dtype=torch.float64, device=f0.device), "sx": sxbar}
 
 
 def _sym_T(gbar, sx, pd):

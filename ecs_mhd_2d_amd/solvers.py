@@ -36,8 +36,8 @@ def newton_gmres_update(input_dict, param_dict, mode, inner=64, outer=1, do_line
     if use_transpose:
         if mode not in loss_functions.REVERSE_MODES:
             raise NotImplementedError(f"use_transpose needs the reverse pass (modes {loss_functions.REVERSE_MODES})")
-        A_T = lambda v: utils.flatten(loss_functions.objective_RPO_vjp(input_dict, utils.unflatten(v, n),
-                                                                        param_dict, mode))
+        # newton.py:168-169: one vjp function of the primal, applied per Krylov step
+        A_T = lambda v: utils.flatten(lin.vjp(utils.unflatten(v, n)))
         precond = [A_T]
     t1 = time.perf_counter()
     step, gres, hist = linalg.gmres(A, b, inner, outer, preconditioner_list=precond, return_history=True)

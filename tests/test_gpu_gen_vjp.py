@@ -167,3 +167,24 @@ def test_objective_vjp_dot_test_in_rk43_mode():
     JTu = utils.flatten(LF.objective_RPO_vjp(inp, utils.unflatten(u, n), pd, "Lawson_RK43"))
     lhs, rhs = float(torch.dot(u, Jv)), float(torch.dot(JTu, v))
     assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+@pytest.mark.parametrize("mode", ["Lawson_RK43", "TDRK4"])
+def test_linearization_vjp_reuses_the_primal(mode):
+    """Linearization(x).vjp (the vjp function of one primal, applied to several
+    cotangents as newton.py:168-169 does) computes the same bits as separate
+    objective_RPO_vjp calls, each of which solves the primal again."""
+    print(f"mode {mode}", flush=True)
+    from ecs_mhd_2d_amd import loss_functions as LF, mhd, utils
+    n = 16
+    pd = mhd.make_param_dict(n, 1 / 40, 1 / 40)
+    pd.update({"shift_reflect_ny": 2, "rot": True, "num_checkpoints": 2, "ministeps": 5,
+               "adaptive_dict": {"atol": 1e-7, "checkpoints": 8, "max_steps_per_checkpoint": 32}})
+    rng = np.random.default_rng(9)
+    inp = {"fields": torch.as_tensor(np.fft.irfft2(spectrum(n, 7))).cuda(), "T": 0.3, "sx": 0.1}
+    lin = LF.Linearization(inp, pd, mode)
+    for _ in range(2):
+        u = utils.unflatten(torch.as_tensor(rng.standard_normal(2 * n * n + 2)).cuda(), n)
+        a = utils.flatten(lin.vjp(u))
+        b = utils.flatten(LF.objective_RPO_vjp(inp, u, pd, mode))
+        assert torch.equal(a, b)
