@@ -185,7 +185,6 @@ struct Args {
   double2* ain;            // adjoint row output AI [B][8][N][KP]
   const double2* prim;     // primal column output for the row-adjoint [B][8][N][KP]
   const double2* gin;      // cotangent input of a standalone RHS adjoint
-  int out_acc;             // standalone RHS adjoint: add its result to out instead of storing it
   // C_GEN, primal: when pdt is set, the stage RHS is stored raw at vraw and
   // scaled by exp(pdt[b] L) before it is stored at out and combined (the
   // RK4(3) reverse pass's carried-k1 factor on stage 1)
@@ -195,22 +194,30 @@ struct Args {
   // launches of global step `step` only while step < nstep[b] (null: all do)
   const int* nstep;
   int step;
-  int include_diss;
-  int adaptive;            // AdaptKind
-  int masked_e;            // fixed step: e = exp(dt/2 L) * mask
-  int skip_inv;            // 1: do not produce the inverse pass (last stage of a run)
-  // tangent replay with the index as an argument: 1 when every candidate runs
-  // accepted step `attempt` (resp. has a step after it), so the kernels need
-  // not read s_nacc before their gathers
-  int run_all;
-  int next_all;
   // row passes: candidates b >= b0_from see b0 = 0 (the second-derivative
   // pass of the two-derivative RK4 tangent, batched with the other passes)
   int b0_from;
+  // flags, one byte each (the struct is every kernel's argument block; it is
+  // kept under 512 bytes)
+  int8_t include_diss;
+  int8_t adaptive;         // AdaptKind
+  int8_t masked_e;         // fixed step: e = exp(dt/2 L) * mask
+  int8_t skip_inv;         // 1: do not produce the inverse pass (last stage of a run)
+  // tangent replay with the index as an argument: 1 when every candidate runs
+  // accepted step `attempt` (resp. has a step after it), so the kernels need
+  // not read s_nacc before their gathers
+  int8_t run_all;
+  int8_t next_all;
+  int8_t out_acc;          // standalone RHS adjoint: add its result to out instead of storing it
   // two-derivative RK4 tangent: J(z) v, J(z) dv and H[dz, v] of the half step
   double2* tdw;
   double2* tdjdv;
   double2* tdh;
 };
+// Every kernel takes Args by value.  Above 512 bytes the launches of the
+// latency-bound batch-1 kernels measured 1.5-3.5% slower (520 against 496
+// bytes: GMRES matvecs 14.57 against 14.78/s, Lawson RK4 at N=128 22.8k
+// against 23.6k steps/s, alternating runs on one MI355X).
+static_assert(sizeof(Args) <= 512, "kernel argument block above 512 bytes");
 
 }  // namespace ecs
