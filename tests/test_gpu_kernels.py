@@ -229,3 +229,22 @@ def test_batched_uneven_rk43_jvp_equals_solo_and_oracle(n):
             sched = [(infos[b]["hs"][i], bool(infos[b]["acc"][i])) for i in range(infos[b]["attempts"])]
             _, rdx = O.lawson_rk43_jvp(x0[b], dx0[b], pd, Ts[b], dTs[b], sched)
             assert rel(host(dxj[b]), rdx) < 1e-10, b
+
+
+@pytest.mark.parametrize("n", [32, 128])
+def test_reverse_pass_batch_16_equals_solo(n):
+    """The adjoint kernels take one column / row per block below batch 16 and
+    wider tiles from 16 on; tiles do not change the arithmetic, so every
+    candidate of a batch of 16 gets the bits of its one-candidate run."""
+    print(f"n={n}", flush=True)
+    pd, ctx = setup(n)
+    B, steps = 16, 6
+    f = np.stack([spectrum(n, 70 + b) for b in range(B)])
+    fbar = np.stack([spectrum(n, 90 + b) for b in range(B)])
+    dts = [0.003 + 0.0002 * b for b in range(B)]
+    gb, hb = ctx.lawson_rk4_vjp(dev(f), dev(fbar), dts, steps, True, 4)
+    gb = host(gb)
+    for b in (0, 7, 15):
+        g1, h1 = ctx.lawson_rk4_vjp(dev(f[b:b + 1]), dev(fbar[b:b + 1]), [dts[b]], steps, True, 4)
+        assert np.array_equal(gb[b], host(g1)[0]), b
+        assert hb[b] == h1[0], b
