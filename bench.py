@@ -288,9 +288,11 @@ def batched_leg():
     ru, tu = batch.run_unbatched(cands)
     sb, su = batch.accepted_steps(rb) / tb, batch.accepted_steps(ru) / tu
     same = all(torch.equal(a[0], b[0]) for a, b in zip(rb, ru))
+    fevals = int(sum(info["fevals"] for _, info in rb))   # RHS evaluations, 1 + 4 per attempt
     return {"workload": "64 candidates, N in {64,128,256} (16/32/16), T~U[2,8], RK4(3) atol 1e-8",
-            "accepted_steps": batch.accepted_steps(rb), "batched_s": tb, "unbatched_s": tu,
+            "accepted_steps": batch.accepted_steps(rb), "rhs_evals": fevals, "batched_s": tb, "unbatched_s": tu,
             "batched_steps_per_s": sb, "unbatched_steps_per_s": su, "speedup": sb / su,
+            "batched_rhs_per_s": fevals / tb, "unbatched_rhs_per_s": fevals / tu,
             "bitwise_equal_to_unbatched": bool(same)}
 
 
