@@ -163,6 +163,14 @@ template <> struct Plan<32, 8> { static constexpr int R[3] = {4, 4, 2}; };
 template <> struct Plan<64, 8> { static constexpr int R[3] = {4, 4, 4}; };
 template <> struct Plan<128, 8> { static constexpr int R[3] = {8, 4, 4}; };
 template <> struct Plan<256, 8> { static constexpr int R[3] = {8, 8, 4}; };
+// P = 9: the narrow plans reversed, the forward transform of the row pass
+// whose first pass takes the values the inverse's last narrow pass leaves in
+// registers (row_kernel with P = 8)
+template <> struct Plan<16, 9> { static constexpr int R[3] = {4, 4, 1}; };
+template <> struct Plan<32, 9> { static constexpr int R[3] = {2, 4, 4}; };
+template <> struct Plan<64, 9> { static constexpr int R[3] = {4, 4, 4}; };
+template <> struct Plan<128, 9> { static constexpr int R[3] = {4, 4, 8}; };
+template <> struct Plan<256, 9> { static constexpr int R[3] = {4, 8, 8}; };
 
 // Twiddles in LDS, one table per pass with NS > 1: entry (r - 1) * NS + k
 // holds exp(-2 pi i r k / (NS R)) = tw[r k N / (NS R)], the factor of input r
@@ -280,6 +288,45 @@ __device__ __forceinline__ void fft_lds_gather(double2* __restrict__ buf, const 
   __syncthreads();
   if constexpr (R1 > 1) stockham_pass<N, R1, R0, NSEQ, SSTRIDE, NT, INV, TL::OFF2>(buf, twl);
   if constexpr (R2 > 1) stockham_pass<N, R2, R0 * R1, NSEQ, SSTRIDE, NT, INV, TL::OFF3>(buf, twl);
+}
+
+// fft_lds_gather without the last pass: the gather pass and, for a
+// three-pass plan, the middle pass.  The caller runs the last pass itself.
+template <int N, int NSEQ, int SSTRIDE, int NT, bool INV, int P, class F, class H>
+__device__ __forceinline__ void fft_lds_gather_head(double2* __restrict__ buf, const double2* __restrict__ twl,
+                                                    F&& gather, H&& before_barrier) {
+  using TL = TwLayout<N, P>;
+  constexpr int R0 = TL::R0, R1 = TL::R1, R2 = TL::R2;
+  static_assert(R1 > 1, "a head needs a plan of at least two passes");
+  constexpr int NB = NSEQ * (N / R0);
+  constexpr int BPT = (NB + NT - 1) / NT;
+  const int tid = threadIdx.x;
+  double2 v[BPT][R0];
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int bfly = tid + q * NT;
+    if (bfly < NB) {
+      const int s = bfly / (N / R0);
+      const int j = bfly - s * (N / R0);
+#pragma unroll
+      for (int r = 0; r < R0; ++r) v[q][r] = gather(s, j + r * (N / R0));
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int bfly = tid + q * NT;
+    if (bfly < NB) {
+      const int s = bfly / (N / R0);
+      const int j = bfly - s * (N / R0);
+      Dft<R0, INV>::run(v[q]);
+      double2* dst = buf + s * SSTRIDE;
+#pragma unroll
+      for (int r = 0; r < R0; ++r) dst[pidx(j * R0 + r)] = v[q][r];
+    }
+  }
+  before_barrier();
+  __syncthreads();
+  if constexpr (R2 > 1) stockham_pass<N, R1, R0, NSEQ, SSTRIDE, NT, INV, TL::OFF2>(buf, twl);
 }
 
 template <int N, int NSEQ, int SSTRIDE, int NT, bool INV, int P, bool SEQFAST = false, class F>

@@ -634,6 +634,29 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   }
 }
 
+// -advection[0] + advection[1] and u x B (mhd_jax.py:152-163) at one point
+// from the packed real fields (wx, wy), (u, v), (jx, jy), (Bx, By) and, with
+// TAN, their linearisation along the tangent fields z4..z7
+template <bool TAN>
+__device__ __forceinline__ void rhs_products(double2 z0, double2 z1, double2 z2, double2 z3, double2 z4, double2 z5,
+                                             double2 z6, double2 z7, double b0x, double b0y, double2& q, double2& dq) {
+  const double wx = z0.x, wy = z0.y, u = z1.x, v = z1.y, jx = z2.x, jy = z2.y;
+  const double Bx = z3.x + b0x, By = z3.y + b0y;
+  q = make_double2(-(u * wx + v * wy) + (Bx * jx + By * jy), u * By - v * Bx);
+  if (TAN) {
+    const double twx = z4.x, twy = z4.y, tu = z5.x, tv = z5.y, tjx = z6.x, tjy = z6.y, ta = z7.x, tb = z7.y;
+    dq = make_double2(-(tu * wx + u * twx + tv * wy + v * twy) + (ta * jx + Bx * tjx + tb * jy + By * tjy),
+                      tu * By + u * tb - tv * Bx - v * ta);
+  }
+}
+
+// Row pass.  With the narrow plan (P = 8) the RHS row pass is fused across its
+// middle: the inverse transform's last pass, the products and the forward
+// transform's first pass run in registers, one thread per row position j and
+// all of the row's sequences, and the forward transform takes the narrow plan
+// reversed (P = 9), whose first radix is the inverse's last.  That saves the
+// LDS round trips and barriers of three steps; the forward arithmetic is that
+// of the reversed plan, the same for every tile and batch.
 template <int N, int TX, int NT, int RMODE, bool TAN, int P, int AK>
 __global__ __launch_bounds__(NT) void row_kernel(Args a) {
   using G = Geo<N>;
@@ -643,10 +666,14 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
   constexpr int ND = (RMODE == R_ERR) ? 2 : 8 * NF;
   constexpr int NHALF = N / 2 + 1;
   constexpr int SS = Seq<N>::S;
+  constexpr bool FUSED = P == 8 && RMODE == R_RHS;
+  constexpr int PF = FUSED ? 9 : P;
   __shared__ double2 sm[TX * NP * SS];
   __shared__ double2 twl[N];
   TwFetch<N, NT, P> twf;
   twf.fetch(a.tw);
+  [[maybe_unused]] TwFetch<N, NT, PF> twff;
+  if constexpr (FUSED) twff.fetch(a.tw);
   int lx, ly, lz;
   xcd_tile(lx, ly, lz);
   const int b = lz, xbase = lx * TX, tid = threadIdx.x;
@@ -657,20 +684,21 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
   // bin are dropped, as numpy's c2r does (ky = N/2 lies in the dealiased gap).
   // Every lane loads (gap lanes re-read bin 0 and discard it), so all loads of
   // the pass are in flight before the first wait.
-  fft_lds_gather<N, TX * NP, SS, NT, true, P>(
-      sm, twl,
-      [&](int s, int m) -> double2 {
-        const int p = s % NP;
-        const int x = xbase + s / NP;
-        const bool lo = m <= G::KMAX, hi = m >= N - G::KMAX;
-        const int k = lo ? m : (hi ? N - m : 0);
-        const double2 A = I[((size_t)(2 * p) * N + x) * KP + k];
-        const double2 B = I[((size_t)(2 * p + 1) * N + x) * KP + k];
-        const double2 vlo = m == 0 ? make_double2(A.x, B.x) : make_double2(A.x - B.y, A.y + B.x);
-        const double2 vhi = make_double2(A.x + B.y, B.x - A.y);
-        return lo ? vlo : (hi ? vhi : make_double2(0.0, 0.0));
-      },
-      [&] { twf.store(twl); });
+  auto gather = [&](int s, int m) -> double2 {
+    const int p = s % NP;
+    const int x = xbase + s / NP;
+    const bool lo = m <= G::KMAX, hi = m >= N - G::KMAX;
+    const int k = lo ? m : (hi ? N - m : 0);
+    const double2 A = I[((size_t)(2 * p) * N + x) * KP + k];
+    const double2 B = I[((size_t)(2 * p + 1) * N + x) * KP + k];
+    const double2 vlo = m == 0 ? make_double2(A.x, B.x) : make_double2(A.x - B.y, A.y + B.x);
+    const double2 vhi = make_double2(A.x + B.y, B.x - A.y);
+    return lo ? vlo : (hi ? vhi : make_double2(0.0, 0.0));
+  };
+  if constexpr (FUSED)
+    fft_lds_gather_head<N, TX * NP, SS, NT, true, P>(sm, twl, gather, [&] { twf.store(twl); });
+  else
+    fft_lds_gather<N, TX * NP, SS, NT, true, P>(sm, twl, gather, [&] { twf.store(twl); });
 
   if (RMODE == R_ERR) {
     unsigned long long mx = 0ull;
@@ -695,41 +723,101 @@ __global__ __launch_bounds__(NT) void row_kernel(Args a) {
     return;
   }
 
-  constexpr int EPT = (TX * N + NT - 1) / NT;
-  double2 q[EPT], dq[EPT];
   const bool nob0 = a.b0_from > 0 && b >= a.b0_from;
   const double b0x = nob0 ? 0.0 : a.b0x, b0y = nob0 ? 0.0 : a.b0y;
+  const double2 zz = make_double2(0.0, 0.0);
+  if constexpr (FUSED) {
+    using TL = TwLayout<N, P>;
+    using TF = TwLayout<N, PF>;
+    constexpr int RL = TL::R2 > 1 ? TL::R2 : TL::R1;        // the inverse's last radix
+    constexpr int TWL = TL::R2 > 1 ? TL::OFF3 : TL::OFF2;   // its twiddle table
+    constexpr int NSL = N / RL;                              // its sub-transform length
+    static_assert(TF::R0 == RL, "the reversed plan starts with the inverse's last radix");
+    constexpr int NPAIR = TX * NSL;
+    constexpr int EPP = (NPAIR + NT - 1) / NT;
+    double2 fq[EPP][RL], fdq[EPP][TAN ? RL : 1];
 #pragma unroll
-  for (int e = 0; e < EPT; ++e) {
-    const int idx = tid + e * NT;
-    if (idx < TX * N) {
-      const int xl = idx / N, yy = idx - xl * N;
-      const double2* zb = sm + xl * NP * SS + pidx(yy);
-      const double2 z0 = zb[0], z1 = zb[SS], z2 = zb[2 * SS], z3 = zb[3 * SS];
-      const double wx = z0.x, wy = z0.y, u = z1.x, v = z1.y, jx = z2.x, jy = z2.y;
-      const double Bx = z3.x + b0x, By = z3.y + b0y;
-      // -advection[0] + advection[1] and u x B (mhd_jax.py:152-163)
-      q[e] = make_double2(-(u * wx + v * wy) + (Bx * jx + By * jy), u * By - v * Bx);
-      if (TAN) {
-        const double2 z4 = zb[4 * SS], z5 = zb[5 * SS], z6 = zb[6 * SS], z7 = zb[7 * SS];
-        const double twx = z4.x, twy = z4.y, tu = z5.x, tv = z5.y, tjx = z6.x, tjy = z6.y, ta = z7.x, tb = z7.y;
-        dq[e] = make_double2(-(tu * wx + u * twx + tv * wy + v * twy) + (ta * jx + Bx * tjx + tb * jy + By * tjy),
-                             tu * By + u * tb - tv * Bx - v * ta);
+    for (int e = 0; e < EPP; ++e) {
+      const int pair = tid + e * NT;
+      if (pair < NPAIR) {
+        const int xl = pair / NSL, j = pair - xl * NSL;
+        // the inverse's last pass over the row's sequences: outputs at
+        // positions j + r NSL (the pass's output index, with k = j)
+        double2 z[NP][RL];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          const double2* src = sm + (xl * NP + p) * SS;
+#pragma unroll
+          for (int r = 0; r < RL; ++r) z[p][r] = src[pidx(j + r * NSL)];
+#pragma unroll
+          for (int r = 1; r < RL; ++r) {
+            double2 w = twl[TWL + (r - 1) * NSL + j];
+            w.y = -w.y;
+            z[p][r] = cmul(z[p][r], w);
+          }
+          Dft<RL, true>::run(z[p]);
+        }
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+          double2 dqv = zz;
+          if constexpr (TAN)
+            rhs_products<true>(z[0][r], z[1][r], z[2][r], z[3][r], z[4][r], z[5][r], z[6][r], z[7][r], b0x, b0y,
+                               fq[e][r], dqv);
+          else
+            rhs_products<false>(z[0][r], z[1][r], z[2][r], z[3][r], zz, zz, zz, zz, b0x, b0y, fq[e][r], dqv);
+          if (TAN) fdq[e][TAN ? r : 0] = dqv;
+        }
+        // the forward's first pass (inputs at j + r NSL = j + r N / RL)
+        Dft<RL, false>::run(fq[e]);
+        if constexpr (TAN) Dft<RL, false>::run(fdq[e]);
       }
     }
-  }
-  __syncthreads();
+    __syncthreads();  // every read of the inverse's sequences and twiddles is done
 #pragma unroll
-  for (int e = 0; e < EPT; ++e) {
-    const int idx = tid + e * NT;
-    if (idx < TX * N) {
-      const int xl = idx / N, yy = idx - xl * N;
-      sm[(xl * NF + 0) * SS + pidx(yy)] = q[e];
-      if (TAN) sm[(xl * NF + 1) * SS + pidx(yy)] = dq[e];
+    for (int e = 0; e < EPP; ++e) {
+      const int pair = tid + e * NT;
+      if (pair < NPAIR) {
+        const int xl = pair / NSL, j = pair - xl * NSL;
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+          sm[(xl * NF + 0) * SS + pidx(j * RL + r)] = fq[e][r];
+          if constexpr (TAN) sm[(xl * NF + 1) * SS + pidx(j * RL + r)] = fdq[e][r];
+        }
+      }
     }
+    twff.store(twl);
+    __syncthreads();
+    if constexpr (TF::R1 > 1) stockham_pass<N, TF::R1, TF::R0, TX * NF, SS, NT, false, TF::OFF2>(sm, twl);
+    if constexpr (TF::R2 > 1) stockham_pass<N, TF::R2, TF::R0 * TF::R1, TX * NF, SS, NT, false, TF::OFF3>(sm, twl);
+  } else {
+    constexpr int EPT = (TX * N + NT - 1) / NT;
+    double2 q[EPT], dq[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int idx = tid + e * NT;
+      if (idx < TX * N) {
+        const int xl = idx / N, yy = idx - xl * N;
+        const double2* zb = sm + xl * NP * SS + pidx(yy);
+        if constexpr (TAN)
+          rhs_products<true>(zb[0], zb[SS], zb[2 * SS], zb[3 * SS], zb[4 * SS], zb[5 * SS], zb[6 * SS], zb[7 * SS],
+                             b0x, b0y, q[e], dq[e]);
+        else
+          rhs_products<false>(zb[0], zb[SS], zb[2 * SS], zb[3 * SS], zz, zz, zz, zz, b0x, b0y, q[e], dq[e]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int idx = tid + e * NT;
+      if (idx < TX * N) {
+        const int xl = idx / N, yy = idx - xl * N;
+        sm[(xl * NF + 0) * SS + pidx(yy)] = q[e];
+        if (TAN) sm[(xl * NF + 1) * SS + pidx(yy)] = dq[e];
+      }
+    }
+    __syncthreads();
+    fft_lds<N, TX * NF, SS, NT, false, P>(sm, twl);
   }
-  __syncthreads();
-  fft_lds<N, TX * NF, SS, NT, false, P>(sm, twl);
   double2* O = a.rowout + (size_t)b * (2 * NF) * N * KP;
   for (int idx = tid; idx < TX * NF * KC; idx += NT) {
     const int k = idx % KC;
