@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 # 1024); the runs are summarised in profiles/pmc_traffic_tangent_row_kernel_n128_b1.md
 # and profiles/pmc_batched_row_kernel_n256_b64.md.
 TRAFFIC_ROW_T_N128_B1 = 847.0 * 2047.3 + 352.0 * 1024
-TRAFFIC_ROW_N256_B64 = 90197.6 * 2047.3 + 44032.0 * 1024
+TRAFFIC_ROW_N256_B64 = 90205.2 * 2047.3 + 44032.0 * 1024
 METRIC = "FP64 RK steps/s at N=256 + GMRES matvecs/s; p50/p99 RPO solve latency"
 
 
