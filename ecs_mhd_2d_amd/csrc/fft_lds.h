@@ -290,6 +290,62 @@ __device__ __forceinline__ void fft_lds_gather(double2* __restrict__ buf, const 
   if constexpr (R2 > 1) stockham_pass<N, R2, R0 * R1, NSEQ, SSTRIDE, NT, INV, TL::OFF3>(buf, twl);
 }
 
+// The transform's last pass run in registers, its outputs handed to
+// emit(s, i, value) (element i of sequence s) instead of written back to LDS.
+// The last pass of a Stockham plan puts each output where its butterfly read
+// its inputs (positions j + r N / R), so the arithmetic and the values are
+// those of fft_lds.  Lanes are dealt GRP sequences fastest (sequences g GRP +
+// 0 .. GRP-1 at one butterfly), then butterflies, then groups: with GRP
+// neighbouring columns per group, neighbouring lanes emit neighbouring words.
+// fft_lds_head<N, ...> must have run the earlier passes (its last barrier
+// orders their writes); the caller orders later writes to buf after a barrier.
+template <int N, int NSEQ, int SSTRIDE, int NT, bool INV, int P>
+__device__ __forceinline__ void fft_lds_head(double2* __restrict__ buf, const double2* __restrict__ twl) {
+  using TL = TwLayout<N, P>;
+  constexpr int R0 = TL::R0, R1 = TL::R1, R2 = TL::R2;
+  static_assert(R0 * R1 * R2 == N, "radix plan must factor N");
+  if constexpr (R1 > 1) stockham_pass<N, R0, 1, NSEQ, SSTRIDE, NT, INV, 0>(buf, twl);
+  if constexpr (R2 > 1) stockham_pass<N, R1, R0, NSEQ, SSTRIDE, NT, INV, TL::OFF2>(buf, twl);
+}
+
+template <int N, int NSEQ, int SSTRIDE, int NT, bool INV, int P, int GRP, class E>
+__device__ __forceinline__ void fft_lds_tail(const double2* __restrict__ buf, const double2* __restrict__ twl,
+                                             E&& emit) {
+  using TL = TwLayout<N, P>;
+  constexpr int RL = TL::R2 > 1 ? TL::R2 : (TL::R1 > 1 ? TL::R1 : TL::R0);
+  constexpr int TWOFF = TL::R2 > 1 ? TL::OFF3 : TL::OFF2;
+  constexpr int NSL = N / RL;  // the pass's NS: its butterflies' sub-transform length
+  static_assert(NSEQ % GRP == 0, "sequence groups must tile the sequences");
+  constexpr int NB = NSEQ * NSL;
+  constexpr int BPT = (NB + NT - 1) / NT;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int bfly = tid + q * NT;
+    if (bfly < NB) {
+      const int g = bfly % GRP;
+      const int rest = bfly / GRP;
+      const int j = rest % NSL;
+      const int s = (rest / NSL) * GRP + g;
+      const double2* src = buf + s * SSTRIDE;
+      double2 v[RL];
+#pragma unroll
+      for (int r = 0; r < RL; ++r) v[r] = src[pidx(j + r * NSL)];
+      if (NSL > 1) {
+#pragma unroll
+        for (int r = 1; r < RL; ++r) {
+          double2 w = twl[TWOFF + (r - 1) * NSL + j];
+          if (INV) w.y = -w.y;
+          v[r] = cmul(v[r], w);
+        }
+      }
+      Dft<RL, INV>::run(v);
+#pragma unroll
+      for (int r = 0; r < RL; ++r) emit(s, j + r * NSL, v[r]);
+    }
+  }
+}
+
 // fft_lds_gather without the last pass: the gather pass and, for a
 // three-pass plan, the middle pass.  The caller runs the last pass itself.
 template <int N, int NSEQ, int SSTRIDE, int NT, bool INV, int P, class F, class H>

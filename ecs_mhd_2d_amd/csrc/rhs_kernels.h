@@ -619,18 +619,17 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
       sm[s * SS + pidx(KMAX + 1 + g)] = make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_lds<N, NSEQ, SS, NT, true, P>(sm, twl);
-    for (int idx = tid; idx < NSEQ * N; idx += NT) {
-      const int cy = idx % TY;
-      const int t = idx / TY;
-      const int x = t % N;
-      const int s2 = t / N;
+    // all passes but the last through LDS; the last in registers, its outputs
+    // stored straight to the derived arrays (neighbouring lanes: neighbouring ky)
+    fft_lds_head<N, NSEQ, SS, NT, true, P>(sm, twl);
+    fft_lds_tail<N, NSEQ, SS, NT, true, P, TY>(sm, twl, [&](int s, int x, double2 v) {
+      const int cy = s % TY, s2 = s / TY;
       const int ky = ky0 + cy;
-      if (ky >= KC) continue;
+      if (ky >= KC) return;
       const int d = sp * DPS + ch * DPC + s2 / NFO, f = s2 % NFO;
       const int g = (NDO == 1) ? c : (f * 8 + 4 * c + d);
-      a.inter[(((size_t)b * ND + g) * N + x) * KP + ky] = sm[(s2 * TY + cy) * SS + pidx(x)];
-    }
+      a.inter[(((size_t)b * ND + g) * N + x) * KP + ky] = v;
+    });
   }
 }
 
