@@ -183,15 +183,13 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
     sm[s * SS + pidx(KMAX + 1 + gg)] = make_double2(0.0, 0.0);
   }
   __syncthreads();
-  fft_lds<N, TY, SS, NT, true, PL>(sm, twl);
+  // the last pass in registers, stored straight to the x-space cotangent
+  fft_lds_head<N, TY, SS, NT, true, PL>(sm, twl);
   double2* AO = a.aout + (size_t)b * 2 * N * G::KP;
-  for (int idx = tid; idx < N * TY; idx += NT) {
-    const int cy = idx % TY;
-    const int x = idx / TY;
+  fft_lds_tail<N, TY, SS, NT, true, PL, TY>(sm, twl, [&](int cy, int x, double2 v) {
     const int ky = ky0 + cy;
-    if (ky >= KC) continue;
-    AO[((size_t)c * N + x) * G::KP + ky] = sm[cy * SS + pidx(x)];
-  }
+    if (ky < KC) AO[((size_t)c * N + x) * G::KP + ky] = v;
+  });
 }
 
 // Row-adjoint kernel: weighted c2r of (Pbar_omega, Pbar_j), transposed

@@ -29,8 +29,9 @@ __device__ __forceinline__ double2 td_derived(int d, int kx, int ky, double2 y) 
 // Writes the derived spectra of the pair (p, t) of this block's rows into the
 // 8 LDS sequences (derived d of p in sequence 2 d, of t in 2 d + 1), zeroes
 // the dealiased gap, runs the inverse x-FFT and stores the 16 derived arrays
-// of candidate `cand` in the row pass's input layout.  Starts and ends with a
-// barrier.
+// of candidate `cand` in the row pass's input layout.  Starts with a barrier,
+// which orders the previous call's last reads of the LDS sequences; the last
+// transform pass reads them without a barrier after it.
 template <int N, int NT, int P, int EPT>
 __device__ __forceinline__ void td_emit_pair(const Args& a, double2* sm, const double2* twl, int c, int ky, int cand,
                                              const double2 (&p)[EPT], const double2 (&t)[EPT]) {
@@ -54,13 +55,13 @@ __device__ __forceinline__ void td_emit_pair(const Args& a, double2* sm, const d
     sm[s * SS + pidx(KMAX + 1 + (idx - s * GAP))] = make_double2(0.0, 0.0);
   }
   __syncthreads();
-  fft_lds<N, 8, SS, NT, true, P>(sm, twl);
-  for (int idx = tid; idx < 8 * N; idx += NT) {
-    const int x = idx % N, s2 = idx / N;
+  // the last pass in registers, stored straight to the derived arrays
+  fft_lds_head<N, 8, SS, NT, true, P>(sm, twl);
+  fft_lds_tail<N, 8, SS, NT, true, P, 1>(sm, twl, [&](int s2, int x, double2 v) {
     const int d = s2 / 2, f = s2 % 2;
     const int g = f * 8 + 4 * c + d;
-    a.inter[(((size_t)cand * 16 + g) * N + x) * G::KP + ky] = sm[s2 * SS + pidx(x)];
-  }
+    a.inter[(((size_t)cand * 16 + g) * N + x) * G::KP + ky] = v;
+  });
 }
 
 template <int N, int NT, int P>
