@@ -298,7 +298,9 @@ __device__ __forceinline__ void fft_lds_gather(double2* __restrict__ buf, const 
 // 0 .. GRP-1 at one butterfly), then butterflies, then groups: with GRP
 // neighbouring columns per group, neighbouring lanes emit neighbouring words.
 // fft_lds_head<N, ...> must have run the earlier passes (its last barrier
-// orders their writes); the caller orders later writes to buf after a barrier.
+// orders their writes; for a one-pass plan it runs none, and the caller's
+// barrier after writing the inputs does); the caller orders later writes to
+// buf after a barrier.
 template <int N, int NSEQ, int SSTRIDE, int NT, bool INV, int P>
 __device__ __forceinline__ void fft_lds_head(double2* __restrict__ buf, const double2* __restrict__ twl) {
   using TL = TwLayout<N, P>;
