@@ -2,10 +2,11 @@
 
 Candidates are grouped by N; each group advances as one batch through the
 fused kernels (per-candidate T and adaptive controller, finished candidates
-turn into no-ops), the groups run concurrently on their own streams, and the
-adaptive attempts run as replays of one captured hipGraph chunk.  Tile sizes
-come from the library's autotuner.  The unbatched path (one candidate at a
-time) is kept for comparison.
+turn into no-ops), and the groups run concurrently on their own streams.
+The adaptive attempts are launched eagerly, 32 attempts per host poll; the
+library replays them as captured hipGraph chunks only when ECSMHD_GRAPH=1 is
+set.  Tile sizes come from the library's autotuner.  The unbatched path (one
+candidate at a time) is kept for comparison.
 """
 from __future__ import annotations
 

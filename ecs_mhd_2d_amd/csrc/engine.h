@@ -215,19 +215,19 @@ static int time_variants(ecs_ctx* c, int B, int kind, const int* vs, int nv, flo
   return rc;
 }
 
-// Picks, per kernel family (column, tangent column, row, tangent row), the
-// FFT radix plan once per grid size and the tile per batch bucket (powers of
-// two).  The plan decides the arithmetic, so it is fixed at the first call
-// for a grid size, whatever that call's batch: both plans are timed at a
-// latency batch (1) and a throughput batch (16), and the plan whose worst
-// slowdown against the faster plan at either batch is smaller wins.  Every
-// batch size then computes with the same arithmetic, so a candidate's result
-// does not depend on the batch it rides in.  Tiles do not change the
-// arithmetic and are timed per bucket within the fixed plan.  Runs at the
-// start of an entry point, before any input is packed (it overwrites the
-// workspace and may grow it to 16 candidates).
+// Fixes, per kernel family (column, tangent column, row, tangent row), the
+// FFT radix plan of a grid size and picks the tile per batch bucket (powers
+// of two).  The plan decides the arithmetic, so it comes from a fixed table
+// (kPlanTable: the plan that timed fastest on MI355X at batch 1 and 16), and
+// every process, batch size and run computes the same bits; a candidate's
+// result does not depend on the batch it rides in or on timing noise.  Tiles
+// and splits do not change the arithmetic and are timed per bucket within the
+// fixed plan.  Runs at the start of an entry point, before any input is
+// packed (it overwrites the workspace and may grow it to 16 candidates).
 // ECSMHD_NO_TUNE=1 keeps the default tiles and the wide plan;
-// ECSMHD_PLAN=wide|narrow fixes the plan and tunes the tiles only;
+// ECSMHD_PLAN=wide|narrow fixes the plan of every family; ECSMHD_PLAN=tune
+// chooses it by timing (both plans at batch 1 and 16, the plan with the
+// smaller worst-case slowdown wins), which is how the table was measured;
 // ECSMHD_VARIANTS="N:c,ct,r,rt[;N:...]" fixes the variants (var_* in host.h)
 // of the four families for grid N at every batch, so a profiled run can use
 // the configuration an unprofiled run tuned.
@@ -249,6 +249,20 @@ static bool forced_variants(int n, int v[4]) {
   return false;
 }
 
+// FFT plan (0 wide, 1 narrow) of each family {column, tangent column, row,
+// tangent row} per grid size.
+static int plan_table(int n, int kind) {
+  static const int kPlanTable[5][4] = {
+      {0, 0, 0, 0},   // N = 16
+      {0, 0, 0, 0},   // N = 32
+      {0, 0, 0, 0},   // N = 64
+      {1, 1, 1, 0},   // N = 128
+      {1, 1, 1, 0},   // N = 256
+  };
+  const int row = n == 16 ? 0 : n == 32 ? 1 : n == 64 ? 2 : n == 128 ? 3 : 4;
+  return kPlanTable[row][kind];
+}
+
 template <int N>
 static int autotune(ecs_ctx* c, int B, hipStream_t st) {
   if (getenv("ECSMHD_NO_TUNE")) return 0;
@@ -264,7 +278,8 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
   static std::mutex run_mu;  // one tuning pass per grid size at a time
   std::lock_guard<std::mutex> rl(run_mu);
   const char* force = getenv("ECSMHD_PLAN");
-  const int forced = force ? (strcmp(force, "narrow") == 0 ? 1 : 0) : -1;
+  const bool timed = force && strcmp(force, "tune") == 0;
+  const int forced = force && !timed ? (strcmp(force, "narrow") == 0 ? 1 : 0) : -1;
   int plan[4] = {-1, -1, -1, -1};
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -272,6 +287,10 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
     for (int kind = 0; kind < 4; ++kind) {
       auto it = g_plan.find(plan_key(N, kind));
       if (it != g_plan.end()) plan[kind] = it->second;
+      if (plan[kind] < 0 && !timed) {
+        plan[kind] = forced >= 0 ? forced : plan_table(N, kind);
+        g_plan[plan_key(N, kind)] = plan[kind];
+      }
     }
   }
   if (plan[0] < 0 || plan[1] < 0 || plan[2] < 0 || plan[3] < 0) {

@@ -28,8 +28,21 @@ def gmres(A, b, inner, outer=1, preconditioner_list=(), return_history=False):
     A: callable on flat float64 CUDA vectors; b: flat float64 CUDA vector.
     Returns (x, rel_res) like the reference (rel_res = |M(A x) - b| / |b|
     with one extra matvec, linalg.py:97).  With ``return_history`` also
-    returns a dict with H, the per-Krylov-step least-squares residuals
-    min_y |H_k y - Q_k^T b| / |b| (k = 1..inner) and the matvec count.
+    returns a dict with H, the matvec count and two per-Krylov-step residual
+    histories (k = 1..inner):
+
+    * ``krylov_residuals``: min_y |H_k y - |b| e_1| / |b|, the Arnoldi
+      residual estimate.  For modified Gram-Schmidt it follows the true
+      residual |A Q_k y - b| / |b| down to rounding level even after the
+      basis has lost orthogonality;
+    * ``krylov_residuals_qtb``: min_y |H_k y - (Q^T b)_k| / |b| with the
+      right-hand side the reference's solve uses (b2 = Q^T b, linalg.py:92).
+      Once the basis has lost orthogonality, Q^T b is no longer |b| e_1 and
+      this value stops tracking the true residual.
+
+    ``orthogonality_loss`` = |Q^T Q - I|_F of the final basis says how far
+    the second form has drifted; ``Q`` is the basis itself, rows q_0..q_inner
+    on the device.
     """
     def precond(v):
         for M in preconditioner_list:
@@ -57,12 +70,22 @@ def gmres(A, b, inner, outer=1, preconditioner_list=(), return_history=False):
     rel_res = float(torch.linalg.vector_norm(precond(A(x)) - b) / nb)
     if not return_history:
         return x, rel_res
-    nbh = float(nb)
-    hist = []
+    return x, rel_res, _history(H, b2, float(nb), Q, inner)
+
+
+def _history(H, b2, nb, Q, inner):
+    e1 = np.zeros(inner + 1)
+    e1[0] = nb
+    hist, hist_qtb = [], []
     for k in range(1, inner + 1):
-        yk = np.linalg.lstsq(H[: k + 1, :k], b2[: k + 1], rcond=None)[0]
-        hist.append(float(np.linalg.norm(H[: k + 1, :k] @ yk - b2[: k + 1]) / nbh))
-    return x, rel_res, {"H": H, "krylov_residuals": hist, "matvecs": inner + 1}
+        Hk = H[: k + 1, :k]
+        for rhs, out in ((e1, hist), (b2, hist_qtb)):
+            yk = np.linalg.lstsq(Hk, rhs[: k + 1], rcond=None)[0]
+            out.append(float(np.linalg.norm(Hk @ yk - rhs[: k + 1]) / nb))
+    gram = Q @ Q.T
+    gram.diagonal().sub_(1.0)
+    return {"H": H, "Q": Q, "krylov_residuals": hist, "krylov_residuals_qtb": hist_qtb,
+            "orthogonality_loss": float(torch.linalg.matrix_norm(gram)), "matvecs": inner + 1}
 
 
 # ------------------------------------------------------------------ hookstep

@@ -176,6 +176,16 @@ class Linearization:
             self.fT, self.info = integrate(self.f0, self.T, param_dict, mode)
         self.vt, self.vx = phase_vectors(self.f0, param_dict)
 
+    def objective(self):
+        """objective_RPO at this input, from the integration the linearisation
+        already holds (mode Lawson_RK43: the same solve whose schedule the
+        JVPs replay) -- the same operations as objective_RPO, so the same bits."""
+        if self.fT is None:
+            self.fT, _ = integrate(self.f0, self.T, self.pd, self.mode)
+        g = symmetry.apply(self.fT, self.sx, self.pd)
+        z = torch.zeros((), dtype=torch.float64, device=self.f0.device)
+        return {"fields": _irfft2(self.f0 - g, self.n), "T": z, "sx": z.clone()}
+
     def jvp(self, tangent):
         """J(x) t for the tangent dict t (utils.py:108)."""
         dfields = _fields(tangent)

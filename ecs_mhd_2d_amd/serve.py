@@ -1,12 +1,15 @@
 """Continuous-batching server for Newton-GMRES and adjoint-descent jobs on one GPU.
 
-Jobs arrive over time (e.g. Poisson), are admitted while their estimated
-device memory fits an HBM budget, and then advance together: every job is a
-generator that yields the next device operation it needs; each scheduler tick
-gathers the pending operations of all admitted jobs, groups the compatible
-ones (same kind and grid size; same step count for the fixed-step gradient)
-into one batched call of the fused kernels, runs the groups concurrently on
-their own streams, and sends each job its slice of the result.  Krylov bases
+Jobs arrive over time (e.g. Poisson) and are admitted while their estimated
+device memory fits an HBM budget (a job larger than the whole budget is
+rejected; one that does not fit yet lets smaller later jobs pass for a
+while).  Every job is a generator that yields the next device operation it
+needs.  Whenever a group slot is free, the scheduler gathers the pending
+operations of the admitted jobs that are not running, groups the compatible
+ones (same kind and grid size; gradients of whole 64-step segments across
+step counts) into one batched call of the fused kernels, and runs it on a
+context lane of its own with that lane's stream; when a group finishes, its
+jobs advance at once, without waiting for the other groups.  Krylov bases
 and reverse-pass checkpoints stay on the device for the life of the job.
 
 A job's arithmetic does not depend on the batch it rides in (every kernel
@@ -17,7 +20,6 @@ arrival time; the server reports throughput and p50/p99 latency.
 from __future__ import annotations
 
 import math
-import threading
 import time
 
 import numpy as np
@@ -118,19 +120,33 @@ def _batch_objective(ops, pd, lane):
 
 
 class _Lin:
-    """Primal data of one Newton iteration: adaptive schedule and phase vectors
-    (the content of loss_functions.Linearization, built on a given lane)."""
+    """Primal data of one Newton iteration from one adaptive solve: the
+    objective (loss_functions.py:132-148), the schedule the JVPs replay and
+    the phase vectors -- the content of loss_functions.Linearization."""
 
-    def __init__(self, fields, T, sx, adaptive, pd, lane):
-        ctx = mhd.context(pd, lane)
-        n = fields.shape[-1]
+    def __init__(self, f0, fT, info, T, sx, pd, ctx):
+        n = f0.shape[-2]
         self.T, self.sx = float(T), float(sx)
-        self.f0 = torch.fft.rfft2(fields).contiguous()
-        _, infos = ctx.lawson_rk43(self.f0, self.T, 1e-2, adaptive["atol"], adaptive["max_steps_per_checkpoint"],
-                                   adaptive["checkpoints"])
-        self.info = infos[0]
-        self.vt = torch.fft.irfft2(ctx.state_vel(self.f0, True), s=(n, n))
-        self.vx = torch.fft.irfft2(1j * pd["kx"] * self.f0, s=(n, n))
+        self.f0 = f0
+        self.info = info
+        g = symmetry.apply(fT, self.sx, pd)
+        self.objective = {"fields": torch.fft.irfft2(f0 - g, s=(n, n)),
+                          "T": torch.zeros((), dtype=torch.float64, device="cuda"),
+                          "sx": torch.zeros((), dtype=torch.float64, device="cuda")}
+        self.vt = torch.fft.irfft2(ctx.state_vel(f0, True), s=(n, n))
+        self.vx = torch.fft.irfft2(1j * pd["kx"] * f0, s=(n, n))
+
+
+def _batch_linearize(ops, pd, lane):
+    """One batched adaptive solve per Newton iteration: each job's objective
+    and the schedule its JVPs replay come from the same integration."""
+    ctx = mhd.context(pd, lane)
+    fields = torch.stack([o.kw["fields"] for o in ops]).contiguous()
+    f0 = torch.fft.rfft2(fields).contiguous()
+    ad = ops[0].kw["adaptive"]
+    fT, infos = ctx.lawson_rk43(f0, [float(o.kw["T"]) for o in ops], 1e-2, ad["atol"],
+                                ad["max_steps_per_checkpoint"], ad["checkpoints"])
+    return [_Lin(f0[b].contiguous(), fT[b], infos[b], o.kw["T"], o.kw["sx"], pd, ctx) for b, o in enumerate(ops)]
 
 
 def _batch_jvp(ops, pd, lane):
@@ -179,16 +195,40 @@ def adjoint_descent_job(fields, T, sx, n, steps, iters, lr=1e-2):
     return {"params": params, "losses": losses}
 
 
-def newton_job(fields, T, sx, n, adaptive, inner, iters):
+def _dump(dump_dir, tag, why, x, lin=None, **extra):
+    """Write the input of a failed Newton iteration (fields, T, sx, and the
+    adaptive schedule when there is one) so it can be replayed on the CPU."""
+    if not dump_dir:
+        return None
+    import os
+    os.makedirs(dump_dir, exist_ok=True)
+    path = os.path.join(dump_dir, f"newton_failure_{tag}_{os.getpid()}.npz")
+    data = {"fields": torch.as_tensor(x["fields"]).detach().cpu().numpy(), "T": float(x["T"]),
+            "sx": float(x["sx"]), "why": why}
+    if lin is not None:
+        data.update(hs=lin.info["hs"], acc=lin.info["acc"], attempts=lin.info["attempts"])
+    data.update({k: np.asarray(v) for k, v in extra.items()})
+    np.savez(path, **data)
+    return path
+
+
+def newton_job(fields, T, sx, n, adaptive, inner, iters, dump_dir=None, tag="job"):
     """Newton-GMRES (newton.py:151-215) with per-job Krylov basis on the device;
-    matvecs are yielded so they batch with other jobs'."""
-    from . import linalg  # noqa: F401  (same algorithm, written inline to yield matvecs)
+    matvecs are yielded so they batch with other jobs'.  The objective and
+    the schedule of the JVPs come from one adaptive solve per iteration.  An
+    adaptive solve that does not complete within its attempt budget, or a
+    non-finite matvec or least-squares problem, ends the job with an error
+    (after writing its input to ``dump_dir``, when given)."""
+    from . import linalg
     x = {"fields": fields, "T": T, "sx": sx}
     logs = []
     for it in range(iters):
-        f, _ = yield Op("objective", n, fields=x["fields"], T=x["T"], sx=x["sx"], adaptive=adaptive)
-        b = _flat(f)
         lin = yield Op("linearize", n, fields=x["fields"], T=x["T"], sx=x["sx"], adaptive=adaptive)
+        if not lin.info["completed"]:
+            path = _dump(dump_dir, tag, "incomplete", x, lin)
+            raise RuntimeError(f"Newton iteration {it}: the adaptive integration stopped at s = {lin.info['s']:.6g} "
+                               f"after {lin.info['attempts']} attempts (dump: {path})")
+        b = _flat(lin.objective)
         nb = torch.linalg.vector_norm(b)
         Q = torch.zeros((inner + 1, b.numel()), dtype=torch.float64, device="cuda")
         Hd = torch.zeros((inner, inner + 2), dtype=torch.float64, device="cuda")
@@ -197,14 +237,18 @@ def newton_job(fields, T, sx, n, adaptive, inner, iters):
             w = yield Op("jvp", n, lin=lin, tangent=_unflat(Q[k], n))
             w = _flat(w).contiguous().clone()
             if not bool(torch.isfinite(w).all()):
+                path = _dump(dump_dir, tag, "nonfinite_matvec", x, lin, tangent=Q[k].cpu().numpy(), it=it, k=k)
                 raise FloatingPointError(f"non-finite matvec in Newton iteration {it}, Krylov column {k} "
-                                         f"(|b| = {float(nb):.3e}, T = {float(x['T']):.6g})")
+                                         f"(|b| = {float(nb):.3e}, T = {float(x['T']):.6g}, dump: {path})")
             linalg._mgs(Q, k, w, Hd[k])
         H = np.zeros((inner + 1, inner))
         Hh = Hd.cpu().numpy()
         for k in range(inner):
             H[: k + 2, k] = Hh[k, : k + 2]
         b2 = torch.mv(Q, b).cpu().numpy()
+        if not (np.isfinite(H).all() and np.isfinite(b2).all()):
+            path = _dump(dump_dir, tag, "nonfinite_hessenberg", x, lin, H=H, b2=b2, it=it)
+            raise FloatingPointError(f"non-finite Hessenberg matrix in Newton iteration {it} (dump: {path})")
         y = np.linalg.lstsq(H, b2, rcond=None)[0]
         step = torch.mv(Q[:inner].T.contiguous(), torch.as_tensor(y, device="cuda"))
         Ax = yield Op("jvp", n, lin=lin, tangent=_unflat(step, n))
@@ -213,9 +257,9 @@ def newton_job(fields, T, sx, n, adaptive, inner, iters):
         damp = 1.0
         for _ in range(20):   # line search (utils.py:17-58)
             xt = xv - damp * step
-            ft, _ = yield Op("objective", n, fields=xt[1:-1].reshape(2, n, n).contiguous(), T=float(xt[0]),
-                             sx=float(xt[-1]), adaptive=adaptive)
-            if float(torch.linalg.vector_norm(_flat(ft))) < float(nb):
+            ft, info = yield Op("objective", n, fields=xt[1:-1].reshape(2, n, n).contiguous(), T=float(xt[0]),
+                                sx=float(xt[-1]), adaptive=adaptive)
+            if info["completed"] and float(torch.linalg.vector_norm(_flat(ft))) < float(nb):
                 xv = xt
                 break
             damp /= 2.0
@@ -234,6 +278,7 @@ class Job:
         self.error = None
         self.start = None
         self.finish = None
+        self.in_flight = False
 
 
 def job_memory(kind, n, inner=64, steps=0, budget=4096):
@@ -248,19 +293,69 @@ def job_memory(kind, n, inner=64, steps=0, budget=4096):
     return nseg * comp + 64 * 5 * comp + 64 * 4 * 8 * n * kp * 16 + 6 * vec
 
 
+def op_rk_steps(op, result):
+    """Stepper steps one operation integrates (the forward and the reverse
+    sweep of a gradient each count its steps; an adaptive solve counts its
+    attempts; a tangent replay its accepted steps)."""
+    if op.kind == "grad":
+        return 2 * int(op.kw["steps"])
+    if op.kind == "objective":
+        return int(result[1]["attempts"])
+    if op.kind == "linearize":
+        return int(result.info["attempts"])
+    if op.kind == "jvp":
+        return int(op.kw["lin"].info["accepted"])
+    return 0
+
+
+def _own(obj, stream):
+    """Mark every CUDA tensor reachable from ``obj`` as used on ``stream``, so
+    the caching allocator does not hand its memory to the stream that made it
+    while work queued on ``stream`` may still read it."""
+    if isinstance(obj, torch.Tensor):
+        if obj.is_cuda:
+            obj.record_stream(stream)
+    elif isinstance(obj, dict):
+        for v in obj.values():
+            _own(v, stream)
+    elif isinstance(obj, (list, tuple)):
+        for v in obj:
+            _own(v, stream)
+    elif isinstance(obj, _Lin):
+        _own(vars(obj), stream)
+
+
 # ------------------------------------------------------------------ scheduler
 class Server:
-    """``executor(key, ops, lane) -> results`` runs one group of compatible ops;
-    the default runs them on the GPU through the fused kernels."""
+    """Continuous batching of job operations on one GPU.
 
-    def __init__(self, hbm_budget=200e9, nu=1 / 40, eta=1 / 40, b0=(0.0, 0.1), max_batch=64, executor=None):
+    ``executor(key, ops, lane) -> results`` runs one group of compatible ops;
+    the default runs them on the GPU through the fused kernels.  Up to
+    ``max_groups`` groups run at once, each on a context lane of its grid size
+    and that lane's stream; a group's jobs advance as soon as it finishes, so
+    a slow group delays only its own jobs.
+
+    Stream order: a job's generator code runs on the caller's current stream;
+    a group's stream first waits for an event recorded on that stream after
+    the group was formed (so it sees everything the jobs queued, such as the
+    Gram-Schmidt sweep that wrote the next Krylov vector), and the caller's
+    stream waits for the group's completion event before the jobs read the
+    results, whose memory is then marked as in use on the caller's stream.
+    """
+
+    def __init__(self, hbm_budget=200e9, nu=1 / 40, eta=1 / 40, b0=(0.0, 0.1), max_batch=64, executor=None,
+                 max_groups=4, bypass_wait_s=30.0):
         self.executor = executor
         self.budget = hbm_budget
         self.nu, self.eta, self.b0 = nu, eta, b0
         self.max_batch = max_batch
+        self.max_groups = max_groups
+        self.bypass_wait_s = bypass_wait_s
         self._pds = {}
+        self._streams = {}
         self.ticks = 0
         self.batched_ops = 0
+        self.rk_steps = 0
 
     def pd(self, n):
         if n not in self._pds:
@@ -275,125 +370,173 @@ class Server:
         if kind == "objective":
             return _batch_objective(ops, pd, lane)
         if kind == "linearize":
-            return [_Lin(o.kw["fields"], o.kw["T"], o.kw["sx"], o.kw["adaptive"], pd, lane) for o in ops]
+            return _batch_linearize(ops, pd, lane)
         if kind == "jvp":
             return _batch_jvp(ops, pd, lane)
         raise ValueError(kind)
 
+    def _work(self, key, ops, lane, ready):
+        """Runs in a worker thread; returns (results, completion event)."""
+        if self.executor is not None:
+            return self.executor(key, ops, lane), None
+        s = self._streams[(ops[0].n, lane)]
+        with torch.cuda.stream(s):
+            s.wait_event(ready)
+            out = self._run_group(key, ops, lane)
+            done = torch.cuda.Event()
+            done.record(s)
+        return out, done
+
+    def _admit(self, waiting, now, used):
+        """Jobs to admit now: every arrived job whose memory fits, in arrival
+        order; a job that does not fit lets later ones pass unless it has
+        waited more than bypass_wait_s (then it holds the queue until memory
+        frees up).  Returns (admitted, rejected)."""
+        admitted, rejected = [], []
+        for j in list(waiting):
+            if j.arrival > now:
+                break
+            if j.mem > self.budget:
+                rejected.append(j)
+                waiting.remove(j)
+                continue
+            if used + j.mem <= self.budget:
+                admitted.append(j)
+                waiting.remove(j)
+                used += j.mem
+            elif now - j.arrival > self.bypass_wait_s:
+                break
+        return admitted, rejected
+
     def run(self, arrivals, concurrent=True, log=None):
         """arrivals: list of (t_arrival_seconds, kind, make_generator, mem_bytes, n).
         Returns the finished jobs and summary statistics; `log(msg)`, if given,
-        hears of every finished job and, every 10 s, of the ticks run so far."""
+        hears of every finished job and, every 10 s, of the groups run so far.
+        A job whose memory estimate exceeds the budget is rejected at once
+        (error "exceeds the HBM budget")."""
+        from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
         t0 = time.perf_counter()
-        waiting = [Job(i, k, t, g(), m, n) for i, (t, k, g, m, n) in enumerate(sorted(arrivals, key=lambda a: a[0]))]
+        waiting = [Job(i, k, t, g, m, n) for i, (t, k, g, m, n) in enumerate(sorted(arrivals, key=lambda a: a[0]))]
         active, done = [], []
-        used = 0.0
-        peak = 0.0
-        last_log = 0.0
-        while waiting or active:
-            now = time.perf_counter() - t0
-            while waiting and waiting[0].arrival <= now and used + waiting[0].mem <= self.budget:
-                j = waiting.pop(0)
-                j.start = now
-                used += j.mem
-                peak = max(peak, used)
-                try:
-                    j.pending = next(j.gen)
-                except StopIteration as e:
-                    j.result, j.finish = e.value, now
-                    done.append(j)
-                    used -= j.mem
-                    continue
-                except Exception as e:
-                    j.error, j.finish = repr(e), now
-                    done.append(j)
-                    used -= j.mem
-                    continue
-                active.append(j)
-            if not active:
-                if waiting:
-                    time.sleep(max(0.0, min(0.01, waiting[0].arrival - now)))
-                continue
-            groups = {}
-            for j in active:
-                groups.setdefault(j.pending.group_key(), []).append(j)
-            tasks = []
-            lanes = []
-            per_n = {}
-            for key, js in groups.items():
-                for i in range(0, len(js), self.max_batch):
-                    n = js[0].n
-                    lanes.append(per_n.get(n, 0))   # concurrent tasks on one N get their own context
-                    per_n[n] = per_n.get(n, 0) + 1
-                    tasks.append((key, js[i:i + self.max_batch]))
-            outs = [None] * len(tasks)
-            errs = []
+        used, peak, last_log = 0.0, 0.0, 0.0
+        slots = self.max_groups if concurrent else 1
+        main = torch.cuda.current_stream() if self.executor is None else None
+        inflight = {}
+        busy = set()
+        pool = ThreadPoolExecutor(max_workers=slots)
 
-            def work(ti):
-                key, js = tasks[ti]
-                try:
-                    if self.executor is not None:
-                        outs[ti] = self.executor(key, [j.pending for j in js], lanes[ti])
-                        return
-                    s = torch.cuda.Stream()
-                    with torch.cuda.stream(s):
-                        outs[ti] = self._run_group(key, [j.pending for j in js], lanes[ti])
-                        s.synchronize()
-                except Exception as e:
-                    errs.append(e)
+        def finish(j, now, result=None, error=None):
+            nonlocal used
+            j.result, j.error, j.finish = result, error, now
+            done.append(j)
+            used -= j.mem
+            if j in active:
+                active.remove(j)
+            if log is not None:
+                state = "done" if error is None else f"FAILED: {error}"
+                log(f"job {j.id} ({j.kind}, N={j.n}) {state} at {now:.1f} s, latency {now - j.arrival:.1f} s")
 
-            if concurrent and len(tasks) > 1:
-                ths = [threading.Thread(target=work, args=(ti,)) for ti in range(len(tasks))]
-                for t in ths:
-                    t.start()
-                for t in ths:
-                    t.join()
-            else:
-                for ti in range(len(tasks)):
-                    work(ti)
-            if errs:
-                raise errs[0]
-            if self.executor is None:
-                torch.cuda.synchronize()   # results cross from the worker streams to the jobs' stream
-            self.ticks += 1
-            self.batched_ops += sum(len(js) for _, js in tasks)
-            now = time.perf_counter() - t0
-            if log is not None and now - last_log >= 10.0:
-                last_log = now
-                kinds = ", ".join(f"{k[0]} N={k[1]} x{len(js)}" for k, js in tasks)
-                log(f"{now:.1f} s: tick {self.ticks}, {len(active)} active jobs, {len(waiting)} waiting; groups: {kinds}")
-            still = []
-            for (key, js), res in zip(tasks, outs):
-                for j, r in zip(js, res):
+        def advance(j, value, now, first=False):
+            try:
+                j.pending = next(j.gen) if first else j.gen.send(value)
+            except StopIteration as e:
+                finish(j, now, result=e.value)
+            except Exception as e:   # the job's own failure ends that job, not the server
+                finish(j, now, error=repr(e))
+
+        try:
+            while waiting or active:
+                now = time.perf_counter() - t0
+                adm, rej = self._admit(waiting, now, used)
+                for j in rej:
+                    j.start = now
+                    j.gen = None
+                    used += j.mem
+                    finish(j, now, error=f"job {j.id} needs {j.mem:.3g} bytes, which exceeds the HBM budget "
+                                         f"of {self.budget:.3g} bytes")
+                for j in adm:
+                    j.start = now
+                    j.gen = j.gen()
+                    used += j.mem
+                    peak = max(peak, used)
+                    active.append(j)
+                    advance(j, None, now, first=True)
+                # form groups from the jobs whose next operation is not running, oldest job first
+                ready = [j for j in active if not j.in_flight]
+                groups = {}
+                for j in sorted(ready, key=lambda j: j.arrival):
+                    groups.setdefault(j.pending.group_key(), []).append(j)
+                for key, js in groups.items():
+                    for i in range(0, len(js), self.max_batch):
+                        if len(inflight) >= slots:
+                            break
+                        part = js[i:i + self.max_batch]
+                        n = part[0].n
+                        lane = 0
+                        while (n, lane) in busy:
+                            lane += 1
+                        busy.add((n, lane))
+                        ev = None
+                        if self.executor is None:
+                            if (n, lane) not in self._streams:
+                                self._streams[(n, lane)] = torch.cuda.Stream()
+                            ev = torch.cuda.Event()
+                            ev.record(main)
+                        for j in part:
+                            j.in_flight = True
+                        fut = pool.submit(self._work, key, [j.pending for j in part], lane, ev)
+                        inflight[fut] = (key, part, (n, lane))
+                        self.ticks += 1
+                        self.batched_ops += len(part)
+                # the next arrival still in the future (an arrived job that does not fit
+                # waits for a group to finish, which frees no memory by itself but ends jobs)
+                now = time.perf_counter() - t0
+                upcoming = [j.arrival for j in waiting if j.arrival > now]
+                if not inflight:
+                    if waiting:
+                        time.sleep(max(0.0, min(0.01, (upcoming[0] if upcoming else now) - now)))
+                    continue
+                timeout = max(0.0, upcoming[0] - now) if upcoming else None
+                fin, _ = wait(list(inflight), timeout=timeout, return_when=FIRST_COMPLETED)
+                now = time.perf_counter() - t0
+                for fut in fin:
+                    key, part, ln = inflight.pop(fut)
+                    busy.discard(ln)
+                    for j in part:
+                        j.in_flight = False
                     try:
-                        j.pending = j.gen.send(r)
-                        still.append(j)
-                    except StopIteration as e:
-                        j.result, j.finish = e.value, now
-                        done.append(j)
-                        used -= j.mem
-                        if log is not None:
-                            log(f"job {j.id} ({j.kind}, N={j.n}) done at {now:.1f} s, latency {now - j.arrival:.1f} s")
-                    except Exception as e:  # the job's own failure ends that job, not the server
-                        j.error, j.finish = repr(e), now
-                        done.append(j)
-                        used -= j.mem
-                        if log is not None:
-                            log(f"job {j.id} ({j.kind}, N={j.n}) FAILED at {now:.1f} s: {j.error}")
-            active = still
+                        res, ev = fut.result()
+                    except Exception as e:   # a failed group ends its jobs
+                        for j in part:
+                            finish(j, now, error=f"group {key}: {e!r}")
+                        continue
+                    if ev is not None:
+                        main.wait_event(ev)
+                        _own(res, main)
+                    for j, r in zip(part, res):
+                        self.rk_steps += op_rk_steps(j.pending, r)
+                        advance(j, r, now)
+                if log is not None and now - last_log >= 10.0:
+                    last_log = now
+                    log(f"{now:.1f} s: {self.ticks} groups run, {len(active)} active jobs, {len(waiting)} waiting")
+        finally:
+            pool.shutdown(wait=True)
+        if self.executor is None:
+            torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        self.max_concurrent = getattr(self, "max_concurrent", 0)
         ok = [j for j in done if j.error is None]
         lat = np.array([j.finish - j.arrival for j in ok]) if ok else np.array([np.nan])
         return done, {"jobs": len(done), "failed_jobs": len(done) - len(ok), "wall_s": wall,
                       "throughput_jobs_per_s": len(ok) / wall,
                       "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
                       "ticks": self.ticks, "mean_batch": self.batched_ops / max(1, self.ticks),
+                      "rk_steps": self.rk_steps, "rk_steps_per_s": self.rk_steps / wall,
                       "peak_resident_bytes": peak, "hbm_budget_bytes": self.budget}
 
 
 def make_workload(count, rate=4.0, seed=7, sizes=(128, 256), T_range=(2.0, 8.0), adjoint_frac=0.7,
-                  descent_iters=64, newton_iters=5, inner=64, dt=0.004, atol=1e-8, adaptive_budget=(64, 64)):
+                  descent_iters=64, newton_iters=5, inner=64, dt=0.004, atol=1e-8, adaptive_budget=(64, 64),
+                  dump_dir=None):
     """Poisson arrivals at `rate` per second of mixed jobs (the serving workload)."""
     rng = np.random.default_rng(seed)
     t = 0.0
@@ -410,7 +553,8 @@ def make_workload(count, rate=4.0, seed=7, sizes=(128, 256), T_range=(2.0, 8.0),
             gen = (lambda f=f, T=T, n=n, steps=steps: adjoint_descent_job(f, T, 0.0, n, steps, descent_iters))
             mem = job_memory("adjoint", n, steps=steps)
         else:
-            gen = (lambda f=f, T=T, n=n: newton_job(f, T, 0.0, n, adaptive, inner, newton_iters))
+            gen = (lambda f=f, T=T, n=n, i=i: newton_job(f, T, 0.0, n, adaptive, inner, newton_iters,
+                                                         dump_dir=dump_dir, tag=f"job{i}"))
             mem = job_memory("newton", n, inner=inner)
         out.append((t, kind, gen, mem, n))
     return out

@@ -26,7 +26,11 @@ def compile_objective_and_Jacobian(input_dict, param_dict, obj_mode):
 
     ``obj_mode`` is the integration mode of objective_RPO.  The linearisation
     data of a primal (adaptive schedule, phase vectors) is computed once and
-    reused while the same primal is passed again (one Newton iteration).
+    reused while the same primal is passed again (one Newton iteration).  The
+    cache key holds the fields tensor's storage and version counter (an
+    in-place update of the fields bumps it) with T and sx, and the cached
+    entry keeps a reference to that tensor, so a freed and reused address
+    cannot match either; fields that are not tensors are never cached.
     """
     mode = obj_mode
 
@@ -36,8 +40,13 @@ def compile_objective_and_Jacobian(input_dict, param_dict, obj_mode):
     cache = {}
 
     def jac(primal, tangent):
-        key = (id(primal), primal["fields"].data_ptr() if isinstance(primal["fields"], torch.Tensor) else 0,
-               float(primal["T"]), float(primal["sx"]))
+        fl = primal["fields"]
+        if not isinstance(fl, torch.Tensor):
+            return loss_functions.Linearization(primal, param_dict, mode).jvp(tangent)
+        key = (fl.data_ptr(), fl._version, tuple(fl.shape), float(primal["T"]), float(primal["sx"]))
+        if cache.get("fields") is not fl:
+            cache.pop("key", None)
+        cache["fields"] = fl
         lin = cache.get("lin")
         if lin is None or cache.get("key") != key:
             lin = loss_functions.Linearization(primal, param_dict, mode)

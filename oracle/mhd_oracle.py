@@ -355,7 +355,7 @@ def lawson_rk4_vjp(f0, t, steps, pd, fbar, mask=True):
 # adaptive Lawson RK4(3)  (timestepping.py:151-335)
 # --------------------------------------------------------------------------
 def lawson_rk43(x, pd, t, h=1e-2, atol=1e-4, max_steps_per_checkpoint=1024,
-                checkpoints=1, record=False):
+                checkpoints=1, record=False, _plant_k1_restart=False):
     """Adaptive Lawson RK4 with embedded 3rd-order error (timestepping.py:151-335).
 
     Reproduces: pseudo-time s in [0,1] with mod_L = t L and v' = t v
@@ -370,6 +370,10 @@ def lawson_rk43(x, pd, t, h=1e-2, atol=1e-4, max_steps_per_checkpoint=1024,
     err at the boundary flips the decision, so only invariants of the
     decision sequence are pinned.  With ``record`` the per-attempt
     (h, accepted) schedule is returned in info['schedule'].
+
+    ``_plant_k1_restart`` is a planted mistake for the tests only: a rejected
+    attempt then restarts from k1 = v'(x0) instead of carrying e^2 k1, so a
+    test can show that it tells the two apart.
     """
     L = dissipation(pd)
     mod_L = t * L
@@ -403,6 +407,8 @@ def lawson_rk43(x, pd, t, h=1e-2, atol=1e-4, max_steps_per_checkpoint=1024,
             x, k1, s, acc = xf, k5, s + h, acc + 1
         else:
             x, rej = x0, rej + 1
+            if _plant_k1_restart:
+                k1 = v(x0)
         with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
             h = h / 2 if np.isnan(err) else h * 0.9 * (atol / err) ** (1 / 4)
             over = s + h - 1
@@ -1095,3 +1101,50 @@ def adam_update(params, grads, m, v, t, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-
 def dealias_fields(fields, pd):
     """irfft2(mask * rfft2(fields)) (newton.py:211-214, adjoint_descent.py:111-114)."""
     return np.fft.irfft2(pd["mask"] * np.fft.rfft2(fields))
+
+
+# --------------------------------------------------------------------------
+# one Newton-GMRES iteration and one adjoint-descent iteration
+# (newton.py:151-215, adjoint_descent.py:93-123)
+# --------------------------------------------------------------------------
+def newton_iteration(inp, pd, mode, inner, do_line_search=True, default_damp=0.1):
+    """newton_gmres_update of newton.py:151-215 without preconditioners.
+
+    b = flatten(objective(x)) (:159-162); A = flatten o jac(x, .) o unflatten
+    (:165); (step, rel_res) = gmres(A, b, inner) (:181-182); halving line
+    search on |objective(x - damp step)| < |b| (utils.py:17-58) or the fixed
+    default_damp; rel_err = |f.fields| / |x_new.fields| before dealiasing
+    (:195, relative_error_RPO at :140-142); the fields are dealiased last
+    (:211-214).  Returns (x_new, log) with the logged values (:197-208) plus
+    H, Q and the GMRES step.
+    """
+    n = inp["fields"].shape[-1]
+    f, info = objective_RPO(inp, pd, mode)
+    b = flatten(f)
+    sched = info["schedule"] if info is not None else None
+    A = lambda v: flatten(objective_RPO_jvp(inp, unflatten(v, n), pd, mode, sched))
+    step, rel_res, H, Q = gmres(A, b, inner)
+    x = flatten(inp)
+    if do_line_search:
+        obj_norm = lambda xv: float(np.linalg.norm(flatten(objective_RPO(unflatten(xv, n), pd, mode)[0])))
+        x, damp = line_search(x, step, obj_norm, float(np.linalg.norm(b)))
+    else:
+        damp = default_damp
+        x = x - damp * step
+    new = unflatten(x, n)
+    rel_err = float(np.linalg.norm(f["fields"]) / np.linalg.norm(new["fields"]))
+    new = {"fields": dealias_fields(new["fields"], pd), "T": new["T"], "sx": new["sx"]}
+    log = {"rel_err": rel_err, "f_norm": float(np.linalg.norm(b)), "gmres_rel_res": rel_res, "damp": damp,
+           "T": new["T"], "sx": new["sx"], "H": H, "Q": Q, "step": step}
+    return new, log
+
+
+def adjoint_descent_iteration(params, m, v, t, pd, lr=1e-2):
+    """One iteration of adjoint_descent.py:93-123 in mode Lawson_RK4: loss and
+    gradient (:96), adam_update at step t+1 with betas (0.9, 0.999) and eps
+    1e-6 (:108), then the fields dealiased (:111-114).  ``t`` counts from 0.
+    Returns (loss, params, m, v)."""
+    loss, grad = loss_RPO_value_and_grad(params, pd)
+    params, m, v = adam_update(params, grad, m, v, t + 1, lr=lr, beta1=0.9, beta2=0.999, eps=1e-6)
+    params = dict(params, fields=dealias_fields(params["fields"], pd))
+    return loss, params, m, v

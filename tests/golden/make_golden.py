@@ -13,6 +13,11 @@
 3. dictio/Re40_EQ1.npz: one solution file with every entry of the reference's
    dictionaryIO layout (input_keys, param_keys, forcing_str, ...), re-saved
    entry by entry, for the load_dicts format test.
+4. final_states.npz: for each committed solution, the oracle's final state
+   of the Lawson-RK4 trajectory of mismatch_RPO (before the symmetry), as the
+   compact spectrum of the 2/3 region, rows kx = 0..kmax then -kmax..-1,
+   columns ky = 0..kmax (the masked stepper leaves every other mode zero).
+   Made from the committed fixtures alone: ``python make_golden.py final``.
 """
 import json
 import os
@@ -54,5 +59,26 @@ def main():
     np.savez(os.path.join(HERE, "dictio", "Re40_EQ1.npz"), **{k: np.asarray(d[k]) for k in d.files})
 
 
+def final_states():
+    out = {}
+    for name in PICK:
+        tag = name.replace("/", "_")
+        d = np.load(os.path.join(HERE, "solutions", tag + ".npz"), allow_pickle=False)
+        n = d["fields"].shape[-1]
+        pd = O.make_params(n, float(d["nu"]), float(d["eta"]), tuple(d["b0"]))
+        nc, ms = int(d["num_checkpoints"]), int(d["ministeps"])
+        f = np.fft.rfft2(d["fields"])
+        for _ in range(nc):
+            f = O.lawson_rk4(f, float(d["T"]) / nc, ms, pd, True)
+        kmax = (n - 1) // 3
+        out[tag] = np.concatenate([f[:, : kmax + 1, : kmax + 1], f[:, n - kmax:, : kmax + 1]], axis=1)
+        print(tag, n, nc * ms, np.linalg.norm(out[tag]), flush=True)
+    np.savez(os.path.join(HERE, "final_states.npz"), **out)
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "final":
+        final_states()
+    else:
+        main()
+        final_states()

@@ -57,3 +57,36 @@ def test_admission_respects_the_memory_budget():
     assert stats["peak_resident_bytes"] <= budget
     free, _ = serve.Server().run(specs)
     assert stats["wall_s"] >= 0.0 and len(free) == len(specs)
+
+
+def config4_jobs():
+    """Three jobs at the serving workload's per-job sizes of grid and period
+    (N in {128, 256}, T in [2, 8], atol 1e-8, gradient steps = round(T/0.004)
+    rounded up to whole 64-step segments), with fewer iterations each."""
+    adaptive = {"atol": 1e-8, "max_steps_per_checkpoint": 64, "checkpoints": 64}
+    specs = []
+    for i, (kind, n, T) in enumerate([("adjoint", 256, 2.3), ("newton", 128, 2.1), ("adjoint", 128, 4.8)]):
+        f = torch.as_tensor(synth_fields(n, 70 + i)).cuda()
+        if kind == "adjoint":
+            steps = int(np.ceil(round(T / 0.004) / 64) * 64)
+            gen = (lambda f=f, T=T, n=n, steps=steps: serve.adjoint_descent_job(f, T, 0.0, n, steps, 2))
+            mem = serve.job_memory("adjoint", n, steps=steps)
+        else:
+            gen = (lambda f=f, T=T, n=n: serve.newton_job(f, T, 0.0, n, adaptive, 4, 1))
+            mem = serve.job_memory("newton", n, inner=4)
+        specs.append((0.1 * i, kind, gen, mem, n))
+    return specs
+
+
+def test_serving_sizes_each_job_same_alone_and_in_the_mix():
+    print("serving, 3 jobs at N in {128, 256}, T in [2, 8]", flush=True)
+    specs = config4_jobs()
+    done, stats = serve.Server().run(specs)
+    print(stats, flush=True)
+    assert stats["jobs"] == 3 and stats["failed_jobs"] == 0
+    mixed = {j.id: key(j) for j in done}
+    for i, spec in enumerate(sorted(specs, key=lambda a: a[0])):
+        solo, _ = serve.Server().run([(0.0,) + spec[1:]])
+        a, b = key(solo[0]), mixed[i]
+        assert torch.equal(a[0], b[0]), i
+        assert np.array_equal(np.asarray(a[1]), np.asarray(b[1])), i

@@ -511,3 +511,196 @@ def test_dense_vjp_reproduces_the_lawson_rk4_adjoint():
     fb, tb = O.dense_vjp(lambda d, dt: O.lawson_rk4_jvp(f, d, T, dt, steps, pd, True)[1], f.shape, c)
     rf, rt = O.lawson_rk4_vjp(f, T, steps, pd, c, True)
     assert rel(fb, rf) < 1e-12 and abs(tb - rt) < 1e-12 * abs(rt)
+
+
+# ------------------------------------------------------------ RK4(3) rejection quirk
+def _rk43_torch(x0, n, T, h0, atol, budget):
+    from tests import torch_restatement as TR
+    d = TR.domain(n)
+    forcing = -4 * torch.cos(4 * d["y"])
+    xt, att, s = TR.lawson_rk43(torch.as_tensor(x0), T, d, 1 / 40, 1 / 40, (0.0, 0.1), forcing, h=h0, atol=atol,
+                                max_steps_per_checkpoint=budget[0], checkpoints=budget[1])
+    return xt.numpy(), att, s
+
+
+def _schedules_agree(a, b):
+    """Same accept/reject sequence; h equal up to the rounding of the error
+    estimate, a difference of nearly equal stages (h follows err^(-1/4))."""
+    if [ok for _, ok in a] != [ok for _, ok in b]:
+        return False
+    return all(abs(ha - hb) <= 1e-6 * hb for (ha, _), (hb, _) in zip(a, b))
+
+
+@pytest.mark.parametrize("n,T,h0,atol,seed", [(16, 0.15, 0.7, 1e-6, 41), (32, 0.4, 0.8, 1e-7, 3)])
+def test_rk43_rejection_quirk_matches_independent_restatement(n, T, h0, atol, seed):
+    """Q1 (timestepping.py:267,276,297): after a rejection the next attempt
+    starts from the doubly multiplied e^2 k1, not from v'(x0).  A first step
+    far too large (h0) forces several rejections.  The oracle and the torch
+    restatement (written separately from the reference's loop body) agree
+    on the decisions, the step sizes and the final state; the oracle with a
+    planted restart k1 = v'(x0) after each rejection does not."""
+    pd = params(n)
+    x0 = spectrum(n, seed)
+    x, info = O.lawson_rk43(x0, pd, T, h=h0, atol=atol, max_steps_per_checkpoint=64, checkpoints=4, record=True)
+    assert info["completed"] and info["rejected"] >= 3
+    xt, att, s = _rk43_torch(x0, n, T, h0, atol, (64, 4))
+    assert abs(s - 1.0) < 1e-10
+    assert _schedules_agree(att, info["schedule"])
+    assert rel(xt, x) < 1e-13
+    xm, infom = O.lawson_rk43(x0, pd, T, h=h0, atol=atol, max_steps_per_checkpoint=64, checkpoints=4, record=True,
+                              _plant_k1_restart=True)
+    assert not (_schedules_agree(att, infom["schedule"]) and rel(xt, xm) < 1e-13)
+    assert rel(xt, xm) > 1e-7
+
+
+def _rk43_alfven_mode(a, b, fh, T, h, atol, budget, n, nu, eta, restart=False):
+    """The RK4(3) scheme of timestepping.py:151-335 on the two amplitudes
+    (omega_hat, j_hat) of the (kx, ky) = (0, 4) mode of y-only fields, where
+    state_vel reduces to the closed form (mhd_jax.py:136-164 with kx = 0,
+    v = 0, By = b0_y = 0.1): omega_t = F_hat + 0.1 i 4 j_hat,
+    j_t = 0.1 k^2 i (4 / k^2) omega_hat.  The error norm is the grid maximum
+    of a single-mode cosine: (2 / N^2) |c| max(|cos phi|, |sin phi|) on the
+    N = 16 grid, where 4 y_m = pi m / 2."""
+    lam = np.array([-nu * 16.0, -eta * 16.0])
+    vel = lambda z: T * np.array([fh + 0.4j * z[1], 0.4j * z[0]])
+    norm = lambda c: 2.0 / n ** 2 * abs(c) * max(abs(np.cos(np.angle(c))), abs(np.sin(np.angle(c))))
+    x = np.array([a, b], dtype=complex)
+    s, k1, sched = 0.0, vel(x), []
+    for _ in range(budget):
+        if abs(s - 1.0) < 1e-10:
+            break
+        e = np.exp(T * lam * h / 2)
+        xa, ka = e * x, e * k1
+        k2 = vel(xa + h * ka / 2)
+        k3 = vel(xa + h * k2 / 2)
+        xa, ka, k2, k3 = e * xa, e * ka, e * k2, e * k3
+        k4 = vel(xa + h * k3)
+        xf = xa + h * (ka / 6 + k2 / 3 + k3 / 3 + k4 / 6)
+        k5 = vel(xf)
+        d = h * (k5 - k4) / 6
+        err = max(norm(d[0]), norm(d[1]))
+        ok = err < atol
+        sched.append((h, ok))
+        if ok:
+            x, k1, s = xf, k5, s + h
+        else:
+            k1 = vel(x) if restart else ka
+        h = h * 0.9 * (atol / err) ** 0.25
+        h = h - max(s + h - 1.0, 0.0)
+    return x, sched, s
+
+
+def test_rk43_rejection_quirk_on_a_closed_form_linear_mode():
+    """Fields that depend on y alone, on the forced (0, 4) mode, with b0 =
+    (0, 0.1): the nonlinearity vanishes and the b0 terms couple omega and j
+    linearly (an Alfven oscillation), so the whole adaptive integration is a
+    recursion on two complex amplitudes with a closed-form right-hand side
+    and error norm.  Several rejections occur (h0 = 0.9); the oracle's
+    decisions and final state follow that recursion, which carries e^2 k1
+    across each rejection (Q1), and the oracle with the planted restart
+    k1 = v'(x0) no longer does."""
+    n = 16
+    pd = params(n)
+    F = (pd["mask"] * np.fft.rfft2(pd["forcing"]))[0, 4]
+    x0 = np.zeros((2, n, n // 2 + 1), dtype=complex)
+    x0[0, 0, 4] = 0.3 * F
+    x0[1, 0, 4] = 40.0j
+    T, h0, atol = 2.0, 0.9, 1e-7
+    x, info = O.lawson_rk43(x0, pd, T, h=h0, atol=atol, max_steps_per_checkpoint=64, checkpoints=4, record=True)
+    assert info["completed"] and info["rejected"] >= 2
+    c, sched, s = _rk43_alfven_mode(x0[0, 0, 4], x0[1, 0, 4], F, T, h0, atol, 256, n, 1 / 40, 1 / 40)
+    assert abs(s - 1.0) < 1e-10
+    assert [ok for _, ok in sched] == [ok for _, ok in info["schedule"]]
+    # h follows err^(-1/4), and err is h (k5 - k4)/6 of nearly equal stages:
+    # its rounding differs between an FFT and the closed form by ~1e-6 relative
+    np.testing.assert_allclose([h for h, _ in sched], [h for h, _ in info["schedule"]], rtol=1e-5)
+    np.testing.assert_allclose([x[0, 0, 4], x[1, 0, 4]], c, rtol=1e-10)
+    others = x.copy()
+    others[:, 0, 4] = 0
+    assert np.abs(others).max() < 1e-10 * np.abs(c).max()
+    xm, _ = O.lawson_rk43(x0, pd, T, h=h0, atol=atol, max_steps_per_checkpoint=64, checkpoints=4,
+                          _plant_k1_restart=True)
+    cm, _, _ = _rk43_alfven_mode(x0[0, 0, 4], x0[1, 0, 4], F, T, h0, atol, 256, n, 1 / 40, 1 / 40, restart=True)
+    np.testing.assert_allclose([xm[0, 0, 4], xm[1, 0, 4]], cm, rtol=1e-10)
+    assert abs(cm[0] - c[0]) > 1e-7 * abs(c[0])
+
+
+# ------------------------------------------------------------ line search, dealias, ADAM, Newton
+def test_line_search_halves_until_the_norm_drops():
+    """utils.py:17-58: damp = 1, 1/2, 1/4, ... and the first x - damp step whose
+    objective norm is below |f(x)| is returned with that damp."""
+    c = np.array([1.0, -2.0, 0.5])
+    x = np.array([3.0, 1.0, -1.0])
+    obj_norm = lambda z: float(np.linalg.norm(z - c))
+    f_norm = obj_norm(x)
+    # f(x - d step) = (x - c)(1 - 2.2 d): |1 - 2.2 d| < 1 first at d = 1/2
+    xn, damp = O.line_search(x, 2.2 * (x - c), obj_norm, f_norm)
+    assert damp == 0.5 and np.allclose(xn, x - 1.1 * (x - c))
+    # |1 - 7 d| < 1 first at d = 1/4
+    _, damp = O.line_search(x, 7.0 * (x - c), obj_norm, f_norm)
+    assert damp == 0.25
+    # a step that never helps: x unchanged, damp = 2^-max_iters
+    xn, damp = O.line_search(x, -(x - c), obj_norm, f_norm, max_iters=20)
+    assert np.array_equal(xn, x) and damp == 2.0 ** -20
+
+
+def test_dealias_fields_projects_onto_the_mask():
+    """newton.py:211-214: keeps |kx|, |ky| < N/3 except the mean, drops the rest."""
+    n = 32
+    pd = params(n)
+    x, y = pd["x"], pd["y"]
+    kept = np.cos(3 * x + 2 * y) + 0.5 * np.sin(10 * y)
+    dropped = 0.7 + np.cos(11 * x) + np.sin(5 * x + 12 * y)
+    fields = np.stack([kept + dropped, 2 * kept])
+    out = O.dealias_fields(fields, pd)
+    assert np.allclose(out[0], kept, atol=1e-13) and np.allclose(out[1], 2 * kept, atol=1e-13)
+    assert np.allclose(O.dealias_fields(out, pd), out, atol=1e-14)   # idempotent
+
+
+def test_adam_moves_by_lr_under_a_constant_gradient():
+    """With the same gradient g at every step the bias-corrected moments are
+    exactly g and g^2, so each step moves by lr g / (|g| + eps) -- a wrong
+    bias-correction power or moment decay breaks this from step 2 on."""
+    p = {"a": np.array([1.0, -2.0, 3.0]), "b": np.array(0.5)}
+    g = {"a": np.array([0.3, -4.0, 1e-3]), "b": np.array(-2.0)}
+    m = {k: np.zeros_like(v) for k, v in p.items()}
+    v = {k: np.zeros_like(w) for k, w in p.items()}
+    lr, eps = 1e-2, 1e-6
+    for t in range(1, 6):
+        q, m, v = O.adam_update(p, g, m, v, t, lr=lr, beta1=0.9, beta2=0.999, eps=eps)
+        for k in p:
+            np.testing.assert_allclose(p[k] - q[k], lr * g[k] / (np.abs(g[k]) + eps), rtol=1e-9, atol=0)
+        p = q
+
+
+def test_newton_iteration_takes_the_exact_newton_step_on_a_full_krylov_space():
+    """newton.py:151-215 on a grid of 8 (n = 2*64 + 2 = 130 unknowns) with a
+    Krylov space of the full dimension: GMRES then solves J step = f exactly,
+    the line search returns the first halving that lowers |f| (checked by
+    evaluating the objective along the step), and the new fields are the
+    dealiased x - damp step."""
+    n = 8
+    pd = params(n)
+    pd.update({"num_checkpoints": 2, "ministeps": 4, "shift_reflect_ny": 0, "rot": False})
+    rng = np.random.default_rng(2)
+    fields = np.fft.irfft2(pd["mask"] * np.fft.rfft2(rng.standard_normal((2, n, n))))
+    inp = {"fields": fields, "T": 0.3, "sx": 0.05}
+    nvec = 2 * n * n + 2
+    new, log = O.newton_iteration(inp, pd, "Lawson_RK4", inner=nvec - 2)
+    f, _ = O.objective_RPO(inp, pd, "Lawson_RK4")
+    b = O.flatten(f)
+    step = log["step"]
+    J = lambda v: O.flatten(O.objective_RPO_jvp(inp, O.unflatten(v, n), pd, "Lawson_RK4"))
+    assert np.linalg.norm(J(step) - b) < 1e-8 * np.linalg.norm(b)
+    assert log["gmres_rel_res"] < 1e-8
+    x = O.flatten(inp)
+    nrm = lambda d: float(np.linalg.norm(O.flatten(O.objective_RPO(O.unflatten(x - d * step, n), pd,
+                                                                   "Lawson_RK4")[0])))
+    d = 1.0
+    while nrm(d) >= np.linalg.norm(b):
+        d /= 2
+    assert log["damp"] == d
+    xs = O.unflatten(x - d * step, n)
+    assert np.allclose(new["fields"], O.dealias_fields(xs["fields"], pd), atol=1e-14)
+    assert new["T"] == xs["T"] and new["sx"] == xs["sx"]
+    assert abs(log["rel_err"] - np.linalg.norm(f["fields"]) / np.linalg.norm(xs["fields"])) < 1e-15

@@ -1,7 +1,11 @@
 """Host logic of the serving loop (no GPU): admission against the memory
-budget, grouping of compatible operations across jobs, arrival order and the
-latency statistics, with a CPU executor standing in for the kernels."""
+budget, grouping of compatible operations across jobs, arrival order, groups
+that advance independently and the latency statistics, with a CPU executor
+standing in for the kernels."""
 import threading
+import time
+
+import pytest
 
 import numpy as np
 
@@ -41,10 +45,10 @@ def test_results_and_batching():
     res = {j.id: j.result for j in done}
     for i in range(6):
         assert res[i] == 3 * 2.0 * i
-    # all six jobs arrive at t=0: each tick has one group per grid size
+    # all six jobs arrive at t=0: one group per grid size, each advancing on its own
     assert all(c[1] == 3 for c in ex.calls)
     assert {c[0] for c in ex.calls} == {("grad", 32), ("grad", 64)}
-    assert stats["mean_batch"] == 6.0
+    assert stats["mean_batch"] == 3.0 and stats["ticks"] == 6
 
 
 def test_gradient_jobs_of_different_segment_counts_batch_together():
@@ -105,3 +109,74 @@ def test_a_failing_job_ends_alone():
     res = {j.id: j.result for j in done if j.error is None}
     assert sorted(res.values()) == [0.0, 6.0, 12.0, 18.0]
     assert np.isfinite(stats["p50_latency_s"])
+
+
+@pytest.mark.timeout(60)
+def test_a_job_larger_than_the_budget_is_rejected():
+    """A job whose memory exceeds the whole budget ends at once with an error
+    (it would never fit), and the jobs behind it are served."""
+    ex = Exec()
+    arr = arrivals(3, mem=10.0) + [(0.0, "newton", (lambda: toy_job(64, 2, 1.0)), 30.0, 64)]
+    done, stats = serve.Server(hbm_budget=25.0, executor=ex).run(arr)
+    assert stats["jobs"] == 4 and stats["failed_jobs"] == 1
+    bad = [j for j in done if j.error is not None]
+    assert len(bad) == 1 and "exceeds the HBM budget" in bad[0].error and bad[0].mem == 30.0
+    assert sorted(j.result for j in done if j.error is None) == [0.0, 6.0, 12.0]
+
+
+class SlowExec(Exec):
+    def __init__(self, delay):
+        super().__init__()
+        self.delay = delay
+
+    def __call__(self, key, ops, lane):
+        time.sleep(self.delay.get(key[1], 0.0))
+        return super().__call__(key, ops, lane)
+
+
+@pytest.mark.timeout(60)
+def test_smaller_jobs_pass_a_waiting_large_one():
+    """Budget 25: a resident job of 20 keeps a second job of 20 waiting; a job
+    of 5 that arrives later is admitted in the meantime."""
+    ex = SlowExec({64: 0.05})
+    arr = [(0.0, "adjoint", (lambda: toy_job(64, 4, 1.0)), 20.0, 64),
+           (0.0, "adjoint", (lambda: toy_job(64, 1, 2.0)), 20.0, 64),
+           (0.01, "adjoint", (lambda: toy_job(32, 1, 3.0)), 5.0, 32)]
+    done, stats = serve.Server(hbm_budget=25.0, executor=ex).run(arr)
+    by = {j.id: j for j in done}
+    assert stats["failed_jobs"] == 0 and stats["peak_resident_bytes"] <= 25.0
+    assert by[2].start < by[1].start and by[2].finish < by[1].start
+
+
+@pytest.mark.timeout(60)
+def test_groups_advance_independently():
+    """A slow group (N=64, 0.5 s per operation) does not hold back a fast job
+    on another grid size: the fast job's five operations all run while the
+    slow one is in flight."""
+    ex = SlowExec({64: 0.5})
+    arr = [(0.0, "adjoint", (lambda: toy_job(64, 1, 1.0)), 1.0, 64),
+           (0.0, "adjoint", (lambda: toy_job(32, 5, 2.0)), 1.0, 32)]
+    done, stats = serve.Server(executor=ex).run(arr)
+    by = {j.id: j for j in done}
+    assert by[1].result == 20.0 and by[0].result == 2.0
+    assert by[1].finish < by[0].finish and by[1].finish - by[1].arrival < 0.4
+
+
+@pytest.mark.timeout(60)
+def test_group_slots_bound_concurrency():
+    """At most max_groups groups run at once."""
+    running, peak = [0], [0]
+    lock = threading.Lock()
+
+    def ex(key, ops, lane):
+        with lock:
+            running[0] += 1
+            peak[0] = max(peak[0], running[0])
+        time.sleep(0.02)
+        with lock:
+            running[0] -= 1
+        return [o.kw["x"] for o in ops]
+
+    arr = [(0.0, "adjoint", (lambda i=i: toy_job(16 * (i + 1), 2, 1.0)), 1.0, 16 * (i + 1)) for i in range(6)]
+    done, stats = serve.Server(executor=ex, max_groups=2).run(arr)
+    assert stats["failed_jobs"] == 0 and peak[0] == 2

@@ -2,8 +2,8 @@
 to cross-check the oracle's hand-derived derivatives with torch autograd.
 
 Written directly from the reference (mhd_jax.py:105-175 state_vel,
-timestepping.py:46-81 lawson_rk4, symmetry.py:4-25, loss_functions.py:99-111),
-sharing no code with oracle/mhd_oracle.py.
+timestepping.py:46-81 lawson_rk4, :151-335 lawson_rk43, symmetry.py:4-25,
+loss_functions.py:99-111), sharing no code with oracle/mhd_oracle.py.
 """
 import math
 
@@ -139,3 +139,45 @@ def tdrk4(f, t, steps, d, nu, eta, b0, forcing):
         a2 = h * jac(g, k2)
         f = f + k1 + a1 / 6 + a2 / 3
     return f
+
+
+def lawson_rk43(x, t, d, nu, eta, b0, forcing, h=1e-2, atol=1e-4, max_steps_per_checkpoint=1024,
+                checkpoints=1):
+    """timestepping.py:151-335 written from the reference's loop body.
+
+    Dynamics in pseudo-time s in [0, 1]: dx/ds = t (L x + v(x)) (:242-243).
+    Each attempt multiplies the carried k1 by e = exp(t L h / 2) twice
+    (:267, :276); a rejected attempt keeps x0 but carries that e^2 k1 into
+    the next attempt (:297).
+    Returns (x, [(h, accepted), ...], s).
+    """
+    ksq = d["kx"] ** 2 + d["ky"] ** 2
+    tL = t * (-torch.stack([nu * ksq, eta * ksq]))
+    vel = lambda z: t * state_vel(z, d, nu, eta, b0, forcing, False)
+    s = 0.0
+    k1 = vel(x)
+    attempts = []
+    for _ in range(int(max_steps_per_checkpoint) * int(checkpoints)):
+        if abs(s - 1.0) < 1e-10:
+            break
+        e = torch.exp(tL * h / 2)
+        xa = e * x
+        ka = e * k1
+        k2 = vel(xa + h * ka / 2)
+        k3 = vel(xa + h * k2 / 2)
+        xa, ka, k2, k3 = e * xa, e * ka, e * k2, e * k3
+        k4 = vel(xa + h * k3)
+        xf = xa + h * (ka / 6 + k2 / 3 + k3 / 3 + k4 / 6)
+        k5 = vel(xf)
+        err = float(torch.fft.irfft2(h * (k5 - k4) / 6).abs().max())
+        ok = err < atol
+        attempts.append((h, ok))
+        if ok:
+            x, k1, s = xf, k5, s + h
+        else:
+            k1 = ka
+        h = h / 2 if math.isnan(err) else h * 0.9 * (atol / err) ** 0.25
+        over = s + h - 1.0
+        if math.isnan(over) or over > 0:
+            h = h - over
+    return x, attempts, s
