@@ -333,7 +333,9 @@ class Server:
     the default runs them on the GPU through the fused kernels.  Up to
     ``max_groups`` groups run at once, each on a context lane of its grid size
     and that lane's stream; a group's jobs advance as soon as it finishes, so
-    a slow group delays only its own jobs.
+    a slow group delays only its own jobs.  A job whose operation matches a
+    group still running (same key, batch not full) waits for that group and
+    then rides with its jobs, which keeps jobs of one kind in phase.
 
     Stream order: a job's generator code runs on the caller's current stream;
     a group's stream first waits for an event recorded on that stream after
@@ -467,6 +469,11 @@ class Server:
                 for j in sorted(ready, key=lambda j: j.arrival):
                     groups.setdefault(j.pending.group_key(), []).append(j)
                 for key, js in groups.items():
+                    # a group of this key still running with room to spare: its jobs
+                    # come back soon, and these wait to ride with them (batches stay
+                    # whole instead of splitting into out-of-phase singletons)
+                    if any(k == key and len(p) < self.max_batch for k, p, _ in inflight.values()):
+                        continue
                     for i in range(0, len(js), self.max_batch):
                         if len(inflight) >= slots:
                             break

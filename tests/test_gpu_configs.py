@@ -100,11 +100,25 @@ def test_newton_configuration_full_iteration_properties():
         hq = torch.mv(Q[: k + 2].T.contiguous(), torch.as_tensor(H[: k + 2, k], device="cuda"))
         assert float(torch.linalg.vector_norm(w - hq) / torch.linalg.vector_norm(w)) < 1e-10, k
     res = hist["krylov_residuals"]
-    assert all(res[k + 1] <= res[k] * (1 + 1e-9) for k in range(len(res) - 1))
-    # the Arnoldi estimate tracks the true residual |A x - b| / |b| (one extra matvec, linalg.py:97)
+    print("residual history", ["%.3e" % r for r in res], flush=True)
+    # non-increasing in exact arithmetic; each value is a least-squares solve that
+    # rounds at ~eps cond(H_k), far below the 1e-10 slack
+    assert all(res[k + 1] <= res[k] + 1e-10 for k in range(len(res) - 1))
     print("rel_res", rr, "estimate", res[-1], "qtb", hist["krylov_residuals_qtb"][-1], "orth",
           hist["orthogonality_loss"], flush=True)
-    assert rr < 1e-3 and 0.1 < (rr + 1e-13) / (res[-1] + 1e-13) < 10
+    # while it is above the floor that rounding sets (~1e-8 here), the Arnoldi
+    # estimate is the true residual |A x_k - b| / |b| of the iterate it describes,
+    # x_k = Q_k y_k with y_k = argmin |H_k y - |b| e_1|
+    k = 20
+    nb = float(torch.linalg.vector_norm(b))
+    e1 = np.zeros(k + 1)
+    e1[0] = nb
+    yk = np.linalg.lstsq(H[: k + 1, :k], e1, rcond=None)[0]
+    xk = torch.mv(Q[:k].T.contiguous(), torch.as_tensor(yk, device="cuda"))
+    true_k = float(torch.linalg.vector_norm(A(xk) - b)) / nb
+    assert abs(true_k - res[k - 1]) < 1e-6 * res[k - 1], (true_k, res[k - 1])
+    # the 64-vector solve (the reference's Q^T b right-hand side, linalg.py:92-97) reaches that floor
+    assert rr < 1e-6 and res[-1] <= rr
     new, log = solvers.newton_gmres_update(inp, pd, "Lawson_RK43", inner=64)
     assert log["completed"] and log["gmres_rel_res"] == rr      # same bits as the solve above
     f_new = utils.flatten(LF.objective_RPO(new, pd, "Lawson_RK43"))

@@ -11,7 +11,9 @@ from ecs_mhd_2d_amd.synthetic import fields as synth_fields
 pytestmark = pytest.mark.gpu
 
 
-def jobs():
+def jobs(gap=0.0):
+    """Five small jobs of two kinds on two grid sizes; with gap = 0 they all
+    arrive together, so same-kind jobs share their first launches."""
     adaptive = {"atol": 1e-6, "max_steps_per_checkpoint": 32, "checkpoints": 8}
     specs = []
     for i, (kind, n, T) in enumerate([("adjoint", 32, 0.4), ("newton", 32, 0.3), ("adjoint", 32, 0.5),
@@ -24,7 +26,7 @@ def jobs():
         else:
             gen = (lambda f=f, T=T, n=n: serve.newton_job(f, T, 0.1, n, adaptive, 6, 2))
             mem = serve.job_memory("newton", n, inner=6)
-        specs.append((0.05 * i, kind, gen, mem, n))
+        specs.append((gap * i, kind, gen, mem, n))
     return specs
 
 
@@ -84,6 +86,8 @@ def test_serving_sizes_each_job_same_alone_and_in_the_mix():
     done, stats = serve.Server().run(specs)
     print(stats, flush=True)
     assert stats["jobs"] == 3 and stats["failed_jobs"] == 0
+    # later arrivals were admitted while earlier jobs were on the GPU
+    assert any(a.start < b.finish and b.start < a.finish for a in done for b in done if a.id < b.id)
     mixed = {j.id: key(j) for j in done}
     for i, spec in enumerate(sorted(specs, key=lambda a: a[0])):
         solo, _ = serve.Server().run([(0.0,) + spec[1:]])

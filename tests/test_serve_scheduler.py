@@ -180,3 +180,15 @@ def test_group_slots_bound_concurrency():
     arr = [(0.0, "adjoint", (lambda i=i: toy_job(16 * (i + 1), 2, 1.0)), 1.0, 16 * (i + 1)) for i in range(6)]
     done, stats = serve.Server(executor=ex, max_groups=2).run(arr)
     assert stats["failed_jobs"] == 0 and peak[0] == 2
+
+
+@pytest.mark.timeout(60)
+def test_a_late_job_joins_the_running_group_of_its_kind():
+    """A job arriving while a group of its kind is running waits for it and
+    then rides in the same batch, instead of running beside it alone."""
+    ex = SlowExec({64: 0.05})
+    arr = [(0.0, "adjoint", (lambda: toy_job(64, 4, 1.0)), 1.0, 64),
+           (0.02, "adjoint", (lambda: toy_job(64, 3, 2.0)), 1.0, 64)]
+    done, stats = serve.Server(executor=ex).run(arr)
+    assert stats["failed_jobs"] == 0
+    assert [c[1] for c in ex.calls] == [1, 2, 2, 2]
