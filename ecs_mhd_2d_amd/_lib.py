@@ -32,6 +32,7 @@ _SIGS = {
     "ecs_last_error": (ctypes.c_char_p, []),
     "ecs_ctx_n": (c_int, [c_void_p]),
     "ecs_state_vel": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    "ecs_state_vel_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     "ecs_state_vel_jvp": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     "ecs_state_vel_vjp": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     "ecs_lawson_rk4": (c_int, [c_void_p, c_void_p, c_int, P_double, c_int, c_int, c_void_p]),

@@ -7,6 +7,27 @@
 #include "recurrence_kernels.h"
 
 static thread_local std::string g_err;
+
+// Workspace of ecs_state_vel_noalloc: one copy per GPU, held until the process
+// ends.  Host code never uses this name as an address: each context keeps the
+// device address hipGetSymbolAddress gives on its GPU (ctx->noalloc).
+static __device__ __align__(256) double2 scratch[ECS_NOALLOC_SCRATCH_BYTES / sizeof(double2)];
+static std::mutex g_scratch_mu;
+static std::map<int, double2*> g_scratch_addr;  // device id -> address of `scratch` there
+
+// Address of `scratch` on the current device (looked up once per device).
+static int scratch_address(int dev, double2** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_addr.find(dev);
+  if (it == g_scratch_addr.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(2, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    it = g_scratch_addr.emplace(dev, (double2*)p).first;
+  }
+  *out = it->second;
+  return 0;
+}
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
@@ -88,19 +109,34 @@ static int bucket_of(int B) {
 }
 long long tune_key(int n, int kind, int B) { return ((long long)n * 8 + kind) * 64 + bucket_of(B); }
 long long plan_key(int n, int kind) { return ((long long)n * 8 + kind) * 64 + 63; }
+
+int plan_table(int n, int kind) {
+  // families: column, tangent column, row, tangent row
+  static const int kPlanTable[5][4] = {
+      {1, 1, 1, 0},   // N = 16
+      {0, 0, 0, 0},   // N = 32
+      {1, 1, 0, 0},   // N = 64
+      {1, 1, 1, 0},   // N = 128
+      {1, 1, 1, 0},   // N = 256
+  };
+  const int row = n == 16 ? 0 : n == 32 ? 1 : n == 64 ? 2 : n == 128 ? 3 : 4;
+  return kPlanTable[row][kind & 3];
+}
+
 int tuned(int n, int kind, int B, int dflt) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto it = g_tune.find(tune_key(n, kind, B));
   if (it != g_tune.end()) return it->second;
   // an untuned bucket keeps the default tile within the grid size's fixed plan
   auto ip = g_plan.find(plan_key(n, kind));
-  return ip == g_plan.end() ? dflt : 3 * ip->second + dflt % 3;
+  const int plan = ip == g_plan.end() ? plan_table(n, kind) : ip->second;
+  return 3 * plan + dflt % 3;
 }
 
 int plan_of(int n, int kind) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto ip = g_plan.find(plan_key(n, kind));
-  return ip == g_plan.end() ? 0 : ip->second;
+  return ip == g_plan.end() ? plan_table(n, kind) : ip->second;
 }
 
 int upload_scalars(ecs_ctx* c, int slot, const double* vals, int B, hipStream_t st) {
@@ -250,7 +286,8 @@ int ecs_ctx_create(ecs_ctx** out, int n, double nu, double eta, double b0x, doub
   if (!dg.ok) return fail(2, "ecs_ctx_create: cannot make the device current");
   ecs_ctx* c = new ecs_ctx();
   c->n = n; c->dev = device; c->nu = nu; c->eta = eta; c->b0x = b0x; c->b0y = b0y;
-  const int rc = ctx_tables(c, fhat_full);
+  int rc = ctx_tables(c, fhat_full);
+  if (!rc) rc = scratch_address(device, &c->noalloc);
   if (rc) {
     const std::string msg = g_err;
     ecs_ctx_destroy(c);  // frees what was allocated before the failure
@@ -297,6 +334,14 @@ int ecs_state_vel(ecs_ctx* c, const double* in, double* out, int batch, int diss
   GUARD(c);
   hipStream_t st = (hipStream_t)stream;
   return OPS(c->n)->state_vel(c, in, nullptr, out, nullptr, batch, diss, st);
+}
+
+int ecs_state_vel_noalloc(ecs_ctx* c, const double* in, double* out, int batch, int diss, void* stream) {
+  if (!c || !in || !out || batch < 1) return fail(1, "ecs_state_vel_noalloc: bad arguments");
+  GUARD(c);
+  if (!c->noalloc) return fail(2, "ecs_state_vel_noalloc: no scratch address for the context's device");
+  return OPS(c->n)->state_vel_noalloc(c, in, out, batch, diss, c->noalloc,
+                                      ECS_NOALLOC_SCRATCH_BYTES / sizeof(double2), (hipStream_t)stream);
 }
 
 int ecs_state_vel_jvp(ecs_ctx* c, const double* in, const double* din, double* out, double* dout, int batch,

@@ -218,13 +218,13 @@ static int time_variants(ecs_ctx* c, int B, int kind, const int* vs, int nv, flo
 // Fixes, per kernel family (column, tangent column, row, tangent row), the
 // FFT radix plan of a grid size and picks the tile per batch bucket (powers
 // of two).  The plan decides the arithmetic, so it comes from a fixed table
-// (kPlanTable: the plan that timed fastest on MI355X at batch 1 and 16), and
+// (plan_table: the plan that timed fastest on MI355X at batch 1 and 16), and
 // every process, batch size and run computes the same bits; a candidate's
 // result does not depend on the batch it rides in or on timing noise.  Tiles
 // and splits do not change the arithmetic and are timed per bucket within the
 // fixed plan.  Runs at the start of an entry point, before any input is
 // packed (it overwrites the workspace and may grow it to 16 candidates).
-// ECSMHD_NO_TUNE=1 keeps the default tiles and the wide plan;
+// ECSMHD_NO_TUNE=1 keeps the default tiles in the table's plans;
 // ECSMHD_PLAN=wide|narrow fixes the plan of every family; ECSMHD_PLAN=tune
 // chooses it by timing (both plans at batch 1 and 16, the plan with the
 // smaller worst-case slowdown wins), which is how the table was measured;
@@ -247,20 +247,6 @@ static bool forced_variants(int n, int v[4]) {
     if (s) ++s;
   }
   return false;
-}
-
-// FFT plan (0 wide, 1 narrow) of each family {column, tangent column, row,
-// tangent row} per grid size.
-static int plan_table(int n, int kind) {
-  static const int kPlanTable[5][4] = {
-      {0, 0, 0, 0},   // N = 16
-      {0, 0, 0, 0},   // N = 32
-      {0, 0, 0, 0},   // N = 64
-      {1, 1, 1, 0},   // N = 128
-      {1, 1, 1, 0},   // N = 256
-  };
-  const int row = n == 16 ? 0 : n == 32 ? 1 : n == 64 ? 2 : n == 128 ? 3 : 4;
-  return kPlanTable[row][kind];
 }
 
 template <int N>
@@ -368,6 +354,41 @@ static int run_state_vel(ecs_ctx* c, const double* in, const double* din, double
     TRY((col<N, C_RHS, false>(a, B, st)));
   }
   TRY(unpack<N>(c->out, out, B, 1, st));
+  return 0;
+}
+
+// state_vel with its workspace in caller-provided scratch (B candidates):
+// [x0: B comp][out: B comp][I: B 8 N KP][O: B 2 N KP]; no autotuning, no
+// allocation.  The three kernels (C_LOAD, R_RHS, C_RHS) and pack/unpack read
+// and write only these arrays, the twiddles and the forcing spectrum.  Kernel variants are the tuned ones where a call of the same
+// batch bucket has tuned them, else the default tile in the grid size's fixed
+// FFT plan, so the arithmetic is that of run_state_vel.
+template <int N>
+static int run_state_vel_noalloc(ecs_ctx* c, const double* in, double* out, int B, int diss, double2* scr,
+                                 size_t scr_elems, hipStream_t st) {
+  using G = Geo<N>;
+  const size_t comp = c->compact_elems();
+  const size_t per = 2 * comp + (size_t)10 * N * G::KP;
+  if ((size_t)B * per > scr_elems)
+    return fail(1, "ecs_state_vel_noalloc: batch " + std::to_string(B) + " at N=" + std::to_string(N) +
+                       " needs " + std::to_string((size_t)B * per * sizeof(double2)) + " bytes of scratch; the limit is " +
+                       std::to_string(scr_elems * sizeof(double2)) + " bytes (at most " +
+                       std::to_string(scr_elems / per) + " candidates)");
+  Args a = base_args(c);
+  a.include_diss = diss;
+  // the context's scalar table may hold fewer than B candidates (or none);
+  // these kernels take no step size
+  a.dt = nullptr;
+  a.ddt = nullptr;
+  a.x0 = scr;
+  a.out = scr + (size_t)B * comp;
+  a.inter = scr + (size_t)2 * B * comp;
+  a.rowout = a.inter + (size_t)B * 8 * N * G::KP;
+  TRY(pack<N>(in, a.x0, B, st));
+  TRY((col<N, C_LOAD, false>(a, B, st)));
+  TRY((row<N, R_RHS, false>(a, B, st)));
+  TRY((col<N, C_RHS, false>(a, B, st)));
+  TRY(unpack<N>(a.out, out, B, 1, st));
   return 0;
 }
 
@@ -1706,6 +1727,6 @@ template <int N> static int gen_t(ecs_ctx* c, double* f, double* df, int B, cons
                                 &run_rk43_jvp<NV>,  &run_state_vel_vjp<NV>, &run_rk4_ckpt<NV>,            \
                                 &run_rk4_rev<NV>,   &run_rk4_vjp<NV>,    &run_time_kernel<NV>,            \
                                 &gen_p<NV>,         &gen_t<NV>,          &run_gen_vjp<NV>,                \
-                                &run_rk43_vjp<NV>};                                                   \
+                                &run_rk43_vjp<NV>,  &run_state_vel_noalloc<NV>};                          \
     return &ops;                                                                                         \
   }

@@ -131,6 +131,7 @@ struct ecs_ctx {
   std::map<std::string, hipGraphExec_t> graphs;
   double* red = nullptr;       // reductions [cap][64]
   double* hred = nullptr;      // pinned [cap]
+  double2* noalloc = nullptr;  // device address of the library's static scratch on this context's GPU
   size_t compact_elems() const {
     int kmax = (n - 1) / 3;
     return (size_t)2 * (2 * kmax + 1) * (kmax + 1);
@@ -197,8 +198,13 @@ extern std::map<long long, int> g_plan;  // (n, kind) -> plan index
 long long tune_key(int n, int kind, int B);
 long long plan_key(int n, int kind);
 int tuned(int n, int kind, int B, int dflt);
-// the FFT plan fixed for (n, kind): 0 wide, 1 narrow; 0 before tuning
+// the FFT plan fixed for (n, kind): 0 wide, 1 narrow; the plan table's entry
+// until the autotuner (or ECSMHD_PLAN / ECSMHD_VARIANTS) has fixed it
 int plan_of(int n, int kind);
+// FFT plan (0 wide, 1 narrow) of kernel family `kind` for grid size n: the
+// plan that timed fastest on MI355X at batch 1 and 16, fixed so that every
+// process computes the same bits
+int plan_table(int n, int kind);
 
 // ------------------------------------------------------------------ hipGraph chunks
 // Captures the launches `record` makes on the context's own stream once per
@@ -264,6 +270,7 @@ struct GridOps {
   int (*gen_vjp)(ecs_ctx*, const double*, double*, double*, int, const double*, int, int, int, int, hipStream_t);
   int (*rk43_vjp)(ecs_ctx*, const double*, double*, double*, int, const double*, const double*, const int8_t*,
                   const int*, int, int, hipStream_t);
+  int (*state_vel_noalloc)(ecs_ctx*, const double*, double*, int, int, double2*, size_t, hipStream_t);
 };
 const GridOps* grid_ops(int n);
 const GridOps* grid_ops_16();

@@ -46,6 +46,21 @@ int ecs_ctx_n(const ecs_ctx* ctx);
 int ecs_state_vel(ecs_ctx* ctx, const double* in, double* out, int batch, int include_dissipation,
                   void* stream);
 
+/* Allocation-free variant of ecs_state_vel: the same kernels and the same
+ * result.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Its workspace is a static device array of
+ * ECS_NOALLOC_SCRATCH_BYTES that the library holds on each GPU until the
+ * process ends; the call allocates nothing and needs no earlier call.
+ * A batch needs batch * (2 C + 10 N KP) * 16 bytes, C = 2 (2K+1)(K+1),
+ * K = (N-1)/3, KP = K+1 rounded up to a multiple of 8: at most 14 candidates
+ * at N=256, 55 at N=128; a larger batch returns 1.  Every call on one GPU
+ * shares the array: calls that overlap in time on one GPU (on other streams
+ * or other contexts) race. */
+#define ECS_NOALLOC_SCRATCH_BYTES (64u << 20)
+int ecs_state_vel_noalloc(ecs_ctx* ctx, const double* in, double* out, int batch, int include_dissipation,
+                          void* stream);
+
 /* Tangent of state_vel: out = v(in), dout = J(in) din  (linearisation of
  * mhd_jax.py:105-175, as jax.jvp of state_vel would give). */
 int ecs_state_vel_jvp(ecs_ctx* ctx, const double* in, const double* din, double* out, double* dout,
