@@ -54,30 +54,58 @@ def build_guess(n, seed, steps, h, pd):
     return torch.fft.irfft2(f, s=(n, n)).contiguous()
 
 
-def cpu_baseline_matvec(fields_np, T, sx, lin_info, budget_s=15.0):
-    """Oracle (numpy, 1 thread of pocketfft) JVP through the first S accepted
-    steps of the same schedule; scaled to the full trajectory."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_matvec(fields_np, T, sx, lin_info, sample_attempts=64, runs=5):
+    """The oracle's JVP (the CPU restatement of the reference's lines, FP64) on
+    the first ``sample_attempts`` attempts of the same N=128 schedule, on the
+    host's cores: numpy's single-threaded FFT and scipy.fft with one worker
+    per core available to this process; each timed ``runs`` times after one
+    warm-up run, median kept, scaled to the whole trajectory.  The faster
+    configuration is the baseline; both are reported."""
     from oracle import mhd_oracle as O
     n = fields_np.shape[-1]
     pd = O.make_params(n, 1 / 40, 1 / 40)
     acc = lin_info["acc"][: lin_info["attempts"]]
     hs = lin_info["hs"][: lin_info["attempts"]]
     sched = [(float(h), bool(a)) for h, a in zip(hs, acc)]
+    S = min(sample_attempts, len(sched))
     f0 = np.fft.rfft2(fields_np)
     rng = np.random.default_rng(5)
     df0 = np.fft.rfft2(rng.standard_normal((2, n, n)))
-    # probe cost of a few steps, then size the sample to ~budget_s
-    t0 = time.perf_counter()
-    O.lawson_rk43_jvp(f0, df0, pd, T, 0.1, sched[:4])
-    per = (time.perf_counter() - t0) / 4
-    S = int(max(8, min(len(sched), budget_s / max(per, 1e-6))))
-    t0 = time.perf_counter()
-    O.lawson_rk43_jvp(f0, df0, pd, T, 0.1, sched[:S])
-    t = time.perf_counter() - t0
-    full = t * len(sched) / S
-    return {"value": 1.0 / full, "unit": "matvecs/s", "cores": 1, "kind": "port",
-            "sample": f"oracle JVP through the first {S} of {len(sched)} attempts of the same N=128 schedule "
-                      f"({t:.1f} s), scaled linearly to the full trajectory"}
+    cores = len(os.sched_getaffinity(0))
+    lim = os.environ.get("OMP_NUM_THREADS")
+    if lim and lim.isdigit():
+        cores = min(cores, int(lim))
+    res = {}
+    for workers in sorted({1, cores}):
+        O.set_fft_workers(workers)
+        try:
+            O.lawson_rk43_jvp(f0, df0, pd, T, 0.1, sched[:S])
+            ts = []
+            for _ in range(runs):
+                t0 = time.perf_counter()
+                O.lawson_rk43_jvp(f0, df0, pd, T, 0.1, sched[:S])
+                ts.append(time.perf_counter() - t0)
+        finally:
+            O.set_fft_workers(1)
+        res[workers] = float(np.median(ts)) * len(sched) / S
+    best = min(res, key=res.get)
+    return {"value": 1.0 / res[best], "unit": "matvecs/s", "cores": best, "kind": "port",
+            "sample": (f"oracle JVP (numpy/scipy FP64 restatement) through the first {S} of {len(sched)} attempts "
+                       f"of the same N=128 schedule, median of {runs} runs after a warm-up, scaled to the whole "
+                       f"trajectory; {'numpy FFT, 1 thread' if best == 1 else f'scipy.fft, {best} workers'} "
+                       f"was the faster: " + ", ".join(f"{w} thread(s) {1.0 / t:.4f}/s" for w, t in res.items())
+                       + f"; host {_cpu_model()}, {len(os.sched_getaffinity(0))} cores available")}
 
 
 def probe():
@@ -212,8 +240,13 @@ def newton_leg(inp, pd):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return {"workload": "one Newton-GMRES iteration, inner 64, same guess as the headline", "wall_s": wall,
+            "fwall_s": log["fwall"], "gmreswall_s": log["gmreswall"],
+            "gmres_matvecs_per_s": log["matvecs"] / log["gmreswall"],
             "f_norm": log["f_norm"], "gmres_rel_res": log["gmres_rel_res"], "damp": log["damp"],
-            "krylov_residual_last": log["krylov_residuals"][-1]}
+            "rel_err": log["rel_err"],
+            "arnoldi_residual_estimate_last": log["krylov_residuals"][-1],
+            "qtb_residual_last": log["krylov_residuals_qtb"][-1],
+            "basis_orthogonality_loss": log["orthogonality_loss"]}
 
 
 def guess_generation_leg():
@@ -335,16 +368,18 @@ def serving_full_leg(jobs):
 
 
 def probe_roofline(batched=False, kernel=None):
-    """The matvec's dominant kernel (tangent row kernel, N=128, batch 1) or,
+    """The matvec's dominant kernel (tangent row kernel of the adaptive tangent
+    replay, N=128, batch 1) or,
     with ``batched``, the batched roofline kernel (row kernel, N=256, batch
     64), or ``kernel`` = (N, batch, which) with which as in ecs_time_kernel
-    (0 column, 1 row, 2 tangent column, 3 tangent row kernel): 50 launches
+    (0 column, 1 row, 2 tangent column, 3 tangent row kernel, 4 tangent row
+    kernel of the tangent replay): 50 launches
     through ecs_time_kernel; then a plain device copy of 64 MiB (read 64 MiB,
     write 64 MiB), whose counter readings scale FETCH_SIZE / WRITE_SIZE to
     bytes."""
     import torch
     from ecs_mhd_2d_amd import _lib, _runtime, mhd
-    n, B, which = kernel if kernel else ((256, 64, 1) if batched else (128, 1, 3))
+    n, B, which = kernel if kernel else ((256, 64, 1) if batched else (128, 1, 4))
     pd = mhd.make_param_dict(n, 1 / 40, 1 / 40)
     ctx = mhd.context(pd)
     ms = ctypes.c_double()
@@ -374,7 +409,8 @@ def main():
     ap.add_argument("--probe-roofline-batched", action="store_true",
                     help="the same for the batched row kernel (N=256, batch 64)")
     ap.add_argument("--probe-kernel", default="", metavar="N,BATCH,WHICH",
-                    help="the same for one kernel family (WHICH: 0 column, 1 row, 2 tangent column, 3 tangent row)")
+                    help="the same for one kernel family (WHICH: 0 column, 1 row, 2 tangent column, 3 tangent row, "
+                         "4 tangent row of the tangent replay)")
     ap.add_argument("--probe", action="store_true",
                     help="run a short fixed kernel workload for rocprofv3 and exit")
     args = ap.parse_args()
@@ -413,16 +449,54 @@ def main():
         v = torch.randn(2 * n * n + 2, generator=gen, device="cuda", dtype=torch.float64)
         return utils.unflatten(v / torch.linalg.vector_norm(v), n)
 
-    tans = [tangent() for _ in range(args.warmup + args.steps)]
-    for i in range(args.warmup):
-        lin.jvp(tans[i])
+    # one step = one Arnoldi step of GMRES (linalg.py:66-85): the JVP matvec through
+    # the whole adaptive trajectory and the Gram-Schmidt sweep of its result,
+    # on the Krylov basis started from b = flatten(objective) (Krylov dimension 64;
+    # past 64 steps a new basis is started from b again)
+    from ecs_mhd_2d_amd import linalg
+    b = utils.flatten(lin.objective())
+    A = lambda v: utils.flatten(lin.jvp(utils.unflatten(v, n)))
+    inner = 64
+    Q = torch.zeros((inner + 1, b.numel()), dtype=torch.float64, device="cuda")
+    Hd = torch.zeros((inner, inner + 2), dtype=torch.float64, device="cuda")
+    kstep = [0]
+
+    def arnoldi_step():
+        k = kstep[0] % inner
+        if k == 0:
+            Q.zero_()
+            Q[0] = b / torch.linalg.vector_norm(b)
+        w = A(Q[k]).contiguous().clone()
+        linalg._mgs(Q, k, w, Hd[k])
+        kstep[0] += 1
+
+    for _ in range(args.warmup):
+        arnoldi_step()
+    lib = _lib.load()
+    ctx = mhd.context(pd)
     torch.cuda.synchronize()
+    # HIP events around the first tangent row pass of every 8th accepted step, inside the
+    # timed matvecs, on their stream (up to 1024 samples)
+    _lib.check(lib.ecs_replay_probe(ctx.handle, 1, 8, None, None), "ecs_replay_probe")
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        out = lin.jvp(tans[args.warmup + i])
+    for _ in range(args.steps):
+        arnoldi_step()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     value = args.steps / dt
+    probe_ms, probe_n = ctypes.c_double(), ctypes.c_int()
+    _lib.check(lib.ecs_replay_probe(ctx.handle, 0, 0, ctypes.byref(probe_ms), ctypes.byref(probe_n)),
+               "ecs_replay_probe")
+    # the isolated JVP (objective_RPO's Jacobian along a dict tangent), as the reference's
+    # benchmarking.m times it
+    tans = [tangent() for _ in range(4)]
+    lin.jvp(tans[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in tans:
+        lin.jvp(t)
+    torch.cuda.synchronize()
+    jvp_rate = len(tans) / (time.perf_counter() - t0)
 
     # Lawson RK4 steps/s at N=256
     from ecs_mhd_2d_amd.synthetic import spectrum
@@ -438,11 +512,13 @@ def main():
         torch.cuda.synchronize()
         rk4[B] = B * steps / (time.perf_counter() - t1)
 
-    # live roofline of the dominant kernel of the matvec (tangent row kernel, N=128, batch 1)
-    lib = _lib.load()
-    ctx = mhd.context(pd)
+    # live roofline of the dominant kernel of the matvec: the tangent row kernel of the
+    # adaptive tangent replay (N=128, batch 1), the instance the matvec launches, timed
+    # over 200 back-to-back launches (agrees with its rocprofv3 kernel-trace duration
+    # inside the matvec); the event pairs sampled inside the timed matvecs above also
+    # hold the kernel's dispatch latency and the marker packets, an upper bound
     ms = ctypes.c_double()
-    _lib.check(lib.ecs_time_kernel(ctx.handle, 1, 3, 200, ctypes.byref(ms), _runtime._stream()),
+    _lib.check(lib.ecs_time_kernel(ctx.handle, 1, 4, 200, ctypes.byref(ms), _runtime._stream()),
                "ecs_time_kernel")
     row_t_ms = ms.value
     bytes_row_t = 20 * zbytes(n)   # reads 16 Z (8 primal + 8 tangent derived fields), writes 4 Z
@@ -487,15 +563,20 @@ def main():
         "metric": METRIC, "value": value, "unit": "matvecs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "GMRES matvec (JVP of objective_RPO through the adaptive Lawson RK4(3) "
-                               "trajectory) on one RPO guess, N=128, T=4.3, sx=0.1, atol=1e-8, Krylov dim 64",
+        "jvp_matvecs_per_s": jvp_rate,
+        "config": {"workload": "GMRES matvec: one Arnoldi step of GMRES (JVP of objective_RPO through the "
+                               "adaptive Lawson RK4(3) trajectory + Gram-Schmidt sweep) on one RPO guess, "
+                               "N=128, T=4.3, sx=0.1, atol=1e-8, Krylov dim 64",
                    "n": 128, "attempts": info["attempts"], "accepted": info["accepted"],
                    "rejected": info["rejected"], "parallelism": "single GPU"},
         "rk4_steps_per_s_n256_b1": rk4[1], "rk4_steps_per_s_n256_b64": rk4[64],
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC_ROW_T_N128_B1,
-                     "kernel": f"row_kernel<128,TX={tiles['n128_b1']['row_tan_tx']},tangent> (batch 1)",
+                     "kernel": f"row_kernel<128,TX={tiles['n128_b1']['row_tan_tx']},tangent,replay> (batch 1)",
                      "avg_ms": row_t_ms,
+                     "timing": "HIP events on the launching stream around 200 back-to-back launches of the "
+                               "replay instance",
+                     "avg_ms_in_matvec_with_dispatch": probe_ms.value, "in_matvec_samples": probe_n.value,
                      "algorithmic_bytes": bytes_row_t},
         "roofline_batched": {"bound": "hbm", "achieved": ach_b, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": ach_b / HBM_PEAK_GBS, "traffic": TRAFFIC_ROW_N256_B64,

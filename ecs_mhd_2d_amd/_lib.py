@@ -50,6 +50,7 @@ _SIGS = {
     "ecs_lawson_rk43_jvp": (c_int, [c_void_p, c_void_p, c_void_p, c_int, P_double, P_double, P_double, P_int8,
                                     P_int, c_int, c_void_p]),
     "ecs_time_kernel": (c_int, [c_void_p, c_int, c_int, c_int, P_double, c_void_p]),
+    "ecs_replay_probe": (c_int, [c_void_p, c_int, c_int, P_double, P_int]),
     "ecs_rk4": (c_int, [c_void_p, c_void_p, c_int, P_double, c_int, c_void_p]),
     "ecs_rk4_jvp": (c_int, [c_void_p, c_void_p, c_void_p, c_int, P_double, P_double, c_int, c_void_p]),
     "ecs_lawson_rk6": (c_int, [c_void_p, c_void_p, c_int, P_double, c_int, c_int, c_void_p]),

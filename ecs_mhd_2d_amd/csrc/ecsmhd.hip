@@ -320,6 +320,8 @@ int ecs_ctx_destroy(ecs_ctx* c) {
   if (c->nstep) (void)hipFree(c->nstep);
   if (c->hnstep) (void)hipHostFree(c->hnstep);
   if (c->ev_nstep) (void)hipEventDestroy(c->ev_nstep);
+  for (hipEvent_t e : c->probe_ev)
+    if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->ev_scal)
     if (e) (void)hipEventDestroy(e);
   delete c;
@@ -632,10 +634,38 @@ extern "C" int ecs_lawson_rk4_vjp_ckpt(ecs_ctx* c, const double* ck, double* fba
 }
 
 extern "C" int ecs_time_kernel(ecs_ctx* c, int batch, int which, int reps, double* ms, void* stream) {
-  if (!c || !ms || batch < 1 || reps < 1 || which < 0 || which > 3) return fail(1, "ecs_time_kernel: bad arguments");
+  if (!c || !ms || batch < 1 || reps < 1 || which < 0 || which > 4) return fail(1, "ecs_time_kernel: bad arguments");
   GUARD(c);
   hipStream_t st = (hipStream_t)stream;
   return OPS(c->n)->time_kernel(c, batch, which, reps, ms, st);
+}
+
+extern "C" int ecs_replay_probe(ecs_ctx* c, int enable, int every, double* avg_ms, int* samples) {
+  if (!c || (enable && every < 1)) return fail(1, "ecs_replay_probe: bad arguments");
+  GUARD(c);
+  constexpr size_t kPairs = 1024;
+  if (enable) {
+    if (c->probe_ev.empty()) {
+      c->probe_ev.resize(2 * kPairs, nullptr);
+      for (auto& e : c->probe_ev) HIPCHK(hipEventCreate(&e));
+    }
+    c->probe_every = every;
+    c->probe_used = 0;
+    return 0;
+  }
+  double sum = 0.0;
+  const size_t used = c->probe_used;
+  if (used) HIPCHK(hipEventSynchronize(c->probe_ev[used - 1]));
+  for (size_t i = 0; i + 1 < used; i += 2) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, c->probe_ev[i], c->probe_ev[i + 1]));
+    sum += t;
+  }
+  if (avg_ms) *avg_ms = used ? sum / (double)(used / 2) : 0.0;
+  if (samples) *samples = (int)(used / 2);
+  c->probe_every = 0;
+  c->probe_used = 0;
+  return 0;
 }
 
 extern "C" int ecs_tuned_tiles(int n, int batch, int* out) {

@@ -611,12 +611,15 @@ static int run_rk43_jvp(ecs_ctx* c, double* x, double* dx, int B, const double* 
   a.adaptive = AD_TANGENT;
   // one accepted step: [A] row S2 row S3 row S4 row S5; eagerly, S5 of step m
   // is fused with the first kernel (A) of step m+1 (C_S5A)
-  auto step = [&](const Args& am, hipStream_t s2, bool first, bool with_a, bool fuse_next) -> int {
+  auto step = [&](const Args& am, hipStream_t s2, bool first, bool with_a, bool fuse_next,
+                  hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) -> int {
     if (with_a) {
       if (first) TRY((col<N, C_A0, true>(am, B, s2)));
       else TRY((col<N, C_A, true>(am, B, s2)));
     }
+    if (e0) HIPCHK(hipEventRecord(e0, s2));
     TRY((row<N, R_RHS, true>(am, B, s2)));
+    if (e1) HIPCHK(hipEventRecord(e1, s2));
     TRY((col<N, C_S2, true>(am, B, s2)));
     TRY((row<N, R_RHS, true>(am, B, s2)));
     TRY((col<N, C_S3, true>(am, B, s2)));
@@ -654,7 +657,13 @@ static int run_rk43_jvp(ecs_ctx* c, double* x, double* dx, int B, const double* 
       am.attempt = m;
       am.run_all = m < minacc;
       am.next_all = m + 1 < minacc;
-      TRY(step(am, st, m == 0, m == 0, m + 1 < maxacc));
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (c->probe_every > 0 && m % c->probe_every == 0 && c->probe_used + 2 <= c->probe_ev.size()) {
+        e0 = c->probe_ev[c->probe_used];
+        e1 = c->probe_ev[c->probe_used + 1];
+        c->probe_used += 2;
+      }
+      TRY(step(am, st, m == 0, m == 0, m + 1 < maxacc, e0, e1));
     }
   }
   TRY(unpack<N>(c->x0, x, B, 0, st));
@@ -900,7 +909,8 @@ static int run_rk4_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
 // events on `stream` over `reps` back-to-back launches of the same kernel on
 // batch spectra already resident in the context's workspace.
 // which: 0 = column kernel (stage 2 glue), 1 = row kernel, 2 = tangent column
-// kernel, 3 = tangent row kernel.
+// kernel, 3 = tangent row kernel (fixed-step instance), 4 = tangent row kernel
+// of the adaptive tangent replay (the instance a GMRES matvec runs).
 template <int N>
 static int run_time_kernel(ecs_ctx* c, int B, int which, int reps, double* ms, hipStream_t st) {
   TRY(ensure_cap(c, B));
@@ -919,12 +929,19 @@ static int run_time_kernel(ecs_ctx* c, int B, int which, int reps, double* ms, h
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
+  // the tangent row pass as the tangent replay of an adaptive schedule runs it
+  // (stepper kind AD_TANGENT, every candidate active: no schedule is read)
+  Args ar = a;
+  ar.adaptive = AD_TANGENT;
+  ar.run_all = 1;
+  ar.attempt = 0;
   auto launch = [&]() -> int {
     switch (which) {
       case 0: return col<N, C_S2, false>(a, B, st);
       case 1: return row<N, R_RHS, false>(a, B, st);
       case 2: return col<N, C_S2, true>(a, B, st);
-      default: return row<N, R_RHS, true>(a, B, st);
+      case 3: return row<N, R_RHS, true>(a, B, st);
+      default: return row<N, R_RHS, true>(ar, B, st);
     }
   };
   for (int i = 0; i < 3; ++i) TRY(launch());

@@ -132,6 +132,11 @@ struct ecs_ctx {
   double* red = nullptr;       // reductions [cap][64]
   double* hred = nullptr;      // pinned [cap]
   double2* noalloc = nullptr;  // device address of the library's static scratch on this context's GPU
+  // measurement hook: event pairs around the first tangent row pass of every
+  // probe_every-th accepted step of the tangent replay (ecs_replay_probe)
+  int probe_every = 0;
+  std::vector<hipEvent_t> probe_ev;
+  size_t probe_used = 0;
   size_t compact_elems() const {
     int kmax = (n - 1) / 3;
     return (size_t)2 * (2 * kmax + 1) * (kmax + 1);

@@ -205,8 +205,18 @@ int ecs_recurrence(const double* fs, int nsnap, long long modes, double* dist, v
  * code): average duration in ms of one launch of a fused RHS kernel over
  * `reps` back-to-back launches on `batch` resident spectra, timed with HIP
  * events on `stream`.  which: 0 column kernel, 1 row kernel, 2 tangent
- * column kernel, 3 tangent row kernel.  Synchronous. */
+ * column kernel, 3 tangent row kernel, 4 tangent row kernel of the adaptive
+ * tangent replay (the instance of a GMRES matvec).  Synchronous. */
 int ecs_time_kernel(ecs_ctx* ctx, int batch, int which, int reps, double* ms, void* stream);
+
+/* Measurement hook (no counterpart in the reference): with enable = 1, the
+ * following ecs_lawson_rk43_jvp calls on ctx record HIP events on their
+ * stream around the first tangent row pass of every `every`-th accepted step
+ * (up to 1024 samples); enable = 0 stops recording, waits for the last event
+ * and returns the average duration in ms of the sampled launches, as they
+ * run inside the replay (after a column kernel, caches as the replay leaves
+ * them), and the number of samples. */
+int ecs_replay_probe(ecs_ctx* ctx, int enable, int every, double* avg_ms, int* samples);
 
 /* Tile choice of the autotuner for grid size n and this batch bucket (no
  * reference counterpart): out[0] columns per column-kernel block, out[1] the

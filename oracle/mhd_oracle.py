@@ -35,6 +35,31 @@ from __future__ import annotations
 
 import numpy as np
 
+# 2D real transforms: numpy's pocketfft (the default), or scipy.fft's with
+# several worker threads for the CPU baseline timing (set_fft_workers).
+_FFT = {"workers": 1}
+
+
+def _rfft2(x, s=None):
+    if _FFT["workers"] > 1:
+        import scipy.fft
+        return scipy.fft.rfft2(x, s=s, workers=_FFT["workers"])
+    return np.fft.rfft2(x, s=s)
+
+
+def _irfft2(x, s=None):
+    if _FFT["workers"] > 1:
+        import scipy.fft
+        return scipy.fft.irfft2(x, s=s, workers=_FFT["workers"])
+    return np.fft.irfft2(x, s=s)
+
+
+def set_fft_workers(k: int) -> None:
+    """Route the oracle's 2D transforms through scipy.fft with k threads (k > 1)
+    or numpy.fft (k = 1).  Same transforms; only the CPU timing changes."""
+    _FFT["workers"] = max(1, int(k))
+
+
 # mhd_jax.py:47 hard-codes pi as this literal; symmetry.py:11 uses jnp.pi.
 # Both round to the same double.
 PI_LITERAL = 3.14159265358979323846
@@ -102,10 +127,10 @@ def _real_fields(f: np.ndarray, pd: dict):
     Returns fx, fy, fu, fv, each [..., 2, N, N]; fu/fv of the current j get
     the mean field b0 added (:145-149).
     """
-    fx = np.fft.irfft2(1j * pd["kx"] * f)
-    fy = np.fft.irfft2(1j * pd["ky"] * f)
-    fu = np.fft.irfft2(1j * pd["to_u"] * f)
-    fv = np.fft.irfft2(1j * pd["to_v"] * f)
+    fx = _irfft2(1j * pd["kx"] * f)
+    fy = _irfft2(1j * pd["ky"] * f)
+    fu = _irfft2(1j * pd["to_u"] * f)
+    fv = _irfft2(1j * pd["to_v"] * f)
     return fx, fy, fu, fv
 
 
@@ -119,9 +144,9 @@ def state_vel(f: np.ndarray, pd: dict, include_dissipation: bool) -> np.ndarray:
     adv = fu * fx + fv * fy
     k_sq = (kx * kx + ky * ky) * mask
     dwdt = -adv[..., 0, :, :] + adv[..., 1, :, :] + pd["forcing"]
-    dwdt = mask * np.fft.rfft2(dwdt)
+    dwdt = mask * _rfft2(dwdt)
     djdt = fu[..., 0, :, :] * fv[..., 1, :, :] - fv[..., 0, :, :] * fu[..., 1, :, :]
-    djdt = k_sq * np.fft.rfft2(djdt)
+    djdt = k_sq * _rfft2(djdt)
     out = np.stack([dwdt, djdt], axis=-3)
     if include_dissipation:
         out = out + dissipation(pd) * f
@@ -143,8 +168,8 @@ def state_vel_jvp(f, df, pd, include_dissipation=False):
     tx, ty, tu, tv = _real_fields(df, pd)
     dadv = tu * fx + fu * tx + tv * fy + fv * ty
     k_sq = (kx * kx + ky * ky) * mask
-    dw = mask * np.fft.rfft2(-dadv[..., 0, :, :] + dadv[..., 1, :, :])
-    dj = k_sq * np.fft.rfft2(tu[..., 0, :, :] * fv[..., 1, :, :] + fu[..., 0, :, :] * tv[..., 1, :, :]
+    dw = mask * _rfft2(-dadv[..., 0, :, :] + dadv[..., 1, :, :])
+    dj = k_sq * _rfft2(tu[..., 0, :, :] * fv[..., 1, :, :] + fu[..., 0, :, :] * tv[..., 1, :, :]
                              - tv[..., 0, :, :] * fu[..., 1, :, :] - fv[..., 0, :, :] * tu[..., 1, :, :])
     out = np.stack([dw, dj], axis=-3)
     if include_dissipation:
@@ -168,7 +193,7 @@ def irfft2_T(g: np.ndarray) -> np.ndarray:
     (w'/N^2) * rfft2(g).
     """
     n = g.shape[-1]
-    return _col_weight(n, 2) / (n * n) * np.fft.rfft2(g)
+    return _col_weight(n, 2) / (n * n) * _rfft2(g)
 
 
 def rfft2_T(c: np.ndarray) -> np.ndarray:
@@ -177,7 +202,7 @@ def rfft2_T(c: np.ndarray) -> np.ndarray:
     w = np.full((1, n // 2 + 1), 0.5)
     w[0, 0] = 1.0
     w[0, -1] = 1.0
-    return n * n * np.fft.irfft2(w * c, s=(n, n))
+    return n * n * _irfft2(w * c, s=(n, n))
 
 
 def state_vel_vjp(f, g, pd, include_dissipation=False):
@@ -399,7 +424,7 @@ def lawson_rk43(x, pd, t, h=1e-2, atol=1e-4, max_steps_per_checkpoint=1024,
         xf = xx + h * (k1 / 6 + k2 / 3 + k3 / 3 + k4 / 6)
         k5 = v(xf)
         fevals += 4
-        err = np.max(np.abs(np.fft.irfft2(h * (k5 - k4) / 6)))
+        err = np.max(np.abs(_irfft2(h * (k5 - k4) / 6)))
         hs[acc + rej] = h
         accepted = bool(err < atol)
         schedule.append((h, accepted))
@@ -760,20 +785,20 @@ def create_state_from_turb(fs, idx, pd, dt, ministeps, nx=64):
     f1, f2 = fs[idx[0] - 1], fs[idx[1] - 1]
     g = find_optimal_symmetry(f1, f2, pd, nx)
     T = dt * ministeps * (idx[1] - idx[0])
-    return {"fields": np.fft.irfft2(f1), "T": T, "sx": g[0]}, int(ministeps * (idx[1] - idx[0])), g[1], g[2]
+    return {"fields": _irfft2(f1), "T": T, "sx": g[0]}, int(ministeps * (idx[1] - idx[0])), g[1], g[2]
 
 
 def floquet_block_power(inp, pd, tang, maxit, mode="TDRK4"):
     """Block power iteration of floquet.py:42-99 with the oracle's tangents:
     J t = irfft2(mask * g dPhi_T rfft2(t)) at fixed T and sx; numpy (LAPACK
     Householder) QR after each application; R = Q J(Q)^T."""
-    f0 = np.fft.rfft2(inp["fields"])
+    f0 = _rfft2(inp["fields"])
     T, sx = float(inp["T"]), float(inp["sx"])
 
     def jac(t):
         out = []
         for b in range(t.shape[0]):
-            df0 = np.fft.rfft2(t[b])
+            df0 = _rfft2(t[b])
             if mode in ("RK4", "TDRK4", "Lawson_RK4"):
                 nc, ms = int(pd["num_checkpoints"]), int(pd["ministeps"])
                 step = {"RK4": lambda f, d, a, c: rk4_jvp(f, d, a, c, ms, pd),
@@ -785,7 +810,7 @@ def floquet_block_power(inp, pd, tang, maxit, mode="TDRK4"):
             else:
                 raise ValueError(mode)
             _, dg = apply_symmetry_jvp(f, df, sx, 0.0, pd)
-            out.append(np.fft.irfft2(dg * pd["mask"]))
+            out.append(_irfft2(dg * pd["mask"]))
         return np.stack(out)
 
     B = tang.shape[0]
@@ -802,7 +827,7 @@ def floquet_block_power(inp, pd, tang, maxit, mode="TDRK4"):
 # --------------------------------------------------------------------------
 def mismatch_RPO(inp, pd, mode):
     """rfft2 of fields (unmasked, :45), integrate by mode (:48-88), apply symmetries (:91-94)."""
-    f0 = np.fft.rfft2(inp["fields"])
+    f0 = _rfft2(inp["fields"])
     T = float(inp["T"])
     if mode == "Lawson_RK4":
         nc, ms = int(pd["num_checkpoints"]), int(pd["ministeps"])
@@ -834,14 +859,14 @@ def mismatch_RPO(inp, pd, mode):
 def loss_RPO(inp, pd, mode):
     """mean(irfft2(f - f0)^2) (loss_functions.py:99-111)."""
     f0, f, info = mismatch_RPO(inp, pd, mode)
-    diff = np.fft.irfft2(f - f0)
+    diff = _irfft2(f - f0)
     return float(np.mean(np.square(diff))), info
 
 
 def phase_vectors(f0, pd):
     """Stop-gradient phase-condition vectors vt, vx in real space (loss_functions.py:113-122)."""
-    vt = np.fft.irfft2(state_vel(f0, pd, True))
-    vx = np.fft.irfft2(1j * pd["kx"] * f0)
+    vt = _irfft2(state_vel(f0, pd, True))
+    vx = _irfft2(1j * pd["kx"] * f0)
     return vt, vx
 
 
@@ -851,7 +876,7 @@ def objective_RPO(inp, pd, mode):
     pt and px are x - stop_gradient(x), i.e. exactly 0 in value (:127-128).
     """
     f0, f, info = mismatch_RPO(inp, pd, mode)
-    return {"fields": np.fft.irfft2(f0 - f), "T": 0.0, "sx": 0.0}, info
+    return {"fields": _irfft2(f0 - f), "T": 0.0, "sx": 0.0}, info
 
 
 def objective_RPO_jvp(inp, tan, pd, mode, schedule=None):
@@ -860,8 +885,8 @@ def objective_RPO_jvp(inp, tan, pd, mode, schedule=None):
     The phase conditions differentiate to mean(vt * dfields), mean(vx * dfields)
     (loss_functions.py:115-128; vt, vx are stop-gradient).
     """
-    f0 = np.fft.rfft2(inp["fields"])
-    df0 = np.fft.rfft2(tan["fields"])
+    f0 = _rfft2(inp["fields"])
+    df0 = _rfft2(tan["fields"])
     T, dT = float(inp["T"]), float(tan["T"])
     if mode == "Lawson_RK4":
         nc, ms = int(pd["num_checkpoints"]), int(pd["ministeps"])
@@ -885,7 +910,7 @@ def objective_RPO_jvp(inp, tan, pd, mode, schedule=None):
         raise ValueError(mode)
     _, dg = apply_symmetry_jvp(f, df, float(inp["sx"]), float(tan["sx"]), pd)
     vt, vx = phase_vectors(f0, pd)
-    return {"fields": np.fft.irfft2(df0 - dg),
+    return {"fields": _irfft2(df0 - dg),
             "T": float(np.mean(vt * tan["fields"])),
             "sx": float(np.mean(vx * tan["fields"]))}
 
@@ -898,13 +923,13 @@ def loss_RPO_value_and_grad(inp, pd):
     n = inp["fields"].shape[-1]
     nc, ms = int(pd["num_checkpoints"]), int(pd["ministeps"])
     T, sx = float(inp["T"]), float(inp["sx"])
-    f0 = np.fft.rfft2(inp["fields"])
+    f0 = _rfft2(inp["fields"])
     segs = [f0]
     for _ in range(nc):
         segs.append(lawson_rk4(segs[-1], T / nc, ms, pd, True))
     fT = segs[-1]
     g = apply_symmetry(fT, sx, pd)
-    diff = np.fft.irfft2(g - f0)
+    diff = _irfft2(g - f0)
     loss = float(np.mean(np.square(diff)))
     dbar = irfft2_T(diff / (n * n))        # d loss / d (g - f0)
     fTbar, sxbar = apply_symmetry_vjp(fT, sx, pd, dbar)
@@ -923,7 +948,7 @@ def objective_RPO_vjp(inp, cot, pd):
     n = inp["fields"].shape[-1]
     nc, ms = int(pd["num_checkpoints"]), int(pd["ministeps"])
     T, sx = float(inp["T"]), float(inp["sx"])
-    f0 = np.fft.rfft2(inp["fields"])
+    f0 = _rfft2(inp["fields"])
     segs = [f0]
     for _ in range(nc):
         segs.append(lawson_rk4(segs[-1], T / nc, ms, pd, True))
@@ -1100,7 +1125,7 @@ def adam_update(params, grads, m, v, t, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-
 
 def dealias_fields(fields, pd):
     """irfft2(mask * rfft2(fields)) (newton.py:211-214, adjoint_descent.py:111-114)."""
-    return np.fft.irfft2(pd["mask"] * np.fft.rfft2(fields))
+    return _irfft2(pd["mask"] * _rfft2(fields))
 
 
 # --------------------------------------------------------------------------
