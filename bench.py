@@ -329,41 +329,36 @@ def batched_leg():
             "bitwise_equal_to_unbatched": bool(same)}
 
 
-def serving_leg():
-    """Poisson arrivals (4 req/s, rng seed 7) of mixed jobs, 70% adjoint descent /
-    30% Newton-GMRES, admitted against a 200 GB HBM budget; reduced in size so the
-    leg runs in seconds: N in {32, 64}, T ~ U[0.5, 1.5], 8 descent steps, 2 Newton
-    iterations with Krylov dimension 16, 24 jobs."""
-    from ecs_mhd_2d_amd import serve
-    w = serve.make_workload(4, rate=100.0, seed=1, sizes=(32, 64), T_range=(0.3, 0.4), descent_iters=1,
-                            newton_iters=1, inner=2, atol=1e-6, adaptive_budget=(64, 16))
-    serve.Server().run(w)
-    wl = serve.make_workload(24, rate=4.0, seed=7, sizes=(32, 64), T_range=(0.5, 1.5), descent_iters=8,
-                             newton_iters=2, inner=16, atol=1e-8, adaptive_budget=(64, 32))
-    _, stats = serve.Server().run(wl)
-    stats["workload"] = ("24 jobs, Poisson 4/s, 70% adjoint descent (8 ADAM steps, dt 0.004) / 30% Newton "
-                         "(2 iterations, Krylov 16, RK4(3) atol 1e-8), N in {32,64}, T~U[0.5,1.5]")
-    return stats
-
-
-def serving_full_leg(jobs):
-    """The serving workload at its stated size: Poisson arrivals at 4 req/s (rng
-    seed 7), 70% adjoint descent (64 ADAM steps, Lawson RK4 with steps =
+def serving_leg(jobs, dump_dir=None, log_to_stderr=False):
+    """The serving workload at its stated per-job size: Poisson arrivals at 4 req/s
+    (rng seed 7), 70% adjoint descent (64 ADAM steps, Lawson RK4 with steps =
     round(T/0.004) rounded up to a multiple of 64) / 30% Newton-GMRES (Krylov 64,
-    5 iterations, RK4(3) atol 1e-8), N in {128, 256}, T ~ U[2, 8], 200 GB budget."""
+    5 iterations, RK4(3) atol 1e-8), N in {128, 256}, T ~ U[2, 8], 200 GB budget;
+    the first ``jobs`` arrivals of that stream.  Beside the served throughput,
+    the unbatched GPU path for the same jobs (each job's operations one at a time,
+    alone on the GPU) estimated by timing each job's first operation of each kind
+    alone, scaled by its stepper steps (serve.unbatched_estimate)."""
     from ecs_mhd_2d_amd import serve
     w = serve.make_workload(4, rate=100.0, seed=1, sizes=(128, 256), T_range=(0.3, 0.4), descent_iters=1,
                             newton_iters=1, inner=2, atol=1e-6, adaptive_budget=(64, 16))
     serve.Server().run(w)   # warm-up: contexts, workspaces, autotuner
     wl = serve.make_workload(jobs, rate=4.0, seed=7, sizes=(128, 256), T_range=(2.0, 8.0), descent_iters=64,
-                             newton_iters=5, inner=64, atol=1e-8, adaptive_budget=(64, 64))
-    log = lambda m: print(m, file=sys.stderr, flush=True)
-    done, stats = serve.Server().run(wl, log=log)
+                             newton_iters=5, inner=64, atol=1e-8, adaptive_budget=(64, 64), dump_dir=dump_dir)
+    log = (lambda m: print(m, file=sys.stderr, flush=True)) if log_to_stderr else None
+    srv = serve.Server(keep_samples=True)
+    done, stats = srv.run(wl, log=log)
     kinds = [j.kind for j in done]
-    stats["workload"] = (f"{jobs} jobs, Poisson 4/s, 70% adjoint descent (64 ADAM steps, dt 0.004) / 30% Newton "
-                         "(5 iterations, Krylov 64, RK4(3) atol 1e-8), N in {128,256}, T~U[2,8], 200 GB budget")
+    ub_s, ub_steps = serve.unbatched_estimate(srv, done)
+    stats["workload"] = (f"first {jobs} arrivals: Poisson 4/s, 70% adjoint descent (64 ADAM steps, dt 0.004) / "
+                         "30% Newton (5 iterations, Krylov 64, RK4(3) atol 1e-8), N in {128,256}, T~U[2,8], "
+                         "200 GB budget")
     stats["adjoint_jobs"] = kinds.count("adjoint")
     stats["newton_jobs"] = kinds.count("newton")
+    stats["jobs_detail"] = [{"id": j.id, "kind": j.kind, "n": j.n, "latency_s": (j.finish - j.arrival),
+                             "error": j.error} for j in done]
+    stats["unbatched_estimate_s"] = ub_s
+    stats["unbatched_rk_steps_per_s"] = ub_steps / ub_s if ub_s > 0 else None
+    stats["served_over_unbatched"] = (stats["rk_steps_per_s"] / (ub_steps / ub_s)) if ub_s > 0 else None
     return stats
 
 
@@ -404,6 +399,10 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the batched-candidates and serving legs")
     ap.add_argument("--serving-full", type=int, default=0, metavar="JOBS",
                     help="run only the serving workload at its full size with JOBS jobs and print its statistics")
+    ap.add_argument("--serving-jobs", type=int, default=0, metavar="JOBS",
+                    help="jobs of the serving leg of the default run (0: the leg is skipped)")
+    ap.add_argument("--dump-dir", default=None,
+                    help="where a failing Newton job of the serving workload writes its input")
     ap.add_argument("--probe-roofline", action="store_true",
                     help="launch only the matvec's dominant kernel (50x) and a 64 MiB copy, for rocprofv3 --pmc")
     ap.add_argument("--probe-roofline-batched", action="store_true",
@@ -430,7 +429,7 @@ def main():
         probe_roofline(batched=args.probe_roofline_batched, kernel=kernel)
         return
     if args.serving_full:
-        stats = serving_full_leg(args.serving_full)
+        stats = serving_leg(args.serving_full, dump_dir=args.dump_dir, log_to_stderr=True)
         print(json.dumps({"metric": "p50/p99 RPO solve latency (serving, full size)", "unit": "s",
                           "value": stats["p99_latency_s"], "higher_is_better": False, "n_gpus": 1,
                           "dtype": "f64", "data": "synthetic", "serving": stats}))
@@ -553,7 +552,8 @@ def main():
         extras["adjoint_gradient"] = gradient_leg()
         extras["guess_generation"] = guess_generation_leg()
         extras["batched_candidates"] = batched_leg()
-        extras["serving"] = serving_leg()
+        if args.serving_jobs:
+            extras["serving"] = serving_leg(args.serving_jobs)
 
     cpu = None
     if not args.no_cpu:

@@ -279,6 +279,9 @@ class Job:
         self.start = None
         self.finish = None
         self.in_flight = False
+        self.op_counts = {}   # kind -> operations run
+        self.op_steps = {}    # kind -> stepper steps of those operations
+        self.first_op = {}    # kind -> (operation, its steps): the sample timed alone afterwards
 
 
 def job_memory(kind, n, inner=64, steps=0, budget=4096):
@@ -346,13 +349,14 @@ class Server:
     """
 
     def __init__(self, hbm_budget=200e9, nu=1 / 40, eta=1 / 40, b0=(0.0, 0.1), max_batch=64, executor=None,
-                 max_groups=4, bypass_wait_s=30.0):
+                 max_groups=4, bypass_wait_s=30.0, keep_samples=False):
         self.executor = executor
         self.budget = hbm_budget
         self.nu, self.eta, self.b0 = nu, eta, b0
         self.max_batch = max_batch
         self.max_groups = max_groups
         self.bypass_wait_s = bypass_wait_s
+        self.keep_samples = keep_samples   # keep each job's first operation of each kind (unbatched_estimate)
         self._pds = {}
         self._streams = {}
         self.ticks = 0
@@ -521,7 +525,13 @@ class Server:
                         main.wait_event(ev)
                         _own(res, main)
                     for j, r in zip(part, res):
-                        self.rk_steps += op_rk_steps(j.pending, r)
+                        st = op_rk_steps(j.pending, r)
+                        self.rk_steps += st
+                        k = j.pending.kind
+                        j.op_counts[k] = j.op_counts.get(k, 0) + 1
+                        j.op_steps[k] = j.op_steps.get(k, 0) + st
+                        if self.keep_samples and k not in j.first_op:
+                            j.first_op[k] = (j.pending, st)
                         advance(j, r, now)
                 if log is not None and now - last_log >= 10.0:
                     last_log = now
@@ -539,6 +549,28 @@ class Server:
                       "ticks": self.ticks, "mean_batch": self.batched_ops / max(1, self.ticks),
                       "rk_steps": self.rk_steps, "rk_steps_per_s": self.rk_steps / wall,
                       "peak_resident_bytes": peak, "hbm_budget_bytes": self.budget}
+
+
+def unbatched_estimate(server, done):
+    """The unbatched GPU path for the same jobs: each job's operations run one
+    at a time, alone on the GPU.  Each job's first operation of each kind
+    (kept by a Server made with keep_samples=True) is run again alone and
+    timed, scaled by the stepper steps of all its operations of that kind.
+    Returns (seconds, stepper steps) summed over the jobs that finished."""
+    total_s, total_steps = 0.0, 0
+    for j in done:
+        if j.error is not None:
+            continue
+        for kind, (op, st) in j.first_op.items():
+            server._run_group(op.group_key(), [op], 0)   # warm: workspace of a batch of one
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            server._run_group(op.group_key(), [op], 0)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            total_s += dt * (j.op_steps[kind] / max(st, 1))
+            total_steps += j.op_steps[kind]
+    return total_s, total_steps
 
 
 def make_workload(count, rate=4.0, seed=7, sizes=(128, 256), T_range=(2.0, 8.0), adjoint_frac=0.7,
