@@ -27,6 +27,12 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The serving leg runs up to four groups at once, each on a stream of its own,
+# beside the caller's stream; with HIP's default of four hardware queues per
+# process two of them would share a queue and run one after the other.  Read
+# when HIP initialises (the first GPU call), so it is set before anything
+# touches the GPU.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the two roofline kernels, from rocprofv3 --pmc
@@ -333,7 +339,8 @@ def serving_leg(jobs, dump_dir=None, log_to_stderr=False):
     """The serving workload at its stated per-job size: Poisson arrivals at 4 req/s
     (rng seed 7), 70% adjoint descent (64 ADAM steps, Lawson RK4 with steps =
     round(T/0.004) rounded up to a multiple of 64) / 30% Newton-GMRES (Krylov 64,
-    5 iterations, RK4(3) atol 1e-8), N in {128, 256}, T ~ U[2, 8], 200 GB budget;
+    5 iterations, RK4(3) atol 1e-8 with up to 64 x 256 attempts), N in {128, 256},
+    T ~ U[2, 8], 200 GB budget;
     the first ``jobs`` arrivals of that stream.  Beside the served throughput,
     the unbatched GPU path for the same jobs (each job's operations one at a time,
     alone on the GPU) estimated by timing each job's first operation of each kind
@@ -343,7 +350,7 @@ def serving_leg(jobs, dump_dir=None, log_to_stderr=False):
                             newton_iters=1, inner=2, atol=1e-6, adaptive_budget=(64, 16))
     serve.Server().run(w)   # warm-up: contexts, workspaces, autotuner
     wl = serve.make_workload(jobs, rate=4.0, seed=7, sizes=(128, 256), T_range=(2.0, 8.0), descent_iters=64,
-                             newton_iters=5, inner=64, atol=1e-8, adaptive_budget=(64, 64), dump_dir=dump_dir)
+                             newton_iters=5, inner=64, atol=1e-8, adaptive_budget=(64, 256), dump_dir=dump_dir)
     log = (lambda m: print(m, file=sys.stderr, flush=True)) if log_to_stderr else None
     srv = serve.Server(keep_samples=True)
     done, stats = srv.run(wl, log=log)

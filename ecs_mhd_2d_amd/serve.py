@@ -49,6 +49,17 @@ class Op:
             return ("jvp", self.n, len(self.kw["lin"].info["hs"]))
         return (self.kind, self.n)
 
+    def length(self):
+        """Relative cost of the operation within its group key: a batch runs as
+        long as its longest member, so members of similar length batch best
+        (stepper steps; the period for adaptive solves, whose steps are not
+        known in advance)."""
+        if self.kind == "grad":
+            return float(self.kw["steps"])
+        if self.kind == "jvp":
+            return float(self.kw["lin"].info["accepted"])
+        return float(self.kw.get("T", 0.0))
+
 
 def _pd(n, nu, eta, b0):
     return mhd.make_param_dict(n, nu, eta, b0)
@@ -334,11 +345,17 @@ class Server:
 
     ``executor(key, ops, lane) -> results`` runs one group of compatible ops;
     the default runs them on the GPU through the fused kernels.  Up to
-    ``max_groups`` groups run at once, each on a context lane of its grid size
-    and that lane's stream; a group's jobs advance as soon as it finishes, so
-    a slow group delays only its own jobs.  A job whose operation matches a
-    group still running (same key, batch not full) waits for that group and
-    then rides with its jobs, which keeps jobs of one kind in phase.
+    ``max_groups`` groups run at once, each in a slot (lane) of its own: the
+    lane's context of the group's grid size and the lane's stream, so groups
+    in different lanes overlap on the device.  A group's jobs advance as soon
+    as it finishes, so a slow group delays only its own jobs.
+
+    Grouping: whenever a lane is free, the ready jobs are grouped by operation
+    key, each key's jobs sorted by operation length (a batch runs as long as
+    its longest member) and cut into batches of at most ``max_batch``; while
+    there are more free lanes than batches, the largest batch is halved (into
+    its shorter and its longer jobs), so every free lane runs a group.  The
+    oldest batches start first.
 
     Stream order: a job's generator code runs on the caller's current stream;
     a group's stream first waits for an event recorded on that stream after
@@ -385,7 +402,7 @@ class Server:
         """Runs in a worker thread; returns (results, completion event)."""
         if self.executor is not None:
             return self.executor(key, ops, lane), None
-        s = self._streams[(ops[0].n, lane)]
+        s = self._streams[lane]
         with torch.cuda.stream(s):
             s.wait_event(ready)
             out = self._run_group(key, ops, lane)
@@ -413,6 +430,27 @@ class Server:
             elif now - j.arrival > self.bypass_wait_s:
                 break
         return admitted, rejected
+
+    def _batches(self, ready, free):
+        """(key, jobs) batches for the ``free`` lanes, oldest first."""
+        if free <= 0 or not ready:
+            return []
+        groups = {}
+        for j in ready:
+            groups.setdefault(j.pending.group_key(), []).append(j)
+        parts = []
+        for key, js in groups.items():
+            js = sorted(js, key=lambda j: (j.pending.length(), j.arrival))
+            parts += [(key, js[i:i + self.max_batch]) for i in range(0, len(js), self.max_batch)]
+        while len(parts) < free:
+            big = max(parts, key=lambda p: len(p[1]))
+            if len(big[1]) < 2:
+                break
+            parts.remove(big)
+            h = len(big[1]) // 2
+            parts += [(big[0], big[1][:h]), (big[0], big[1][h:])]
+        parts.sort(key=lambda p: min(j.arrival for j in p[1]))
+        return parts[:free]
 
     def run(self, arrivals, concurrent=True, log=None):
         """arrivals: list of (t_arrival_seconds, kind, make_generator, mem_bytes, n).
@@ -467,38 +505,24 @@ class Server:
                     peak = max(peak, used)
                     active.append(j)
                     advance(j, None, now, first=True)
-                # form groups from the jobs whose next operation is not running, oldest job first
-                ready = [j for j in active if not j.in_flight]
-                groups = {}
-                for j in sorted(ready, key=lambda j: j.arrival):
-                    groups.setdefault(j.pending.group_key(), []).append(j)
-                for key, js in groups.items():
-                    # a group of this key still running with room to spare: its jobs
-                    # come back soon, and these wait to ride with them (batches stay
-                    # whole instead of splitting into out-of-phase singletons)
-                    if any(k == key and len(p) < self.max_batch for k, p, _ in inflight.values()):
-                        continue
-                    for i in range(0, len(js), self.max_batch):
-                        if len(inflight) >= slots:
-                            break
-                        part = js[i:i + self.max_batch]
-                        n = part[0].n
-                        lane = 0
-                        while (n, lane) in busy:
-                            lane += 1
-                        busy.add((n, lane))
-                        ev = None
-                        if self.executor is None:
-                            if (n, lane) not in self._streams:
-                                self._streams[(n, lane)] = torch.cuda.Stream()
-                            ev = torch.cuda.Event()
-                            ev.record(main)
-                        for j in part:
-                            j.in_flight = True
-                        fut = pool.submit(self._work, key, [j.pending for j in part], lane, ev)
-                        inflight[fut] = (key, part, (n, lane))
-                        self.ticks += 1
-                        self.batched_ops += len(part)
+                # form groups from the jobs whose next operation is not running
+                for key, part in self._batches([j for j in active if not j.in_flight], slots - len(inflight)):
+                    lane = 0
+                    while lane in busy:
+                        lane += 1
+                    busy.add(lane)
+                    ev = None
+                    if self.executor is None:
+                        if lane not in self._streams:
+                            self._streams[lane] = torch.cuda.Stream()
+                        ev = torch.cuda.Event()
+                        ev.record(main)
+                    for j in part:
+                        j.in_flight = True
+                    fut = pool.submit(self._work, key, [j.pending for j in part], lane, ev)
+                    inflight[fut] = (key, part, lane)
+                    self.ticks += 1
+                    self.batched_ops += len(part)
                 # the next arrival still in the future (an arrived job that does not fit
                 # waits for a group to finish, which frees no memory by itself but ends jobs)
                 now = time.perf_counter() - t0
@@ -574,9 +598,15 @@ def unbatched_estimate(server, done):
 
 
 def make_workload(count, rate=4.0, seed=7, sizes=(128, 256), T_range=(2.0, 8.0), adjoint_frac=0.7,
-                  descent_iters=64, newton_iters=5, inner=64, dt=0.004, atol=1e-8, adaptive_budget=(64, 64),
+                  descent_iters=64, newton_iters=5, inner=64, dt=0.004, atol=1e-8, adaptive_budget=(64, 256),
                   dump_dir=None):
-    """Poisson arrivals at `rate` per second of mixed jobs (the serving workload)."""
+    """Poisson arrivals at `rate` per second of mixed jobs (the serving workload).
+
+    ``adaptive_budget`` = (max_steps_per_checkpoint, checkpoints) of the Newton
+    jobs' RK4(3) solves.  The seeded N=256 states take steps of ~1e-4 T at
+    atol 1e-8, so a period up to 8 needs up to ~9000 attempts: the default
+    budget of 16384 attempts completes them, where the reference's 128 x 32 =
+    4096 (newton.py:75-79) stops early."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -594,6 +624,6 @@ def make_workload(count, rate=4.0, seed=7, sizes=(128, 256), T_range=(2.0, 8.0),
         else:
             gen = (lambda f=f, T=T, n=n, i=i: newton_job(f, T, 0.0, n, adaptive, inner, newton_iters,
                                                          dump_dir=dump_dir, tag=f"job{i}"))
-            mem = job_memory("newton", n, inner=inner)
+            mem = job_memory("newton", n, inner=inner, budget=adaptive_budget[0] * adaptive_budget[1])
         out.append((t, kind, gen, mem, n))
     return out

@@ -45,10 +45,12 @@ def test_results_and_batching():
     res = {j.id: j.result for j in done}
     for i in range(6):
         assert res[i] == 3 * 2.0 * i
-    # all six jobs arrive at t=0: one group per grid size, each advancing on its own
-    assert all(c[1] == 3 for c in ex.calls)
+    # all six jobs arrive at t=0: two keys (one per grid size), their batches
+    # halved until the four lanes are busy
     assert {c[0] for c in ex.calls} == {("grad", 32), ("grad", 64)}
-    assert stats["mean_batch"] == 3.0 and stats["ticks"] == 6
+    assert sorted(c[1] for c in ex.calls[:4]) == [1, 1, 2, 2]
+    assert sorted(c[2] for c in ex.calls[:4]) == [0, 1, 2, 3]
+    assert stats["mean_batch"] > 1.0
 
 
 def test_gradient_jobs_of_different_segment_counts_batch_together():
@@ -56,7 +58,7 @@ def test_gradient_jobs_of_different_segment_counts_batch_together():
     candidate runs its own count); other counts batch only with equal ones."""
     ex = Exec()
     arr = [(0.0, "adjoint", (lambda k=k: toy_job(32, 2, 1.0, traj=k)), 1.0, 32) for k in (64, 128, 192, 100, 100, 90)]
-    serve.Server(executor=ex).run(arr)
+    serve.Server(executor=ex, max_groups=3).run(arr)   # three keys, three lanes: no batch is halved
     keys = sorted({(c[0], c[1]) for c in ex.calls}, key=str)
     assert keys == sorted({(("grad", 32), 3), (("grad", 32, 100), 2), (("grad", 32, 90), 1)}, key=str)
 
@@ -81,7 +83,7 @@ def test_latency_percentiles_follow_arrivals():
 
 def test_max_batch_splits_groups_into_lanes():
     ex = Exec()
-    srv = serve.Server(executor=ex, max_batch=2)
+    srv = serve.Server(executor=ex, max_batch=2, max_groups=3)
     done, stats = srv.run([(0.0, "adjoint", (lambda i=i: toy_job(64, 1, float(i))), 1.0, 64) for i in range(5)])
     assert stats["jobs"] == 5
     sizes = sorted(c[1] for c in ex.calls)
@@ -182,13 +184,32 @@ def test_group_slots_bound_concurrency():
     assert stats["failed_jobs"] == 0 and peak[0] == 2
 
 
-@pytest.mark.timeout(60)
-def test_a_late_job_joins_the_running_group_of_its_kind():
-    """A job arriving while a group of its kind is running waits for it and
-    then rides in the same batch, instead of running beside it alone."""
-    ex = SlowExec({64: 0.05})
-    arr = [(0.0, "adjoint", (lambda: toy_job(64, 4, 1.0)), 1.0, 64),
-           (0.02, "adjoint", (lambda: toy_job(64, 3, 2.0)), 1.0, 64)]
-    done, stats = serve.Server(executor=ex).run(arr)
+class StepsExec(Exec):
+    def __call__(self, key, ops, lane):
+        with self.lock:
+            self.calls.append((key, sorted(o.kw["steps"] for o in ops), lane))
+        return [2.0 * o.kw["x"] for o in ops]
+
+
+def test_batches_keep_operations_of_similar_length_together():
+    """Two lanes, four gradient jobs of one key with trajectories of 64..256
+    steps: the batch is halved into its two shorter and its two longer
+    members (a batch runs as long as its longest member)."""
+    ex = StepsExec()
+    arr = [(0.0, "adjoint", (lambda k=k: toy_job(32, 1, 1.0, traj=k)), 1.0, 32) for k in (256, 64, 192, 128)]
+    done, stats = serve.Server(executor=ex, max_groups=2).run(arr)
     assert stats["failed_jobs"] == 0
-    assert [c[1] for c in ex.calls] == [1, 2, 2, 2]
+    assert sorted(c[1] for c in ex.calls) == [[64, 128], [192, 256]]
+
+
+@pytest.mark.timeout(60)
+def test_free_lanes_take_single_jobs_of_one_kind():
+    """Four jobs of one key and four lanes: each runs in a lane of its own and
+    advances on its own (a job that finishes an operation starts the next at
+    once instead of waiting for the others)."""
+    ex = SlowExec({32: 0.02})
+    arr = [(0.0, "adjoint", (lambda i=i: toy_job(32, 3, float(i))), 1.0, 32) for i in range(4)]
+    done, stats = serve.Server(executor=ex).run(arr)
+    assert stats["failed_jobs"] == 0 and stats["ticks"] == 12
+    assert all(c[1] == 1 for c in ex.calls)
+    assert sorted({c[2] for c in ex.calls}) == [0, 1, 2, 3]
