@@ -3,14 +3,18 @@
 Headline workload (the Newton-GMRES configuration): N=128, nu=eta=1/40,
 b0=(0,0.1), F=-4cos(4y); guess = synthetic spectrum G(128, seed 0) evolved
 8192 Lawson-RK4 steps of h=0.005; T=4.3, sx=0.1; adaptive Lawson RK4(3) with
-atol=1e-8, h0=1e-2 (budget 64 x 64 attempts).  One "step" is one GMRES
-matvec: the JVP of objective_RPO through the whole adaptive trajectory, on a
-random unit tangent.  value = matvecs/s.
+atol=1e-8, h0=1e-2 (budget 64 x 64 attempts).  One "step" is one Arnoldi step
+of GMRES with Krylov dimension 64: the JVP of objective_RPO through the whole
+adaptive trajectory (the matvec) and the Gram-Schmidt sweep of its result.
+value = GMRES matvecs/s (Arnoldi steps/s); the JVP alone is reported beside it.
 
-Also reported: Lawson RK4 steps/s at N=128 on four seeds (batch 4 and 1, with
-the oracle's rate beside it) and at N=256 (batch 1 and batch 64), the
-roofline of the dominant kernel of the matvec (tangent row kernel), measured
-live with HIP events, and the CPU oracle's matvec rate on a bounded sample.
+Also reported: one whole Newton iteration, Lawson RK4 steps/s at N=128 on four
+seeds (batch 4 and 1, with the oracle's rate beside it) and at N=256 (batch 1
+and batch 64), the roofline of the dominant kernel of the matvec (the tangent
+row kernel of the replay), measured live with HIP events, the batched
+candidates, the other steppers, the gradient, the first arrivals of the
+serving workload at their full per-job sizes, and the CPU oracle's matvec
+rate on a bounded sample.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -39,8 +43,9 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE passes over `bench.py --probe-roofline[-batched]`
 # (KiB per dispatch scaled by a 64 MiB device copy: FETCH x 2047.3, WRITE x
 # 1024); the runs are summarised in profiles/pmc_traffic_tangent_row_kernel_n128_b1.md
-# and profiles/pmc_batched_row_kernel_n256_b64.md.
-TRAFFIC_ROW_T_N128_B1 = 847.0 * 2047.3 + 352.0 * 1024
+# (the tangent replay's instance of the row kernel) and
+# profiles/pmc_batched_row_kernel_n256_b64.md.
+TRAFFIC_ROW_T_N128_B1 = 848.0 * 2047.3 + 352.0 * 1024
 TRAFFIC_ROW_N256_B64 = 90205.2 * 2047.3 + 44032.0 * 1024
 METRIC = "FP64 RK steps/s at N=256 + GMRES matvecs/s; p50/p99 RPO solve latency"
 
@@ -406,8 +411,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the batched-candidates and serving legs")
     ap.add_argument("--serving-full", type=int, default=0, metavar="JOBS",
                     help="run only the serving workload at its full size with JOBS jobs and print its statistics")
-    ap.add_argument("--serving-jobs", type=int, default=0, metavar="JOBS",
-                    help="jobs of the serving leg of the default run (0: the leg is skipped)")
+    ap.add_argument("--serving-jobs", type=int, default=4, metavar="JOBS",
+                    help="first arrivals of the serving workload run by the default run's serving leg "
+                         "(0: the leg is skipped)")
     ap.add_argument("--dump-dir", default=None,
                     help="where a failing Newton job of the serving workload writes its input")
     ap.add_argument("--probe-roofline", action="store_true",
