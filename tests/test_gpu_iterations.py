@@ -154,3 +154,42 @@ def test_rk43_closed_form_linear_mode_on_gpu():
     assert [bool(a) for a in gi["acc"][: gi["attempts"]]] == [ok for _, ok in sched]
     xh = host(x)
     np.testing.assert_allclose([xh[0, 0, 4], xh[1, 0, 4]], c, rtol=1e-10)
+
+
+def test_newton_with_transpose_in_tdrk4_mode_matches_oracle():
+    """The reference's default Newton setting (newton.py:70,84,87: mode TDRK4,
+    use_transpose=True; the VJP of the objective is GMRES's preconditioner,
+    newton.py:167-170, so GMRES works on J^T J with right-hand side J^T b) at
+    N=16, 8 steps, Krylov 8.  The oracle forms J as the dense matrix of its own
+    JVP (514 columns) and uses its exact transpose, so the device's reverse pass is
+    compared with the transpose itself: |f| 1e-11, H 1e-9, rel_res 1e-6, damp
+    equal, the new input 1e-9."""
+    print("newton TDRK4 transpose n=16 inner=8 vs oracle", flush=True)
+    n, inner = 16, 8
+    items = {"num_checkpoints": 2, "ministeps": 4, "shift_reflect_ny": 0, "rot": False}
+    pd, opd = problem(n, items)
+    f = synth_fields(n, 7)
+    T, sx = 0.25, 0.1
+    new, log = solvers.newton_gmres_update({"fields": torch.as_tensor(f).cuda(), "T": T, "sx": sx}, pd, "TDRK4",
+                                           inner=inner, use_transpose=True)
+    inp = {"fields": f, "T": T, "sx": sx}
+    b = O.flatten(O.objective_RPO(inp, opd, "TDRK4")[0])
+    nd = b.size
+    J = np.empty((nd, nd))
+    for i in range(nd):
+        e = np.zeros(nd)
+        e[i] = 1.0
+        J[:, i] = O.flatten(O.objective_RPO_jvp(inp, O.unflatten(e, n), opd, "TDRK4"))
+    step, rr, H, _ = O.gmres(lambda v: J @ v, b, inner, preconditioner_list=[lambda v: J.T @ v])
+    obj_norm = lambda xv: float(np.linalg.norm(O.flatten(O.objective_RPO(O.unflatten(xv, n), opd, "TDRK4")[0])))
+    xn, damp = O.line_search(O.flatten(inp), step, obj_norm, float(np.linalg.norm(b)))
+    onew = O.unflatten(xn, n)
+    print("f", log["f_norm"], np.linalg.norm(b), "rel_res", log["gmres_rel_res"], rr, "damp", log["damp"], damp,
+          flush=True)
+    assert abs(log["f_norm"] - np.linalg.norm(b)) < 1e-11 * np.linalg.norm(b)
+    assert rel(log["H"], H) < 1e-9
+    assert abs(log["gmres_rel_res"] - rr) < 1e-6 * rr
+    assert log["damp"] == damp
+    assert rel(host(new["fields"]), O.dealias_fields(onew["fields"], opd)) < 1e-9
+    assert abs(float(new["T"]) - float(onew["T"])) < 1e-9 * abs(float(onew["T"]))
+    assert abs(float(new["sx"]) - float(onew["sx"])) < 1e-9 * max(abs(float(onew["sx"])), 1e-3)
