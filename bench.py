@@ -251,7 +251,8 @@ def newton_leg(inp, pd):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return {"workload": "one Newton-GMRES iteration, inner 64, same guess as the headline", "wall_s": wall,
-            "fwall_s": log["fwall"], "gmreswall_s": log["gmreswall"],
+            "fwall_s": log["fwall"], "gmreswall_s": log["gmreswall"], "line_search_s": log["lswall"],
+            "line_search_window": log["line_search_window"],
             "gmres_matvecs_per_s": log["matvecs"] / log["gmreswall"],
             "f_norm": log["f_norm"], "gmres_rel_res": log["gmres_rel_res"], "damp": log["damp"],
             "rel_err": log["rel_err"],

@@ -158,6 +158,46 @@ def objective_RPO(input_dict, param_dict, mode):
     return {"fields": _irfft2(f0 - f, f0.shape[-2]), "T": pt, "sx": px}
 
 
+def integrate_batch(f0s, Ts, param_dict, mode):
+    """The integration of mismatch_RPO for several inputs of one grid size at
+    once (each with its own period), as one batched call of the fused
+    steppers.  Every candidate's result is bitwise its solo result (the kernels
+    treat candidates independently).  Returns (f [B, 2, N, N/2+1], infos)."""
+    ctx = mhd.context(param_dict)
+    f0 = torch.stack(f0s).contiguous()
+    if mode == "Lawson_RK43":
+        ad = param_dict["adaptive_dict"]
+        return ctx.lawson_rk43(f0, list(Ts), 1e-2, ad["atol"], ad["max_steps_per_checkpoint"], ad["checkpoints"])
+    if mode in ("Lawson_RK4", "RK4", "TDRK4"):
+        _, _, steps = _rk4_steps(param_dict)
+        hs = [_rk4_h(T, param_dict) for T in Ts]
+        if mode == "Lawson_RK4":
+            return ctx.lawson_rk4(f0, hs, steps, masked=True), [None] * len(Ts)
+        if mode == "RK4":
+            return ctx.rk4(f0, hs, steps), [None] * len(Ts)
+        return ctx.tdrk4(f0, hs, steps), [None] * len(Ts)
+    if mode == "Lawson_RK6":
+        steps = int(param_dict["steps"])
+        return ctx.lawson_rk6(f0, [T / steps for T in Ts], steps, masked=True), [None] * len(Ts)
+    raise ValueError(f"You selected mode = {mode}, which does not exist.")
+
+
+def objective_RPO_batch(input_dicts, param_dict, mode):
+    """objective_RPO of several inputs of one grid size, their integrations run
+    as one batch; returns the list of objective dicts, each bitwise the one
+    objective_RPO returns for that input alone (the transforms, the symmetry
+    and the difference are formed per input, with the shapes of the solo
+    path)."""
+    f0s = [_rfft2(_fields(d)) for d in input_dicts]
+    f, _ = integrate_batch(f0s, [_f(d["T"]) for d in input_dicts], param_dict, mode)
+    out = []
+    for b, d in enumerate(input_dicts):
+        fb = symmetry.apply(f[b], _f(d["sx"]), param_dict)
+        pt, px = phase_conditions(f0s[b], param_dict)
+        out.append({"fields": _irfft2(f0s[b] - fb, f0s[b].shape[-2]), "T": pt, "sx": px})
+    return out
+
+
 class Linearization:
     """Primal data shared by every JVP at one input (one Newton iteration):
     the adaptive schedule (decisions follow the primal) and the phase vectors."""

@@ -58,6 +58,8 @@ class Op:
             return float(self.kw["steps"])
         if self.kind == "jvp":
             return float(self.kw["lin"].info["accepted"])
+        if self.kind == "objective":
+            return max(float(t["T"]) for t in self.kw["trials"])
         return float(self.kw.get("T", 0.0))
 
 
@@ -113,20 +115,28 @@ def _batch_grad(ops, pd, lane):
 
 
 def _batch_objective(ops, pd, lane):
-    """objective_RPO in mode Lawson_RK43 for a batch (adaptive, per-candidate T)."""
+    """objective_RPO in mode Lawson_RK43 (adaptive, per-candidate T) of every
+    trial input of a batch of operations (an operation holds the tries of one
+    window of a job's line search), as one batched solve; each operation gets
+    the list of its trials' (objective, info)."""
     ctx = mhd.context(pd, lane)
     n = ops[0].n
-    fields = torch.stack([o.kw["fields"] for o in ops]).contiguous()
+    trials = [t for o in ops for t in o.kw["trials"]]
+    fields = torch.stack([t["fields"] for t in trials]).contiguous()
     f0 = torch.fft.rfft2(fields).contiguous()
     ad = ops[0].kw["adaptive"]
-    fT, infos = ctx.lawson_rk43(f0, [float(o.kw["T"]) for o in ops], 1e-2, ad["atol"],
+    fT, infos = ctx.lawson_rk43(f0, [float(t["T"]) for t in trials], 1e-2, ad["atol"],
                                 ad["max_steps_per_checkpoint"], ad["checkpoints"])
-    res = []
-    for b, o in enumerate(ops):
-        g = symmetry.apply(fT[b], float(o.kw["sx"]), pd)
-        res.append(({"fields": torch.fft.irfft2(f0[b] - g, s=(n, n)),
-                     "T": torch.zeros((), dtype=torch.float64, device="cuda"),
-                     "sx": torch.zeros((), dtype=torch.float64, device="cuda")}, infos[b]))
+    res, i = [], 0
+    for o in ops:
+        out = []
+        for t in o.kw["trials"]:
+            g = symmetry.apply(fT[i], float(t["sx"]), pd)
+            out.append(({"fields": torch.fft.irfft2(f0[i] - g, s=(n, n)),
+                         "T": torch.zeros((), dtype=torch.float64, device="cuda"),
+                         "sx": torch.zeros((), dtype=torch.float64, device="cuda")}, infos[i]))
+            i += 1
+        res.append(out)
     return res
 
 
@@ -223,14 +233,17 @@ def _dump(dump_dir, tag, why, x, lin=None, **extra):
     return path
 
 
-def newton_job(fields, T, sx, n, adaptive, inner, iters, dump_dir=None, tag="job"):
+def newton_job(fields, T, sx, n, adaptive, inner, iters, dump_dir=None, tag="job", window=None):
     """Newton-GMRES (newton.py:151-215) with per-job Krylov basis on the device;
     matvecs are yielded so they batch with other jobs'.  The objective and
     the schedule of the JVPs come from one adaptive solve per iteration.  An
     adaptive solve that does not complete within its attempt budget, or a
     non-finite matvec or least-squares problem, ends the job with an error
-    (after writing its input to ``dump_dir``, when given)."""
+    (after writing its input to ``dump_dir``, when given).  The line search's
+    tries go out ``window`` at a time (default 8 for N <= 128, 4 above), each
+    window one operation; the damp taken is the one-at-a-time search's."""
     from . import linalg
+    window = window if window is not None else (8 if n <= 128 else 4)
     x = {"fields": fields, "T": T, "sx": sx}
     logs = []
     for it in range(iters):
@@ -265,15 +278,26 @@ def newton_job(fields, T, sx, n, adaptive, inner, iters, dump_dir=None, tag="job
         Ax = yield Op("jvp", n, lin=lin, tangent=_unflat(step, n))
         rel_res = float(torch.linalg.vector_norm(_flat(Ax) - b) / nb)
         xv = _flat(x)
-        damp = 1.0
-        for _ in range(20):   # line search (utils.py:17-58)
-            xt = xv - damp * step
-            ft, info = yield Op("objective", n, fields=xt[1:-1].reshape(2, n, n).contiguous(), T=float(xt[0]),
-                                sx=float(xt[-1]), adaptive=adaptive)
-            if info["completed"] and float(torch.linalg.vector_norm(_flat(ft))) < float(nb):
-                xv = xt
+        # line search (utils.py:17-58): damps 1, 1/2, ... (20 tries), the first that
+        # lowers |f| taken; the tries are integrated a window at a time as one batch
+        damps = [1.0]
+        for _ in range(19):
+            damps.append(damps[-1] / 2.0)
+        damp = damps[-1] / 2.0
+        found = None
+        for w0 in range(0, 20, window):
+            tried = damps[w0:w0 + window]
+            xts = [xv - d * step for d in tried]
+            outs = yield Op("objective", n, adaptive=adaptive, f_norm=float(nb),
+                            trials=[{"fields": xt[1:-1].reshape(2, n, n).contiguous(), "T": float(xt[0]),
+                                     "sx": float(xt[-1])} for xt in xts])
+            for d, xt, (ft, info) in zip(tried, xts, outs):
+                if info["completed"] and float(torch.linalg.vector_norm(_flat(ft))) < float(nb):
+                    found = (xt, d)
+                    break
+            if found is not None:
+                xv, damp = found
                 break
-            damp /= 2.0
         mask = mhd.construct_domain(n)["mask"]
         x = {"fields": torch.fft.irfft2(mask * torch.fft.rfft2(xv[1:-1].reshape(2, n, n)), s=(n, n)),
              "T": float(xv[0]), "sx": float(xv[-1])}
@@ -307,14 +331,26 @@ def job_memory(kind, n, inner=64, steps=0, budget=4096):
     return nseg * comp + 64 * 5 * comp + 64 * 4 * 8 * n * kp * 16 + 6 * vec
 
 
+def useful_trials(op, result):
+    """Tries of a line-search window that the one-at-a-time search evaluates:
+    up to and including the first completed one that lowers |f| below the
+    job's f_norm (all of them when none does)."""
+    for i, (ft, info) in enumerate(result):
+        if info["completed"] and float(torch.linalg.vector_norm(_flat(ft))) < op.kw["f_norm"]:
+            return i + 1
+    return len(result)
+
+
 def op_rk_steps(op, result):
     """Stepper steps one operation integrates (the forward and the reverse
     sweep of a gradient each count its steps; an adaptive solve counts its
-    attempts; a tangent replay its accepted steps)."""
+    attempts; a tangent replay its accepted steps).  A line-search window
+    counts the tries the one-at-a-time search would make, not the ones the
+    window evaluated past the first success."""
     if op.kind == "grad":
         return 2 * int(op.kw["steps"])
     if op.kind == "objective":
-        return int(result[1]["attempts"])
+        return sum(int(info["attempts"]) for _, info in result[:useful_trials(op, result)])
     if op.kind == "linearize":
         return int(result.info["attempts"])
     if op.kind == "jvp":
@@ -586,10 +622,19 @@ def unbatched_estimate(server, done):
         if j.error is not None:
             continue
         for kind, (op, st) in j.first_op.items():
-            server._run_group(op.group_key(), [op], 0)   # warm: workspace of a batch of one
+            # a line-search window runs as the one-at-a-time search would: its
+            # tries one by one, up to the first that lowers |f|
+            if kind == "objective":
+                res = server._run_group(op.group_key(), [op], 0)[0]
+                singles = [Op("objective", op.n, adaptive=op.kw["adaptive"], f_norm=op.kw["f_norm"], trials=[t])
+                           for t in op.kw["trials"][:useful_trials(op, res)]]
+            else:
+                singles = [op]
+            server._run_group(op.group_key(), [singles[0]], 0)   # warm: workspace of a batch of one
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            server._run_group(op.group_key(), [op], 0)
+            for o in singles:
+                server._run_group(op.group_key(), [o], 0)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
             total_s += dt * (j.op_steps[kind] / max(st, 1))

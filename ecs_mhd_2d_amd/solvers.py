@@ -16,7 +16,7 @@ def dealias(fields, param_dict):
 
 
 def newton_gmres_update(input_dict, param_dict, mode, inner=64, outer=1, do_line_search=True,
-                        default_damp=0.1, use_transpose=False):
+                        default_damp=0.1, use_transpose=False, line_search_window=None):
     """One Newton-GMRES iteration (newton.py:151-215).
 
     Returns (new_input_dict, log) with the reference's logged quantities
@@ -26,6 +26,10 @@ def newton_gmres_update(input_dict, param_dict, mode, inner=64, outer=1, do_line
     whether the adaptive integration completed (the reference goes on either
     way, newton.py:159; so does this function).  fwall covers the objective
     and the primal data of the linearisation, one integration for both.
+
+    The line search's tries are integrated ``line_search_window`` at a time as
+    one batch (default 8 for N <= 128, 4 above; 1 integrates them one by one):
+    the same damp and new input as the reference's one-at-a-time search.
     """
     n = input_dict["fields"].shape[-1]
     t0 = time.perf_counter()
@@ -50,16 +54,24 @@ def newton_gmres_update(input_dict, param_dict, mode, inner=64, outer=1, do_line
     x = utils.flatten(input_dict)
     obj = lambda d: loss_functions.objective_RPO(d, param_dict, mode)
     unravel = lambda v: utils.unflatten(v, n)
+    t2 = time.perf_counter()
+    window = 1
     if do_line_search:
-        x, damp = utils.line_search_unravel(x, step, obj, unravel, b, max_iters=20)
+        window = line_search_window if line_search_window is not None else (8 if n <= 128 else 4)
+        obj_batch = lambda ds: loss_functions.objective_RPO_batch(ds, param_dict, mode)
+        x, damp = utils.line_search_unravel(x, step, obj, unravel, b, max_iters=20, obj_batch=obj_batch,
+                                            window=window)
     else:
         damp = default_damp
         x = x - damp * step
+    torch.cuda.synchronize()
+    lswall = time.perf_counter() - t2
     new = unravel(x)
     rel_err = float(torch.linalg.vector_norm(f["fields"]) / torch.linalg.vector_norm(new["fields"]))
     new = {"fields": dealias(new["fields"].contiguous(), param_dict), "T": new["T"].clone(), "sx": new["sx"].clone()}
     log = {"rel_err": rel_err, "f_norm": float(torch.linalg.vector_norm(b)), "fwall": fwall, "gmreswall": gwall,
            "gmres_rel_res": gres, "damp": damp, "T": float(new["T"]), "sx": float(new["sx"]),
+           "lswall": lswall, "line_search_window": window,
            "krylov_residuals": hist["krylov_residuals"], "krylov_residuals_qtb": hist["krylov_residuals_qtb"],
            "orthogonality_loss": hist["orthogonality_loss"], "H": hist["H"], "matvecs": hist["matvecs"],
            "completed": True if lin.info is None else bool(lin.info["completed"])}

@@ -65,16 +65,34 @@ def compile_objective_and_Jacobian(input_dict, param_dict, obj_mode):
     return objective, jac
 
 
-def line_search_unravel(x, step, obj, unravel_fn, f_vec, max_iters=20):
+def line_search_unravel(x, step, obj, unravel_fn, f_vec, max_iters=20, obj_batch=None, window=1):
     """Halving line search (utils.py:17-58): the first damp in 1, 1/2, 1/4, ...
     (at most max_iters tries) with |f(x - damp step)| < |f(x)|; returns (x, damp).
-    If none improves, x is returned unchanged and damp = 2^-max_iters."""
+    If none improves, x is returned unchanged and damp = 2^-max_iters.
+
+    With ``obj_batch`` (a function of a list of input dicts returning their
+    objectives, each the same as ``obj`` of that input) and ``window`` > 1, the
+    tries are evaluated ``window`` damps at a time, in order, and the first
+    improving one is taken: the same (x, damp) as the one-at-a-time search,
+    with the latency of one batched evaluation per window."""
     norm_orig = torch.linalg.vector_norm(f_vec)
-    damp = 1.0
-    for _ in range(max_iters):
-        xt = x - damp * step
-        ft = flatten(obj(unravel_fn(xt)))
-        if bool(torch.linalg.vector_norm(ft) < norm_orig):
-            return xt, damp
-        damp = damp / 2.0
-    return x, damp
+    if obj_batch is None or window <= 1:
+        damp = 1.0
+        for _ in range(max_iters):
+            xt = x - damp * step
+            ft = flatten(obj(unravel_fn(xt)))
+            if bool(torch.linalg.vector_norm(ft) < norm_orig):
+                return xt, damp
+            damp = damp / 2.0
+        return x, damp
+    damps = [1.0]
+    for _ in range(max_iters - 1):
+        damps.append(damps[-1] / 2.0)
+    for w in range(0, max_iters, window):
+        trial = damps[w:w + window]
+        xts = [x - d * step for d in trial]
+        fts = obj_batch([unravel_fn(xt) for xt in xts])
+        for d, xt, ft in zip(trial, xts, fts):
+            if bool(torch.linalg.vector_norm(flatten(ft)) < norm_orig):
+                return xt, d
+    return x, damps[-1] / 2.0

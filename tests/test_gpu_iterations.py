@@ -193,3 +193,71 @@ def test_newton_with_transpose_in_tdrk4_mode_matches_oracle():
     assert rel(host(new["fields"]), O.dealias_fields(onew["fields"], opd)) < 1e-9
     assert abs(float(new["T"]) - float(onew["T"])) < 1e-9 * abs(float(onew["T"]))
     assert abs(float(new["sx"]) - float(onew["sx"])) < 1e-9 * max(abs(float(onew["sx"])), 1e-3)
+
+
+@pytest.mark.parametrize("mode", ["Lawson_RK43", "Lawson_RK4", "RK4", "TDRK4", "Lawson_RK6"])
+def test_objective_batch_is_bitwise_the_solo_objective(mode):
+    """objective_RPO_batch (one batched integration of several inputs with
+    their own T and sx) gives each input the bits of objective_RPO alone."""
+    from ecs_mhd_2d_amd import loss_functions as LF
+    print(f"objective batch vs solo {mode} n=32", flush=True)
+    n = 32
+    pd, _ = problem(n, dict(ITEMS, steps=32))
+    inputs = [{"fields": torch.as_tensor(synth_fields(n, 20 + i)).cuda(), "T": 0.35 + 0.1 * i, "sx": 0.05 * i}
+              for i in range(3)]
+    outs = LF.objective_RPO_batch(inputs, pd, mode)
+    for d, o in zip(inputs, outs):
+        s = LF.objective_RPO(d, pd, mode)
+        assert torch.equal(o["fields"], s["fields"])
+        assert float(o["T"]) == float(s["T"]) and float(o["sx"]) == float(s["sx"])
+
+
+@pytest.mark.parametrize("mode", ["Lawson_RK43", "Lawson_RK4"])
+def test_newton_line_search_window_gives_the_one_at_a_time_result(mode):
+    """newton_gmres_update with the line search's tries integrated 3 or 8 at a
+    time returns the damp and the bits of the new input of the one-at-a-time
+    search (inputs far from periodic, so the search halves several times; the
+    fixed-step mode keeps T small enough for its 32 steps to stay stable)."""
+    print(f"newton line search window {mode} n=32 inner=8", flush=True)
+    n = 32
+    pd, _ = problem(n, ITEMS)
+    scale, T = (3.0, 1.1) if mode == "Lawson_RK43" else (1.0, 0.6)
+    inp = {"fields": torch.as_tensor(synth_fields(n, 11)).cuda() * scale, "T": T, "sx": 0.3}
+    res = {w: solvers.newton_gmres_update(inp, pd, mode, inner=8, line_search_window=w) for w in (1, 3, 8)}
+    print(mode, "damp", [res[w][1]["damp"] for w in (1, 3, 8)], flush=True)
+    new1, log1 = res[1]
+    for w in (3, 8):
+        new, log = res[w]
+        assert log["damp"] == log1["damp"]
+        assert torch.equal(new["fields"], new1["fields"])
+        assert float(new["T"]) == float(new1["T"]) and float(new["sx"]) == float(new1["sx"])
+
+
+@pytest.mark.parametrize("mode", ["Lawson_RK43", "Lawson_RK4"])
+def test_line_search_windows_cross_on_the_rpo_objective(mode):
+    """The windowed line search on the RPO objective along the steepest-descent
+    direction J^T f (the reverse pass) scaled to norm 100: the first improving
+    damp is 2^-7, found by windows of 1, 3, 4 and 8 alike with the same bits;
+    along the ascent direction nothing improves within 20 tries (damp 2^-20,
+    x unchanged) for every window."""
+    from ecs_mhd_2d_amd import loss_functions as LF, utils
+    print(f"line search windows on the RPO objective {mode} n=32", flush=True)
+    n = 32
+    pd, _ = problem(n, ITEMS)
+    inp = {"fields": torch.as_tensor(synth_fields(n, 11)).cuda(), "T": 0.6, "sx": 0.3}
+    lin = LF.Linearization(inp, pd, mode)
+    f = lin.objective()
+    b = utils.flatten(f)
+    g = utils.flatten(lin.vjp(f))
+    x = utils.flatten(inp)
+    obj = lambda d: LF.objective_RPO(d, pd, mode)
+    obj_batch = lambda ds: LF.objective_RPO_batch(ds, pd, mode)
+    unravel = lambda v: utils.unflatten(v, n)
+    step = g / torch.linalg.vector_norm(g) * 100.0
+    res = {w: utils.line_search_unravel(x, step, obj, unravel, b, 20, obj_batch, w) for w in (1, 3, 4, 8)}
+    assert res[1][1] == 2.0 ** -7
+    for w in (3, 4, 8):
+        assert res[w][1] == res[1][1] and torch.equal(res[w][0], res[1][0])
+    for w in (1, 8):
+        xn, damp = utils.line_search_unravel(x, -step / 1e3, obj, unravel, b, 20, obj_batch, w)
+        assert damp == 2.0 ** -20 and torch.equal(xn, x)
