@@ -135,7 +135,34 @@ def _flat_objective(objective, input_dict):
     return x, (lambda v: utils.flatten(objective(utils.unflatten(v, n)))), (lambda v: utils.unflatten(v, n))
 
 
-def newton_gmres_hookstep_v2(A, b, m, s, objective, input_dict, max_halvings=64):
+def _first_decrease(x, tries, objective, objective_batch, window, nref, unravel):
+    """The first (step, s) of ``tries`` (an iterator of the steps a trust-region
+    loop tries, in order, with the size s each was made for) whose
+    |objective(x - step)| < nref, or None.  With ``objective_batch`` and
+    window > 1 the tries are integrated ``window`` at a time as one batch;
+    the answer is that of evaluating them one by one."""
+    from . import utils
+    if objective_batch is None or window <= 1:
+        for dx, s in tries:
+            if float(torch.linalg.vector_norm(utils.flatten(objective(unravel(x - dx))))) < nref:
+                return dx, s
+        return None
+    while True:
+        win = []
+        for item in tries:
+            win.append(item)
+            if len(win) == window:
+                break
+        if not win:
+            return None
+        outs = objective_batch([unravel(x - dx) for dx, _ in win])
+        for (dx, s), o in zip(win, outs):
+            if float(torch.linalg.vector_norm(utils.flatten(o))) < nref:
+                return dx, s
+
+
+def newton_gmres_hookstep_v2(A, b, m, s, objective, input_dict, max_halvings=64, objective_batch=None,
+                             window=1):
     """Newton step with a bound on its size (linalg.py:217-337).
 
     Krylov space of A of dimension m from b, SVD of the (m+1) x m Hessenberg
@@ -144,38 +171,48 @@ def newton_gmres_hookstep_v2(A, b, m, s, objective, input_dict, max_halvings=64)
     when |objective(x - dx)| < |b| (then s doubles), otherwise s halves.
     Returns (new_input_dict, s, rel_res) with rel_res = |A dx - b| / |b|.
     The reference loops until a step decreases |f|; here more than
-    ``max_halvings`` halvings raise RuntimeError.
+    ``max_halvings`` halvings raise RuntimeError.  The steps tried depend only
+    on the sequence of sizes, so with ``objective_batch`` (objectives of a
+    list of input dicts) and window > 1 they are integrated a window at a
+    time, with the same result.
     """
     Q, H = _arnoldi_reorth(A, b, m)
     b2 = torch.mv(Q, b).cpu().numpy()
     U, S, Vh = np.linalg.svd(H, full_matrices=False)
-    y = Vh.T @ ((U.T @ b2) / S)
+    y0 = Vh.T @ ((U.T @ b2) / S)
     x, f_vec, unravel = _flat_objective(objective, input_dict)
     nb = float(torch.linalg.vector_norm(b))
-    for _ in range(max_halvings):
-        if np.linalg.norm(y) > s * s:
-            b4 = Vh @ (H.T @ b2)
-            g = lambda t: s - np.linalg.norm(b4 / (S * S + t))
-            t = _bisection(g, 0.0, np.sqrt(m) * np.linalg.norm(b4) / s)
-            y = Vh.T @ (b4 / (S * S + t))
-        dx = torch.mv(Q[:m].T.contiguous(), torch.as_tensor(y, device=b.device))
-        if float(torch.linalg.vector_norm(f_vec(x - dx))) < nb:
-            s = s * 2
-            break
-        s = s / 2
-    else:
+
+    def tries():
+        y, sk = y0, s
+        for _ in range(max_halvings):
+            if np.linalg.norm(y) > sk * sk:
+                b4 = Vh @ (H.T @ b2)
+                g = lambda t: sk - np.linalg.norm(b4 / (S * S + t))
+                t = _bisection(g, 0.0, np.sqrt(m) * np.linalg.norm(b4) / sk)
+                y = Vh.T @ (b4 / (S * S + t))
+            yield torch.mv(Q[:m].T.contiguous(), torch.as_tensor(y, device=b.device)), sk
+            sk = sk / 2
+
+    hit = _first_decrease(x, tries(), objective, objective_batch, window, nb, unravel)
+    if hit is None:
         raise RuntimeError("hookstep: no step decreased |f|")
+    dx, s = hit
+    s = s * 2
     rel_res = float(torch.linalg.vector_norm(A(dx) - b)) / nb
     return unravel(x - dx), s, rel_res
 
 
-def newton_gmres_hookstep(AtA, Atb, m, s, f, f0, J, b, input_dict, max_halvings=64):
+def newton_gmres_hookstep(AtA, Atb, m, s, f, f0, J, b, input_dict, max_halvings=64, objective_batch=None,
+                          window=1):
     """Hookstep on the normal equations (linalg.py:102-213): Krylov space of
     A^T A from A^T b, eigendecomposition of the m x m Hessenberg matrix (lower
     triangle, as jnp/numpy eigh read it), bisection for a step of norm s on
     [0, |b| / s].  f maps input dicts to output dicts, f0 is the flattened
     objective at input_dict.  Returns (new_input_dict, s, rel_res_1,
-    rel_res_2) = (..., |A^T A step - A^T b| / |A^T b|, |J step + b| / |b|)."""
+    rel_res_2) = (..., |A^T A step - A^T b| / |A^T b|, |J step + b| / |b|).
+    With ``objective_batch`` and window > 1 the tries are integrated a window
+    at a time (the same result)."""
     Q, Hf = _arnoldi_reorth(AtA, Atb, m)
     H = Hf[:m, :m]
     D, V = np.linalg.eigh(H)
@@ -183,16 +220,20 @@ def newton_gmres_hookstep(AtA, Atb, m, s, f, f0, J, b, input_dict, max_halvings=
     x0, f_vec, unravel = _flat_objective(f, input_dict)
     nf0 = float(torch.linalg.vector_norm(f0))
     nb = float(torch.linalg.vector_norm(b))
-    for _ in range(max_halvings):
-        g = lambda t: s - np.linalg.norm(b2 / (D + t))
-        t = 0.0 if g(0.0) > 0 else _bisection(g, 0.0, nb / s)
-        step = torch.mv(Q[:m].T.contiguous(), torch.as_tensor(V @ (b2 / (D + t)), device=Atb.device))
-        if float(torch.linalg.vector_norm(f_vec(x0 - step))) < nf0:
-            s = s * 2
-            break
-        s = s / 2
-    else:
+
+    def tries():
+        sk = s
+        for _ in range(max_halvings):
+            g = lambda t: sk - np.linalg.norm(b2 / (D + t))
+            t = 0.0 if g(0.0) > 0 else _bisection(g, 0.0, nb / sk)
+            yield torch.mv(Q[:m].T.contiguous(), torch.as_tensor(V @ (b2 / (D + t)), device=Atb.device)), sk
+            sk = sk / 2
+
+    hit = _first_decrease(x0, tries(), f, objective_batch, window, nf0, unravel)
+    if hit is None:
         raise RuntimeError("hookstep: no step decreased |f|")
+    step, s = hit
+    s = s * 2
     rr1 = float(torch.linalg.vector_norm(AtA(step) - Atb) / torch.linalg.vector_norm(Atb))
     rr2 = float(torch.linalg.vector_norm(J(step) + b)) / nb
     return unravel(x0 - step), s, rr1, rr2

@@ -78,9 +78,11 @@ def newton_gmres_update(input_dict, param_dict, mode, inner=64, outer=1, do_line
     return new, log
 
 
-def newton_hookstep_update(input_dict, param_dict, mode, s, inner=64):
+def newton_hookstep_update(input_dict, param_dict, mode, s, inner=64, window=None):
     """One iteration of newton_hookstep_v2.py:95-121: objective, Jacobian
     matvec, newton_gmres_hookstep_v2 with trust-region size s, dealias.
+    The trust-region tries are integrated ``window`` at a time as one batch
+    (default 8 for N <= 128, 4 above; 1: one by one), with the same result.
     Returns (new_input_dict, s, log)."""
     n = input_dict["fields"].shape[-1]
     t0 = time.perf_counter()
@@ -93,7 +95,10 @@ def newton_hookstep_update(input_dict, param_dict, mode, s, inner=64):
     A = lambda v: utils.flatten(lin.jvp(utils.unflatten(v, n)))
     obj = lambda d: loss_functions.objective_RPO(d, param_dict, mode)
     t1 = time.perf_counter()
-    new, s, rel_res = linalg.newton_gmres_hookstep_v2(A, b, inner, s, obj, input_dict)
+    window = window if window is not None else (8 if n <= 128 else 4)
+    obj_batch = lambda ds: loss_functions.objective_RPO_batch(ds, param_dict, mode)
+    new, s, rel_res = linalg.newton_gmres_hookstep_v2(A, b, inner, s, obj, input_dict, objective_batch=obj_batch,
+                                                      window=window)
     torch.cuda.synchronize()
     gwall = time.perf_counter() - t1
     rel_err = float(torch.linalg.vector_norm(f["fields"]) / torch.linalg.vector_norm(new["fields"]))

@@ -77,3 +77,33 @@ def test_newton_hookstep_iteration_reduces_residual():
     r1 = float(torch.linalg.vector_norm(LF.objective_RPO(x1, pd, "Lawson_RK4")["fields"]))
     print("hookstep", r0, r1, log["rel_res"], s, flush=True)
     assert r1 < 0.5 * r0 and s == 200.0
+
+
+def test_hookstep_windows_give_the_one_by_one_result():
+    """Both hookstep forms with their trust-region tries integrated 3, 4 or 8
+    at a time (objective_RPO_batch) return the bits, s and residuals of the
+    one-by-one loop; from s0 = 100 the v2 form fails four times (s = 100 ..
+    12.5) and takes the fifth try (s = 6.25, returned doubled)."""
+    pd, opd, inp, oinp = small_problem()
+    n = 16
+    f = LF.objective_RPO(inp, pd, "Lawson_RK4")
+    b = utils.flatten(f)
+    lin = LF.Linearization(inp, pd, "Lawson_RK4")
+    J = lambda v: utils.flatten(lin.jvp(utils.unflatten(v, n)))
+    obj = lambda d: LF.objective_RPO(d, pd, "Lawson_RK4")
+    ob = lambda ds: LF.objective_RPO_batch(ds, pd, "Lawson_RK4")
+    res = {w: linalg.newton_gmres_hookstep_v2(J, b, 10, 100.0, obj, inp, objective_batch=ob, window=w)
+           for w in (1, 3, 4, 8)}
+    assert res[1][1] == 12.5
+    for w in (3, 4, 8):
+        assert res[w][1] == res[1][1] and res[w][2] == res[1][2]
+        assert torch.equal(utils.flatten(res[w][0]), utils.flatten(res[1][0]))
+    JT = lambda v: utils.flatten(lin.vjp(utils.unflatten(v, n)))
+    AtA = lambda v: JT(J(v))
+    Atb = JT(b)
+    r1 = linalg.newton_gmres_hookstep(AtA, Atb, 8, 100.0, obj, b, J, -b, inp, objective_batch=ob, window=1)
+    for w in (3, 8):
+        rw = linalg.newton_gmres_hookstep(AtA, Atb, 8, 100.0, obj, b, J, -b, inp, objective_batch=ob, window=w)
+        assert rw[1] == r1[1] and rw[2] == r1[2] and rw[3] == r1[3]
+        assert torch.equal(utils.flatten(rw[0]), utils.flatten(r1[0]))
+    print("hookstep windows: v2 s", res[1][1], "v1 s", r1[1], flush=True)
