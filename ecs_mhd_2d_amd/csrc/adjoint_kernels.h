@@ -15,6 +15,7 @@
 // (de/d(dt) = L/2 e) per mode into hb, summed once at the end.
 #pragma once
 #include "rhs_kernels.h"
+#include "comb_kernels.h"
 
 namespace ecs {
 
@@ -43,8 +44,10 @@ __device__ __forceinline__ void adj_glue(const Args& a, int b, size_t ci, int r,
   if (MODE == B_OUT) {
     double2 o = yb;
     if (a.include_diss) o = cadd(o, cscale(L, a.gin[ci]));
-    // added as the combination kernel adds a term of weight 1
-    a.out[ci] = a.out_acc ? cadd(a.out[ci], cscale(1.0, o)) : o;
+    // added as the combination kernel adds a term of weight 1; g returns the
+    // value stored (the input of a combination fused after it)
+    g = a.out_acc ? cadd(a.out[ci], cscale(1.0, o)) : o;
+    a.out[ci] = g;
     return;
   }
   const double dt = a.dt[b];
@@ -113,9 +116,10 @@ __device__ __forceinline__ void adj_glue(const Args& a, int b, size_t ci, int r,
 }
 
 // Column-adjoint kernel: [x-FFT of the row-adjoint output -> ybar] -> glue ->
-// [m g -> inverse x-FFT -> AO].
-template <int N, int TY, int NT, int MODE, int PL>
-__global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
+// [m g -> inverse x-FFT -> AO].  `after(b, ci, c, kx, ky, v)` runs on every
+// mode after a B_OUT glue has stored v there.
+template <int N, int TY, int NT, int MODE, int PL, class F>
+__device__ __forceinline__ void adj_col_body(const Args& a, F&& after) {
   using G = Geo<N>;
   constexpr int KC = G::KC, R = G::R, KMAX = G::KMAX;
   constexpr bool FWD = !(MODE == B_G || MODE == B_ENTRY);
@@ -168,6 +172,7 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
     }
     double2 g;
     adj_glue<N, MODE>(a, b, ci, r, c, kx, ky, yb, g);
+    if constexpr (MODE == B_OUT) after(b, ci, c, kx, ky, g);
     if (INV) {
       const bool dc = (kx == 0 && ky == 0);
       const double m = dc ? 0.0 : (c == 0 ? 1.0 : (double)(kx * kx + ky * ky));
@@ -189,6 +194,22 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
   fft_lds_tail<N, TY, SS, NT, true, PL, TY>(sm, twl, [&](int cy, int x, double2 v) {
     const int ky = ky0 + cy;
     if (ky < KC) AO[((size_t)c * N + x) * G::KP + ky] = v;
+  });
+}
+
+template <int N, int TY, int NT, int MODE, int PL>
+__global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
+  adj_col_body<N, TY, NT, MODE, PL>(a, [](int, size_t, int, int, int, double2) {});
+}
+
+// The last kernel of an RHS-VJP pass (B_OUT) with the transposed stage
+// combination that reads its output applied in the same launch, mode by
+// mode: one launch instead of two, the same arithmetic.  The table comes as a
+// second argument (the Args block stays within its 512 bytes).
+template <int N, int TY, int NT, int PL>
+__global__ __launch_bounds__(NT) void adj_out_comb_kernel(Args a, CombAdj cb) {
+  adj_col_body<N, TY, NT, B_OUT, PL>(a, [&](int b, size_t ci, int c, int kx, int ky, double2 v) {
+    comb_adj_elem(cb, b, ci, c, kx, ky, v);
   });
 }
 

@@ -43,6 +43,39 @@ struct CombAdj {
   size_t elems;
 };
 
+// The transposed combination at one mode (c, kx, ky) of candidate b, compact
+// index i, for the stage cotangent value yb (cb.yb[i]).  The combination
+// kernel below and the RHS-VJP pass's last kernel (which produces yb and
+// applies the combination that follows it in the same launch) share it.
+__device__ __forceinline__ void comb_adj_elem(const CombAdj& cb, int b, size_t i, int c, int kx, int ky,
+                                              double2 yb) {
+  const bool dc = (kx == 0 && ky == 0);
+  const double L = -(double)(kx * kx + ky * ky) * cb.nu[c];
+  const double h = cb.h[b];
+  const double E1 = exp(h / cb.ediv * L);
+  double hacc = 0.0;
+  for (int t = 0; t < cb.nterm; ++t) {
+    double w = cb.coef[t];
+    for (int q = 0; q < cb.hp[t]; ++q) w *= h;
+    double em = 1.0;
+    if (cb.ep[t] > 0) {
+      double Ep = E1;
+      for (int q = 1; q < cb.ep[t]; ++q) Ep *= E1;
+      em = (cb.masked && dc) ? 0.0 : Ep;
+      w *= em;
+    }
+    if (cb.out[t]) cb.out[t][i] = ((cb.assign >> t) & 1u) ? cscale(w, yb) : cadd(cb.out[t][i], cscale(w, yb));
+    if (cb.in[t] && cb.hb) {
+      double dw = cb.coef[t] * (double)cb.hp[t];
+      for (int q = 1; q < cb.hp[t]; ++q) dw *= h;
+      if (cb.ep[t] > 0) dw = dw * em + w * ((double)cb.ep[t] * L / cb.ediv);
+      const double2 v = cb.in[t][i];
+      hacc += dw * (yb.x * v.x + yb.y * v.y);
+    }
+  }
+  if (cb.hb) cb.hb[i] += cb.hscale ? cb.hscale[b] * hacc : hacc;
+}
+
 template <int N>
 __global__ __launch_bounds__(256) void comb_adj_kernel(CombAdj cb, int B) {
   using G = Geo<N>;
@@ -54,33 +87,7 @@ __global__ __launch_bounds__(256) void comb_adj_kernel(CombAdj cb, int B) {
     const int ky = (int)(e % KC);
     const int r = (int)((e / KC) % R);
     const int c = (int)(e / ((size_t)KC * R));
-    const int kx = G::row_kx(r);
-    const bool dc = (kx == 0 && ky == 0);
-    const double L = -(double)(kx * kx + ky * ky) * cb.nu[c];
-    const double h = cb.h[b];
-    const double E1 = exp(h / cb.ediv * L);
-    const double2 yb = cb.yb[i];
-    double hacc = 0.0;
-    for (int t = 0; t < cb.nterm; ++t) {
-      double w = cb.coef[t];
-      for (int q = 0; q < cb.hp[t]; ++q) w *= h;
-      double em = 1.0;
-      if (cb.ep[t] > 0) {
-        double Ep = E1;
-        for (int q = 1; q < cb.ep[t]; ++q) Ep *= E1;
-        em = (cb.masked && dc) ? 0.0 : Ep;
-        w *= em;
-      }
-      if (cb.out[t]) cb.out[t][i] = ((cb.assign >> t) & 1u) ? cscale(w, yb) : cadd(cb.out[t][i], cscale(w, yb));
-      if (cb.in[t] && cb.hb) {
-        double dw = cb.coef[t] * (double)cb.hp[t];
-        for (int q = 1; q < cb.hp[t]; ++q) dw *= h;
-        if (cb.ep[t] > 0) dw = dw * em + w * ((double)cb.ep[t] * L / cb.ediv);
-        const double2 v = cb.in[t][i];
-        hacc += dw * (yb.x * v.x + yb.y * v.y);
-      }
-    }
-    if (cb.hb) cb.hb[i] += cb.hscale ? cb.hscale[b] * hacc : hacc;
+    comb_adj_elem(cb, b, i, c, G::row_kx(r), ky, cb.yb[i]);
   }
 }
 

@@ -700,6 +700,19 @@ template <int N, int MODE>
 static int acol(const Args& a, int B, hipStream_t st) {
   return B < 16 ? acol_t<N, MODE, 1>(a, B, st) : acol_t<N, MODE, Tiles<N>::TY>(a, B, st);
 }
+// B_OUT and the transposed stage combination `cb` (whose yb is B_OUT's output) in one launch
+template <int N, int TY>
+static int acol_comb_t(const Args& a, const CombAdj& cb, int B, hipStream_t st) {
+  dim3 grid((Geo<N>::KC + TY - 1) / TY, 2, B);
+  if (plan_of(N, K_COL) == 1) hipLaunchKernelGGL((adj_out_comb_kernel<N, TY, NT, 8>), grid, dim3(NT), 0, st, a, cb);
+  else hipLaunchKernelGGL((adj_out_comb_kernel<N, TY, NT, 16>), grid, dim3(NT), 0, st, a, cb);
+  LAUNCHCHK();
+  return 0;
+}
+template <int N>
+static int acol_comb(const Args& a, const CombAdj& cb, int B, hipStream_t st) {
+  return B < 16 ? acol_comb_t<N, 1>(a, cb, B, st) : acol_comb_t<N, Tiles<N>::TY>(a, cb, B, st);
+}
 template <int N, int TX>
 static int arow_t(const Args& a, int B, hipStream_t st) {
   dim3 grid(N / TX, 1, B);
@@ -1262,10 +1275,13 @@ struct CombAdjBuilder {
 // hess: out = (J_0(x))^T g with J_0 the Jacobian of the nonlinearity without
 // b0 (the transpose of y -> H[y, x]).  prim: the column pass's output at x
 // (the 8 derived fields in x-space), when a forward pass left it; otherwise
-// it is recomputed.  acc: out += instead of out =.
+// it is recomputed.  acc: out += instead of out =.  then: a transposed stage
+// combination whose yb is `out`, applied by the pass's last kernel (the same
+// arithmetic as comb_adj_launch after the pass, one launch fewer).
 template <int N>
 static int vjp_pass(ecs_ctx* c, const Args& base, double2* x, const double2* g, double2* out, int B, int diss,
-                    hipStream_t st, bool hess = false, const double2* prim = nullptr, bool acc = false) {
+                    hipStream_t st, bool hess = false, const double2* prim = nullptr, bool acc = false,
+                    const CombAdj* then = nullptr) {
   Args a = base;
   a.x0 = x; a.gin = g; a.out = out; a.out_acc = acc ? 1 : 0;
   a.include_diss = hess ? 0 : diss;
@@ -1275,7 +1291,10 @@ static int vjp_pass(ecs_ctx* c, const Args& base, double2* x, const double2* g, 
   if (!prim) TRY((col<N, C_LOAD, false>(a, B, st)));
   TRY((acol<N, B_G>(a, B, st)));
   TRY(arow<N>(a, B, st));
+  // per-candidate step counts end a candidate's B_OUT early, not its combination
+  if (then && !a.nstep && then->yb == out) return acol_comb<N>(a, *then, B, st);
   TRY((acol<N, B_OUT>(a, B, st)));
+  if (then) TRY(comb_adj_launch<N>(*then, B, st));
   return 0;
 }
 
@@ -1463,7 +1482,6 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
         TRY(comb_adj_launch<N>(fin.cb, B, st));
         for (int k = S - 1; k >= 0; --k) {
           if (!touched[k]) HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));  // a stage no combination reads
-          TRY(vjp_pass<N>(c, a, k == 0 ? xs : y(k), vb(k), yb, B, diss, st, false, ti(k)));
           CombAdjBuilder st_(c, ce, ediv, msk, yb, c->hb);
           st_.term(xbn, k == 0 ? nullptr : xs, 1.0, 0, tb.p[k]);
           for (int j = 0; j < k; ++j)
@@ -1471,7 +1489,7 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
               st_.term(vb(j), v(j), tb.A[k][j], 1, tb.p[k] - tb.p[j], !touched[j]);
               touched[j] = 1;
             }
-          TRY(comb_adj_launch<N>(st_.cb, B, st));
+          TRY(vjp_pass<N>(c, a, k == 0 ? xs : y(k), vb(k), yb, B, diss, st, false, ti(k), false, &st_.cb));
         }
       } else {
         // x' = x + h v1 + h^2/6 w1 + h^2/3 w2 (vbar2 is written whole by its VJP pass below)
@@ -1482,11 +1500,10 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
         // w2 = J(y) v2, v2 = f(y): vbar2 = J(y)^T wbar2; ybar = J(y)^T vbar2 + J_0(v2)^T wbar2
         TRY(vjp_pass<N>(c, a, y(0), wb(1), vb(1), B, diss, st, false, ti(1)));
         TRY(vjp_pass<N>(c, a, y(0), vb(1), yb, B, diss, st, false, ti(1)));
-        TRY(vjp_pass<N>(c, a, v(1), wb(1), yb, B, diss, st, true, nullptr, true));
-        // y = x + h/2 v1 + h^2/8 w1
+        // y = x + h/2 v1 + h^2/8 w1 (the transposed combination applied by the pass's last kernel)
         CombAdjBuilder cy(c, ce, 1.0, 0, yb, c->hb);
         cy.term(xbn, nullptr, 1.0, 0, 0).term(vb(0), v(0), 0.5, 1, 0).term(wb(0), w(0), 0.125, 2, 0);
-        TRY(comb_adj_launch<N>(cy.cb, B, st));
+        TRY(vjp_pass<N>(c, a, v(1), wb(1), yb, B, diss, st, true, nullptr, true, &cy.cb));
         // w1 = J(x) v1: vbar1 += J(x)^T wbar1, xbar += J_0(v1)^T wbar1
         TRY(vjp_pass<N>(c, a, xs, wb(0), vb(0), B, diss, st, false, ti(0), true));
         TRY(vjp_pass<N>(c, a, v(0), wb(0), xbn, B, diss, st, true, nullptr, true));
@@ -1695,17 +1712,6 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
         if (!touched[k]) HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));  // a stage no combination reads
         Args am = a;
         am.dt = DT(m);
-        if (k == 0) {
-          // v1 = P vraw: vbraw = P vbar1, d/dT through T s
-          CombAdjBuilder pc(c, ce, 1.0, 0, vb(0), c->hb);
-          pc.cb.h = PDT(m);
-          pc.cb.hscale = SIG(m);
-          pc.term(vbraw, vraw, 1.0, 0, 1, true);
-          TRY(comb_adj_launch<N>(pc.cb, B, st));
-          TRY(vjp_pass<N>(c, am, xs, vbraw, yb, B, 0, st, false, ti(0)));
-        } else {
-          TRY(vjp_pass<N>(c, am, y(k), vb(k), yb, B, 0, st, false, ti(k)));
-        }
         CombAdjBuilder sa(c, ce, ediv, 0, yb, c->hb);
         sa.cb.h = DT(m);
         sa.cb.hscale = HSC(m);
@@ -1715,7 +1721,18 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
             sa.term(vb(j), v(j), tb.A[k][j], 1, tb.p[k] - tb.p[j], !touched[j]);
             touched[j] = 1;
           }
-        TRY(comb_adj_launch<N>(sa.cb, B, st));
+        // the pass's last kernel applies sa
+        if (k == 0) {
+          // v1 = P vraw: vbraw = P vbar1, d/dT through T s
+          CombAdjBuilder pc(c, ce, 1.0, 0, vb(0), c->hb);
+          pc.cb.h = PDT(m);
+          pc.cb.hscale = SIG(m);
+          pc.term(vbraw, vraw, 1.0, 0, 1, true);
+          TRY(comb_adj_launch<N>(pc.cb, B, st));
+          TRY(vjp_pass<N>(c, am, xs, vbraw, yb, B, 0, st, false, ti(0), false, &sa.cb));
+        } else {
+          TRY(vjp_pass<N>(c, am, y(k), vb(k), yb, B, 0, st, false, ti(k), false, &sa.cb));
+        }
       }
       std::swap(xb, xbn);
     }
