@@ -161,7 +161,31 @@ struct Pre {
   double2 acc, dacc;      // RK accumulator
   double2 xf, dxf;        // tangent replay C_S5A: the accepted state
   double2 v1, dv1;        // tangent replay C_A: the carried k1
+  // C_GEN with one glue element per thread: the operands of the combination's
+  // terms (in_t, din_t), loaded with the rest; the terms reading this
+  // kernel's own output (in_t == out) are formed in the glue instead
+  double2 gv[COMB_MAX], gdv[COMB_MAX];
 };
+
+// The C_GEN operands of Pre (element ci), when each thread has one glue element.
+template <bool TAN>
+__device__ __forceinline__ void prefetch_comb(const Args& a, size_t ci, Pre& p) {
+  const Comb* cm = a.gc;
+  const int nt = cm->nterm;
+#pragma unroll
+  for (int t = 0; t < COMB_MAX; ++t) {
+    p.gv[t] = make_double2(0.0, 0.0);
+    p.gdv[t] = p.gv[t];
+    if (t < nt) {
+      const double2* in = cm->in[t];
+      if (in && in != a.out) p.gv[t] = in[ci];
+      if (TAN) {
+        const double2* din = cm->din[t];
+        if (din && din != a.dout) p.gdv[t] = din[ci];
+      }
+    }
+  }
+}
 
 template <int MODE, bool TAN, int AK>
 __device__ __forceinline__ Pre prefetch(const Args& a, size_t ci, size_t fi, int c) {
@@ -222,7 +246,7 @@ __device__ __forceinline__ void replay_rejections(const Args& a, int bb, unsigne
 // Lawson RK4 in the form of timestepping.py:70-80 and :263-282:
 //   y2 = e x0 + e k1/2, y3 = e x0 + k2/2, y4 = e e x0 + e k3,
 //   xf = e e x0 + e e k1/6 + e k2/3 + e k3/3 + k4/6,   k_s = dt v(y_s).
-template <int N, int MODE, bool TAN, int AK>
+template <int N, int MODE, bool TAN, int AK, bool GPF = false>
 __device__ __forceinline__ void glue(const Args& a, bool wr, const Blk& bk, const Pre& p, int bb, size_t ci, int r,
                                      int c, int kx, int ky, double2 rh, double2 drh, double2& y, double2& dy) {
   const double k2 = (double)(kx * kx + ky * ky);
@@ -288,7 +312,8 @@ __device__ __forceinline__ void glue(const Args& a, bool wr, const Blk& bk, cons
     const double dh = TAN ? cm->dh[bb] : 0.0;
     const double E1 = exp(h / cm->ediv * L);
     double2 acc = make_double2(0.0, 0.0), dacc = acc;
-    for (int t = 0; t < cm->nterm; ++t) {
+    // term t with its operands prefetched (vpre, dvpre; GPF) or loaded here
+    auto term = [&](int t, double2 vpre, double2 dvpre) {
       double w = cm->coef[t];
       for (int q = 0; q < cm->hp[t]; ++q) w *= h;
       if (cm->ep[t] > 0) {
@@ -296,7 +321,7 @@ __device__ __forceinline__ void glue(const Args& a, bool wr, const Blk& bk, cons
         for (int q = 1; q < cm->ep[t]; ++q) Ep *= E1;
         w *= (cm->masked && dc) ? 0.0 : Ep;
       }
-      const double2 v = cm->in[t] == a.out ? o : cm->in[t][ci];
+      const double2 v = cm->in[t] == a.out ? o : (GPF ? vpre : cm->in[t][ci]);
       acc = cadd(acc, cscale(w, v));
       if (TAN) {
         double dw = cm->coef[t] * (double)cm->hp[t];
@@ -308,8 +333,17 @@ __device__ __forceinline__ void glue(const Args& a, bool wr, const Blk& bk, cons
           dw = dw * em + w * ((double)cm->ep[t] * L / cm->ediv);
         }
         dacc = cadd(dacc, cscale(dh * dw, v));
-        if (cm->din[t]) dacc = cadd(dacc, cscale(w, cm->din[t] == a.dout ? od : cm->din[t][ci]));
+        if (cm->din[t]) dacc = cadd(dacc, cscale(w, cm->din[t] == a.dout ? od : (GPF ? dvpre : cm->din[t][ci])));
       }
+    };
+    const int nt = cm->nterm;
+    if constexpr (GPF) {
+      // unrolled, so the prefetched operands stay in registers
+#pragma unroll
+      for (int t = 0; t < COMB_MAX; ++t)
+        if (t < nt) term(t, p.gv[t], p.gdv[t]);
+    } else {
+      for (int t = 0; t < nt; ++t) term(t, make_double2(0.0, 0.0), make_double2(0.0, 0.0));
     }
     a.xf[ci] = acc;
     y = acc;
@@ -493,6 +527,10 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   static_assert(SPL == 1 || (col_splittable<MODE>() && NDO % SPL == 0), "column split not allowed for this mode");
   constexpr int DPS = NDO / SPL;               // derived outputs per block
   constexpr int EPT = (R * TY + NT - 1) / NT;  // glue elements per thread
+  // C_GEN loads its combination operands before the FFT when a thread has one
+  // glue element (the small tiles of small batches, where a kernel is a chain
+  // of dependent memory round trips); wider tiles load them in the glue
+  constexpr bool GPF = MODE == C_GEN && EPT == 1;
   // A block's derived outputs are transformed in TCH chunks of at most 8
   // sequences that reuse one LDS region (the glue results wait in registers),
   // so tiles of 2-4 columns keep the LDS of a block small enough for several
@@ -528,6 +566,7 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
     if (idx < R * TY && ky < KC) {
       const size_t ci = (((size_t)b * 2 + c) * R + r) * KC + ky;
       pre[q] = prefetch<MODE, TAN, AK>(a, ci, (size_t)r * KC + ky, c);
+      if constexpr (GPF) prefetch_comb<TAN>(a, ci, pre[q]);
     }
   }
 
@@ -573,7 +612,7 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
       if (NFI == 2) drh = sm[(1 * TY + cy) * SS + pidx(xr)];
     }
     double2 y, dy;
-    glue<N, MODE, TAN, AK>(a, wr, bk, pre[q], b, ci, r, c, kx, ky, rh, drh, y, dy);
+    glue<N, MODE, TAN, AK, GPF>(a, wr, bk, pre[q], b, ci, r, c, kx, ky, rh, drh, y, dy);
     if (TCH > 1) {
       yv[q] = y;
       dyv[q] = dy;
