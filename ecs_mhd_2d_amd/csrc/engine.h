@@ -43,7 +43,9 @@ static int row_v(const Args& a, int B, hipStream_t st) {
 // sequences split over 2 (even v) or 4 (odd v) blocks, plan wide (6, 7) or
 // narrow (8, 9); for N >= 128 v = 10, 11: tiles of 8 columns, plan wide (10)
 // or narrow (11); v = 12..15: tiles of 2 columns split over 2 (12, 14) or 4
-// (13, 15) blocks, plan wide (12, 13) or narrow (14, 15).
+// (13, 15) blocks, plan wide (12, 13) or narrow (14, 15); for N >= 128 and the
+// primal family v = 16: tiles of 8 columns, narrow plan, the middle in
+// registers (col_kernel_fm; variant 11 for the modes it does not cover).
 template <int N, int MODE, bool TAN, int AK>
 static int col_var(int v, const Args& a, int B, hipStream_t st) {
   switch (v) {
@@ -61,6 +63,18 @@ static int col_var(int v, const Args& a, int B, hipStream_t st) {
       if constexpr (N >= 128) {
         if (v == 10) return col_v<N, 8, MODE, TAN, 16, AK>(a, B, st);
         if (v == 11) return col_v<N, 8, MODE, TAN, 8, AK>(a, B, st);
+        if (v == 16) {
+          // tiles of 8 columns, narrow plan, the middle in registers where the
+          // mode allows it (the same arithmetic as variant 11 otherwise)
+          if constexpr (!TAN && col_fm_mode<MODE>()) {
+            dim3 grid((Geo<N>::KC + 7) / 8, 2, B);
+            hipLaunchKernelGGL((col_kernel_fm<N, 8, NT, MODE, 8, AK>), grid, dim3(NT), 0, st, a);
+            LAUNCHCHK();
+            return 0;
+          } else {
+            return col_v<N, 8, MODE, TAN, 8, AK>(a, B, st);
+          }
+        }
       }
       switch (v) {
         case 12: return col_v<N, 2, MODE, TAN, 16, AK, 2>(a, B, st);

@@ -350,7 +350,8 @@ __device__ __forceinline__ void fft_lds_tail(const double2* __restrict__ buf, co
 
 // fft_lds_gather without the last pass: the gather pass and, for a
 // three-pass plan, the middle pass.  The caller runs the last pass itself.
-template <int N, int NSEQ, int SSTRIDE, int NT, bool INV, int P, class F, class H>
+// SEQFAST as in fft_lds_gather.
+template <int N, int NSEQ, int SSTRIDE, int NT, bool INV, int P, bool SEQFAST = false, class F, class H>
 __device__ __forceinline__ void fft_lds_gather_head(double2* __restrict__ buf, const double2* __restrict__ twl,
                                                     F&& gather, H&& before_barrier) {
   using TL = TwLayout<N, P>;
@@ -364,8 +365,8 @@ __device__ __forceinline__ void fft_lds_gather_head(double2* __restrict__ buf, c
   for (int q = 0; q < BPT; ++q) {
     const int bfly = tid + q * NT;
     if (bfly < NB) {
-      const int s = bfly / (N / R0);
-      const int j = bfly - s * (N / R0);
+      const int s = SEQFAST ? bfly % NSEQ : bfly / (N / R0);
+      const int j = SEQFAST ? bfly / NSEQ : bfly - s * (N / R0);
 #pragma unroll
       for (int r = 0; r < R0; ++r) v[q][r] = gather(s, j + r * (N / R0));
     }
@@ -374,8 +375,8 @@ __device__ __forceinline__ void fft_lds_gather_head(double2* __restrict__ buf, c
   for (int q = 0; q < BPT; ++q) {
     const int bfly = tid + q * NT;
     if (bfly < NB) {
-      const int s = bfly / (N / R0);
-      const int j = bfly - s * (N / R0);
+      const int s = SEQFAST ? bfly % NSEQ : bfly / (N / R0);
+      const int j = SEQFAST ? bfly / NSEQ : bfly - s * (N / R0);
       Dft<R0, INV>::run(v[q]);
       double2* dst = buf + s * SSTRIDE;
 #pragma unroll

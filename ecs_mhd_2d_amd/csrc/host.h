@@ -173,20 +173,26 @@ int upload_ints(ecs_ctx* c, const int* vals, int B, hipStream_t st);
 // the padded x-space arrays), plan v - 10; they pay at large batches, where
 // wider tiles move the same bytes in fewer, fuller transactions.  And v =
 // 12..15: tiles of 2 columns split over 2 (even v) or 4 (odd v) blocks, plan
-// (v - 12) / 2, for small batches (32-byte runs instead of 16-byte ones).  The autotuner picks the
+// (v - 12) / 2, for small batches (32-byte runs instead of 16-byte ones).  For
+// N >= 128 the primal column family also has v = 16: tiles of 8 columns, narrow
+// plan, with the forward transform's last pass, the glue and the inverse's
+// first pass in registers (the same arithmetic as v = 11).  The autotuner picks the
 // plan once per (N, kernel family), so every batch size computes with the
 // same arithmetic, and the variant within that plan per (N, kernel family,
 // batch bucket).
 enum TuneKind : int { K_COL = 0, K_COLT = 1, K_ROW = 2, K_ROWT = 3 };
-constexpr int VMAX = 16;  // variant indices of one family lie in [0, VMAX)
+constexpr int VMAX = 17;  // variant indices of one family lie in [0, VMAX)
 inline bool var_ok(int kind, int n, int v) {
   if (v < 0 || v >= VMAX) return false;
   if (kind == K_ROW || kind == K_ROWT) return v < 6;
+  if (v == 16) return kind == K_COL && n >= 128;
   return v < 10 || v >= 12 || n >= 128;
 }
-inline int var_plan(int v) { return v < 6 ? v / 3 : v < 10 ? (v - 6) / 2 : v < 12 ? v - 10 : (v - 12) / 2; }
-inline int var_tile(int v) { return v < 6 ? 1 << (v % 3) : v < 10 ? 1 : v < 12 ? 8 : 2; }
-inline int var_split(int v) { return (v < 6 || v == 10 || v == 11) ? 1 : (v % 2 == 0 ? 2 : 4); }
+inline int var_plan(int v) {
+  return v < 6 ? v / 3 : v < 10 ? (v - 6) / 2 : v < 12 ? v - 10 : v < 16 ? (v - 12) / 2 : 1;
+}
+inline int var_tile(int v) { return v < 6 ? 1 << (v % 3) : v < 10 ? 1 : v < 12 ? 8 : v < 16 ? 2 : 8; }
+inline int var_split(int v) { return (v < 6 || v == 10 || v == 11 || v == 16) ? 1 : (v % 2 == 0 ? 2 : 4); }
 // padded row stride of the x-space arrays of grid n (Geo<N>::KP)
 inline int kp_of(int n) { return (((n - 1) / 3 + 1) + 7) & ~7; }
 // the variants of family `kind` of grid n in plan p (every plan for p < 0);

@@ -672,6 +672,140 @@ __global__ __launch_bounds__(NT) void col_kernel(Args a) {
   }
 }
 
+// Modes the column kernel with its middle in registers (col_kernel_fm) runs:
+// primal, forward x-FFT, glue and four derived outputs transformed back.
+template <int MODE>
+constexpr bool col_fm_mode() {
+  return MODE == C_S1F || MODE == C_S2 || MODE == C_S3 || MODE == C_S4 || MODE == C_A0 || MODE == C_A ||
+         MODE == C_GEN;
+}
+
+// Column kernel for tiles of TY columns (primal, three-pass plan) with its
+// middle in registers.  The forward x-FFT's last pass (radix RF, in place:
+// butterfly jf reads and writes positions jf + r N/RF) and the inverse x-FFT's
+// first pass (radix R0, butterfly j reads positions j + m N/R0) meet when one
+// thread takes the G = R0/RF forward butterflies jf = j + g N/R0: it then holds
+// the R0 positions p_m = j + m N/R0 (m = g + G r) of its column, applies the
+// stage glue to them and, per derived output, the inverse's first butterfly,
+// and writes only that butterfly's outputs to LDS.  The forward output never
+// goes through LDS and the derived inputs are never read back from it: per
+// block one LDS round trip and its barriers fewer for the forward transform
+// and for each of the four derived outputs.  Every butterfly, twiddle and
+// glue operation is the one col_kernel performs, in the same order, so the
+// results are the same bits.  Positions in the dealiased gap are zero (slots
+// m whose whole range lies in the gap are skipped at compile time).
+template <int N, int TY, int NT, int MODE, int P, int AK>
+__global__ __launch_bounds__(NT) void col_kernel_fm(Args a) {
+  using G = Geo<N>;
+  using TL = TwLayout<N, P>;
+  constexpr int KC = G::KC, KP = G::KP, KMAX = G::KMAX;
+  constexpr int RF = TL::R2, R0 = TL::R0, R1 = TL::R1;
+  static_assert(RF > 1, "a plan of three passes");
+  static_assert(R0 % RF == 0, "the forward's last radix must divide the inverse's first");
+  static_assert(col_fm_mode<MODE>(), "mode without the register middle");
+  constexpr int GG = R0 / RF;    // forward butterflies per work item
+  constexpr int NI = N / R0;     // inverse first-pass butterflies per sequence
+  constexpr int NSF = N / RF;    // the forward last pass's sub-transform length (its NS)
+  static_assert(TY * NI <= NT, "one work item per thread");
+  constexpr int SS = Seq<N>::S;
+  __shared__ double2 sm[TY * SS];
+  __shared__ double2 twl[N];
+  TwFetch<N, NT, P> twf;
+  twf.fetch(a.tw);
+  int lx, ly, lz;
+  xcd_tile(lx, ly, lz);
+  const int b = lz, c = ly, ky0 = lx * TY, tid = threadIdx.x;
+  Blk bk = prologue<MODE, false, AK>(a, b, lx == 0 && ly == 0);
+  if (!bk.run) return;
+  // work item: column cy, inverse butterfly j
+  const int cy = tid % TY, j = tid / TY;
+  const int ky = ky0 + cy;
+  const bool act = tid < TY * NI && ky < KC;
+  // slot m of the item may hold a kept mode (p_m <= KMAX or p_m >= N - KMAX for some j)
+  auto maybe = [](int m) constexpr { return m * NI <= KMAX || m * NI + NI - 1 >= N - KMAX; };
+  auto kept = [&](int m) { const int p = j + m * NI; return act && (p <= KMAX || p >= N - KMAX); };
+  auto crow = [&](int m) { const int p = j + m * NI; return p <= KMAX ? p : p - N + G::R; };
+  Pre pre[R0];
+#pragma unroll
+  for (int m = 0; m < R0; ++m) {
+    if (!maybe(m) || !kept(m)) continue;
+    const int r = crow(m);
+    const size_t ci = (((size_t)b * 2 + c) * G::R + r) * KC + ky;
+    pre[m] = prefetch<MODE, false, AK>(a, ci, (size_t)r * KC + ky, c);
+  }
+  const double2* O = a.rowout + (size_t)b * 2 * N * KP;
+  fft_lds_gather_head<N, TY, SS, NT, false, P, (TY > 1)>(
+      sm, twl,
+      [&](int s, int x) -> double2 {
+        const int kys = ky0 + s;
+        return kys < KC ? O[((size_t)c * N + x) * KP + kys] : make_double2(0.0, 0.0);
+      },
+      [&] { twf.store(twl); });
+  finish<AK>(bk);
+  // the forward's last pass, G butterflies of this item's column, into registers
+  double2 vf[R0];
+  if (tid < TY * NI) {
+    const double2* src = sm + cy * SS;
+#pragma unroll
+    for (int g = 0; g < GG; ++g) {
+      const int jf = j + g * NI;
+      double2 v[RF];
+#pragma unroll
+      for (int r = 0; r < RF; ++r) v[r] = src[pidx(jf + r * NSF)];
+#pragma unroll
+      for (int r = 1; r < RF; ++r) v[r] = cmul(v[r], twl[TL::OFF3 + (r - 1) * NSF + jf]);
+      Dft<RF, false>::run(v);
+#pragma unroll
+      for (int r = 0; r < RF; ++r) vf[g + GG * r] = v[r];
+    }
+  }
+  const bool inv = bk.stage && !a.skip_inv;
+  constexpr double invN2 = 1.0 / ((double)N * (double)N);
+  auto derived = [&](int d, int kx, int kyy, double2 y) -> double2 {
+    const bool dc = (kx == 0 && kyy == 0);
+    const double k2s = dc ? 1.0 : (double)(kx * kx + kyy * kyy);
+    const double md = d == 0 ? (double)kx : d == 1 ? (double)kyy : d == 2 ? (double)kyy / k2s : -(double)kx / k2s;
+    return cscale(md * invN2, cmuli(y));
+  };
+  // glue on the kept modes of the item
+  double2 yv[R0];
+#pragma unroll
+  for (int m = 0; m < R0; ++m) {
+    yv[m] = make_double2(0.0, 0.0);
+    if (!maybe(m) || !kept(m)) continue;
+    const int r = crow(m);
+    const int kx = G::row_kx(r);
+    const size_t ci = (((size_t)b * 2 + c) * G::R + r) * KC + ky;
+    double2 dy;
+    glue<N, MODE, false, AK>(a, true, bk, pre[m], b, ci, r, c, kx, ky, vf[m], make_double2(0.0, 0.0), yv[m], dy);
+  }
+  if (!inv) return;
+  __syncthreads();  // every forward value is in registers before the buffer is rewritten
+  for (int d = 0; d < 4; ++d) {
+    if (d > 0) __syncthreads();  // the previous derived output's last pass has read the buffer
+    if (tid < TY * NI) {
+      // the inverse's first pass (no twiddles) on this item's butterfly j
+      double2 u[R0];
+#pragma unroll
+      for (int m = 0; m < R0; ++m) {
+        u[m] = make_double2(0.0, 0.0);
+        if (maybe(m) && kept(m)) u[m] = derived(d, G::row_kx(crow(m)), ky, yv[m]);
+      }
+      Dft<R0, true>::run(u);
+      double2* dst = sm + cy * SS;
+#pragma unroll
+      for (int m = 0; m < R0; ++m) dst[pidx(j * R0 + m)] = u[m];
+    }
+    __syncthreads();
+    stockham_pass<N, R1, R0, TY, SS, NT, true, TL::OFF2>(sm, twl);
+    fft_lds_tail<N, TY, SS, NT, true, P, TY>(sm, twl, [&](int s, int x, double2 v) {
+      const int kys = ky0 + s;
+      if (kys >= KC) return;
+      a.inter[(((size_t)b * 8 + 4 * c + d) * N + x) * KP + kys] = v;
+    });
+  }
+}
+
 // -advection[0] + advection[1] and u x B (mhd_jax.py:152-163) at one point
 // from the packed real fields (wx, wy), (u, v), (jx, jy), (Bx, By) and, with
 // TAN, their linearisation along the tangent fields z4..z7

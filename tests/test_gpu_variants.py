@@ -25,6 +25,9 @@ CASES = [
     ("128:0,3,0,0;256:3,0,3,3", "128:10,11,0,0;256:11,10,3,3"),
     # tiles of 2 columns split over 2 or 4 blocks: 12, 13 wide, 14, 15 narrow
     ("32:0,3,0,0;64:3,0,3,3;128:3,0,3,3", "32:12,15,0,0;64:14,13,3,3;128:15,12,3,3"),
+    # tiles of 8 columns with the middle in registers (16, primal family only)
+    # against the same tiles through LDS (11)
+    ("128:11,0,3,3;256:11,0,3,3", "128:16,0,3,3;256:16,0,3,3"),
 ]
 
 

@@ -4,7 +4,8 @@ own process (variants are fixed once per process and grid size):
     ECSMHD_VARIANTS="64:9,9,3,3" python tests/variant_check.py out.npz
 
 For every grid size named in ECSMHD_VARIANTS: the RHS and its tangent at
-batch 3, a 32-step Lawson RK4 trajectory with its tangent, and an adaptive
+batch 3, a 32-step Lawson RK4 trajectory with its tangent, the primal
+trajectory at batch 3, 16 steps of classical RK4 and of Lawson RK6, and an adaptive
 Lawson RK4(3) solve with the tangent replay of its schedule (batch 2, uneven
 periods, so one candidate finishes early).  The fixed-step trajectory is also
 checked against the oracle (1e-11, tangent 1e-10), and the primal of the
@@ -47,6 +48,13 @@ def main():
         xr, dxr = O.lawson_rk4_jvp(x0, dx0, dt * steps, ddt * steps, steps, pd, True)
         res[f"rk4{n}"], res[f"rk4_t{n}"] = xg.cpu().numpy(), dxg.cpu().numpy()
         e_x, e_dx = rel(res[f"rk4{n}"], xr), rel(res[f"rk4_t{n}"], dxr)
+        # primal fixed-step Lawson RK4, classical RK4 and Lawson RK6 at batch 3
+        # (the column kernels' primal stage and generic-stage modes)
+        xp = ctx.lawson_rk4(torch.as_tensor(np.stack([x0, spectrum(n, 13), spectrum(n, 14)])).cuda(), dt, steps)
+        res[f"rk4p{n}"] = xp.cpu().numpy()
+        e_p = rel(res[f"rk4p{n}"][0], xr)
+        res[f"crk4{n}"] = ctx.rk4(f, dt, 16).cpu().numpy()
+        res[f"lrk6{n}"] = ctx.lawson_rk6(f, dt, 16).cpu().numpy()
         xa = torch.as_tensor(np.stack([spectrum(n, 9), spectrum(n, 10)])).cuda()
         dxa = torch.as_tensor(np.stack([spectrum(n, 11), spectrum(n, 12)])).cuda()
         T = np.array([0.3, 0.5])
@@ -55,9 +63,9 @@ def main():
         res[f"rk43{n}"], res[f"rk43_jvp{n}"], res[f"rk43_jvp_t{n}"] = (xT.cpu().numpy(), xj.cpu().numpy(),
                                                                      dxj.cpu().numpy())
         e_rep = rel(res[f"rk43_jvp{n}"], res[f"rk43{n}"])
-        good = e_x < 1e-11 and e_dx < 1e-10 and e_rep < 1e-12 and all(i["completed"] for i in infos)
+        good = e_x < 1e-11 and e_p < 1e-11 and e_dx < 1e-10 and e_rep < 1e-12 and all(i["completed"] for i in infos)
         ok = ok and good
-        print(f"n={n} rk4 {e_x:.2e} tangent {e_dx:.2e} replayed primal {e_rep:.1e} "
+        print(f"n={n} rk4 {e_x:.2e} primal batch {e_p:.2e} tangent {e_dx:.2e} replayed primal {e_rep:.1e} "
               f"steps {[i['accepted'] for i in infos]} {'ok' if good else 'FAIL'}", flush=True)
     np.savez(out_path, **res)
     return 0 if ok else 1
