@@ -540,6 +540,10 @@ def main():
     row_t_ms = ms.value
     bytes_row_t = 20 * zbytes(n)   # reads 16 Z (8 primal + 8 tangent derived fields), writes 4 Z
     ach = bytes_row_t / (row_t_ms * 1e-3) / 1e9
+    # the launch floor beside it: an empty kernel with the same grid and argument block
+    _lib.check(lib.ecs_time_kernel(ctx.handle, 1, 5, 200, ctypes.byref(ms), _runtime._stream()),
+               "ecs_time_kernel")
+    floor_ms = ms.value
     # the same kernel family at full occupancy: primal row kernel, N=256, batch 64
     _lib.check(lib.ecs_time_kernel(ctx256.handle, 64, 1, 50, ctypes.byref(ms), _runtime._stream()),
                "ecs_time_kernel")
@@ -595,7 +599,10 @@ def main():
                      "timing": "HIP events on the launching stream around 200 back-to-back launches of the "
                                "replay instance",
                      "avg_ms_in_matvec_with_dispatch": probe_ms.value, "in_matvec_samples": probe_n.value,
-                     "algorithmic_bytes": bytes_row_t},
+                     "algorithmic_bytes": bytes_row_t,
+                     "launch_floor_ms": floor_ms,
+                     "launch_floor": "an empty kernel with the same grid and argument block, 200 back-to-back "
+                                     "launches timed the same way: what a launch costs before any work"},
         "roofline_batched": {"bound": "hbm", "achieved": ach_b, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": ach_b / HBM_PEAK_GBS, "traffic": TRAFFIC_ROW_N256_B64,
                              "kernel": f"row_kernel<256,TX={tiles['n256_b64']['row_tx']}> (batch 64)",

@@ -634,7 +634,7 @@ extern "C" int ecs_lawson_rk4_vjp_ckpt(ecs_ctx* c, const double* ck, double* fba
 }
 
 extern "C" int ecs_time_kernel(ecs_ctx* c, int batch, int which, int reps, double* ms, void* stream) {
-  if (!c || !ms || batch < 1 || reps < 1 || which < 0 || which > 4) return fail(1, "ecs_time_kernel: bad arguments");
+  if (!c || !ms || batch < 1 || reps < 1 || which < 0 || which > 5) return fail(1, "ecs_time_kernel: bad arguments");
   GUARD(c);
   hipStream_t st = (hipStream_t)stream;
   return OPS(c->n)->time_kernel(c, batch, which, reps, ms, st);

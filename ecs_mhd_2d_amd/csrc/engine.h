@@ -932,12 +932,18 @@ static int run_rk4_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
 }
 
 // ------------------------------------------------------------------ timing hook
+// An empty kernel with the fused kernels' argument block: back to back, its
+// launches measure the floor a kernel launch costs on this stream.
+template <int N>
+__global__ __launch_bounds__(NT) void empty_kernel(Args a) {}
+
 // Average duration (ms) of one launch of a fused RHS kernel, timed with HIP
 // events on `stream` over `reps` back-to-back launches of the same kernel on
 // batch spectra already resident in the context's workspace.
 // which: 0 = column kernel (stage 2 glue), 1 = row kernel, 2 = tangent column
 // kernel, 3 = tangent row kernel (fixed-step instance), 4 = tangent row kernel
-// of the adaptive tangent replay (the instance a GMRES matvec runs).
+// of the adaptive tangent replay (the instance a GMRES matvec runs), 5 = the
+// empty kernel with the tangent row kernel's grid (the launch floor).
 template <int N>
 static int run_time_kernel(ecs_ctx* c, int B, int which, int reps, double* ms, hipStream_t st) {
   TRY(ensure_cap(c, B));
@@ -968,7 +974,13 @@ static int run_time_kernel(ecs_ctx* c, int B, int which, int reps, double* ms, h
       case 1: return row<N, R_RHS, false>(a, B, st);
       case 2: return col<N, C_S2, true>(a, B, st);
       case 3: return row<N, R_RHS, true>(a, B, st);
-      default: return row<N, R_RHS, true>(ar, B, st);
+      case 4: return row<N, R_RHS, true>(ar, B, st);
+      default: {
+        const int tx = var_tile(tuned(N, K_ROWT, B, DefTile<N>::ROW));
+        hipLaunchKernelGGL(empty_kernel<N>, dim3(N / tx, 1, B), dim3(NT), 0, st, ar);
+        LAUNCHCHK();
+        return 0;
+      }
     }
   };
   for (int i = 0; i < 3; ++i) TRY(launch());

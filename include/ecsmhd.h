@@ -206,7 +206,9 @@ int ecs_recurrence(const double* fs, int nsnap, long long modes, double* dist, v
  * `reps` back-to-back launches on `batch` resident spectra, timed with HIP
  * events on `stream`.  which: 0 column kernel, 1 row kernel, 2 tangent
  * column kernel, 3 tangent row kernel, 4 tangent row kernel of the adaptive
- * tangent replay (the instance of a GMRES matvec).  Synchronous. */
+ * tangent replay (the instance of a GMRES matvec), 5 an empty kernel with the
+ * tangent row kernel's grid and argument block (the launch floor).
+ * Synchronous. */
 int ecs_time_kernel(ecs_ctx* ctx, int batch, int which, int reps, double* ms, void* stream);
 
 /* Measurement hook (no counterpart in the reference): with enable = 1, the
