@@ -368,10 +368,15 @@ def serving_leg(jobs, dump_dir=None, log_to_stderr=False):
     stats["adjoint_jobs"] = kinds.count("adjoint")
     stats["newton_jobs"] = kinds.count("newton")
     stats["jobs_detail"] = [{"id": j.id, "kind": j.kind, "n": j.n, "latency_s": (j.finish - j.arrival),
-                             "error": j.error} for j in done]
+                             "unbatched_s": getattr(j, "unbatched_s", None), "error": j.error} for j in done]
     stats["unbatched_estimate_s"] = ub_s
     stats["unbatched_rk_steps_per_s"] = ub_steps / ub_s if ub_s > 0 else None
     stats["served_over_unbatched"] = (stats["rk_steps_per_s"] / (ub_steps / ub_s)) if ub_s > 0 else None
+    # a job's operations run one after another, so the wall time is at least the longest
+    # job's own unbatched time: the ratio above is bounded by this
+    longest = max((getattr(j, "unbatched_s", 0.0) for j in done if j.error is None), default=0.0)
+    stats["longest_job_unbatched_s"] = longest
+    stats["served_over_unbatched_bound"] = ub_s / longest if longest > 0 else None
     return stats
 
 

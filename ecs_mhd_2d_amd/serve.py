@@ -616,11 +616,16 @@ def unbatched_estimate(server, done):
     at a time, alone on the GPU.  Each job's first operation of each kind
     (kept by a Server made with keep_samples=True) is run again alone and
     timed, scaled by the stepper steps of all its operations of that kind.
-    Returns (seconds, stepper steps) summed over the jobs that finished."""
+    Returns (seconds, stepper steps) summed over the jobs that finished; each
+    such job's own share is left in ``job.unbatched_s``.  A job's operations
+    run one after another however they are served, so the served wall time is
+    at least the longest job's share: the served / unbatched ratio cannot
+    exceed (seconds) / max(unbatched_s)."""
     total_s, total_steps = 0.0, 0
     for j in done:
         if j.error is not None:
             continue
+        j.unbatched_s = 0.0
         for kind, (op, st) in j.first_op.items():
             # a line-search window runs as the one-at-a-time search would: its
             # tries one by one, up to the first that lowers |f|
@@ -637,8 +642,9 @@ def unbatched_estimate(server, done):
                 server._run_group(op.group_key(), [o], 0)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
-            total_s += dt * (j.op_steps[kind] / max(st, 1))
+            j.unbatched_s += dt * (j.op_steps[kind] / max(st, 1))
             total_steps += j.op_steps[kind]
+        total_s += j.unbatched_s
     return total_s, total_steps
 
 
