@@ -540,19 +540,23 @@ def main():
     # inside the matvec); the event pairs sampled inside the timed matvecs above also
     # hold the kernel's dispatch latency and the marker packets, an upper bound
     ms = ctypes.c_double()
-    _lib.check(lib.ecs_time_kernel(ctx.handle, 1, 4, 200, ctypes.byref(ms), _runtime._stream()),
-               "ecs_time_kernel")
-    row_t_ms = ms.value
+
+    def timed(c, B, which, reps):
+        """median over three rounds of `reps` back-to-back launches"""
+        vals = []
+        for _ in range(3):
+            _lib.check(lib.ecs_time_kernel(c.handle, B, which, reps, ctypes.byref(ms), _runtime._stream()),
+                       "ecs_time_kernel")
+            vals.append(ms.value)
+        return float(np.median(vals))
+
+    row_t_ms = timed(ctx, 1, 4, 200)
     bytes_row_t = 20 * zbytes(n)   # reads 16 Z (8 primal + 8 tangent derived fields), writes 4 Z
     ach = bytes_row_t / (row_t_ms * 1e-3) / 1e9
     # the launch floor beside it: an empty kernel with the same grid and argument block
-    _lib.check(lib.ecs_time_kernel(ctx.handle, 1, 5, 200, ctypes.byref(ms), _runtime._stream()),
-               "ecs_time_kernel")
-    floor_ms = ms.value
+    floor_ms = timed(ctx, 1, 5, 200)
     # the same kernel family at full occupancy: primal row kernel, N=256, batch 64
-    _lib.check(lib.ecs_time_kernel(ctx256.handle, 64, 1, 50, ctypes.byref(ms), _runtime._stream()),
-               "ecs_time_kernel")
-    row_b_ms = ms.value
+    row_b_ms = timed(ctx256, 64, 1, 50)
     ach_b = 64 * 10 * zbytes(256) / (row_b_ms * 1e-3) / 1e9
 
     tiles = {}
@@ -602,7 +606,7 @@ def main():
                      "kernel": f"row_kernel<128,TX={tiles['n128_b1']['row_tan_tx']},tangent,replay> (batch 1)",
                      "avg_ms": row_t_ms,
                      "timing": "HIP events on the launching stream around 200 back-to-back launches of the "
-                               "replay instance",
+                               "replay instance, median of three rounds",
                      "avg_ms_in_matvec_with_dispatch": probe_ms.value, "in_matvec_samples": probe_n.value,
                      "algorithmic_bytes": bytes_row_t,
                      "launch_floor_ms": floor_ms,
