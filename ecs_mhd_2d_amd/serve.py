@@ -190,9 +190,10 @@ def _batch_jvp(ops, pd, lane):
 
 # ------------------------------------------------------------------ jobs
 def _flat(d):
-    return torch.cat([torch.as_tensor(d["T"], dtype=torch.float64, device="cuda").reshape(1),
+    dev = d["fields"].device
+    return torch.cat([torch.as_tensor(d["T"], dtype=torch.float64, device=dev).reshape(1),
                       d["fields"].reshape(-1),
-                      torch.as_tensor(d["sx"], dtype=torch.float64, device="cuda").reshape(1)])
+                      torch.as_tensor(d["sx"], dtype=torch.float64, device=dev).reshape(1)])
 
 
 def _unflat(v, n):

@@ -213,3 +213,27 @@ def test_free_lanes_take_single_jobs_of_one_kind():
     assert stats["failed_jobs"] == 0 and stats["ticks"] == 12
     assert all(c[1] == 1 for c in ex.calls)
     assert sorted({c[2] for c in ex.calls}) == [0, 1, 2, 3]
+
+
+def test_line_search_window_statistics_count_the_one_at_a_time_tries():
+    """A Newton job's line-search window (one "objective" operation holding
+    several tries) counts, in the served steps, only the tries the
+    one-at-a-time search makes: up to and including the first completed try
+    whose |f| is below the job's f_norm, all of them when none is."""
+    import torch
+
+    def out(scale, attempts, completed=True):
+        f = {"fields": torch.full((2, 4, 4), scale, dtype=torch.float64), "T": torch.tensor(0.0),
+             "sx": torch.tensor(0.0)}
+        return f, {"attempts": attempts, "completed": completed}
+
+    trials = [{"fields": None, "T": t, "sx": 0.0} for t in (1.0, 2.0, 3.0, 4.0)]
+    op = serve.Op("objective", 4, trials=trials, f_norm=10.0, adaptive={"atol": 1e-8, "max_steps_per_checkpoint": 8,
+                                                                       "checkpoints": 2})
+    # |f| of a full((2,4,4), a) field is a * sqrt(32) = 5.66 a
+    res = [out(3.0, 100), out(1.5, 110, completed=False), out(1.5, 120), out(0.1, 130)]
+    assert serve.useful_trials(op, res) == 3
+    assert serve.op_rk_steps(op, res) == 100 + 110 + 120
+    none = [out(3.0, 100), out(2.0, 110), out(2.5, 120), out(4.0, 130)]
+    assert serve.useful_trials(op, none) == 4 and serve.op_rk_steps(op, none) == 460
+    assert op.length() == 4.0
