@@ -31,12 +31,6 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# The serving leg runs up to four groups at once, each on a stream of its own,
-# beside the caller's stream; with HIP's default of four hardware queues per
-# process two of them would share a queue and run one after the other.  Read
-# when HIP initialises (the first GPU call), so it is set before anything
-# touches the GPU.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the two roofline kernels, from rocprofv3 --pmc

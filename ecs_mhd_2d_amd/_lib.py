@@ -31,6 +31,7 @@ _SIGS = {
     "ecs_ctx_destroy": (c_int, [c_void_p]),
     "ecs_last_error": (ctypes.c_char_p, []),
     "ecs_ctx_n": (c_int, [c_void_p]),
+    "ecs_ctx_set_rev_overlap": (c_int, [c_void_p, c_int]),
     "ecs_state_vel": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     "ecs_state_vel_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     "ecs_state_vel_jvp": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),

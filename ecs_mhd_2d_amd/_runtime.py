@@ -72,6 +72,12 @@ class Context:
         except Exception:
             pass
 
+    def set_rev_overlap(self, enable: bool) -> None:
+        """Lawson RK4 reverse passes on this context: recompute the next segment on
+        a second stream while the caller's stream sweeps the current one backward
+        (default), or one stream (the same bits either way)."""
+        _lib.check(self.lib.ecs_ctx_set_rev_overlap(self.handle, int(bool(enable))), "ecs_ctx_set_rev_overlap")
+
     # ------------------------------------------------------------ RHS
     def state_vel(self, f: torch.Tensor, include_dissipation: bool) -> torch.Tensor:
         f = _as_spec(f, self.n)

@@ -305,12 +305,15 @@ int ecs_ctx_destroy(ecs_ctx* c) {
   DevGuard dg(c->dev);
   void* ps[] = {c->tw, c->fhat, c->I, c->O, c->x0, c->acc, c->v1, c->v4, c->v5, c->xf, c->out, c->dx0, c->dacc,
                 c->dv1, c->dv4, c->dxf, c->dout, c->scal, c->hist, c->s_h, c->s_rs, c->s_rc, c->s_rh, c->s_nacc,
-                c->ckpt, c->xtape, c->itape, c->red, c->AO, c->AI, c->xb, c->k1b, c->k2b, c->k3b, c->hb,
-                c->gen, c->tab, c->gcomb};
+                c->ckpt, c->xtape, c->itape, c->xtape2, c->itape2, c->red, c->AO, c->AI, c->xb, c->k1b, c->k2b,
+                c->k3b, c->hb, c->gen, c->tab, c->gcomb};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   drop_graphs(c);
   if (c->gs) (void)hipStreamDestroy(c->gs);
+  if (c->side) (void)hipStreamDestroy(c->side);
+  for (hipEvent_t e : c->ev_rev)
+    if (e) (void)hipEventDestroy(e);
   if (c->ev_in) (void)hipEventDestroy(c->ev_in);
   if (c->ev_out) (void)hipEventDestroy(c->ev_out);
   if (c->cnt) (void)hipFree(c->cnt);
@@ -330,6 +333,12 @@ int ecs_ctx_destroy(ecs_ctx* c) {
 }
 
 int ecs_ctx_n(const ecs_ctx* c) { return c ? c->n : 0; }
+
+extern "C" int ecs_ctx_set_rev_overlap(ecs_ctx* c, int enable) {
+  if (!c) return fail(1, "ecs_ctx_set_rev_overlap: null context");
+  c->rev_overlap = enable != 0;
+  return 0;
+}
 
 int ecs_state_vel(ecs_ctx* c, const double* in, double* out, int batch, int diss, void* stream) {
   if (!c || !in || !out || batch < 1) return fail(1, "ecs_state_vel: bad arguments");

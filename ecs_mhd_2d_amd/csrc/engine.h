@@ -780,6 +780,28 @@ static int adj_buffers(ecs_ctx* c, int B, int ckpt) {
   return 0;
 }
 
+// The second step tape of the reverse pass, with the side stream and events
+// of the overlap, when it fits REV_OVERLAP_BYTES; returns false (and the
+// reverse pass runs its segments one phase after the other) otherwise, when
+// the context's option is off (ecs_ctx_set_rev_overlap) or ECSMHD_REV_OVERLAP=0.
+constexpr size_t REV_OVERLAP_BYTES = (size_t)4 << 30;
+template <int N>
+static int adj_buffers2(ecs_ctx* c, int B, int ckpt, bool* ok) {
+  using G = Geo<N>;
+  const size_t cs = (size_t)B * 2 * G::R * G::KC;
+  const size_t is = (size_t)B * 8 * N * G::KP;
+  const size_t nv = (size_t)ckpt * 5 * cs, ni = (size_t)ckpt * 4 * is;
+  const char* env = getenv("ECSMHD_REV_OVERLAP");
+  *ok = c->rev_overlap && (nv + ni) * sizeof(double2) <= REV_OVERLAP_BYTES && !(env && env[0] == '0');
+  if (!*ok) return 0;
+  TRY(grow(&c->xtape2, &c->vt2_cap, nv));
+  TRY(grow(&c->itape2, &c->it2_cap, ni));
+  if (!c->side) HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+  for (hipEvent_t& e : c->ev_rev)
+    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return 0;
+}
+
 // Forward pass of Lawson RK4 storing the compact state at the start of every
 // segment of ckpt steps into ck [nseg][B][2][R][KC]; f is advanced in place.
 template <int N>
@@ -844,37 +866,64 @@ static int run_rk4_rev(ecs_ctx* c, const double2* ck, double* fbar, double* hbar
   }
   TRY(pack<N>(fbar, c->xb, B, st));
   HIPCHK(hipMemsetAsync(c->hb, 0, cs * sizeof(double), st));
-  auto slot = [&](int t, int s) { return c->itape + ((size_t)t * 4 + s) * is; };
-  auto xt = [&](int t) { return c->xtape + (size_t)t * cs; };
-  auto vt = [&](int t) { return c->vtape + (size_t)t * 4 * cs; };
+  // Segment j's recomputation (forward steps from its checkpoint, storing the
+  // step tape) reads and writes only the forward workspace and the tape; its
+  // reverse sweep reads the tape and writes only the adjoint workspace.  With
+  // two tapes, segment j - 1 is recomputed on the side stream while the
+  // caller's stream sweeps segment j backward (the same kernels in the same
+  // order in each chain, so the same bits): at small batches both chains are
+  // launch-bound and overlap.  A tape is rewritten only after the sweep that
+  // read it (events).
+  bool two = false;
+  TRY(adj_buffers2<N>(c, B, ckpt, &two));
+  two = two && nseg > 1;
+  hipStream_t fs = two ? c->side : st;
+  if (two) {
+    HIPCHK(hipEventRecord(c->ev_rev[0], st));
+    HIPCHK(hipStreamWaitEvent(fs, c->ev_rev[0], 0));
+  }
+  auto itb = [&](int j) { return (two && (j & 1)) ? c->itape2 : c->itape; };
+  auto xtb = [&](int j) { return (two && (j & 1)) ? c->xtape2 : c->xtape; };
   for (int j = nseg - 1; j >= 0; --j) {
     const int t0 = j * ckpt;
     const int L = std::min(ckpt, steps - t0);
-    HIPCHK(hipMemcpyAsync(c->x0, ck + (size_t)j * cs, cs * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    double2* itj = itb(j);
+    double2* xtj = xtb(j);
+    auto slot = [&](int t, int s) { return itj + ((size_t)t * 4 + s) * is; };
+    auto xt = [&](int t) { return xtj + (size_t)t * cs; };
+    auto vt = [&](int t) { return xtj + (size_t)ckpt * cs + (size_t)t * 4 * cs; };
+    const int p = j & 1;
+    // the tape this segment fills was last read by the sweep of segment j + 2
+    if (two && j + 2 < nseg) HIPCHK(hipStreamWaitEvent(fs, c->ev_rev[3 + p], 0));
+    HIPCHK(hipMemcpyAsync(c->x0, ck + (size_t)j * cs, cs * sizeof(double2), hipMemcpyDeviceToDevice, fs));
     Args f = a;
     f.inter = slot(0, 0);
     f.step = t0;
-    TRY((col<N, C_LOAD, false>(f, B, st)));
+    TRY((col<N, C_LOAD, false>(f, B, fs)));
     for (int t = 0; t < L; ++t) {
       f.step = t0 + t;
       f.vtape = nullptr; f.xtape = nullptr;
       f.inter = slot(t, 0);
-      TRY((row<N, R_RHS, false>(f, B, st)));
+      TRY((row<N, R_RHS, false>(f, B, fs)));
       f.inter = slot(t, 1); f.vtape = vt(t); f.xtape = xt(t);
-      TRY((col<N, C_S1F, false>(f, B, st)));
+      TRY((col<N, C_S1F, false>(f, B, fs)));
       f.xtape = nullptr;
-      TRY((row<N, R_RHS, false>(f, B, st)));
+      TRY((row<N, R_RHS, false>(f, B, fs)));
       f.inter = slot(t, 2); f.vtape = vt(t) + cs;
-      TRY((col<N, C_S2, false>(f, B, st)));
-      TRY((row<N, R_RHS, false>(f, B, st)));
+      TRY((col<N, C_S2, false>(f, B, fs)));
+      TRY((row<N, R_RHS, false>(f, B, fs)));
       f.inter = slot(t, 3); f.vtape = vt(t) + 2 * cs;
-      TRY((col<N, C_S3, false>(f, B, st)));
-      TRY((row<N, R_RHS, false>(f, B, st)));
+      TRY((col<N, C_S3, false>(f, B, fs)));
+      TRY((row<N, R_RHS, false>(f, B, fs)));
       Args f4 = f;
       f4.vtape = vt(t) + 3 * cs;
       f4.inter = (t + 1 < L) ? slot(t + 1, 0) : slot(t, 3);
       f4.skip_inv = (t + 1 == L);
-      TRY((col<N, C_S4, false>(f4, B, st)));
+      TRY((col<N, C_S4, false>(f4, B, fs)));
+    }
+    if (two) {
+      HIPCHK(hipEventRecord(c->ev_rev[1 + p], fs));
+      HIPCHK(hipStreamWaitEvent(st, c->ev_rev[1 + p], 0));
     }
     Args b = a;
     b.tx = xt(L - 1);
@@ -904,6 +953,7 @@ static int run_rk4_rev(ecs_ctx* c, const double2* ck, double* fbar, double* hbar
         TRY((acol<N, B_S1>(b, B, st)));
       }
     }
+    if (two) HIPCHK(hipEventRecord(c->ev_rev[3 + p], st));
   }
   TRY(unpack<N>(c->xb, fbar, B, masked ? 1 : 0, st));
   hipLaunchKernelGGL((hb_sum_kernel<N>), dim3(B), dim3(256), 0, st, c->hb, c->red);

@@ -114,6 +114,15 @@ struct ecs_ctx {
   size_t vt_cap = 0;           // elements of xtape + vtape
   double2* itape = nullptr;    // primal column outputs [ckpt][4][B][8][N][KP]
   size_t it_cap = 0;
+  // a second step tape, so the reverse pass recomputes one segment on the
+  // side stream while the caller's stream sweeps the next one backward
+  double2* xtape2 = nullptr;
+  size_t vt2_cap = 0;
+  double2* itape2 = nullptr;
+  size_t it2_cap = 0;
+  hipStream_t side = nullptr;  // default priority, non-blocking
+  bool rev_overlap = true;     // ecs_ctx_set_rev_overlap
+  hipEvent_t ev_rev[5] = {};   // start, recomputed[2], swept[2]
   double2* gen = nullptr;      // stage arrays of the generic steppers
   size_t gen_cap = 0;
   double2* gcomb = nullptr;    // stage-combination tables of the fused generic steppers (Comb records)

@@ -403,8 +403,11 @@ class Server:
     """
 
     def __init__(self, hbm_budget=200e9, nu=1 / 40, eta=1 / 40, b0=(0.0, 0.1), max_batch=64, executor=None,
-                 max_groups=4, bypass_wait_s=30.0, keep_samples=False):
+                 max_groups=4, bypass_wait_s=30.0, keep_samples=False, rev_overlap=False):
         self.executor = executor
+        # reverse passes of the lanes' gradients on one stream: the lanes already
+        # run concurrently, and a second stream per lane measured slower
+        self.rev_overlap = rev_overlap
         self.budget = hbm_budget
         self.nu, self.eta, self.b0 = nu, eta, b0
         self.max_batch = max_batch
@@ -440,6 +443,7 @@ class Server:
         if self.executor is not None:
             return self.executor(key, ops, lane), None
         s = self._streams[lane]
+        mhd.context(self.pd(ops[0].n), lane).set_rev_overlap(self.rev_overlap)
         with torch.cuda.stream(s):
             s.wait_event(ready)
             out = self._run_group(key, ops, lane)
@@ -628,6 +632,9 @@ def unbatched_estimate(server, done):
             continue
         j.unbatched_s = 0.0
         for kind, (op, st) in j.first_op.items():
+            # one at a time, an operation takes the library's fastest form (reverse
+            # passes with the next segment recomputed on a second stream)
+            mhd.context(server.pd(op.n), 0).set_rev_overlap(True)
             # a line-search window runs as the one-at-a-time search would: its
             # tries one by one, up to the first that lowers |f|
             if kind == "objective":
