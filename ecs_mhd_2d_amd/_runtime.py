@@ -73,9 +73,9 @@ class Context:
             pass
 
     def set_rev_overlap(self, enable: bool) -> None:
-        """Lawson RK4 reverse passes on this context: recompute the next segment on
-        a second stream while the caller's stream sweeps the current one backward
-        (default), or one stream (the same bits either way)."""
+        """Reverse passes on this context (every stepper's VJP): recompute the next
+        segment on a second stream while the caller's stream sweeps the current one
+        backward (default), or one stream (the same bits either way)."""
         _lib.check(self.lib.ecs_ctx_set_rev_overlap(self.handle, int(bool(enable))), "ecs_ctx_set_rev_overlap")
 
     # ------------------------------------------------------------ RHS

@@ -1361,7 +1361,7 @@ static int vjp_pass(ecs_ctx* c, const Args& base, double2* x, const double2* g, 
   Args a = base;
   a.x0 = x; a.gin = g; a.out = out; a.out_acc = acc ? 1 : 0;
   a.include_diss = hess ? 0 : diss;
-  a.aout = c->AO; a.ain = c->AI; a.prim = prim ? prim : c->I;
+  a.aout = c->AO; a.ain = c->AI; a.prim = prim ? prim : a.inter;
   a.adaptive = AD_FIXED;
   if (hess) { a.b0x = 0.0; a.b0y = 0.0; }
   if (!prim) TRY((col<N, C_LOAD, false>(a, B, st)));
@@ -1371,6 +1371,174 @@ static int vjp_pass(ecs_ctx* c, const Args& base, double2* x, const double2* g, 
   if (then && !a.nstep && then->yb == out) return acol_comb<N>(a, *then, B, st);
   TRY((acol<N, B_OUT>(a, B, st)));
   if (then) TRY(comb_adj_launch<N>(*then, B, st));
+  return 0;
+}
+
+// Step tape of one segment of the generic reverse passes (the Butcher-form
+// schemes, TDRK4 and the RK4(3) schedule): per step t the step input x(t),
+// the stage inputs y(t, 1..S-1), the stage values v(t, 0..S-1), the raw
+// first-stage value vraw(t) (RK4(3)), `ex` further stage arrays w(t, k)
+// (TDRK4: w_j = J v_j), and the derived fields of the stage inputs
+// d(t, 0..S-1) (the column outputs the RHS-VJP passes read).  One forward
+// pass over the segment fills it, so no step is recomputed before it is
+// transposed.  aux: one derived-field array after the steps, for the column
+// loads of the transpose's own RHS-VJP passes (TDRK4's at points off the
+// tape), so they do not share the forward pass's column output.
+struct StepTape {
+  double2* cm = nullptr;  // [ckpt][uc] compact arrays
+  double2* dm = nullptr;  // [ckpt][S] derived-field arrays, then aux
+  size_t cs = 0, is = 0;
+  int uc = 0, S = 0, ckpt = 0;
+  double2* x(int t) const { return cm + (size_t)t * uc * cs; }
+  double2* y(int t, int k) const { return cm + ((size_t)t * uc + k) * cs; }  // k = 1 .. S-1
+  double2* v(int t, int k) const { return cm + ((size_t)t * uc + S + k) * cs; }
+  double2* vraw(int t) const { return cm + ((size_t)t * uc + 2 * S) * cs; }
+  double2* w(int t, int k) const { return cm + ((size_t)t * uc + 2 * S + 1 + k) * cs; }
+  double2* d(int t, int k) const { return dm + ((size_t)t * S + k) * is; }
+  double2* aux() const { return dm + (size_t)ckpt * S * is; }
+};
+
+// The step tapes of a generic reverse pass with S stages, when one tape of
+// ckpt steps fits SEG_TAPE_BYTES (and ECSMHD_SEG_TAPE is not 0): *ntape = 1,
+// or 2 with the side stream and events of the overlap when the context's
+// option allows it (ecs_ctx_set_rev_overlap, ECSMHD_REV_OVERLAP) and there is
+// more than one segment.  *ntape = 0: the pass recomputes each step from its
+// stored input before transposing it, in the shared stage arrays.
+constexpr size_t SEG_TAPE_BYTES = (size_t)4 << 30;
+template <int N>
+static int step_tapes(ecs_ctx* c, int B, int S, int ex, int ckpt, int nseg, StepTape* tp, int* ntape) {
+  using G = Geo<N>;
+  *ntape = 0;
+  const size_t cs = (size_t)B * 2 * G::R * G::KC;
+  const size_t is = (size_t)B * 8 * N * G::KP;
+  const int uc = 2 * S + 1 + ex;
+  const size_t nc = (size_t)ckpt * uc * cs, ni = ((size_t)ckpt * S + 1) * is;
+  const char* env = getenv("ECSMHD_SEG_TAPE");
+  if ((nc + ni) * sizeof(double2) > SEG_TAPE_BYTES || (env && env[0] == '0')) return 0;
+  const char* ov = getenv("ECSMHD_REV_OVERLAP");
+  const int want = (c->rev_overlap && nseg > 1 && !(ov && ov[0] == '0')) ? 2 : 1;
+  TRY(grow(&c->xtape, &c->vt_cap, nc));
+  TRY(grow(&c->itape, &c->it_cap, ni));
+  if (want == 2) {
+    TRY(grow(&c->xtape2, &c->vt2_cap, nc));
+    TRY(grow(&c->itape2, &c->it2_cap, ni));
+    if (!c->side) HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    for (hipEvent_t& e : c->ev_rev)
+      if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  for (int p = 0; p < want; ++p) {
+    tp[p].cm = p ? c->xtape2 : c->xtape;
+    tp[p].dm = p ? c->itape2 : c->itape;
+    tp[p].cs = cs; tp[p].is = is; tp[p].uc = uc; tp[p].S = S; tp[p].ckpt = ckpt;
+  }
+  *ntape = want;
+  return 0;
+}
+
+// Appends the stage-combination tables of one tape's steps (Butcher form):
+// stage k of step t reads x(t) and v(t, j <= k) and gives y(t, k + 1), its
+// last stage x(t + 1) (`spare` for the tape's last step slot).  h: the
+// candidates' steps (null: the launch's per-step table, as RK4(3) passes it).
+static void tape_tables(std::vector<Comb>& tabs, ecs_ctx* c, size_t ce, const Tableau& tb, int msk,
+                        const double* h, const StepTape& T, int ckpt, double2* spare) {
+  const int S = tb.S;
+  for (int t = 0; t < ckpt; ++t)
+    for (int k = 0; k < S; ++k) {
+      const bool last = k == S - 1;
+      double2* out = last ? (t + 1 < ckpt ? T.x(t + 1) : spare) : T.y(t, k + 1);
+      CombBuilder cb(c, ce, tb.ediv, msk, out, nullptr);
+      cb.cb.h = h;
+      const int pk = last ? tb.p[S] : tb.p[k + 1];
+      cb.term(T.x(t), nullptr, 1.0, 0, pk);
+      for (int j = 0; j <= k; ++j) {
+        const double cj = last ? tb.b[j] : tb.A[k + 1][j];
+        if (cj != 0.0) cb.term(T.v(t, j), nullptr, cj, 1, pk - tb.p[j]);
+      }
+      tabs.push_back(cb.cb);
+    }
+}
+
+// The forward pass of one segment into tape T (Butcher form, C_GEN stages):
+// L steps from the checkpoint ck, step m0 + t of the schedule on slot t.
+// rk43: the RK4(3) reverse pass's per-step tables (dt, and the carried-k1
+// factor on stage 1 with its raw value in vraw(t)).
+template <int N>
+static int tape_forward(const Args& af, const StepTape& T, const Comb* ttab, int S, int ckpt, int L, int m0,
+                        const double2* ck, double2* spare, int B, hipStream_t fs,
+                        const std::function<void(Args&, int, int)>& stage_args) {
+  HIPCHK(hipMemcpyAsync(T.x(0), ck, T.cs * sizeof(double2), hipMemcpyDeviceToDevice, fs));
+  Args al = af;
+  al.x0 = T.x(0);
+  al.inter = T.d(0, 0);
+  TRY((col<N, C_LOAD, false>(al, B, fs)));
+  for (int t = 0; t < L; ++t)
+    for (int k = 0; k < S; ++k) {
+      const bool last = k == S - 1;
+      Args ar = af;
+      ar.inter = T.d(t, k);
+      TRY((row<N, R_RHS, false>(ar, B, fs)));
+      Args ak = af;
+      ak.x0 = k == 0 ? T.x(t) : T.y(t, k);
+      ak.out = T.v(t, k);
+      ak.xf = last ? (t + 1 < ckpt ? T.x(t + 1) : spare) : T.y(t, k + 1);
+      ak.gc = ttab + (size_t)t * S + k;
+      ak.skip_inv = (last && t + 1 == L) ? 1 : 0;
+      ak.inter = !last ? T.d(t, k + 1) : (t + 1 < L ? T.d(t + 1, 0) : T.d(t, k));
+      if (stage_args) stage_args(ak, m0 + t, k);
+      TRY((col<N, C_GEN, false>(ak, B, fs)));
+    }
+  return 0;
+}
+
+// TDRK4's tables for one tape: per step t, y(t, 1) = x + h/2 v1 + h^2/8 w1 and
+// x(t + 1) = x + h v1 + h^2/6 w1 + h^2/3 w2 (`spare` for the last step slot),
+// with x = x(t), v_j = v(t, j - 1), w_j = w(t, j - 1).
+static void tape_tables_td(std::vector<Comb>& tabs, ecs_ctx* c, size_t ce, const StepTape& T, int ckpt,
+                           double2* spare) {
+  for (int t = 0; t < ckpt; ++t) {
+    CombBuilder cy(c, ce, 1.0, 0, T.y(t, 1), nullptr);
+    cy.term(T.x(t), nullptr, 1.0, 0, 0).term(T.v(t, 0), nullptr, 0.5, 1, 0).term(T.w(t, 0), nullptr, 0.125, 2, 0);
+    CombBuilder cx(c, ce, 1.0, 0, t + 1 < ckpt ? T.x(t + 1) : spare, nullptr);
+    cx.term(T.x(t), nullptr, 1.0, 0, 0).term(T.v(t, 0), nullptr, 1.0, 1, 0).term(T.w(t, 0), nullptr, 1.0 / 6, 2, 0)
+      .term(T.w(t, 1), nullptr, 1.0 / 3, 2, 0);
+    tabs.push_back(cy.cb);
+    tabs.push_back(cx.cb);
+  }
+}
+
+// TDRK4's forward pass of one segment into tape T: per half step a row pass,
+// C_TD1 (v = RHS(z)), the tangent row pass and C_TD2 (w = J(z) v, then the
+// combination of table ttab[2 t + half]), as run_gen's fused steps; the
+// derived fields of x(t) and y(t, 1) go to d(t, 0) and d(t, 1).
+template <int N>
+static int tape_forward_td(const Args& af, const StepTape& T, const Comb* ttab, int ckpt, int L, const double2* ck,
+                           double2* spare, int B, hipStream_t fs) {
+  HIPCHK(hipMemcpyAsync(T.x(0), ck, T.cs * sizeof(double2), hipMemcpyDeviceToDevice, fs));
+  Args al = af;
+  al.x0 = T.x(0);
+  al.inter = T.d(0, 0);
+  TRY((col<N, C_LOAD, false>(al, B, fs)));
+  for (int t = 0; t < L; ++t)
+    for (int half = 0; half < 2; ++half) {
+      double2* z = half == 0 ? T.x(t) : T.y(t, 1);
+      Args ar = af;
+      ar.inter = T.d(t, half);
+      TRY((row<N, R_RHS, false>(ar, B, fs)));
+      Args a1 = af;
+      a1.x0 = z;
+      a1.out = T.v(t, half);
+      TRY((col<N, C_TD1, false>(a1, B, fs)));
+      TRY((row<N, R_RHS, true>(af, B, fs)));
+      Args a2 = af;
+      a2.x0 = z;
+      a2.dx0 = T.v(t, half);
+      a2.dout = T.w(t, half);
+      a2.gc = ttab + 2 * (size_t)t + half;
+      a2.xf = half == 0 ? T.y(t, 1) : (t + 1 < ckpt ? T.x(t + 1) : spare);
+      a2.skip_inv = (half == 1 && t + 1 == L) ? 1 : 0;
+      a2.inter = half == 0 ? T.d(t, 1) : (t + 1 < L ? T.d(t + 1, 0) : T.d(t, 1));
+      TRY((col<N, C_TD2, true>(a2, B, fs)));
+    }
   return 0;
 }
 
@@ -1400,9 +1568,15 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   const Tableau& tb = tableau(scheme);
   const int S = td ? 2 : tb.S;
   const int nseg = (steps + ckpt - 1) / ckpt;
-  // units of cs: xw (state of the fused steps), xb, xbn, yb, 3 spare | segment inputs [ckpt] | stages
+  // a step tape per segment when it fits
+  StepTape tp[2];
+  int ntape = 0;
+  TRY(step_tapes<N>(c, B, S, td ? 2 : 0, ckpt, nseg, tp, &ntape));
+  // units of cs: xw (state of the fused steps), xb, xbn, yb, 3 spare | segment inputs [ckpt]
+  // (without a tape) | stages
   const int stage_units = td ? 10 : 3 * S;
-  const int units = 7 + ckpt + stage_units;
+  const int seg_units = ntape ? 0 : ckpt;
+  const int units = 7 + seg_units + stage_units;
   TRY(grow(&c->gen, &c->gen_cap, (size_t)units * cs));
   TRY(grow(&c->ckpt, &c->ckpt_cap, (size_t)nseg * cs));
   TRY(upload_scalars(c, 0, dt, B, st));
@@ -1411,7 +1585,7 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   auto slot = [&](int u) { return g + (size_t)u * cs; };
   double2 *xw = slot(0), *xb = slot(1), *xbn = slot(2), *yb = slot(3);
   auto xt = [&](int t) { return slot(7 + t); };
-  const int s0 = 7 + ckpt;
+  const int s0 = 7 + seg_units;
   const int diss = scheme == G_LRK6 ? 0 : 1;
   const int msk = scheme == G_LRK6 ? masked : 0;
   const double ediv = tb.ediv;
@@ -1453,6 +1627,11 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
     tabs.push_back(cy.cb);
     tabs.push_back(cx.cb);
   }
+  // the tapes' per-step tables (S per step) follow the forward pass's S tables
+  for (int p = 0; p < ntape; ++p) {
+    if (td) tape_tables_td(tabs, c, ce, tp[p], ckpt, xw);
+    else tape_tables(tabs, c, ce, tb, msk, c->scal, tp[p], ckpt, xw);
+  }
   TRY(grow(&c->gcomb, &c->gcomb_cap, (tabs.size() * sizeof(Comb) + sizeof(double2) - 1) / sizeof(double2)));
   Comb* dtabs = (Comb*)c->gcomb;
   HIPCHK(hipMemcpyAsync(dtabs, tabs.data(), tabs.size() * sizeof(Comb), hipMemcpyHostToDevice, st));
@@ -1460,11 +1639,11 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   Args af = a;
   af.adaptive = AD_FIXED;
   af.include_diss = diss;
-  // The derived fields (column pass output) of the stage inputs of the step
-  // being transposed, kept for its RHS-VJP passes: slot k for stage input k
-  // (TDRK4: 0 the step input, 1 the midpoint y).
+  // Without a tape: the derived fields (column pass output) of the stage
+  // inputs of the step being transposed, kept for its RHS-VJP passes: slot k
+  // for stage input k (TDRK4: 0 the step input, 1 the midpoint y).
   const size_t is = (size_t)B * 8 * N * Geo<N>::KP;
-  TRY(grow(&c->itape, &c->it_cap, (size_t)(td ? 2 : S + 1) * is));
+  if (!ntape) TRY(grow(&c->itape, &c->it_cap, (size_t)(td ? 2 : S + 1) * is));
   auto ti = [&](int k) { return c->itape + (size_t)k * is; };
   // n >= 1 steps from the state in xw; after(s), when given, runs once step s
   // has left its result in xw; tape (one step): the stage inputs' derived
@@ -1531,7 +1710,104 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   // reverse
   TRY(pack<N>(fbar, xb, B, st));
   HIPCHK(hipMemsetAsync(c->hb, 0, cs * sizeof(double), st));
-  for (int jseg = nseg - 1; jseg >= 0; --jseg) {
+  // Transpose of one Butcher-form step with input xs, stage inputs ys(k),
+  // stage values vs(k) and the stage inputs' derived fields ds(k).  The
+  // step's cotangent accumulators start with their first contribution
+  // (assign), so they need no clearing.
+  using Ptr = std::function<double2*(int)>;
+  auto rev_step = [&](double2* xs, const Ptr& ys, const Ptr& vs, const Ptr& ds) -> int {
+    std::vector<char> touched(S, 0);
+    CombAdjBuilder fin(c, ce, ediv, msk, xb, c->hb);
+    fin.term(xbn, xs, 1.0, 0, tb.p[S], true);
+    for (int j = 0; j < S; ++j)
+      if (tb.b[j] != 0.0) {
+        fin.term(vb(j), vs(j), tb.b[j], 1, tb.p[S] - tb.p[j], true);
+        touched[j] = 1;
+      }
+    TRY(comb_adj_launch<N>(fin.cb, B, st));
+    for (int k = S - 1; k >= 0; --k) {
+      if (!touched[k]) HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));  // a stage no combination reads
+      CombAdjBuilder st_(c, ce, ediv, msk, yb, c->hb);
+      st_.term(xbn, k == 0 ? nullptr : xs, 1.0, 0, tb.p[k]);
+      for (int j = 0; j < k; ++j)
+        if (tb.A[k][j] != 0.0) {
+          st_.term(vb(j), vs(j), tb.A[k][j], 1, tb.p[k] - tb.p[j], !touched[j]);
+          touched[j] = 1;
+        }
+      TRY(vjp_pass<N>(c, a, k == 0 ? xs : ys(k), vb(k), yb, B, diss, st, false, ds(k), false, &st_.cb));
+    }
+    std::swap(xb, xbn);
+    return 0;
+  };
+  // Transpose of one TDRK4 step with input xs, midpoint ym, stage values v1,
+  // v2, w1 = J(xs) v1, w2 = J(ym) v2 and the derived fields d1, dm of xs and
+  // ym; ab: the base arguments of the RHS-VJP passes (their column output
+  // serves the passes at v1 and v2, which load their point themselves).
+  auto rev_td = [&](const Args& ab, double2* xs, double2* ym, double2* v1, double2* v2, double2* w1, double2* w2,
+                    double2* d1, double2* dm) -> int {
+    // x' = x + h v1 + h^2/6 w1 + h^2/3 w2 (vbar2 is written whole by its VJP pass below)
+    CombAdjBuilder fin(c, ce, 1.0, 0, xb, c->hb);
+    fin.term(xbn, nullptr, 1.0, 0, 0, true).term(vb(0), v1, 1.0, 1, 0, true)
+      .term(wb(0), w1, 1.0 / 6, 2, 0, true).term(wb(1), w2, 1.0 / 3, 2, 0, true);
+    TRY(comb_adj_launch<N>(fin.cb, B, st));
+    // w2 = J(y) v2, v2 = f(y): vbar2 = J(y)^T wbar2; ybar = J(y)^T vbar2 + J_0(v2)^T wbar2
+    TRY(vjp_pass<N>(c, ab, ym, wb(1), vb(1), B, diss, st, false, dm));
+    TRY(vjp_pass<N>(c, ab, ym, vb(1), yb, B, diss, st, false, dm));
+    // y = x + h/2 v1 + h^2/8 w1 (the transposed combination applied by the pass's last kernel)
+    CombAdjBuilder cy(c, ce, 1.0, 0, yb, c->hb);
+    cy.term(xbn, nullptr, 1.0, 0, 0).term(vb(0), v1, 0.5, 1, 0).term(wb(0), w1, 0.125, 2, 0);
+    TRY(vjp_pass<N>(c, ab, v2, wb(1), yb, B, diss, st, true, nullptr, true, &cy.cb));
+    // w1 = J(x) v1: vbar1 += J(x)^T wbar1, xbar += J_0(v1)^T wbar1
+    TRY(vjp_pass<N>(c, ab, xs, wb(0), vb(0), B, diss, st, false, d1, true));
+    TRY(vjp_pass<N>(c, ab, v1, wb(0), xbn, B, diss, st, true, nullptr, true));
+    // v1 = f(x): xbar += J(x)^T vbar1
+    TRY(vjp_pass<N>(c, ab, xs, vb(0), xbn, B, diss, st, false, d1, true));
+    std::swap(xb, xbn);
+    return 0;
+  };
+  if (ntape) {
+    // Segment j's forward pass fills tape j % 2 (writing only the tape, the
+    // forward workspace and the spare slot xw); its transpose reads the tape
+    // and writes only the cotangent slots, the adjoint workspace and the
+    // tape's aux array (TDRK4's column loads of v1, v2).  With
+    // two tapes, segment j - 1 runs forward on the side stream while the
+    // caller's stream transposes segment j (the same kernels in the same
+    // order in each chain, so the same bits); a tape is refilled only after
+    // the transpose that read it (events).
+    const bool two = ntape == 2 && nseg > 1;
+    hipStream_t fs = two ? c->side : st;
+    if (two) {
+      HIPCHK(hipEventRecord(c->ev_rev[0], st));
+      HIPCHK(hipStreamWaitEvent(fs, c->ev_rev[0], 0));
+    }
+    for (int jseg = nseg - 1; jseg >= 0; --jseg) {
+      const int t0 = jseg * ckpt;
+      const int L = std::min(ckpt, steps - t0);
+      const int p = two ? (jseg & 1) : 0;
+      const StepTape& tap = tp[p];
+      if (two && jseg + 2 < nseg) HIPCHK(hipStreamWaitEvent(fs, c->ev_rev[3 + p], 0));
+      const Comb* ttab = dtabs + S + (size_t)p * ckpt * S;
+      const double2* ck = c->ckpt + (size_t)jseg * cs;
+      if (td) TRY(tape_forward_td<N>(af, tap, ttab, ckpt, L, ck, xw, B, fs));
+      else TRY(tape_forward<N>(af, tap, ttab, S, ckpt, L, t0, ck, xw, B, fs, nullptr));
+      if (two) {
+        HIPCHK(hipEventRecord(c->ev_rev[1 + p], fs));
+        HIPCHK(hipStreamWaitEvent(st, c->ev_rev[1 + p], 0));
+      }
+      Args ab = a;
+      ab.inter = tp[0].aux();
+      for (int t = L - 1; t >= 0; --t) {
+        if (td)
+          TRY(rev_td(ab, tap.x(t), tap.y(t, 1), tap.v(t, 0), tap.v(t, 1), tap.w(t, 0), tap.w(t, 1), tap.d(t, 0),
+                     tap.d(t, 1)));
+        else
+          TRY(rev_step(tap.x(t), [&](int k) { return tap.y(t, k); }, [&](int k) { return tap.v(t, k); },
+                       [&](int k) { return tap.d(t, k); }));
+      }
+      if (two) HIPCHK(hipEventRecord(c->ev_rev[3 + p], st));
+    }
+  }
+  for (int jseg = ntape ? -1 : nseg - 1; jseg >= 0; --jseg) {
     const int t0 = jseg * ckpt;
     const int L = std::min(ckpt, steps - t0);
     // the segment's step inputs, then per step (last first) its stages
@@ -1544,49 +1820,8 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
       double2* xs = xt(t);
       TRY(copy(xw, xs));
       TRY(fsteps(1, nullptr, true));
-      // the step's cotangent accumulators start with their first contribution
-      // (assign), so they need no clearing
-      if (!td) {
-        std::vector<char> touched(S, 0);
-        CombAdjBuilder fin(c, ce, ediv, msk, xb, c->hb);
-        fin.term(xbn, xs, 1.0, 0, tb.p[S], true);
-        for (int j = 0; j < S; ++j)
-          if (tb.b[j] != 0.0) {
-            fin.term(vb(j), v(j), tb.b[j], 1, tb.p[S] - tb.p[j], true);
-            touched[j] = 1;
-          }
-        TRY(comb_adj_launch<N>(fin.cb, B, st));
-        for (int k = S - 1; k >= 0; --k) {
-          if (!touched[k]) HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));  // a stage no combination reads
-          CombAdjBuilder st_(c, ce, ediv, msk, yb, c->hb);
-          st_.term(xbn, k == 0 ? nullptr : xs, 1.0, 0, tb.p[k]);
-          for (int j = 0; j < k; ++j)
-            if (tb.A[k][j] != 0.0) {
-              st_.term(vb(j), v(j), tb.A[k][j], 1, tb.p[k] - tb.p[j], !touched[j]);
-              touched[j] = 1;
-            }
-          TRY(vjp_pass<N>(c, a, k == 0 ? xs : y(k), vb(k), yb, B, diss, st, false, ti(k), false, &st_.cb));
-        }
-      } else {
-        // x' = x + h v1 + h^2/6 w1 + h^2/3 w2 (vbar2 is written whole by its VJP pass below)
-        CombAdjBuilder fin(c, ce, 1.0, 0, xb, c->hb);
-        fin.term(xbn, nullptr, 1.0, 0, 0, true).term(vb(0), v(0), 1.0, 1, 0, true)
-          .term(wb(0), w(0), 1.0 / 6, 2, 0, true).term(wb(1), w(1), 1.0 / 3, 2, 0, true);
-        TRY(comb_adj_launch<N>(fin.cb, B, st));
-        // w2 = J(y) v2, v2 = f(y): vbar2 = J(y)^T wbar2; ybar = J(y)^T vbar2 + J_0(v2)^T wbar2
-        TRY(vjp_pass<N>(c, a, y(0), wb(1), vb(1), B, diss, st, false, ti(1)));
-        TRY(vjp_pass<N>(c, a, y(0), vb(1), yb, B, diss, st, false, ti(1)));
-        // y = x + h/2 v1 + h^2/8 w1 (the transposed combination applied by the pass's last kernel)
-        CombAdjBuilder cy(c, ce, 1.0, 0, yb, c->hb);
-        cy.term(xbn, nullptr, 1.0, 0, 0).term(vb(0), v(0), 0.5, 1, 0).term(wb(0), w(0), 0.125, 2, 0);
-        TRY(vjp_pass<N>(c, a, v(1), wb(1), yb, B, diss, st, true, nullptr, true, &cy.cb));
-        // w1 = J(x) v1: vbar1 += J(x)^T wbar1, xbar += J_0(v1)^T wbar1
-        TRY(vjp_pass<N>(c, a, xs, wb(0), vb(0), B, diss, st, false, ti(0), true));
-        TRY(vjp_pass<N>(c, a, v(0), wb(0), xbn, B, diss, st, true, nullptr, true));
-        // v1 = f(x): xbar += J(x)^T vbar1
-        TRY(vjp_pass<N>(c, a, xs, vb(0), xbn, B, diss, st, false, ti(0), true));
-      }
-      std::swap(xb, xbn);
+      if (td) TRY(rev_td(a, xs, y(0), v(0), v(1), w(0), w(1), ti(0), ti(1)));
+      else TRY(rev_step(xs, y, v, ti));
     }
   }
   TRY(unpack<N>(xb, fbar, B, msk, st));
@@ -1665,9 +1900,13 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
   const Tableau& tb = tableau(G_LRK4);
   const int S = tb.S;
   const int nseg = (steps + ckpt - 1) / ckpt;
-  // units: xw (state of the fused steps), xb, xbn, yb, spare, vraw, vbraw | segment inputs [ckpt] |
-  // y[S], v[S], vbar[S]
-  const int units = 7 + ckpt + 3 * S;
+  StepTape tp[2];
+  int ntape = 0;
+  TRY(step_tapes<N>(c, B, S, 0, ckpt, nseg, tp, &ntape));
+  // units: xw (state of the fused steps), xb, xbn, yb, spare, vraw, vbraw | segment inputs [ckpt]
+  // (without a tape) | y[S], v[S], vbar[S]
+  const int seg_units = ntape ? 0 : ckpt;
+  const int units = 7 + seg_units + 3 * S;
   TRY(grow(&c->gen, &c->gen_cap, (size_t)units * cs));
   TRY(grow(&c->ckpt, &c->ckpt_cap, (size_t)nseg * cs));
   Args a = base_args(c);
@@ -1676,7 +1915,7 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
   double2 *xw = slot(0), *xb = slot(1), *xbn = slot(2), *yb = slot(3), *vraw = slot(5);
   double2* vbraw = slot(6);
   auto xt = [&](int t) { return slot(7 + t); };
-  const int s0 = 7 + ckpt;
+  const int s0 = 7 + seg_units;
   auto y = [&](int k) { return slot(s0 + k); };
   auto v = [&](int k) { return slot(s0 + S + k); };
   auto vb = [&](int k) { return slot(s0 + 2 * S + k); };
@@ -1701,13 +1940,15 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
     }
     tabs.push_back(cb.cb);
   }
+  // the tapes' per-step tables (the launch's dt) follow the forward pass's S tables
+  for (int p = 0; p < ntape; ++p) tape_tables(tabs, c, ce, tb, 0, nullptr, tp[p], ckpt, xw);
   TRY(grow(&c->gcomb, &c->gcomb_cap, (tabs.size() * sizeof(Comb) + sizeof(double2) - 1) / sizeof(double2)));
   Comb* dtabs = (Comb*)c->gcomb;
   HIPCHK(hipMemcpyAsync(dtabs, tabs.data(), tabs.size() * sizeof(Comb), hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));  // the host table goes out of scope
-  // derived fields of the stage inputs of the step being transposed
+  // without a tape: derived fields of the stage inputs of the step being transposed
   const size_t is = (size_t)B * 8 * N * Geo<N>::KP;
-  TRY(grow(&c->itape, &c->it_cap, (size_t)(S + 1) * is));
+  if (!ntape) TRY(grow(&c->itape, &c->it_cap, (size_t)(S + 1) * is));
   auto ti = [&](int k) { return c->itape + (size_t)k * is; };
   Args af = a;
   af.adaptive = AD_FIXED;
@@ -1759,7 +2000,86 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
     }, false));
   TRY(pack<N>(xbar, xb, B, st));
   HIPCHK(hipMemsetAsync(c->hb, 0, cs * sizeof(double), st));
-  for (int jseg = nseg - 1; jseg >= 0; --jseg) {
+  // Transpose of accepted step m with input xs, stage inputs ys(k), stage
+  // values vs(k), raw first-stage value vr and the stage inputs' derived
+  // fields ds(k).  The step's cotangent accumulators start with their first
+  // contribution.
+  using Ptr = std::function<double2*(int)>;
+  auto rev_step = [&](int m, double2* xs, const Ptr& ys, const Ptr& vs, double2* vr, const Ptr& ds) -> int {
+    std::vector<char> touched(S, 0);
+    CombAdjBuilder fin(c, ce, ediv, 0, xb, c->hb);
+    fin.cb.h = DT(m);
+    fin.cb.hscale = HSC(m);
+    fin.term(xbn, xs, 1.0, 0, tb.p[S], true);
+    for (int j = 0; j < S; ++j)
+      if (tb.b[j] != 0.0) {
+        fin.term(vb(j), vs(j), tb.b[j], 1, tb.p[S] - tb.p[j], true);
+        touched[j] = 1;
+      }
+    TRY(comb_adj_launch<N>(fin.cb, B, st));
+    for (int k = S - 1; k >= 0; --k) {
+      if (!touched[k]) HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));  // a stage no combination reads
+      Args am = a;
+      am.dt = DT(m);
+      CombAdjBuilder sa(c, ce, ediv, 0, yb, c->hb);
+      sa.cb.h = DT(m);
+      sa.cb.hscale = HSC(m);
+      sa.term(xbn, k == 0 ? nullptr : xs, 1.0, 0, tb.p[k]);
+      for (int j = 0; j < k; ++j)
+        if (tb.A[k][j] != 0.0) {
+          sa.term(vb(j), vs(j), tb.A[k][j], 1, tb.p[k] - tb.p[j], !touched[j]);
+          touched[j] = 1;
+        }
+      // the pass's last kernel applies sa
+      if (k == 0) {
+        // v1 = P vraw: vbraw = P vbar1, d/dT through T s
+        CombAdjBuilder pc(c, ce, 1.0, 0, vb(0), c->hb);
+        pc.cb.h = PDT(m);
+        pc.cb.hscale = SIG(m);
+        pc.term(vbraw, vr, 1.0, 0, 1, true);
+        TRY(comb_adj_launch<N>(pc.cb, B, st));
+        TRY(vjp_pass<N>(c, am, xs, vbraw, yb, B, 0, st, false, ds(0), false, &sa.cb));
+      } else {
+        TRY(vjp_pass<N>(c, am, ys(k), vb(k), yb, B, 0, st, false, ds(k), false, &sa.cb));
+      }
+    }
+    std::swap(xb, xbn);
+    return 0;
+  };
+  if (ntape) {
+    // the segments' forward passes fill the tapes, on the side stream with two
+    // (as in run_gen_vjp)
+    const bool two = ntape == 2 && nseg > 1;
+    hipStream_t fs = two ? c->side : st;
+    if (two) {
+      HIPCHK(hipEventRecord(c->ev_rev[0], st));
+      HIPCHK(hipStreamWaitEvent(fs, c->ev_rev[0], 0));
+    }
+    for (int jseg = nseg - 1; jseg >= 0; --jseg) {
+      const int t0 = jseg * ckpt;
+      const int L = std::min(ckpt, steps - t0);
+      const int p = two ? (jseg & 1) : 0;
+      const StepTape& tap = tp[p];
+      if (two && jseg + 2 < nseg) HIPCHK(hipStreamWaitEvent(fs, c->ev_rev[3 + p], 0));
+      TRY(tape_forward<N>(af, tap, dtabs + S + (size_t)p * ckpt * S, S, ckpt, L, t0, c->ckpt + (size_t)jseg * cs,
+                          xw, B, fs, [&](Args& ak, int m, int k) {
+                            ak.dt = DT(m);
+                            if (k == 0) {
+                              ak.pdt = PDT(m);
+                              ak.vraw = tap.vraw(m - t0);
+                            }
+                          }));
+      if (two) {
+        HIPCHK(hipEventRecord(c->ev_rev[1 + p], fs));
+        HIPCHK(hipStreamWaitEvent(st, c->ev_rev[1 + p], 0));
+      }
+      for (int t = L - 1; t >= 0; --t)
+        TRY(rev_step(t0 + t, tap.x(t), [&](int k) { return tap.y(t, k); }, [&](int k) { return tap.v(t, k); },
+                     tap.vraw(t), [&](int k) { return tap.d(t, k); }));
+      if (two) HIPCHK(hipEventRecord(c->ev_rev[3 + p], st));
+    }
+  }
+  for (int jseg = ntape ? -1 : nseg - 1; jseg >= 0; --jseg) {
     const int t0 = jseg * ckpt;
     const int L = std::min(ckpt, steps - t0);
     TRY(copy(xt(0), c->ckpt + (size_t)jseg * cs));
@@ -1772,45 +2092,7 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
       double2* xs = xt(t);
       TRY(copy(xw, xs));
       TRY(fsteps(m, 1, nullptr, true));
-      // the step's cotangent accumulators start with their first contribution
-      std::vector<char> touched(S, 0);
-      CombAdjBuilder fin(c, ce, ediv, 0, xb, c->hb);
-      fin.cb.h = DT(m);
-      fin.cb.hscale = HSC(m);
-      fin.term(xbn, xs, 1.0, 0, tb.p[S], true);
-      for (int j = 0; j < S; ++j)
-        if (tb.b[j] != 0.0) {
-          fin.term(vb(j), v(j), tb.b[j], 1, tb.p[S] - tb.p[j], true);
-          touched[j] = 1;
-        }
-      TRY(comb_adj_launch<N>(fin.cb, B, st));
-      for (int k = S - 1; k >= 0; --k) {
-        if (!touched[k]) HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));  // a stage no combination reads
-        Args am = a;
-        am.dt = DT(m);
-        CombAdjBuilder sa(c, ce, ediv, 0, yb, c->hb);
-        sa.cb.h = DT(m);
-        sa.cb.hscale = HSC(m);
-        sa.term(xbn, k == 0 ? nullptr : xs, 1.0, 0, tb.p[k]);
-        for (int j = 0; j < k; ++j)
-          if (tb.A[k][j] != 0.0) {
-            sa.term(vb(j), v(j), tb.A[k][j], 1, tb.p[k] - tb.p[j], !touched[j]);
-            touched[j] = 1;
-          }
-        // the pass's last kernel applies sa
-        if (k == 0) {
-          // v1 = P vraw: vbraw = P vbar1, d/dT through T s
-          CombAdjBuilder pc(c, ce, 1.0, 0, vb(0), c->hb);
-          pc.cb.h = PDT(m);
-          pc.cb.hscale = SIG(m);
-          pc.term(vbraw, vraw, 1.0, 0, 1, true);
-          TRY(comb_adj_launch<N>(pc.cb, B, st));
-          TRY(vjp_pass<N>(c, am, xs, vbraw, yb, B, 0, st, false, ti(0), false, &sa.cb));
-        } else {
-          TRY(vjp_pass<N>(c, am, y(k), vb(k), yb, B, 0, st, false, ti(k), false, &sa.cb));
-        }
-      }
-      std::swap(xb, xbn);
+      TRY(rev_step(m, xs, y, v, vraw, ti));
     }
   }
   TRY(unpack<N>(xb, xbar, B, 0, st));

@@ -42,12 +42,13 @@ const char* ecs_last_error(void);
 int ecs_ctx_n(const ecs_ctx* ctx);
 
 /* Execution option of a context (no counterpart in the reference, whose
- * reverse mode is XLA's): with enable = 1 (the default) the reverse passes of
- * Lawson RK4 recompute the next segment on a stream of the context while the
- * caller's stream sweeps the current one backward (two step tapes, when they
- * fit 4 GiB); 0 keeps one stream.  The results are the same bits either way;
- * callers that already run several contexts concurrently (a serving loop)
- * may prefer one stream per call. */
+ * reverse mode is XLA's): with enable = 1 (the default) the reverse passes
+ * (Lawson RK4, classical RK4, Lawson RK6, TDRK4, RK4(3)) recompute the next
+ * segment on a stream of the context while the caller's stream sweeps the
+ * current one backward (two step tapes, each when it fits 4 GiB); 0 keeps one
+ * stream.  The results are the same bits either way; callers that already run
+ * several contexts concurrently (a serving loop) may prefer one stream per
+ * call. */
 int ecs_ctx_set_rev_overlap(ecs_ctx* ctx, int enable);
 
 /* state_vel(fields, param_dict, include_dissipation)  (mhd_jax.py:105-175).

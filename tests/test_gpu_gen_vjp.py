@@ -219,3 +219,71 @@ def test_lawson_rk4_reverse_overlap_gives_the_same_bits(n, B, steps):
         lhs = O.rdot(g[b].cpu().numpy(), jdf[b].cpu().numpy())
         rhs = O.rdot(res[True][0][b].cpu().numpy(), df[b].cpu().numpy())
         assert abs(lhs - rhs) < 1e-10 * abs(lhs), (b, lhs, rhs)
+
+
+@pytest.mark.parametrize("scheme,n,B,steps,ckpt", [("rk4", 128, 1, 70, 16), ("lawson_rk6", 128, 2, 40, 16),
+                                                   ("tdrk4", 128, 2, 40, 16), ("rk4", 256, 1, 33, 8),
+                                                   ("tdrk4", 256, 1, 21, 8)])
+def test_generic_reverse_step_tape_and_overlap(scheme, n, B, steps, ckpt, monkeypatch):
+    """The generic reverse pass with a step tape per segment (one forward pass
+    fills it; no step is recomputed before its transpose) gives the same bits
+    with the next segment's forward pass on a second stream as on one, and
+    matches the pass that recomputes each step (ECSMHD_SEG_TAPE=0) to 1e-12:
+    several segments with a ragged last one, batches 1-2."""
+    print(f"{scheme} reverse step tape n={n} B={B} steps={steps} ckpt={ckpt}", flush=True)
+    pd, ctx = setup(n)
+    f = torch.as_tensor(np.stack([spectrum(n, 70 + s) for s in range(B)])).cuda()
+    c = torch.as_tensor(np.stack([spectrum(n, 80 + s) for s in range(B)])).cuda()
+    dts = [0.004 + 0.001 * s for s in range(B)]
+    vjp = getattr(ctx, scheme + "_vjp")
+    res = {}
+    for ov in (False, True):
+        ctx.set_rev_overlap(ov)
+        res[ov] = vjp(f, c, dts, steps, ckpt=ckpt)
+    assert torch.equal(res[True][0], res[False][0])
+    assert list(res[True][1]) == list(res[False][1])
+    monkeypatch.setenv("ECSMHD_SEG_TAPE", "0")
+    old = vjp(f, c, dts, steps, ckpt=ckpt)
+    monkeypatch.delenv("ECSMHD_SEG_TAPE")
+    print("bitwise equal to the recomputing pass:", torch.equal(old[0], res[True][0]), flush=True)
+    for b in range(B):
+        assert rel(res[True][0][b].cpu().numpy(), old[0][b].cpu().numpy()) < 1e-12
+        assert abs(res[True][1][b] - old[1][b]) < 1e-12 * max(1.0, abs(old[1][b]))
+
+
+@pytest.mark.parametrize("n,B,T", [(64, 2, 0.6), (128, 1, 0.5), (256, 1, 0.15)])
+def test_rk43_reverse_step_tape_and_overlap(n, B, T, monkeypatch):
+    """The same for the RK4(3) reverse pass along a recorded schedule with
+    rejections (the tape holds the raw first-stage value the carried-k1 factor
+    scales): overlap on and off give the same bits, and the recomputing pass
+    agrees to 1e-12; and the dot test against the RK4(3) tangent."""
+    print(f"rk43 reverse step tape n={n} B={B} T={T}", flush=True)
+    pd, ctx = setup(n)
+    x0 = torch.as_tensor(np.stack([(1 + 0.3 * s) * spectrum(n, 90 + s) for s in range(B)])).cuda()
+    c = torch.as_tensor(np.stack([spectrum(n, 100 + s) for s in range(B)])).cuda()
+    d = torch.as_tensor(np.stack([spectrum(n, 110 + s) for s in range(B)])).cuda()
+    Ts, dTs = [T * (1 + 0.2 * s) for s in range(B)], [0.3 - 0.5 * s for s in range(B)]
+    _, infos = ctx.lawson_rk43(x0, Ts, 0.5, 1e-8, 64, 64)
+    assert all(i["completed"] for i in infos)
+    assert sum(i["rejected"] for i in infos) >= 1
+    # three or more segments
+    ckpt = max(2, (max(i["accepted"] for i in infos) + 2) // 3)
+    print("accepted", [i["accepted"] for i in infos], "ckpt", ckpt, flush=True)
+    res = {}
+    for ov in (False, True):
+        ctx.set_rev_overlap(ov)
+        res[ov] = ctx.lawson_rk43_vjp(x0, c, Ts, infos, ckpt=ckpt)
+    assert torch.equal(res[True][0], res[False][0])
+    assert list(res[True][1]) == list(res[False][1])
+    monkeypatch.setenv("ECSMHD_SEG_TAPE", "0")
+    old = ctx.lawson_rk43_vjp(x0, c, Ts, infos, ckpt=ckpt)
+    monkeypatch.delenv("ECSMHD_SEG_TAPE")
+    print("bitwise equal to the recomputing pass:", torch.equal(old[0], res[True][0]), flush=True)
+    for b in range(B):
+        assert rel(res[True][0][b].cpu().numpy(), old[0][b].cpu().numpy()) < 1e-12
+        assert abs(res[True][1][b] - old[1][b]) < 1e-12 * max(1.0, abs(old[1][b]))
+    _, jd = ctx.lawson_rk43_jvp(x0, d, Ts, dTs, infos)
+    for b in range(B):
+        lhs = O.rdot(c[b].cpu().numpy(), jd[b].cpu().numpy())
+        rhs = O.rdot(res[True][0][b].cpu().numpy(), d[b].cpu().numpy()) + res[True][1][b] * dTs[b]
+        assert abs(lhs - rhs) < 1e-10 * abs(lhs), (b, lhs, rhs)
