@@ -128,6 +128,34 @@ def test_newton_configuration_full_iteration_properties():
     assert float(torch.linalg.vector_norm(f_new)) < log["f_norm"] * 1.01
 
 
+def test_newton_configuration_transpose_dot_test():
+    """The transpose of the headline matvec at its full size (VJP through the
+    whole RK4(3) trajectory of T = 4.3, about 1400 attempts, in checkpointed
+    segments whose forward passes run on a second stream) against the matvec:
+    <u, J v> = <J^T u, v>; and the one-stream reverse pass gives the same bits."""
+    print("N=128 transpose dot test, T=4.3", flush=True)
+    n = 128
+    pd = newton_pd(n)
+    inp = {"fields": newton_guess(), "T": 4.3, "sx": 0.1}
+    lin = LF.Linearization(inp, pd, "Lawson_RK43")
+    assert lin.info["completed"] and lin.info["attempts"] > 1000
+    rng = np.random.default_rng(12)
+    v = torch.as_tensor(rng.standard_normal(2 * n * n + 2)).cuda()
+    u = torch.as_tensor(rng.standard_normal(2 * n * n + 2)).cuda()
+    Jv = utils.flatten(lin.jvp(utils.unflatten(v, n)))
+    JTu = utils.flatten(lin.vjp(utils.unflatten(u, n)))
+    lhs, rhs = float(torch.dot(u, Jv)), float(torch.dot(JTu, v))
+    print("dot test", lhs, rhs, flush=True)
+    assert abs(lhs - rhs) < 1e-9 * max(abs(lhs), 1.0)
+    ctx = mhd.context(pd)
+    ctx.set_rev_overlap(False)
+    try:
+        one = utils.flatten(lin.vjp(utils.unflatten(u, n)))
+    finally:
+        ctx.set_rev_overlap(True)
+    assert torch.equal(one, JTu)
+
+
 def test_batched_candidates_equal_one_at_a_time():
     """Eight candidates (N = 64 x2, 128 x3, 256 x3), T ~ U[2, 8], atol 1e-8:
     the batched, grouped, concurrent run gives each candidate the bits and
