@@ -374,6 +374,32 @@ def serving_leg(jobs, dump_dir=None, log_to_stderr=False):
     return stats
 
 
+def probe_reverse():
+    """Short fixed workload for a kernel trace of the reverse passes: the RK4(3)
+    VJP at N=128, batch 1, along the schedule of T = 1 on G(128, 0) (atol 1e-8,
+    checkpoints every 64 accepted steps), twice with the reverse pass on one
+    stream, then twice with the next segment's forward pass on the context's
+    second stream; prints each call's wall time."""
+    import torch
+    from ecs_mhd_2d_amd import mhd
+    from ecs_mhd_2d_amd.synthetic import spectrum
+    n = 128
+    pd = mhd.make_param_dict(n, 1 / 40, 1 / 40)
+    ctx = mhd.context(pd)
+    x0 = torch.as_tensor(spectrum(n, 0)).cuda()[None]
+    g = torch.as_tensor(spectrum(n, 2)).cuda()[None]
+    _, infos = ctx.lawson_rk43(x0, [1.0], 0.01, 1e-8, 256, 64)
+    for ov in (False, True):
+        ctx.set_rev_overlap(ov)
+        for rep in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.lawson_rk43_vjp(x0, g, [1.0], infos, ckpt=64)
+            torch.cuda.synchronize()
+            print(f"probe-reverse overlap={int(ov)} rep={rep} accepted={infos[0]['accepted']} "
+                  f"wall_ms={1e3 * (time.perf_counter() - t0):.3f}", flush=True)
+
+
 def probe_roofline(batched=False, kernel=None):
     """The matvec's dominant kernel (tangent row kernel of the adaptive tangent
     replay, N=128, batch 1) or,
@@ -425,6 +451,8 @@ def main():
                          "4 tangent row of the tangent replay)")
     ap.add_argument("--probe", action="store_true",
                     help="run a short fixed kernel workload for rocprofv3 and exit")
+    ap.add_argument("--probe-reverse", action="store_true",
+                    help="run the RK4(3) reverse pass on one and on two streams for a kernel trace and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build targets a single MI355X); got --gpus {args.gpus}",
@@ -436,6 +464,9 @@ def main():
     torch.cuda.init()
     if args.probe:
         probe()
+        return
+    if args.probe_reverse:
+        probe_reverse()
         return
     if args.probe_roofline or args.probe_roofline_batched or args.probe_kernel:
         kernel = tuple(int(t) for t in args.probe_kernel.split(",")) if args.probe_kernel else None
