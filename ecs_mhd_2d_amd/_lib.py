@@ -65,6 +65,14 @@ _SIGS = {
     "ecs_lawson_rk6_vjp": (c_int, [c_void_p, c_void_p, c_void_p, P_double, c_int, P_double, c_int, c_int, c_int,
                                    c_void_p]),
     "ecs_tdrk4_vjp": (c_int, [c_void_p, c_void_p, c_void_p, P_double, c_int, P_double, c_int, c_int, c_void_p]),
+    "ecs_gen_fwd_ckpt": (c_int, [c_void_p, c_void_p, c_int, P_double, c_int, c_int, c_int, c_int, c_void_p,
+                                 c_void_p]),
+    "ecs_gen_vjp_ckpt": (c_int, [c_void_p, c_void_p, c_void_p, P_double, c_int, P_double, c_int, c_int, c_int,
+                                 c_int, c_void_p]),
+    "ecs_lawson_rk43_fwd_ckpt": (c_int, [c_void_p, c_void_p, c_int, P_double, P_double, P_int8, P_int, c_int,
+                                         c_int, c_void_p, c_void_p]),
+    "ecs_lawson_rk43_vjp_ckpt": (c_int, [c_void_p, c_void_p, c_void_p, P_double, c_int, P_double, P_double,
+                                         P_int8, P_int, c_int, c_int, c_void_p]),
     "ecs_tuned_tiles": (c_int, [c_int, c_int, P_int]),
     "ecs_tuned_plans": (c_int, [c_int, P_int]),
     "ecs_tuned_splits": (c_int, [c_int, c_int, P_int]),

@@ -184,6 +184,33 @@ class Context:
     def tdrk4_vjp(self, f0, fbar, dt, steps: int, ckpt: int = 64):
         return self._fixed_vjp("ecs_tdrk4_vjp", f0, fbar, dt, steps, ckpt)
 
+    # two-phase reverse passes (one forward pass, then a reverse pass per cotangent)
+    SCHEMES = {"rk4": 0, "lawson_rk6": 1, "tdrk4": 2}
+
+    def gen_fwd_ckpt(self, scheme: str, f0, dt, steps: int, ckpt: int = 64, masked: bool = True):
+        """Forward pass of "rk4", "lawson_rk6" or "tdrk4" that keeps the checkpoints of its
+        reverse pass: (final spectra, the bits of rk4 / lawson_rk6 / tdrk4; checkpoints)."""
+        f = _as_spec(f0, self.n).clone()
+        B = _batch(f)
+        ck = torch.empty(int(self.lib.ecs_ckpt_doubles(self.handle, B, int(steps), int(ckpt))),
+                         dtype=torch.float64, device=f.device)
+        _lib.check(self.lib.ecs_gen_fwd_ckpt(self.handle, f.data_ptr(), B, _lib.dptr(_per_batch(dt, B)),
+                                             int(steps), self.SCHEMES[scheme], int(bool(masked)), int(ckpt),
+                                             ck.data_ptr(), _stream()), "ecs_gen_fwd_ckpt")
+        return f, ck
+
+    def gen_vjp_ckpt(self, scheme: str, ck, fbar, dt, steps: int, ckpt: int = 64, masked: bool = True):
+        """(f0bar, hbar[B]) from the checkpoints of gen_fwd_ckpt (the same arguments)."""
+        g = _as_spec(fbar, self.n).clone()
+        B = _batch(g)
+        if ck.numel() != int(self.lib.ecs_ckpt_doubles(self.handle, B, int(steps), int(ckpt))):
+            raise ValueError("gen_vjp_ckpt: checkpoints of another batch, step count or segment length")
+        hbar = (ctypes.c_double * B)()
+        _lib.check(self.lib.ecs_gen_vjp_ckpt(self.handle, ck.data_ptr(), g.data_ptr(), hbar, B,
+                                             _lib.dptr(_per_batch(dt, B)), int(steps), self.SCHEMES[scheme],
+                                             int(bool(masked)), int(ckpt), _stream()), "ecs_gen_vjp_ckpt")
+        return g, list(hbar)
+
     # ------------------------------------------------------------ Lawson RK4(3)
     def lawson_rk43(self, x, T, h0: float, atol: float, max_steps_per_checkpoint: int, checkpoints: int):
         """Returns (x_T, infos) with one info dict per candidate (keys of timestepping.py:327-333
@@ -220,20 +247,57 @@ class Context:
         return x, dx
 
 
+    @staticmethod
+    def _schedule(infos):
+        budget = len(infos[0]["hs"])
+        hs = np.ascontiguousarray(np.stack([i["hs"] for i in infos]).astype(np.float64))
+        acc = np.ascontiguousarray(np.stack([i["acc"] for i in infos]).astype(np.int8))
+        att = np.ascontiguousarray(np.array([i["attempts"] for i in infos], dtype=np.int32))
+        return budget, hs, acc, att
+
     def lawson_rk43_vjp(self, x0, xbar, T, infos, ckpt: int = 64):
         """(x0bar, Tbar[B]) along the schedules ``infos`` of a lawson_rk43 call."""
         x0 = _as_spec(x0, self.n)
         g = _as_spec(xbar, self.n).clone()
         B = _batch(x0)
-        budget = len(infos[0]["hs"])
-        hs = np.ascontiguousarray(np.stack([i["hs"] for i in infos]).astype(np.float64))
-        acc = np.ascontiguousarray(np.stack([i["acc"] for i in infos]).astype(np.int8))
-        att = np.ascontiguousarray(np.array([i["attempts"] for i in infos], dtype=np.int32))
+        budget, hs, acc, att = self._schedule(infos)
         Tbar = (ctypes.c_double * B)()
         _lib.check(self.lib.ecs_lawson_rk43_vjp(self.handle, x0.data_ptr(), g.data_ptr(), Tbar, B,
                                                 _lib.dptr(_per_batch(T, B)), hs.ctypes.data_as(_lib.P_double),
                                                 acc.ctypes.data_as(_lib.P_int8), att.ctypes.data_as(_lib.P_int),
                                                 budget, int(ckpt), _stream()), "ecs_lawson_rk43_vjp")
+        return g, list(Tbar)
+
+    def _rk43_ckpt_doubles(self, B, infos, ckpt):
+        m = max(1, max(int(i["accepted"]) for i in infos))
+        return int(self.lib.ecs_ckpt_doubles(self.handle, B, m, int(ckpt)))
+
+    def lawson_rk43_fwd_ckpt(self, x0, T, infos, ckpt: int = 64):
+        """Checkpoints of the reverse pass along the schedules ``infos`` (x0 not modified)."""
+        x0 = _as_spec(x0, self.n)
+        B = _batch(x0)
+        budget, hs, acc, att = self._schedule(infos)
+        ck = torch.empty(self._rk43_ckpt_doubles(B, infos, ckpt), dtype=torch.float64, device=x0.device)
+        _lib.check(self.lib.ecs_lawson_rk43_fwd_ckpt(self.handle, x0.data_ptr(), B, _lib.dptr(_per_batch(T, B)),
+                                                     hs.ctypes.data_as(_lib.P_double),
+                                                     acc.ctypes.data_as(_lib.P_int8),
+                                                     att.ctypes.data_as(_lib.P_int), budget, int(ckpt),
+                                                     ck.data_ptr(), _stream()), "ecs_lawson_rk43_fwd_ckpt")
+        return ck
+
+    def lawson_rk43_vjp_ckpt(self, ck, xbar, T, infos, ckpt: int = 64):
+        """(x0bar, Tbar[B]) from the checkpoints of lawson_rk43_fwd_ckpt (the same schedules)."""
+        g = _as_spec(xbar, self.n).clone()
+        B = _batch(g)
+        if ck.numel() != self._rk43_ckpt_doubles(B, infos, ckpt):
+            raise ValueError("lawson_rk43_vjp_ckpt: checkpoints of another batch, schedule or segment length")
+        budget, hs, acc, att = self._schedule(infos)
+        Tbar = (ctypes.c_double * B)()
+        _lib.check(self.lib.ecs_lawson_rk43_vjp_ckpt(self.handle, ck.data_ptr(), g.data_ptr(), Tbar, B,
+                                                     _lib.dptr(_per_batch(T, B)), hs.ctypes.data_as(_lib.P_double),
+                                                     acc.ctypes.data_as(_lib.P_int8),
+                                                     att.ctypes.data_as(_lib.P_int), budget, int(ckpt),
+                                                     _stream()), "ecs_lawson_rk43_vjp_ckpt")
         return g, list(Tbar)
 
 

@@ -443,7 +443,8 @@ int ecs_rk4_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar, int ba
   if (!c || !f0 || !fbar || !hbar || !dt || batch < 1 || steps < 1 || ckpt < 1)
     return fail(1, "ecs_rk4_vjp: bad arguments");
   GUARD(c);
-  return OPS(c->n)->gen_vjp(c, f0, fbar, hbar, batch, dt, steps, 0, 0, ckpt, (hipStream_t)stream);
+  return OPS(c->n)->gen_vjp(c, f0, fbar, hbar, batch, dt, steps, 0, 0, ckpt, nullptr, nullptr, nullptr,
+                            (hipStream_t)stream);
 }
 
 int ecs_lawson_rk6_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar, int batch, const double* dt,
@@ -451,7 +452,8 @@ int ecs_lawson_rk6_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   if (!c || !f0 || !fbar || !hbar || !dt || batch < 1 || steps < 1 || ckpt < 1)
     return fail(1, "ecs_lawson_rk6_vjp: bad arguments");
   GUARD(c);
-  return OPS(c->n)->gen_vjp(c, f0, fbar, hbar, batch, dt, steps, 1, masked ? 1 : 0, ckpt, (hipStream_t)stream);
+  return OPS(c->n)->gen_vjp(c, f0, fbar, hbar, batch, dt, steps, 1, masked ? 1 : 0, ckpt, nullptr, nullptr, nullptr,
+                            (hipStream_t)stream);
 }
 
 int ecs_tdrk4_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar, int batch, const double* dt, int steps,
@@ -459,7 +461,8 @@ int ecs_tdrk4_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar, int 
   if (!c || !f0 || !fbar || !hbar || !dt || batch < 1 || steps < 1 || ckpt < 1)
     return fail(1, "ecs_tdrk4_vjp: bad arguments");
   GUARD(c);
-  return OPS(c->n)->gen_vjp(c, f0, fbar, hbar, batch, dt, steps, 2, 0, ckpt, (hipStream_t)stream);
+  return OPS(c->n)->gen_vjp(c, f0, fbar, hbar, batch, dt, steps, 2, 0, ckpt, nullptr, nullptr, nullptr,
+                            (hipStream_t)stream);
 }
 
 int ecs_lawson_rk43_vjp(ecs_ctx* c, const double* x0, double* xbar, double* Tbar, int batch, const double* T,
@@ -470,7 +473,53 @@ int ecs_lawson_rk43_vjp(ecs_ctx* c, const double* x0, double* xbar, double* Tbar
   for (int b = 0; b < batch; ++b)
     if (attempts[b] < 0 || attempts[b] > budget) return fail(1, "ecs_lawson_rk43_vjp: attempts out of range");
   GUARD(c);
-  return OPS(c->n)->rk43_vjp(c, x0, xbar, Tbar, batch, T, hs, acc, attempts, budget, ckpt, (hipStream_t)stream);
+  return OPS(c->n)->rk43_vjp(c, x0, xbar, Tbar, batch, T, hs, acc, attempts, budget, ckpt, nullptr, nullptr,
+                             (hipStream_t)stream);
+}
+
+int ecs_gen_fwd_ckpt(ecs_ctx* c, double* f, int batch, const double* dt, int steps, int scheme, int masked, int ckpt,
+                     double* ck, void* stream) {
+  if (!c || !f || !dt || !ck || batch < 1 || steps < 1 || ckpt < 1 || scheme < 0 || scheme > 2)
+    return fail(1, "ecs_gen_fwd_ckpt: bad arguments");
+  GUARD(c);
+  return OPS(c->n)->gen_vjp(c, f, nullptr, nullptr, batch, dt, steps, scheme, scheme == 1 && masked ? 1 : 0, ckpt,
+                            nullptr, (double2*)ck, f, (hipStream_t)stream);
+}
+
+int ecs_gen_vjp_ckpt(ecs_ctx* c, const double* ck, double* fbar, double* hbar, int batch, const double* dt, int steps,
+                     int scheme, int masked, int ckpt, void* stream) {
+  if (!c || !ck || !fbar || !hbar || !dt || batch < 1 || steps < 1 || ckpt < 1 || scheme < 0 || scheme > 2)
+    return fail(1, "ecs_gen_vjp_ckpt: bad arguments");
+  GUARD(c);
+  return OPS(c->n)->gen_vjp(c, nullptr, fbar, hbar, batch, dt, steps, scheme, scheme == 1 && masked ? 1 : 0, ckpt,
+                            (const double2*)ck, nullptr, nullptr, (hipStream_t)stream);
+}
+
+static int check_rk43(const char* who, const int* attempts, int batch, int budget) {
+  for (int b = 0; b < batch; ++b)
+    if (attempts[b] < 0 || attempts[b] > budget) return fail(1, std::string(who) + ": attempts out of range");
+  return 0;
+}
+
+int ecs_lawson_rk43_fwd_ckpt(ecs_ctx* c, const double* x0, int batch, const double* T, const double* hs,
+                             const int8_t* acc, const int* attempts, int budget, int ckpt, double* ck, void* stream) {
+  if (!c || !x0 || !T || !hs || !acc || !attempts || !ck || batch < 1 || budget < 1 || ckpt < 1)
+    return fail(1, "ecs_lawson_rk43_fwd_ckpt: bad arguments");
+  TRY(check_rk43("ecs_lawson_rk43_fwd_ckpt", attempts, batch, budget));
+  GUARD(c);
+  return OPS(c->n)->rk43_vjp(c, x0, nullptr, nullptr, batch, T, hs, acc, attempts, budget, ckpt, nullptr,
+                             (double2*)ck, (hipStream_t)stream);
+}
+
+int ecs_lawson_rk43_vjp_ckpt(ecs_ctx* c, const double* ck, double* xbar, double* Tbar, int batch, const double* T,
+                             const double* hs, const int8_t* acc, const int* attempts, int budget, int ckpt,
+                             void* stream) {
+  if (!c || !ck || !xbar || !Tbar || !T || !hs || !acc || !attempts || batch < 1 || budget < 1 || ckpt < 1)
+    return fail(1, "ecs_lawson_rk43_vjp_ckpt: bad arguments");
+  TRY(check_rk43("ecs_lawson_rk43_vjp_ckpt", attempts, batch, budget));
+  GUARD(c);
+  return OPS(c->n)->rk43_vjp(c, nullptr, xbar, Tbar, batch, T, hs, acc, attempts, budget, ckpt, (const double2*)ck,
+                             nullptr, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -1554,9 +1554,14 @@ static int tape_forward_td(const Args& af, const StepTape& T, const Comb* ttab, 
 //   TDRK4 (a_j = h^2 w_j, w_j = J(z_j) v_j): wbar_j gives J(z_j)^T wbar_j to
 //     vbar_j and J_0(v_j)^T wbar_j to zbar_j.
 // d/dh of every weight is accumulated per mode and summed at the end.
+// Two-phase use: with ck_out the call only runs the forward pass over all
+// steps, storing the checkpoints into ck_out ([nseg][B][2][R][KC]) and the
+// final spectra into f_out (the forward entry points' result); with ck_in
+// it skips the forward pass and runs the reverse from those checkpoints.
 template <int N>
 static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar, int B, const double* dt, int steps,
-                       int scheme, int masked, int ckpt, hipStream_t st) {
+                       int scheme, int masked, int ckpt, const double2* ck_in, double2* ck_out, double* f_out,
+                       hipStream_t st) {
   const bool td = scheme == G_TDRK4;
   TRY(ensure_adj(c, td ? 2 * B : B));
   TRY(ensure_cap(c, td ? 2 * B : B));
@@ -1568,17 +1573,18 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   const Tableau& tb = tableau(scheme);
   const int S = td ? 2 : tb.S;
   const int nseg = (steps + ckpt - 1) / ckpt;
-  // a step tape per segment when it fits
+  // a step tape per segment when it fits (the reverse pass only)
   StepTape tp[2];
   int ntape = 0;
-  TRY(step_tapes<N>(c, B, S, td ? 2 : 0, ckpt, nseg, tp, &ntape));
+  if (!ck_out) TRY(step_tapes<N>(c, B, S, td ? 2 : 0, ckpt, nseg, tp, &ntape));
   // units of cs: xw (state of the fused steps), xb, xbn, yb, 3 spare | segment inputs [ckpt]
   // (without a tape) | stages
   const int stage_units = td ? 10 : 3 * S;
   const int seg_units = ntape ? 0 : ckpt;
   const int units = 7 + seg_units + stage_units;
   TRY(grow(&c->gen, &c->gen_cap, (size_t)units * cs));
-  TRY(grow(&c->ckpt, &c->ckpt_cap, (size_t)nseg * cs));
+  if (!ck_in && !ck_out) TRY(grow(&c->ckpt, &c->ckpt_cap, (size_t)nseg * cs));
+  const double2* ckb = ck_in ? ck_in : ck_out ? ck_out : c->ckpt;
   TRY(upload_scalars(c, 0, dt, B, st));
   Args a = base_args(c);
   double2* g = c->gen;
@@ -1701,12 +1707,24 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   };
 
   // forward pass to the last segment's start, checkpointing every ckpt steps
-  TRY(pack<N>(f0, xw, B, st));
-  TRY(copy(c->ckpt, xw));
-  if (nseg > 1)
-    TRY(fsteps((nseg - 1) * ckpt, [&](int s) -> int {
-      return (s + 1) % ckpt == 0 ? copy(c->ckpt + (size_t)((s + 1) / ckpt) * cs, xw) : 0;
+  // (the forward phase: over every step, leaving the result in f_out)
+  if (ck_out) {
+    TRY(pack<N>(f0, xw, B, st));
+    TRY(copy(ck_out, xw));
+    TRY(fsteps(steps, [&](int s) -> int {
+      return (s + 1) % ckpt == 0 && s + 1 < steps ? copy(ck_out + (size_t)((s + 1) / ckpt) * cs, xw) : 0;
     }, false));
+    TRY(unpack<N>(xw, f_out, B, msk, st));
+    return 0;
+  }
+  if (!ck_in) {
+    TRY(pack<N>(f0, xw, B, st));
+    TRY(copy(c->ckpt, xw));
+    if (nseg > 1)
+      TRY(fsteps((nseg - 1) * ckpt, [&](int s) -> int {
+        return (s + 1) % ckpt == 0 ? copy(c->ckpt + (size_t)((s + 1) / ckpt) * cs, xw) : 0;
+      }, false));
+  }
   // reverse
   TRY(pack<N>(fbar, xb, B, st));
   HIPCHK(hipMemsetAsync(c->hb, 0, cs * sizeof(double), st));
@@ -1787,7 +1805,7 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
       const StepTape& tap = tp[p];
       if (two && jseg + 2 < nseg) HIPCHK(hipStreamWaitEvent(fs, c->ev_rev[3 + p], 0));
       const Comb* ttab = dtabs + S + (size_t)p * ckpt * S;
-      const double2* ck = c->ckpt + (size_t)jseg * cs;
+      const double2* ck = ckb + (size_t)jseg * cs;
       if (td) TRY(tape_forward_td<N>(af, tap, ttab, ckpt, L, ck, xw, B, fs));
       else TRY(tape_forward<N>(af, tap, ttab, S, ckpt, L, t0, ck, xw, B, fs, nullptr));
       if (two) {
@@ -1811,7 +1829,7 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
     const int t0 = jseg * ckpt;
     const int L = std::min(ckpt, steps - t0);
     // the segment's step inputs, then per step (last first) its stages
-    TRY(copy(xt(0), c->ckpt + (size_t)jseg * cs));
+    TRY(copy(xt(0), ckb + (size_t)jseg * cs));
     if (L > 1) {
       TRY(copy(xw, xt(0)));
       TRY(fsteps(L - 1, [&](int t) -> int { return copy(xt(t + 1), xw); }, false));
@@ -1842,10 +1860,14 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
 // Candidates with fewer accepted steps take zero-length steps (dt = 0: the
 // identity, with zero derivative).  d/dT is accumulated per mode with the
 // weights dt_m/dT = h_m and d(T s_m)/dT = s_m.  Returns Tbar[b] on the host.
+// Two-phase use: with ck_out the call only runs the forward pass to the last
+// segment's start, storing the checkpoints of the accepted steps into ck_out
+// ([nseg][B][2][R][KC], nseg = ceil(max(1, most accepted steps) / ckpt));
+// with ck_in it runs the reverse from those checkpoints.
 template <int N>
 static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tbar, int B, const double* T,
                         const double* hs, const int8_t* accf, const int* attempts, int budget, int ckpt,
-                        hipStream_t st) {
+                        const double2* ck_in, double2* ck_out, hipStream_t st) {
   TRY(ensure_adj(c, B));
   TRY(autotune<N>(c, B, st));
   TRY(ensure_adj(c, B));
@@ -1902,13 +1924,15 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
   const int nseg = (steps + ckpt - 1) / ckpt;
   StepTape tp[2];
   int ntape = 0;
-  TRY(step_tapes<N>(c, B, S, 0, ckpt, nseg, tp, &ntape));
+  if (!ck_out) TRY(step_tapes<N>(c, B, S, 0, ckpt, nseg, tp, &ntape));
   // units: xw (state of the fused steps), xb, xbn, yb, spare, vraw, vbraw | segment inputs [ckpt]
   // (without a tape) | y[S], v[S], vbar[S]
   const int seg_units = ntape ? 0 : ckpt;
   const int units = 7 + seg_units + 3 * S;
   TRY(grow(&c->gen, &c->gen_cap, (size_t)units * cs));
-  TRY(grow(&c->ckpt, &c->ckpt_cap, (size_t)nseg * cs));
+  if (!ck_in && !ck_out) TRY(grow(&c->ckpt, &c->ckpt_cap, (size_t)nseg * cs));
+  double2* ckw = ck_out ? ck_out : c->ckpt;            // where the forward pass writes
+  const double2* ckb = ck_in ? ck_in : ckw;             // where the reverse pass reads
   Args a = base_args(c);
   double2* g = c->gen;
   auto slot = [&](int u) { return g + (size_t)u * cs; };
@@ -1992,12 +2016,15 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
   };
 
   // forward pass to the last segment's start, checkpointing every ckpt steps
-  TRY(pack<N>(x0f, xw, B, st));
-  TRY(copy(c->ckpt, xw));
-  if (nseg > 1)
-    TRY(fsteps(0, (nseg - 1) * ckpt, [&](int s) -> int {
-      return (s + 1) % ckpt == 0 ? copy(c->ckpt + (size_t)((s + 1) / ckpt) * cs, xw) : 0;
-    }, false));
+  if (!ck_in) {
+    TRY(pack<N>(x0f, xw, B, st));
+    TRY(copy(ckw, xw));
+    if (nseg > 1)
+      TRY(fsteps(0, (nseg - 1) * ckpt, [&](int s) -> int {
+        return (s + 1) % ckpt == 0 ? copy(ckw + (size_t)((s + 1) / ckpt) * cs, xw) : 0;
+      }, false));
+  }
+  if (ck_out) return 0;
   TRY(pack<N>(xbar, xb, B, st));
   HIPCHK(hipMemsetAsync(c->hb, 0, cs * sizeof(double), st));
   // Transpose of accepted step m with input xs, stage inputs ys(k), stage
@@ -2061,7 +2088,7 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
       const int p = two ? (jseg & 1) : 0;
       const StepTape& tap = tp[p];
       if (two && jseg + 2 < nseg) HIPCHK(hipStreamWaitEvent(fs, c->ev_rev[3 + p], 0));
-      TRY(tape_forward<N>(af, tap, dtabs + S + (size_t)p * ckpt * S, S, ckpt, L, t0, c->ckpt + (size_t)jseg * cs,
+      TRY(tape_forward<N>(af, tap, dtabs + S + (size_t)p * ckpt * S, S, ckpt, L, t0, ckb + (size_t)jseg * cs,
                           xw, B, fs, [&](Args& ak, int m, int k) {
                             ak.dt = DT(m);
                             if (k == 0) {
@@ -2082,7 +2109,7 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
   for (int jseg = ntape ? -1 : nseg - 1; jseg >= 0; --jseg) {
     const int t0 = jseg * ckpt;
     const int L = std::min(ckpt, steps - t0);
-    TRY(copy(xt(0), c->ckpt + (size_t)jseg * cs));
+    TRY(copy(xt(0), ckb + (size_t)jseg * cs));
     if (L > 1) {
       TRY(copy(xw, xt(0)));
       TRY(fsteps(t0, L - 1, [&](int t) -> int { return copy(xt(t + 1), xw); }, false));

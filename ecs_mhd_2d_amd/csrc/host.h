@@ -287,9 +287,11 @@ struct GridOps {
   int (*time_kernel)(ecs_ctx*, int, int, int, double*, hipStream_t);
   int (*gen)(ecs_ctx*, double*, int, const double*, int, int, int, hipStream_t);
   int (*gen_jvp)(ecs_ctx*, double*, double*, int, const double*, const double*, int, int, int, hipStream_t);
-  int (*gen_vjp)(ecs_ctx*, const double*, double*, double*, int, const double*, int, int, int, int, hipStream_t);
+  // reverse passes: one call, or (checkpoints in / checkpoints out) one phase of the two-phase form
+  int (*gen_vjp)(ecs_ctx*, const double*, double*, double*, int, const double*, int, int, int, int,
+                 const double2*, double2*, double*, hipStream_t);
   int (*rk43_vjp)(ecs_ctx*, const double*, double*, double*, int, const double*, const double*, const int8_t*,
-                  const int*, int, int, hipStream_t);
+                  const int*, int, int, const double2*, double2*, hipStream_t);
   int (*state_vel_noalloc)(ecs_ctx*, const double*, double*, int, int, double2*, size_t, hipStream_t);
 };
 const GridOps* grid_ops(int n);

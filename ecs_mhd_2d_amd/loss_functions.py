@@ -212,6 +212,7 @@ class Linearization:
         self.f0 = _rfft2(self.fields)
         self.info = None
         self.fT = None
+        self._rev = {}   # the reverse passes' forward pass of this primal (vjp), once computed
         if mode == "Lawson_RK43":
             self.fT, self.info = integrate(self.f0, self.T, param_dict, mode)
         self.vt, self.vx = phase_vectors(self.f0, param_dict)
@@ -241,9 +242,11 @@ class Linearization:
     def vjp(self, cot):
         """J(x)^T c for the cotangent dict c: the function jax.vjp(obj, x)
         returns, applied to c (newton.py:168-169); the adaptive schedule and
-        final state of the primal are the ones computed once for this input."""
+        final state of the primal are the ones computed once for this input,
+        and so are the reverse pass's checkpoints (the first call runs the
+        forward pass, later calls only the reverse pass: the same bits)."""
         return _objective_vjp(self.f0, self.T, self.sx, self.n, cot, self.pd, self.mode, self.vt, self.vx,
-                              self.info, self.fT)
+                              self.info, self.fT, self._rev)
 
 
 def objective_RPO_jvp(primal, tangent, param_dict, mode):
@@ -251,15 +254,22 @@ def objective_RPO_jvp(primal, tangent, param_dict, mode):
     return Linearization(primal, param_dict, mode).jvp(tangent)
 
 
-def _rk4_vjp_from(ctx, f0, fbar, h, steps, pd):
-    """Reverse pass of the Lawson RK4 integration from f0 (two-phase C-ABI)."""
+def _rk4_vjp_from(ctx, f0, fbar, h, steps, pd, cache=None):
+    """Reverse pass of the Lawson RK4 integration from f0 (two-phase C-ABI);
+    cache: a dict that keeps the forward pass (final state, checkpoints) for
+    later cotangents at the same primal."""
     lib = _lib.load()
-    nd = lib.ecs_ckpt_doubles(ctx.handle, 1, steps, CKPT)
-    ck = torch.empty(int(nd), dtype=torch.float64, device=f0.device)
-    fw = f0.clone().contiguous()
     hp = _lib.dptr([h])
-    _lib.check(lib.ecs_lawson_rk4_fwd_ckpt(ctx.handle, fw.data_ptr(), 1, hp, steps, 1, CKPT, ck.data_ptr(), None,
-                                           _runtime._stream()), "ecs_lawson_rk4_fwd_ckpt")
+    if cache is not None and "ck" in cache:
+        fw, ck = cache["fT"], cache["ck"]
+    else:
+        nd = lib.ecs_ckpt_doubles(ctx.handle, 1, steps, CKPT)
+        ck = torch.empty(int(nd), dtype=torch.float64, device=f0.device)
+        fw = f0.clone().contiguous()
+        _lib.check(lib.ecs_lawson_rk4_fwd_ckpt(ctx.handle, fw.data_ptr(), 1, hp, steps, 1, CKPT, ck.data_ptr(),
+                                               None, _runtime._stream()), "ecs_lawson_rk4_fwd_ckpt")
+        if cache is not None:
+            cache.update(fT=fw, ck=ck)
     g = fbar.clone().contiguous()
     hb = (ctypes.c_double * 1)()
     _lib.check(lib.ecs_lawson_rk4_vjp_ckpt(ctx.handle, ck.data_ptr(), g.data_ptr(), hb, 1, hp, steps, 1, CKPT,
@@ -270,34 +280,43 @@ def _rk4_vjp_from(ctx, f0, fbar, h, steps, pd):
 REVERSE_MODES = ("Lawson_RK4", "RK4", "TDRK4", "Lawson_RK6", "Lawson_RK43")
 
 
-def _integrate_vjp(ctx, f0, gbar, T, param_dict, mode, info=None, fT=None):
+def _integrate_vjp(ctx, f0, gbar, T, param_dict, mode, info=None, fT=None, cache=None):
     """Reverse pass of mismatch_RPO's integration: (fT, f0bar, Tbar) for the
     cotangent gbar of the integrated spectra (h = T / steps, so Tbar = hbar / steps).
-    Lawson_RK43: info, fT of the primal solve when already known."""
+    Lawson_RK43: info, fT of the primal solve when already known.  cache: a
+    dict that keeps the forward pass of this primal (its final state and the
+    reverse pass's checkpoints), so that later cotangents at the same primal
+    (Linearization.vjp) run only the reverse pass."""
     if mode == "Lawson_RK4":
         _, _, steps = _rk4_steps(param_dict)
-        fT, f0bar, hbar = _rk4_vjp_from(ctx, f0, gbar, _rk4_h(T, param_dict), steps, param_dict)
+        fT, f0bar, hbar = _rk4_vjp_from(ctx, f0, gbar, _rk4_h(T, param_dict), steps, param_dict, cache)
         return fT, f0bar, hbar / steps
-    if mode in ("RK4", "TDRK4"):
-        _, _, steps = _rk4_steps(param_dict)
-        h = _rk4_h(T, param_dict)
-        if mode == "RK4":
-            fT = ctx.rk4(f0, h, steps)
-            f0bar, hb = ctx.rk4_vjp(f0, gbar, h, steps, CKPT)
+    if mode in ("RK4", "TDRK4", "Lawson_RK6"):
+        if mode == "Lawson_RK6":
+            steps = int(param_dict["steps"])
+            h = T / steps
         else:
-            fT = ctx.tdrk4(f0, h, steps)
-            f0bar, hb = ctx.tdrk4_vjp(f0, gbar, h, steps, CKPT)
-        return fT, f0bar, hb[0] / steps
-    if mode == "Lawson_RK6":
-        steps = int(param_dict["steps"])
-        h = T / steps
-        fT = ctx.lawson_rk6(f0, h, steps, True)
-        f0bar, hb = ctx.lawson_rk6_vjp(f0, gbar, h, steps, True, CKPT)
+            _, _, steps = _rk4_steps(param_dict)
+            h = _rk4_h(T, param_dict)
+        scheme = {"RK4": "rk4", "TDRK4": "tdrk4", "Lawson_RK6": "lawson_rk6"}[mode]
+        if cache is not None and "ck" in cache:
+            fT, ck = cache["fT"], cache["ck"]
+        else:
+            # the forward pass gives the final state (the forward integrator's bits) and the checkpoints
+            fT, ck = ctx.gen_fwd_ckpt(scheme, f0, h, steps, CKPT, True)
+            if cache is not None:
+                cache.update(fT=fT, ck=ck)
+        f0bar, hb = ctx.gen_vjp_ckpt(scheme, ck, gbar, h, steps, CKPT, True)
         return fT, f0bar, hb[0] / steps
     if mode == "Lawson_RK43":
         if info is None:
             fT, info = integrate(f0, T, param_dict, mode)
-        f0bar, Tb = ctx.lawson_rk43_vjp(f0, gbar, T, [info], CKPT)
+        if cache is None:
+            f0bar, Tb = ctx.lawson_rk43_vjp(f0, gbar, T, [info], CKPT)
+            return fT, f0bar, Tb[0]
+        if "ck" not in cache:
+            cache["ck"] = ctx.lawson_rk43_fwd_ckpt(f0, T, [info], CKPT)
+        f0bar, Tb = ctx.lawson_rk43_vjp_ckpt(cache["ck"], gbar, T, [info], CKPT)
         return fT, f0bar, Tb[0]
     raise NotImplementedError(f"the reverse pass is built for modes {REVERSE_MODES}, not {mode!r}")
 
@@ -314,11 +333,12 @@ def objective_RPO_vjp(primal, cot, param_dict, mode="Lawson_RK4"):
     return _objective_vjp(f0, _f(primal["T"]), _f(primal["sx"]), fields.shape[-1], cot, param_dict, mode, vt, vx)
 
 
-def _objective_vjp(f0, T, sx, n, cot, param_dict, mode, vt, vx, info=None, fT=None):
+def _objective_vjp(f0, T, sx, n, cot, param_dict, mode, vt, vx, info=None, fT=None, cache=None):
     ctx = mhd.context(param_dict)
     cbar = irfft2_T(_fields(cot), n)                 # cotangent of (f0 - g)
     # g = sym(fT): the symmetry transpose of -cbar does not need fT; sxbar does
-    fT, f0bar, Tbar = _integrate_vjp(ctx, f0, _sym_T(-cbar, sx, param_dict), T, param_dict, mode, info, fT)
+    fT, f0bar, Tbar = _integrate_vjp(ctx, f0, _sym_T(-cbar, sx, param_dict), T, param_dict, mode, info, fT,
+                                     cache)
     _, sxbar = symmetry.apply_vjp(fT, sx, param_dict, -cbar)
     n2 = 2 * n * n
     gfields = rfft2_T(cbar + f0bar, n) + (_f(cot["T"]) * vt + _f(cot["sx"]) * vx) / n2

@@ -180,15 +180,44 @@ int ecs_tdrk4_jvp(ecs_ctx* ctx, double* f, double* df, int batch, const double* 
  * mismatch_RPO's integration gives in modes "RK4", "Lawson_RK6", "TDRK4"),
  * with the conventions of ecs_lawson_rk4_vjp: f0 not modified, fbar_inout
  * the cotangent of the final spectra on entry and of f0 on exit, hbar host
- * [batch] = d/dh.  The state is checkpointed every ckpt steps and each step's
- * stages are recomputed before being transposed; TDRK4's transpose carries
- * the second derivative of the RHS. */
+ * [batch] = d/dh.  The state is checkpointed every ckpt steps; each segment
+ * is run forward again from its checkpoint before it is transposed; TDRK4's
+ * transpose carries the second derivative of the RHS. */
 int ecs_rk4_vjp(ecs_ctx* ctx, const double* f0, double* fbar_inout, double* hbar, int batch, const double* dt,
                 int steps, int ckpt, void* stream);
 int ecs_lawson_rk6_vjp(ecs_ctx* ctx, const double* f0, double* fbar_inout, double* hbar, int batch,
                        const double* dt, int steps, int masked, int ckpt, void* stream);
 int ecs_tdrk4_vjp(ecs_ctx* ctx, const double* f0, double* fbar_inout, double* hbar, int batch, const double* dt,
                   int steps, int ckpt, void* stream);
+
+/* Two-phase form of the reverse passes above and of ecs_lawson_rk43_vjp, for
+ * callers that apply several cotangents at one primal: the vjp function
+ * jax.vjp returns (newton.py:168-169), applied once per Krylov step of GMRES
+ * on the normal equations (use_transpose, newton.py:167-170) or of the
+ * hookstep solvers.  The forward pass runs once and stores the compact state
+ * at the start of every segment of ckpt steps into the caller's device
+ * buffer ck; each reverse pass then starts from ck.  The same bits as the
+ * one-call forms.
+ * ecs_gen_fwd_ckpt: scheme 0 classical RK4, 1 Lawson RK6 (masked as for
+ *   ecs_lawson_rk6), 2 TDRK4; advances f in place to the final spectra (the
+ *   bits of ecs_rk4 / ecs_lawson_rk6 / ecs_tdrk4); ck holds
+ *   ecs_ckpt_doubles(ctx, batch, steps, ckpt) doubles.
+ * ecs_gen_vjp_ckpt: the reverse pass from ck with the forward pass's
+ *   arguments; fbar_inout and hbar as for ecs_rk4_vjp.
+ * ecs_lawson_rk43_fwd_ckpt: along the schedule of a previous ecs_lawson_rk43
+ *   call (x0 not modified); ck holds ecs_ckpt_doubles(ctx, batch, M, ckpt)
+ *   doubles with M the most accepted steps of a candidate (at least 1).
+ * ecs_lawson_rk43_vjp_ckpt: the reverse pass from ck; xbar_inout and Tbar as
+ *   for ecs_lawson_rk43_vjp. */
+int ecs_gen_fwd_ckpt(ecs_ctx* ctx, double* f, int batch, const double* dt, int steps, int scheme, int masked,
+                     int ckpt, double* ck, void* stream);
+int ecs_gen_vjp_ckpt(ecs_ctx* ctx, const double* ck, double* fbar_inout, double* hbar, int batch, const double* dt,
+                     int steps, int scheme, int masked, int ckpt, void* stream);
+int ecs_lawson_rk43_fwd_ckpt(ecs_ctx* ctx, const double* x0, int batch, const double* T, const double* hs,
+                             const int8_t* acc, const int* attempts, int budget, int ckpt, double* ck, void* stream);
+int ecs_lawson_rk43_vjp_ckpt(ecs_ctx* ctx, const double* ck, double* xbar_inout, double* Tbar, int batch,
+                             const double* T, const double* hs, const int8_t* acc, const int* attempts, int budget,
+                             int ckpt, void* stream);
 
 /* GMRES vector kernels (linalg.py:58-97) on flat FP64 vectors of length n.
  * ecs_mgs: one modified-Gram-Schmidt sweep of Arnoldi step k (linalg.py:70-85):

@@ -287,3 +287,39 @@ def test_rk43_reverse_step_tape_and_overlap(n, B, T, monkeypatch):
         lhs = O.rdot(c[b].cpu().numpy(), jd[b].cpu().numpy())
         rhs = O.rdot(res[True][0][b].cpu().numpy(), d[b].cpu().numpy()) + res[True][1][b] * dTs[b]
         assert abs(lhs - rhs) < 1e-10 * abs(lhs), (b, lhs, rhs)
+
+
+@pytest.mark.parametrize("scheme,n,B,steps", [("rk4", 64, 2, 70), ("lawson_rk6", 32, 1, 40), ("tdrk4", 128, 1, 70)])
+def test_two_phase_reverse_pass_gives_the_one_call_bits(scheme, n, B, steps):
+    """ecs_gen_fwd_ckpt leaves the forward integrator's final state (bitwise)
+    and checkpoints from which ecs_gen_vjp_ckpt, applied to two cotangents,
+    returns the bits of the one-call reverse pass for each."""
+    print(f"two-phase {scheme} n={n} B={B} steps={steps}", flush=True)
+    pd, ctx = setup(n)
+    f = torch.as_tensor(np.stack([spectrum(n, 120 + s) for s in range(B)])).cuda()
+    dts = [0.004 + 0.001 * s for s in range(B)]
+    fT, ck = ctx.gen_fwd_ckpt(scheme, f, dts, steps, 16)
+    fwd = getattr(ctx, scheme)(f, dts, steps)
+    assert torch.equal(fT, fwd)
+    vjp = getattr(ctx, scheme + "_vjp")
+    for seed in (130, 140):
+        c = torch.as_tensor(np.stack([spectrum(n, seed + s) for s in range(B)])).cuda()
+        a = ctx.gen_vjp_ckpt(scheme, ck, c, dts, steps, 16)
+        b = vjp(f, c, dts, steps, ckpt=16)
+        assert torch.equal(a[0], b[0]) and list(a[1]) == list(b[1])
+
+
+def test_rk43_two_phase_reverse_pass_gives_the_one_call_bits():
+    print("two-phase rk43 n=64 B=2", flush=True)
+    n = 64
+    pd, ctx = setup(n)
+    x0 = torch.as_tensor(np.stack([spectrum(n, 150), 1.4 * spectrum(n, 151)])).cuda()
+    Ts = [0.6, 0.8]
+    _, infos = ctx.lawson_rk43(x0, Ts, 0.5, 1e-8, 64, 64)
+    assert infos[0]["accepted"] != infos[1]["accepted"]
+    ck = ctx.lawson_rk43_fwd_ckpt(x0, Ts, infos, 16)
+    for seed in (160, 170):
+        c = torch.as_tensor(np.stack([spectrum(n, seed), spectrum(n, seed + 1)])).cuda()
+        a = ctx.lawson_rk43_vjp_ckpt(ck, c, Ts, infos, 16)
+        b = ctx.lawson_rk43_vjp(x0, c, Ts, infos, ckpt=16)
+        assert torch.equal(a[0], b[0]) and list(a[1]) == list(b[1])
