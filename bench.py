@@ -255,6 +255,32 @@ def newton_leg(inp, pd):
             "basis_orthogonality_loss": log["orthogonality_loss"]}
 
 
+def newton_default_leg(inp, pd):
+    """One Newton-GMRES iteration in the reference's default Newton setting
+    (newton.py:70,84-87: mode TDRK4, use_transpose=True, inner 16, line
+    search), whose one GMRES solve the reference times at 16.1 s on an H100
+    (benchmarking.m:43; its trajectory length is not recorded): here on the
+    headline guess (N=128, T=4.3, sx=0.1) with 32 checkpoints of 32 TDRK4 steps
+    (newton.py:52-55), each Krylov step one JVP and one VJP of the same
+    linearisation (normal_matvecs_per_s counts those pairs, the extra one for
+    the final residual included)."""
+    import torch
+    from ecs_mhd_2d_amd import solvers
+    pdt = dict(pd)
+    pdt.update({"ministeps": 32, "num_checkpoints": 32})
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    new, log = solvers.newton_gmres_update(inp, pdt, "TDRK4", inner=16, use_transpose=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {"workload": "one Newton-GMRES iteration, mode TDRK4, A^T A form, inner 16, 1024 steps, "
+                        "same guess as the headline", "wall_s": wall, "fwall_s": log["fwall"],
+            "gmreswall_s": log["gmreswall"], "line_search_s": log["lswall"],
+            "normal_matvecs_per_s": log["matvecs"] / log["gmreswall"], "f_norm": log["f_norm"],
+            "gmres_rel_res": log["gmres_rel_res"], "damp": log["damp"],
+            "reference_published_h100_gmres_s": 16.1}
+
+
 def guess_generation_leg():
     """The reference's published guess-generation workload (generate_turbulence.py:22-151,
     timings in benchmarking.m:11-59): from 10 N(0,1) random fields at N=128
@@ -604,6 +630,7 @@ def main():
     if not args.no_extras:
         extras["transpose_matvec"] = transpose_leg(inp, pd, [tans[i] for i in range(3)])
         extras["newton_iteration"] = newton_leg(inp, pd)
+        extras["newton_default_setting"] = newton_default_leg(inp, pd)
         extras["other_steppers"] = steppers_leg()
         extras["adjoint_gradient"] = gradient_leg()
         extras["guess_generation"] = guess_generation_leg()
