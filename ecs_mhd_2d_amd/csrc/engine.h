@@ -1544,9 +1544,10 @@ static int tape_forward_td(const Args& af, const StepTape& T, const Comb* ttab, 
 
 // Reverse pass of run_gen: f0 (full spectra, not modified), fbar_inout (on
 // entry the cotangent of the final spectra, on exit that of f0), hbar[b] =
-// d/dh.  The state is checkpointed every ckpt steps; each segment is
-// recomputed storing its step inputs, and each step's stages are recomputed
-// from its input before the stage algebra is transposed:
+// d/dh.  The state is checkpointed every ckpt steps; each segment runs
+// forward again from its checkpoint into a step tape (StepTape; without one,
+// its step inputs are stored and each step's stages recomputed from its
+// input) before the stage algebra is transposed:
 //   Butcher form  x' = E^{p_S} x + h sum_j b_j E^{p_S-p_j} v_j,
 //                 y_s = E^{p_s} x + h sum_j A_sj E^{p_s-p_j} v_j,  v_s = f(y_s):
 //     vbar_j = h b_j E^{p_S-p_j} x'bar, then for s = S..1: ybar_s = J(y_s)^T vbar_s

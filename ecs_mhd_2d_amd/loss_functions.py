@@ -403,8 +403,8 @@ class RPOLoss(torch.autograd.Function):
 class RPOLossModes(torch.autograd.Function):
     """loss_RPO in modes "RK4", "TDRK4", "Lawson_RK6" and "Lawson_RK43" with
     the hand-written reverse pass of their integrators (checkpoint every 64
-    steps, stages recomputed per step; RK4(3) along the primal's schedule) as
-    the backward."""
+    steps, each segment run forward again into a step tape before it is
+    transposed; RK4(3) along the primal's schedule) as the backward."""
 
     @staticmethod
     def forward(ctx_, fields, T, sx, param_dict, mode):
