@@ -278,6 +278,26 @@ def _rk4_vjp_from(ctx, f0, fbar, h, steps, pd, cache=None):
 
 
 REVERSE_MODES = ("Lawson_RK4", "RK4", "TDRK4", "Lawson_RK6", "Lawson_RK43")
+_GEN_MODES = {"RK4": "rk4", "TDRK4": "tdrk4", "Lawson_RK6": "lawson_rk6"}
+
+
+def _gen_args(T, pd, mode):
+    """(scheme, h, steps) of a fixed-step mode other than Lawson_RK4 (loss_functions.py:49-84)."""
+    if mode == "Lawson_RK6":
+        steps = int(pd["steps"])
+        return _GEN_MODES[mode], T / steps, steps
+    _, _, steps = _rk4_steps(pd)
+    return _GEN_MODES[mode], _rk4_h(T, pd), steps
+
+
+def _gen_forward(ctx, f0, T, pd, mode, cache):
+    """The integration of a mode of _GEN_MODES as the reverse pass's forward
+    pass: cache gets the final state (the bits of integrate) and the reverse
+    pass's checkpoints; returns cache."""
+    scheme, h, steps = _gen_args(T, pd, mode)
+    fT, ck = ctx.gen_fwd_ckpt(scheme, f0, h, steps, CKPT, True)
+    cache.update(fT=fT, ck=ck)
+    return cache
 
 
 def _integrate_vjp(ctx, f0, gbar, T, param_dict, mode, info=None, fT=None, cache=None):
@@ -291,23 +311,12 @@ def _integrate_vjp(ctx, f0, gbar, T, param_dict, mode, info=None, fT=None, cache
         _, _, steps = _rk4_steps(param_dict)
         fT, f0bar, hbar = _rk4_vjp_from(ctx, f0, gbar, _rk4_h(T, param_dict), steps, param_dict, cache)
         return fT, f0bar, hbar / steps
-    if mode in ("RK4", "TDRK4", "Lawson_RK6"):
-        if mode == "Lawson_RK6":
-            steps = int(param_dict["steps"])
-            h = T / steps
-        else:
-            _, _, steps = _rk4_steps(param_dict)
-            h = _rk4_h(T, param_dict)
-        scheme = {"RK4": "rk4", "TDRK4": "tdrk4", "Lawson_RK6": "lawson_rk6"}[mode]
-        if cache is not None and "ck" in cache:
-            fT, ck = cache["fT"], cache["ck"]
-        else:
-            # the forward pass gives the final state (the forward integrator's bits) and the checkpoints
-            fT, ck = ctx.gen_fwd_ckpt(scheme, f0, h, steps, CKPT, True)
-            if cache is not None:
-                cache.update(fT=fT, ck=ck)
-        f0bar, hb = ctx.gen_vjp_ckpt(scheme, ck, gbar, h, steps, CKPT, True)
-        return fT, f0bar, hb[0] / steps
+    if mode in _GEN_MODES:
+        scheme, h, steps = _gen_args(T, param_dict, mode)
+        if cache is None or "ck" not in cache:
+            cache = _gen_forward(ctx, f0, T, param_dict, mode, {} if cache is None else cache)
+        f0bar, hb = ctx.gen_vjp_ckpt(scheme, cache["ck"], gbar, h, steps, CKPT, True)
+        return cache["fT"], f0bar, hb[0] / steps
     if mode == "Lawson_RK43":
         if info is None:
             fT, info = integrate(f0, T, param_dict, mode)
@@ -411,11 +420,18 @@ class RPOLossModes(torch.autograd.Function):
         n = fields.shape[-1]
         f0 = _rfft2(fields.detach().to(torch.float64))
         Tf, sxf = float(T), float(sx)
-        fT, _ = integrate(f0, Tf, param_dict, mode)
+        # the integration the backward reuses: the fixed-step modes' forward
+        # pass with its checkpoints (the bits of integrate), RK4(3)'s schedule
+        rev, info = {}, None
+        if mode in _GEN_MODES:
+            fT = _gen_forward(mhd.context(param_dict), f0, Tf, param_dict, mode, rev)["fT"]
+        else:
+            fT, info = integrate(f0, Tf, param_dict, mode)
         g = symmetry.apply(fT, sxf, param_dict)
         diff = _irfft2(g - f0, n)
         ctx_.save_for_backward(f0, fT, diff)
         ctx_.pd, ctx_.mode, ctx_.T, ctx_.sx, ctx_.n = param_dict, mode, Tf, sxf, n
+        ctx_.rev, ctx_.info = rev, info
         return torch.mean(diff * diff)
 
     @staticmethod
@@ -424,7 +440,8 @@ class RPOLossModes(torch.autograd.Function):
         pd, n = ctx_.pd, ctx_.n
         dbar = irfft2_T(diff * (gl / (n * n)), n)         # cotangent of g - f0
         fTbar, sxbar = symmetry.apply_vjp(fT, ctx_.sx, pd, dbar)
-        _, f0bar, Tbar = _integrate_vjp(mhd.context(pd), f0, fTbar.contiguous(), ctx_.T, pd, ctx_.mode)
+        _, f0bar, Tbar = _integrate_vjp(mhd.context(pd), f0, fTbar.contiguous(), ctx_.T, pd, ctx_.mode,
+                                        ctx_.info, fT, ctx_.rev)
         gfields = rfft2_T(f0bar - dbar, n)
         return gfields, torch.tensor(Tbar, dtype=torch.float64, device=diff.device), sxbar, None, None
 
@@ -434,7 +451,8 @@ def loss_RPO_value_and_grad(input_dict, param_dict, mode="Lawson_RK4"):
     grad_dict)  (adjoint_descent.py:90-96), modes of REVERSE_MODES."""
     if mode not in REVERSE_MODES:
         raise NotImplementedError(f"the reverse pass is built for modes {REVERSE_MODES}, not {mode!r}")
-    fields = _fields(input_dict).requires_grad_(True)
+    # a leaf of its own: the caller's tensor keeps its requires_grad flag
+    fields = _fields(input_dict).detach().requires_grad_(True)
     T = torch.tensor(_f(input_dict["T"]), dtype=torch.float64, device=fields.device, requires_grad=True)
     sx = torch.tensor(_f(input_dict["sx"]), dtype=torch.float64, device=fields.device, requires_grad=True)
     if mode == "Lawson_RK4":

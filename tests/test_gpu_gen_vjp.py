@@ -323,3 +323,37 @@ def test_rk43_two_phase_reverse_pass_gives_the_one_call_bits():
         a = ctx.lawson_rk43_vjp_ckpt(ck, c, Ts, infos, 16)
         b = ctx.lawson_rk43_vjp(x0, c, Ts, infos, ckpt=16)
         assert torch.equal(a[0], b[0]) and list(a[1]) == list(b[1])
+
+
+def test_rk43_loss_gradient_against_the_frozen_schedule_jvp():
+    """loss_RPO_value_and_grad in mode Lawson_RK43 (its backward reuses the
+    forward's schedule and final state): <grad, d> equals the derivative of
+    the loss along d with the step sizes frozen (h is stop-gradient,
+    timestepping.py:308), 2 mean(o * J d) for the objective o and its JVP."""
+    from ecs_mhd_2d_amd import loss_functions as LF, mhd, utils
+    print("rk43 loss gradient", flush=True)
+    n = 32
+    pd = mhd.make_param_dict(n, 1 / 40, 1 / 40)
+    pd.update({"shift_reflect_ny": 2, "rot": True,
+               "adaptive_dict": {"atol": 1e-8, "checkpoints": 16, "max_steps_per_checkpoint": 64}})
+    rng = np.random.default_rng(21)
+    inp = {"fields": torch.as_tensor(np.fft.irfft2(spectrum(n, 8))).cuda(), "T": 0.7, "sx": 0.15}
+    (loss, _), grad = LF.loss_RPO_value_and_grad(inp, pd, "Lawson_RK43")
+    lin = LF.Linearization(inp, pd, "Lawson_RK43")
+    o = lin.objective()["fields"]
+    assert abs(float(loss) - float(torch.mean(o * o))) < 1e-13 * float(loss)
+    for _ in range(2):
+        d = utils.unflatten(torch.as_tensor(rng.standard_normal(2 * n * n + 2)).cuda(), n)
+        lhs = float(torch.dot(utils.flatten(grad), utils.flatten(d)))
+        rhs = float(2.0 * torch.mean(o * lin.jvp(d)["fields"]))
+        assert abs(lhs - rhs) < 1e-10 * max(abs(rhs), 1e-300), (lhs, rhs)
+
+
+def test_value_and_grad_leaves_the_callers_fields_alone():
+    from ecs_mhd_2d_amd import loss_functions as LF, mhd
+    n = 16
+    pd = mhd.make_param_dict(n, 1 / 40, 1 / 40)
+    pd.update({"num_checkpoints": 2, "ministeps": 4, "shift_reflect_ny": 0, "rot": False})
+    f = torch.as_tensor(np.fft.irfft2(spectrum(n, 9))).cuda()
+    LF.loss_RPO_value_and_grad({"fields": f, "T": 0.2, "sx": 0.0}, pd)
+    assert not f.requires_grad and f.grad is None
