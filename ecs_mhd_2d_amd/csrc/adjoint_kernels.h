@@ -116,10 +116,11 @@ __device__ __forceinline__ void adj_glue(const Args& a, int b, size_t ci, int r,
 }
 
 // Column-adjoint kernel: [x-FFT of the row-adjoint output -> ybar] -> glue ->
-// [m g -> inverse x-FFT -> AO].  `after(b, ci, c, kx, ky, v)` runs on every
-// mode after a B_OUT glue has stored v there.
-template <int N, int TY, int NT, int MODE, int PL, class F>
-__device__ __forceinline__ void adj_col_body(const Args& a, F&& after) {
+// [m g -> inverse x-FFT -> AO].  `before(b, ci, c, kx, ky)` runs on every
+// mode before a B_G glue reads its input there; `after(b, ci, c, kx, ky, v)`
+// runs on every mode after a B_OUT glue has stored v there.
+template <int N, int TY, int NT, int MODE, int PL, class P, class F>
+__device__ __forceinline__ void adj_col_body(const Args& a, P&& before, F&& after) {
   using G = Geo<N>;
   constexpr int KC = G::KC, R = G::R, KMAX = G::KMAX;
   constexpr bool FWD = !(MODE == B_G || MODE == B_ENTRY);
@@ -171,6 +172,7 @@ __device__ __forceinline__ void adj_col_body(const Args& a, F&& after) {
       yb = s;
     }
     double2 g;
+    if constexpr (MODE == B_G) before(b, ci, c, kx, ky);
     adj_glue<N, MODE>(a, b, ci, r, c, kx, ky, yb, g);
     if constexpr (MODE == B_OUT) after(b, ci, c, kx, ky, g);
     if (INV) {
@@ -199,7 +201,19 @@ __device__ __forceinline__ void adj_col_body(const Args& a, F&& after) {
 
 template <int N, int TY, int NT, int MODE, int PL>
 __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
-  adj_col_body<N, TY, NT, MODE, PL>(a, [](int, size_t, int, int, int, double2) {});
+  adj_col_body<N, TY, NT, MODE, PL>(a, [](int, size_t, int, int, int) {}, [](int, size_t, int, int, int, double2) {});
+}
+
+// The first kernel of an RHS-VJP pass (B_G) with the transposed stage
+// combination that produces its input applied before it in the same launch,
+// mode by mode (the combination's outputs include gin; the thread that
+// writes an entry of gin reads it): one launch instead of two, the same
+// arithmetic.
+template <int N, int TY, int NT, int PL>
+__global__ __launch_bounds__(NT) void adj_comb_g_kernel(Args a, CombAdj cb) {
+  adj_col_body<N, TY, NT, B_G, PL>(
+      a, [&](int b, size_t ci, int c, int kx, int ky) { comb_adj_elem(cb, b, ci, c, kx, ky, cb.yb[ci]); },
+      [](int, size_t, int, int, int, double2) {});
 }
 
 // The last kernel of an RHS-VJP pass (B_OUT) with the transposed stage
@@ -208,9 +222,10 @@ __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
 // second argument (the Args block stays within its 512 bytes).
 template <int N, int TY, int NT, int PL>
 __global__ __launch_bounds__(NT) void adj_out_comb_kernel(Args a, CombAdj cb) {
-  adj_col_body<N, TY, NT, B_OUT, PL>(a, [&](int b, size_t ci, int c, int kx, int ky, double2 v) {
-    comb_adj_elem(cb, b, ci, c, kx, ky, v);
-  });
+  adj_col_body<N, TY, NT, B_OUT, PL>(a, [](int, size_t, int, int, int) {},
+                                     [&](int b, size_t ci, int c, int kx, int ky, double2 v) {
+                                       comb_adj_elem(cb, b, ci, c, kx, ky, v);
+                                     });
 }
 
 // Row-adjoint kernel: weighted c2r of (Pbar_omega, Pbar_j), transposed

@@ -727,6 +727,19 @@ template <int N>
 static int acol_comb(const Args& a, const CombAdj& cb, int B, hipStream_t st) {
   return B < 16 ? acol_comb_t<N, 1>(a, cb, B, st) : acol_comb_t<N, Tiles<N>::TY>(a, cb, B, st);
 }
+// the transposed stage combination `cb` (one of whose outputs is gin) and B_G in one launch
+template <int N, int TY>
+static int acol_g_comb_t(const Args& a, const CombAdj& cb, int B, hipStream_t st) {
+  dim3 grid((Geo<N>::KC + TY - 1) / TY, 2, B);
+  if (plan_of(N, K_COL) == 1) hipLaunchKernelGGL((adj_comb_g_kernel<N, TY, NT, 8>), grid, dim3(NT), 0, st, a, cb);
+  else hipLaunchKernelGGL((adj_comb_g_kernel<N, TY, NT, 16>), grid, dim3(NT), 0, st, a, cb);
+  LAUNCHCHK();
+  return 0;
+}
+template <int N>
+static int acol_g_comb(const Args& a, const CombAdj& cb, int B, hipStream_t st) {
+  return B < 16 ? acol_g_comb_t<N, 1>(a, cb, B, st) : acol_g_comb_t<N, Tiles<N>::TY>(a, cb, B, st);
+}
 template <int N, int TX>
 static int arow_t(const Args& a, int B, hipStream_t st) {
   dim3 grid(N / TX, 1, B);
@@ -1353,19 +1366,27 @@ struct CombAdjBuilder {
 // (the 8 derived fields in x-space), when a forward pass left it; otherwise
 // it is recomputed.  acc: out += instead of out =.  then: a transposed stage
 // combination whose yb is `out`, applied by the pass's last kernel (the same
-// arithmetic as comb_adj_launch after the pass, one launch fewer).
+// arithmetic as comb_adj_launch after the pass, one launch fewer).  before:
+// a transposed stage combination one of whose outputs is g, applied by the
+// pass's first kernel before it reads g (as comb_adj_launch before the pass).
 template <int N>
 static int vjp_pass(ecs_ctx* c, const Args& base, double2* x, const double2* g, double2* out, int B, int diss,
                     hipStream_t st, bool hess = false, const double2* prim = nullptr, bool acc = false,
-                    const CombAdj* then = nullptr) {
+                    const CombAdj* then = nullptr, const CombAdj* before = nullptr) {
   Args a = base;
   a.x0 = x; a.gin = g; a.out = out; a.out_acc = acc ? 1 : 0;
   a.include_diss = hess ? 0 : diss;
   a.aout = c->AO; a.ain = c->AI; a.prim = prim ? prim : a.inter;
   a.adaptive = AD_FIXED;
   if (hess) { a.b0x = 0.0; a.b0y = 0.0; }
+  // per-candidate step counts would end the B_G kernel early, not the combination
+  if (before && a.nstep) {
+    TRY(comb_adj_launch<N>(*before, B, st));
+    before = nullptr;
+  }
   if (!prim) TRY((col<N, C_LOAD, false>(a, B, st)));
-  TRY((acol<N, B_G>(a, B, st)));
+  if (before) TRY(acol_g_comb<N>(a, *before, B, st));
+  else TRY((acol<N, B_G>(a, B, st)));
   TRY(arow<N>(a, B, st));
   // per-candidate step counts end a candidate's B_OUT early, not its combination
   if (then && !a.nstep && then->yb == out) return acol_comb<N>(a, *then, B, st);
@@ -1743,7 +1764,7 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
         fin.term(vb(j), vs(j), tb.b[j], 1, tb.p[S] - tb.p[j], true);
         touched[j] = 1;
       }
-    TRY(comb_adj_launch<N>(fin.cb, B, st));
+    // fin is applied by the first RHS-VJP pass's first kernel (stage S - 1 reads vb(S - 1))
     for (int k = S - 1; k >= 0; --k) {
       if (!touched[k]) HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));  // a stage no combination reads
       CombAdjBuilder st_(c, ce, ediv, msk, yb, c->hb);
@@ -1753,7 +1774,8 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
           st_.term(vb(j), vs(j), tb.A[k][j], 1, tb.p[k] - tb.p[j], !touched[j]);
           touched[j] = 1;
         }
-      TRY(vjp_pass<N>(c, a, k == 0 ? xs : ys(k), vb(k), yb, B, diss, st, false, ds(k), false, &st_.cb));
+      TRY(vjp_pass<N>(c, a, k == 0 ? xs : ys(k), vb(k), yb, B, diss, st, false, ds(k), false, &st_.cb,
+                      k == S - 1 ? &fin.cb : nullptr));
     }
     std::swap(xb, xbn);
     return 0;
@@ -1768,9 +1790,9 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
     CombAdjBuilder fin(c, ce, 1.0, 0, xb, c->hb);
     fin.term(xbn, nullptr, 1.0, 0, 0, true).term(vb(0), v1, 1.0, 1, 0, true)
       .term(wb(0), w1, 1.0 / 6, 2, 0, true).term(wb(1), w2, 1.0 / 3, 2, 0, true);
-    TRY(comb_adj_launch<N>(fin.cb, B, st));
     // w2 = J(y) v2, v2 = f(y): vbar2 = J(y)^T wbar2; ybar = J(y)^T vbar2 + J_0(v2)^T wbar2
-    TRY(vjp_pass<N>(c, ab, ym, wb(1), vb(1), B, diss, st, false, dm));
+    // (fin applied by the first pass's first kernel, which reads wbar2)
+    TRY(vjp_pass<N>(c, ab, ym, wb(1), vb(1), B, diss, st, false, dm, false, nullptr, &fin.cb));
     TRY(vjp_pass<N>(c, ab, ym, vb(1), yb, B, diss, st, false, dm));
     // y = x + h/2 v1 + h^2/8 w1 (the transposed combination applied by the pass's last kernel)
     CombAdjBuilder cy(c, ce, 1.0, 0, yb, c->hb);
@@ -2044,7 +2066,7 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
         fin.term(vb(j), vs(j), tb.b[j], 1, tb.p[S] - tb.p[j], true);
         touched[j] = 1;
       }
-    TRY(comb_adj_launch<N>(fin.cb, B, st));
+    // fin is applied by the first RHS-VJP pass's first kernel (stage S - 1 reads vb(S - 1))
     for (int k = S - 1; k >= 0; --k) {
       if (!touched[k]) HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));  // a stage no combination reads
       Args am = a;
@@ -2065,10 +2087,12 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
         pc.cb.h = PDT(m);
         pc.cb.hscale = SIG(m);
         pc.term(vbraw, vr, 1.0, 0, 1, true);
-        TRY(comb_adj_launch<N>(pc.cb, B, st));
-        TRY(vjp_pass<N>(c, am, xs, vbraw, yb, B, 0, st, false, ds(0), false, &sa.cb));
+        if (k == S - 1) TRY(comb_adj_launch<N>(fin.cb, B, st));  // a one-stage table: fin before pc
+        // (pc applied by the pass's first kernel, which reads vbraw)
+        TRY(vjp_pass<N>(c, am, xs, vbraw, yb, B, 0, st, false, ds(0), false, &sa.cb, &pc.cb));
       } else {
-        TRY(vjp_pass<N>(c, am, ys(k), vb(k), yb, B, 0, st, false, ds(k), false, &sa.cb));
+        TRY(vjp_pass<N>(c, am, ys(k), vb(k), yb, B, 0, st, false, ds(k), false, &sa.cb,
+                        k == S - 1 ? &fin.cb : nullptr));
       }
     }
     std::swap(xb, xbn);
