@@ -28,6 +28,7 @@ enum AdjMode : int {
   B_S2 = 5,
   B_S1E = 6,    // ybar1 -> xb; fused entry of the previous step
   B_S1 = 7,     // ybar1 -> xb; no previous step in this segment
+  B_OUTG = 8,   // B_OUT, then (after the fused combinations) the next pass's B_G
 };
 
 template <int N, int MODE>
@@ -118,13 +119,15 @@ __device__ __forceinline__ void adj_glue(const Args& a, int b, size_t ci, int r,
 // Column-adjoint kernel: [x-FFT of the row-adjoint output -> ybar] -> glue ->
 // [m g -> inverse x-FFT -> AO].  `before(b, ci, c, kx, ky)` runs on every
 // mode before a B_G glue reads its input there; `after(b, ci, c, kx, ky, v)`
-// runs on every mode after a B_OUT glue has stored v there.
+// runs on every mode after a B_OUT glue has stored v there and returns the
+// input of B_OUTG's inverse part there (the next pass's B_G input).
 template <int N, int TY, int NT, int MODE, int PL, class P, class F>
 __device__ __forceinline__ void adj_col_body(const Args& a, P&& before, F&& after) {
   using G = Geo<N>;
   constexpr int KC = G::KC, R = G::R, KMAX = G::KMAX;
   constexpr bool FWD = !(MODE == B_G || MODE == B_ENTRY);
   constexpr bool INV = !(MODE == B_OUT || MODE == B_S1);
+  constexpr int GM = MODE == B_OUTG ? B_OUT : MODE;  // the glue of the mode
   constexpr int NSEQ = FWD ? 4 * TY : TY;
   constexpr int SS = Seq<N>::S;
   __shared__ double2 sm[NSEQ * SS];
@@ -173,8 +176,8 @@ __device__ __forceinline__ void adj_col_body(const Args& a, P&& before, F&& afte
     }
     double2 g;
     if constexpr (MODE == B_G) before(b, ci, c, kx, ky);
-    adj_glue<N, MODE>(a, b, ci, r, c, kx, ky, yb, g);
-    if constexpr (MODE == B_OUT) after(b, ci, c, kx, ky, g);
+    adj_glue<N, GM>(a, b, ci, r, c, kx, ky, yb, g);
+    if constexpr (MODE == B_OUT || MODE == B_OUTG) g = after(b, ci, c, kx, ky, g);
     if (INV) {
       const bool dc = (kx == 0 && ky == 0);
       const double m = dc ? 0.0 : (c == 0 ? 1.0 : (double)(kx * kx + ky * ky));
@@ -201,7 +204,8 @@ __device__ __forceinline__ void adj_col_body(const Args& a, P&& before, F&& afte
 
 template <int N, int TY, int NT, int MODE, int PL>
 __global__ __launch_bounds__(NT) void adj_col_kernel(Args a) {
-  adj_col_body<N, TY, NT, MODE, PL>(a, [](int, size_t, int, int, int) {}, [](int, size_t, int, int, int, double2) {});
+  adj_col_body<N, TY, NT, MODE, PL>(a, [](int, size_t, int, int, int) {},
+                                    [](int, size_t, int, int, int, double2 v) { return v; });
 }
 
 // The first kernel of an RHS-VJP pass (B_G) with the transposed stage
@@ -213,7 +217,7 @@ template <int N, int TY, int NT, int PL>
 __global__ __launch_bounds__(NT) void adj_comb_g_kernel(Args a, CombAdj cb) {
   adj_col_body<N, TY, NT, B_G, PL>(
       a, [&](int b, size_t ci, int c, int kx, int ky) { comb_adj_elem(cb, b, ci, c, kx, ky, cb.yb[ci]); },
-      [](int, size_t, int, int, int, double2) {});
+      [](int, size_t, int, int, int, double2 v) { return v; });
 }
 
 // The last kernel of an RHS-VJP pass (B_OUT) with the transposed stage
@@ -225,7 +229,25 @@ __global__ __launch_bounds__(NT) void adj_out_comb_kernel(Args a, CombAdj cb) {
   adj_col_body<N, TY, NT, B_OUT, PL>(a, [](int, size_t, int, int, int) {},
                                      [&](int b, size_t ci, int c, int kx, int ky, double2 v) {
                                        comb_adj_elem(cb, b, ci, c, kx, ky, v);
+                                       return v;
                                      });
+}
+
+// The last kernel of one RHS-VJP pass and the first of the next in one
+// launch: B_OUT, the transposed stage combination `then` that reads its
+// output, the combination `pre` (when it has terms) that produces the next
+// pass's input, then that pass's B_G on gnext (one of their outputs; the
+// thread that writes an entry reads it) into AO.  Two launches fewer than
+// adj_out_comb_kernel, comb_adj_kernel and adj_col_kernel<B_G>, the same
+// arithmetic.
+template <int N, int TY, int NT, int PL>
+__global__ __launch_bounds__(NT) void adj_out_g_kernel(Args a, CombAdj then, CombAdj pre, const double2* gnext) {
+  adj_col_body<N, TY, NT, B_OUTG, PL>(a, [](int, size_t, int, int, int) {},
+                                      [&](int b, size_t ci, int c, int kx, int ky, double2 v) {
+                                        comb_adj_elem(then, b, ci, c, kx, ky, v);
+                                        if (pre.nterm) comb_adj_elem(pre, b, ci, c, kx, ky, pre.yb[ci]);
+                                        return gnext[ci];
+                                      });
 }
 
 // Row-adjoint kernel: weighted c2r of (Pbar_omega, Pbar_j), transposed

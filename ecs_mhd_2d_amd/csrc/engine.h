@@ -740,6 +740,24 @@ template <int N>
 static int acol_g_comb(const Args& a, const CombAdj& cb, int B, hipStream_t st) {
   return B < 16 ? acol_g_comb_t<N, 1>(a, cb, B, st) : acol_g_comb_t<N, Tiles<N>::TY>(a, cb, B, st);
 }
+// B_OUT, the combinations `then` and `pre`, and the next pass's B_G on gnext in one launch
+template <int N, int TY>
+static int acol_out_g_t(const Args& a, const CombAdj& then, const CombAdj& pre, const double2* gnext, int B,
+                        hipStream_t st) {
+  dim3 grid((Geo<N>::KC + TY - 1) / TY, 2, B);
+  if (plan_of(N, K_COL) == 1)
+    hipLaunchKernelGGL((adj_out_g_kernel<N, TY, NT, 8>), grid, dim3(NT), 0, st, a, then, pre, gnext);
+  else
+    hipLaunchKernelGGL((adj_out_g_kernel<N, TY, NT, 16>), grid, dim3(NT), 0, st, a, then, pre, gnext);
+  LAUNCHCHK();
+  return 0;
+}
+template <int N>
+static int acol_out_g(const Args& a, const CombAdj& then, const CombAdj& pre, const double2* gnext, int B,
+                      hipStream_t st) {
+  return B < 16 ? acol_out_g_t<N, 1>(a, then, pre, gnext, B, st)
+                : acol_out_g_t<N, Tiles<N>::TY>(a, then, pre, gnext, B, st);
+}
 template <int N, int TX>
 static int arow_t(const Args& a, int B, hipStream_t st) {
   dim3 grid(N / TX, 1, B);
@@ -1395,6 +1413,19 @@ static int vjp_pass(ecs_ctx* c, const Args& base, double2* x, const double2* g, 
   return 0;
 }
 
+// The arguments of one RHS-VJP pass at a point whose derived fields are prim
+// (out = J^T g, assigned), for the chained passes of a reverse step: the
+// first kernel of stage k's pass (B_G) runs in the last kernel of stage
+// k + 1's (adj_out_g_kernel), so a step of S stages takes 2 S + 1 launches.
+static Args pass_args(ecs_ctx* c, const Args& base, const double2* g, double2* out, int diss, const double2* prim) {
+  Args a = base;
+  a.gin = g; a.out = out; a.out_acc = 0;
+  a.include_diss = diss;
+  a.aout = c->AO; a.ain = c->AI; a.prim = prim;
+  a.adaptive = AD_FIXED;
+  return a;
+}
+
 // Step tape of one segment of the generic reverse passes (the Butcher-form
 // schemes, TDRK4 and the RK4(3) schedule): per step t the step input x(t),
 // the stage inputs y(t, 1..S-1), the stage values v(t, 0..S-1), the raw
@@ -1764,9 +1795,13 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
         fin.term(vb(j), vs(j), tb.b[j], 1, tb.p[S] - tb.p[j], true);
         touched[j] = 1;
       }
-    // fin is applied by the first RHS-VJP pass's first kernel (stage S - 1 reads vb(S - 1))
+    // the stages' RHS-VJP passes, chained (pass_args): fin runs in stage
+    // S - 1's first kernel, which reads vb(S - 1); stage k's combination and
+    // stage k - 1's first kernel in stage k's last
+    if (!touched[S - 1]) HIPCHK(hipMemsetAsync(vb(S - 1), 0, bytes, st));  // a stage no combination reads
+    TRY(acol_g_comb<N>(pass_args(c, a, vb(S - 1), yb, diss, ds(S - 1)), fin.cb, B, st));
+    const CombAdj none{};
     for (int k = S - 1; k >= 0; --k) {
-      if (!touched[k]) HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));  // a stage no combination reads
       CombAdjBuilder st_(c, ce, ediv, msk, yb, c->hb);
       st_.term(xbn, k == 0 ? nullptr : xs, 1.0, 0, tb.p[k]);
       for (int j = 0; j < k; ++j)
@@ -1774,8 +1809,14 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
           st_.term(vb(j), vs(j), tb.A[k][j], 1, tb.p[k] - tb.p[j], !touched[j]);
           touched[j] = 1;
         }
-      TRY(vjp_pass<N>(c, a, k == 0 ? xs : ys(k), vb(k), yb, B, diss, st, false, ds(k), false, &st_.cb,
-                      k == S - 1 ? &fin.cb : nullptr));
+      const Args ak = pass_args(c, a, vb(k), yb, diss, ds(k));
+      TRY(arow<N>(ak, B, st));
+      if (k == 0) {
+        TRY(acol_comb<N>(ak, st_.cb, B, st));
+      } else {
+        if (!touched[k - 1]) HIPCHK(hipMemsetAsync(vb(k - 1), 0, bytes, st));
+        TRY(acol_out_g<N>(ak, st_.cb, none, vb(k - 1), B, st));
+      }
     }
     std::swap(xb, xbn);
     return 0;
@@ -2066,11 +2107,28 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
         fin.term(vb(j), vs(j), tb.b[j], 1, tb.p[S] - tb.p[j], true);
         touched[j] = 1;
       }
-    // fin is applied by the first RHS-VJP pass's first kernel (stage S - 1 reads vb(S - 1))
+    // v1 = P vraw: vbraw = P vbar1, d/dT through T s (the combination that
+    // gives stage 0's input once vbar1 is complete)
+    CombAdjBuilder pc(c, ce, 1.0, 0, vb(0), c->hb);
+    pc.cb.h = PDT(m);
+    pc.cb.hscale = SIG(m);
+    pc.term(vbraw, vr, 1.0, 0, 1, true);
+    auto gin = [&](int k) { return k == 0 ? vbraw : vb(k); };
+    Args am = a;
+    am.dt = DT(m);
+    // the stages' RHS-VJP passes, chained (pass_args): fin runs in stage
+    // S - 1's first kernel; stage k's combination sa, pc when k = 1, and
+    // stage k - 1's first kernel in stage k's last
+    if (!touched[S - 1]) HIPCHK(hipMemsetAsync(vb(S - 1), 0, bytes, st));  // a stage no combination reads
+    const Args a0 = pass_args(c, am, gin(S - 1), yb, 0, ds(S - 1));
+    if (S == 1) {
+      TRY(comb_adj_launch<N>(fin.cb, B, st));   // a one-stage table: fin before pc
+      TRY(acol_g_comb<N>(a0, pc.cb, B, st));
+    } else {
+      TRY(acol_g_comb<N>(a0, fin.cb, B, st));
+    }
+    const CombAdj none{};
     for (int k = S - 1; k >= 0; --k) {
-      if (!touched[k]) HIPCHK(hipMemsetAsync(vb(k), 0, bytes, st));  // a stage no combination reads
-      Args am = a;
-      am.dt = DT(m);
       CombAdjBuilder sa(c, ce, ediv, 0, yb, c->hb);
       sa.cb.h = DT(m);
       sa.cb.hscale = HSC(m);
@@ -2080,19 +2138,13 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
           sa.term(vb(j), vs(j), tb.A[k][j], 1, tb.p[k] - tb.p[j], !touched[j]);
           touched[j] = 1;
         }
-      // the pass's last kernel applies sa
+      const Args ak = pass_args(c, am, gin(k), yb, 0, ds(k));
+      TRY(arow<N>(ak, B, st));
       if (k == 0) {
-        // v1 = P vraw: vbraw = P vbar1, d/dT through T s
-        CombAdjBuilder pc(c, ce, 1.0, 0, vb(0), c->hb);
-        pc.cb.h = PDT(m);
-        pc.cb.hscale = SIG(m);
-        pc.term(vbraw, vr, 1.0, 0, 1, true);
-        if (k == S - 1) TRY(comb_adj_launch<N>(fin.cb, B, st));  // a one-stage table: fin before pc
-        // (pc applied by the pass's first kernel, which reads vbraw)
-        TRY(vjp_pass<N>(c, am, xs, vbraw, yb, B, 0, st, false, ds(0), false, &sa.cb, &pc.cb));
+        TRY(acol_comb<N>(ak, sa.cb, B, st));
       } else {
-        TRY(vjp_pass<N>(c, am, ys(k), vb(k), yb, B, 0, st, false, ds(k), false, &sa.cb,
-                        k == S - 1 ? &fin.cb : nullptr));
+        if (!touched[k - 1]) HIPCHK(hipMemsetAsync(vb(k - 1), 0, bytes, st));
+        TRY(acol_out_g<N>(ak, sa.cb, k == 1 ? pc.cb : none, gin(k - 1), B, st));
       }
     }
     std::swap(xb, xbn);
