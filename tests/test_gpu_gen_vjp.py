@@ -357,3 +357,30 @@ def test_value_and_grad_leaves_the_callers_fields_alone():
     f = torch.as_tensor(np.fft.irfft2(spectrum(n, 9))).cuda()
     LF.loss_RPO_value_and_grad({"fields": f, "T": 0.2, "sx": 0.0}, pd)
     assert not f.requires_grad and f.grad is None
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_reverse_passes_at_batch_16_give_the_solo_bits(n):
+    """From batch 16 the adjoint column kernels (the fused combination + B_G,
+    B_OUT + combinations + next B_G) take multi-column tiles; each candidate
+    of a batch of 16 gets the bits of its solo reverse pass (RK4, TDRK4 and
+    RK4(3) along per-candidate schedules)."""
+    print(f"reverse passes batch 16 vs solo n={n}", flush=True)
+    pd, ctx = setup(n)
+    B = 16
+    f = torch.as_tensor(np.stack([(1 + 0.05 * s) * spectrum(n, 300 + s) for s in range(B)])).cuda()
+    c = torch.as_tensor(np.stack([spectrum(n, 400 + s) for s in range(B)])).cuda()
+    dts = [0.003 + 0.0002 * s for s in range(B)]
+    for scheme in ("rk4", "tdrk4"):
+        vjp = getattr(ctx, scheme + "_vjp")
+        gb, hb = vjp(f, c, dts, 20, ckpt=8)
+        for b in (0, 7, 15):
+            g1, h1 = vjp(f[b:b + 1], c[b:b + 1], [dts[b]], 20, ckpt=8)
+            assert torch.equal(gb[b:b + 1], g1) and hb[b] == h1[0], (scheme, b)
+    Ts = [0.2 + 0.02 * s for s in range(B)]
+    _, infos = ctx.lawson_rk43(f, Ts, 0.5, 1e-7, 64, 16)
+    gb, Tb = ctx.lawson_rk43_vjp(f, c, Ts, infos, ckpt=16)
+    for b in (0, 9, 15):
+        _, inf1 = ctx.lawson_rk43(f[b:b + 1], [Ts[b]], 0.5, 1e-7, 64, 16)
+        g1, T1 = ctx.lawson_rk43_vjp(f[b:b + 1], c[b:b + 1], [Ts[b]], inf1, ckpt=16)
+        assert torch.equal(gb[b:b + 1], g1) and Tb[b] == T1[0], ("rk43", b)
