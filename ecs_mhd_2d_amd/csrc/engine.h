@@ -798,15 +798,7 @@ static int adj_buffers(ecs_ctx* c, int B, int ckpt) {
   using G = Geo<N>;
   const size_t cs = (size_t)B * 2 * G::R * G::KC;
   const size_t is = (size_t)B * 8 * N * G::KP;
-  const size_t need = (size_t)ckpt * 5 * cs;
-  if (need > c->vt_cap) {
-    if (c->xtape) HIPCHK(hipFree(c->xtape));
-    c->xtape = nullptr;
-    size_t cap0 = 0;
-    TRY(grow(&c->xtape, &cap0, need));
-    c->vt_cap = need;
-  }
-  c->vtape = c->xtape + (size_t)ckpt * cs;
+  TRY(grow(&c->xtape, &c->vt_cap, (size_t)ckpt * 5 * cs));
   TRY(grow(&c->itape, &c->it_cap, (size_t)ckpt * 4 * is));
   return 0;
 }

@@ -109,10 +109,13 @@ struct ecs_ctx {
   // adjoint workspace
   double2* ckpt = nullptr;     // checkpoints [nck][B][2][R][KC]
   size_t ckpt_cap = 0;
-  double2* xtape = nullptr;    // [ckpt][B][2][R][KC]
-  double2* vtape = nullptr;    // [ckpt][4][B][2][R][KC]
-  size_t vt_cap = 0;           // elements of xtape + vtape
-  double2* itape = nullptr;    // primal column outputs [ckpt][4][B][8][N][KP]
+  // step tape of a reverse pass: compact arrays (Lawson RK4: the step inputs
+  // [ckpt][B][2][R][KC], then the stage values [ckpt][4][...]; the generic
+  // passes: StepTape's per-step arrays) and the primal column outputs
+  // (Lawson RK4: [ckpt][4][B][8][N][KP]; StepTape: [ckpt][S][...] + aux)
+  double2* xtape = nullptr;
+  size_t vt_cap = 0;           // elements of xtape
+  double2* itape = nullptr;
   size_t it_cap = 0;
   // a second step tape, so the reverse pass recomputes one segment on the
   // side stream while the caller's stream sweeps the next one backward
