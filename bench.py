@@ -655,6 +655,8 @@ def main():
         "rk4_steps_per_s_n256_b1": rk4[1], "rk4_steps_per_s_n256_b64": rk4[64],
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC_ROW_T_N128_B1,
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of bench.py --probe-roofline, "
+                                       "profiles/pmc_traffic_tangent_row_kernel_n128_b1.md",
                      "kernel": f"row_kernel<128,TX={tiles['n128_b1']['row_tan_tx']},tangent,replay> (batch 1)",
                      "avg_ms": row_t_ms,
                      "timing": "HIP events on the launching stream around 200 back-to-back launches of the "
@@ -666,6 +668,8 @@ def main():
                                      "launches timed the same way: what a launch costs before any work"},
         "roofline_batched": {"bound": "hbm", "achieved": ach_b, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": ach_b / HBM_PEAK_GBS, "traffic": TRAFFIC_ROW_N256_B64,
+                             "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of bench.py "
+                                               "--probe-roofline-batched, profiles/pmc_batched_row_kernel_n256_b64.md",
                              "kernel": f"row_kernel<256,TX={tiles['n256_b64']['row_tx']}> (batch 64)",
                              "avg_ms": row_b_ms, "algorithmic_bytes": 64 * 10 * zbytes(256)},
         "cpu_baseline": cpu,
