@@ -23,3 +23,9 @@ def test_argument_checks_return_errors_without_touching_a_gpu():
     assert lib.ecs_lawson_rk43(None, None, 1, None, 0.01, 1e-6, 4, 4, None, None, None, None) == 1
     assert lib.ecs_mgs(None, 1, 1, 0, None, None, None) == 1
     assert lib.ecs_ckpt_doubles(None, 1, 1, 1) == -1
+    # the two-phase reverse passes refuse a missing context, buffer or a scheme outside 0..2
+    assert lib.ecs_gen_fwd_ckpt(None, None, 1, None, 1, 0, 0, 1, None, None) == 1
+    assert lib.ecs_gen_vjp_ckpt(None, None, None, None, 1, None, 1, 3, 0, 1, None) == 1
+    assert lib.ecs_lawson_rk43_fwd_ckpt(None, None, 1, None, None, None, None, 1, 1, None, None) == 1
+    assert lib.ecs_lawson_rk43_vjp_ckpt(None, None, None, None, 1, None, None, None, None, 1, 1, None) == 1
+    assert b"ecs_lawson_rk43_vjp_ckpt" in lib.ecs_last_error()
