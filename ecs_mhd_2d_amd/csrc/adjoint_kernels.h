@@ -235,16 +235,17 @@ __global__ __launch_bounds__(NT) void adj_out_comb_kernel(Args a, CombAdj cb) {
 
 // The last kernel of one RHS-VJP pass and the first of the next in one
 // launch: B_OUT, the transposed stage combination `then` that reads its
-// output, the combination `pre` (when it has terms) that produces the next
-// pass's input, then that pass's B_G on gnext (one of their outputs; the
-// thread that writes an entry reads it) into AO.  Two launches fewer than
+// output and the combination `pre` that produces the next pass's input (each
+// when it has terms), then that pass's B_G on gnext (B_OUT's output, one of
+// the combinations' outputs, or an input already complete; the thread that
+// writes an entry reads it) into AO.  Up to two launches fewer than
 // adj_out_comb_kernel, comb_adj_kernel and adj_col_kernel<B_G>, the same
 // arithmetic.
 template <int N, int TY, int NT, int PL>
 __global__ __launch_bounds__(NT) void adj_out_g_kernel(Args a, CombAdj then, CombAdj pre, const double2* gnext) {
   adj_col_body<N, TY, NT, B_OUTG, PL>(a, [](int, size_t, int, int, int) {},
                                       [&](int b, size_t ci, int c, int kx, int ky, double2 v) {
-                                        comb_adj_elem(then, b, ci, c, kx, ky, v);
+                                        if (then.nterm) comb_adj_elem(then, b, ci, c, kx, ky, v);
                                         if (pre.nterm) comb_adj_elem(pre, b, ci, c, kx, ky, pre.yb[ci]);
                                         return gnext[ci];
                                       });

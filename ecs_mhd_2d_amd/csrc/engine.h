@@ -1406,15 +1406,18 @@ static int vjp_pass(ecs_ctx* c, const Args& base, double2* x, const double2* g, 
 }
 
 // The arguments of one RHS-VJP pass at a point whose derived fields are prim
-// (out = J^T g, assigned), for the chained passes of a reverse step: the
-// first kernel of stage k's pass (B_G) runs in the last kernel of stage
-// k + 1's (adj_out_g_kernel), so a step of S stages takes 2 S + 1 launches.
-static Args pass_args(ecs_ctx* c, const Args& base, const double2* g, double2* out, int diss, const double2* prim) {
+// (out = J^T g; acc: out +=; hess: the b0-free J_0, as vjp_pass), for the
+// chained passes of a reverse step: the first kernel of one pass (B_G) runs
+// in the last kernel of the pass before it (adj_out_g_kernel), so a step of
+// S stages takes 2 S + 1 launches.
+static Args pass_args(ecs_ctx* c, const Args& base, const double2* g, double2* out, int diss, const double2* prim,
+                      bool acc = false, bool hess = false) {
   Args a = base;
-  a.gin = g; a.out = out; a.out_acc = 0;
-  a.include_diss = diss;
+  a.gin = g; a.out = out; a.out_acc = acc ? 1 : 0;
+  a.include_diss = hess ? 0 : diss;
   a.aout = c->AO; a.ain = c->AI; a.prim = prim;
   a.adaptive = AD_FIXED;
+  if (hess) { a.b0x = 0.0; a.b0y = 0.0; }
   return a;
 }
 
@@ -1823,19 +1826,40 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
     CombAdjBuilder fin(c, ce, 1.0, 0, xb, c->hb);
     fin.term(xbn, nullptr, 1.0, 0, 0, true).term(vb(0), v1, 1.0, 1, 0, true)
       .term(wb(0), w1, 1.0 / 6, 2, 0, true).term(wb(1), w2, 1.0 / 3, 2, 0, true);
-    // w2 = J(y) v2, v2 = f(y): vbar2 = J(y)^T wbar2; ybar = J(y)^T vbar2 + J_0(v2)^T wbar2
-    // (fin applied by the first pass's first kernel, which reads wbar2)
-    TRY(vjp_pass<N>(c, ab, ym, wb(1), vb(1), B, diss, st, false, dm, false, nullptr, &fin.cb));
-    TRY(vjp_pass<N>(c, ab, ym, vb(1), yb, B, diss, st, false, dm));
-    // y = x + h/2 v1 + h^2/8 w1 (the transposed combination applied by the pass's last kernel)
+    // y = x + h/2 v1 + h^2/8 w1 (its transposed combination follows the third pass)
     CombAdjBuilder cy(c, ce, 1.0, 0, yb, c->hb);
     cy.term(xbn, nullptr, 1.0, 0, 0).term(vb(0), v1, 0.5, 1, 0).term(wb(0), w1, 0.125, 2, 0);
-    TRY(vjp_pass<N>(c, ab, v2, wb(1), yb, B, diss, st, true, nullptr, true, &cy.cb));
-    // w1 = J(x) v1: vbar1 += J(x)^T wbar1, xbar += J_0(v1)^T wbar1
-    TRY(vjp_pass<N>(c, ab, xs, wb(0), vb(0), B, diss, st, false, d1, true));
-    TRY(vjp_pass<N>(c, ab, v1, wb(0), xbn, B, diss, st, true, nullptr, true));
-    // v1 = f(x): xbar += J(x)^T vbar1
-    TRY(vjp_pass<N>(c, ab, xs, vb(0), xbn, B, diss, st, false, d1, true));
+    // The six RHS-VJP passes, chained (pass_args): each pass's input is
+    // complete when the pass before it ends, so its B_G runs in that pass's
+    // last kernel; the two passes at v1, v2 load their points (into the
+    // base's column output) before that kernel.  fin runs in the first B_G.
+    //   w2 = J(y) v2, v2 = f(y): vbar2 = J(y)^T wbar2; ybar = J(y)^T vbar2 + J_0(v2)^T wbar2
+    //   w1 = J(x) v1: vbar1 += J(x)^T wbar1, xbar += J_0(v1)^T wbar1
+    //   v1 = f(x): xbar += J(x)^T vbar1
+    const Args p1 = pass_args(c, ab, wb(1), vb(1), diss, dm);
+    const Args p2 = pass_args(c, ab, vb(1), yb, diss, dm);
+    Args p3 = pass_args(c, ab, wb(1), yb, diss, ab.inter, true, true);
+    const Args p4 = pass_args(c, ab, wb(0), vb(0), diss, d1, true);
+    Args p5 = pass_args(c, ab, wb(0), xbn, diss, ab.inter, true, true);
+    const Args p6 = pass_args(c, ab, vb(0), xbn, diss, d1, true);
+    p3.x0 = v2;
+    p5.x0 = v1;
+    const CombAdj none{};
+    TRY(acol_g_comb<N>(p1, fin.cb, B, st));
+    TRY(arow<N>(p1, B, st));
+    TRY(acol_out_g<N>(p1, none, none, vb(1), B, st));
+    TRY(arow<N>(p2, B, st));
+    TRY((col<N, C_LOAD, false>(p3, B, st)));
+    TRY(acol_out_g<N>(p2, none, none, wb(1), B, st));
+    TRY(arow<N>(p3, B, st));
+    TRY(acol_out_g<N>(p3, cy.cb, none, wb(0), B, st));
+    TRY(arow<N>(p4, B, st));
+    TRY((col<N, C_LOAD, false>(p5, B, st)));
+    TRY(acol_out_g<N>(p4, none, none, wb(0), B, st));
+    TRY(arow<N>(p5, B, st));
+    TRY(acol_out_g<N>(p5, none, none, vb(0), B, st));
+    TRY(arow<N>(p6, B, st));
+    TRY((acol<N, B_OUT>(p6, B, st)));
     std::swap(xb, xbn);
     return 0;
   };
