@@ -1370,46 +1370,19 @@ struct CombAdjBuilder {
   }
 };
 
-// out = J(x)^T g (with the RHS's dissipation when diss), on compact spectra;
-// hess: out = (J_0(x))^T g with J_0 the Jacobian of the nonlinearity without
-// b0 (the transpose of y -> H[y, x]).  prim: the column pass's output at x
-// (the 8 derived fields in x-space), when a forward pass left it; otherwise
-// it is recomputed.  acc: out += instead of out =.  then: a transposed stage
-// combination whose yb is `out`, applied by the pass's last kernel (the same
-// arithmetic as comb_adj_launch after the pass, one launch fewer).  before:
-// a transposed stage combination one of whose outputs is g, applied by the
-// pass's first kernel before it reads g (as comb_adj_launch before the pass).
-template <int N>
-static int vjp_pass(ecs_ctx* c, const Args& base, double2* x, const double2* g, double2* out, int B, int diss,
-                    hipStream_t st, bool hess = false, const double2* prim = nullptr, bool acc = false,
-                    const CombAdj* then = nullptr, const CombAdj* before = nullptr) {
-  Args a = base;
-  a.x0 = x; a.gin = g; a.out = out; a.out_acc = acc ? 1 : 0;
-  a.include_diss = hess ? 0 : diss;
-  a.aout = c->AO; a.ain = c->AI; a.prim = prim ? prim : a.inter;
-  a.adaptive = AD_FIXED;
-  if (hess) { a.b0x = 0.0; a.b0y = 0.0; }
-  // per-candidate step counts would end the B_G kernel early, not the combination
-  if (before && a.nstep) {
-    TRY(comb_adj_launch<N>(*before, B, st));
-    before = nullptr;
-  }
-  if (!prim) TRY((col<N, C_LOAD, false>(a, B, st)));
-  if (before) TRY(acol_g_comb<N>(a, *before, B, st));
-  else TRY((acol<N, B_G>(a, B, st)));
-  TRY(arow<N>(a, B, st));
-  // per-candidate step counts end a candidate's B_OUT early, not its combination
-  if (then && !a.nstep && then->yb == out) return acol_comb<N>(a, *then, B, st);
-  TRY((acol<N, B_OUT>(a, B, st)));
-  if (then) TRY(comb_adj_launch<N>(*then, B, st));
-  return 0;
-}
-
-// The arguments of one RHS-VJP pass at a point whose derived fields are prim
-// (out = J^T g; acc: out +=; hess: the b0-free J_0, as vjp_pass), for the
-// chained passes of a reverse step: the first kernel of one pass (B_G) runs
-// in the last kernel of the pass before it (adj_out_g_kernel), so a step of
-// S stages takes 2 S + 1 launches.
+// An RHS-VJP pass is out = J(x)^T g (with the RHS's dissipation when diss)
+// on compact spectra, four kernels: the column load of x (skipped when a
+// forward pass left its column output, the 8 derived fields in x-space, at
+// prim), the column adjoint B_G of g, the row adjoint, and the column adjoint
+// output B_OUT; hess: out = J_0(x)^T g with J_0 the Jacobian of the
+// nonlinearity without b0 (the transpose of y -> H[y, x]); acc: out +=
+// instead of out =.  pass_args gives one pass's arguments.  The reverse steps
+// chain their passes: the transposed stage combination that completes a
+// pass's input runs in its B_G (adj_comb_g_kernel), the combination that
+// reads its output in its B_OUT (adj_out_comb_kernel), and the B_G of the
+// next pass as well (adj_out_g_kernel), so a step of S stages takes 2 S + 1
+// launches.  (These passes take no per-candidate step counts, which would end
+// the adjoint column kernels early but not the combinations.)
 static Args pass_args(ecs_ctx* c, const Args& base, const double2* g, double2* out, int diss, const double2* prim,
                       bool acc = false, bool hess = false) {
   Args a = base;
