@@ -1754,20 +1754,34 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   // step's cotangent accumulators start with their first contribution
   // (assign), so they need no clearing.
   using Ptr = std::function<double2*(int)>;
-  auto rev_step = [&](double2* xs, const Ptr& ys, const Ptr& vs, const Ptr& ds) -> int {
-    std::vector<char> touched(S, 0);
-    CombAdjBuilder fin(c, ce, ediv, msk, xb, c->hb);
-    fin.term(xbn, xs, 1.0, 0, tb.p[S], true);
+  // The final combination of a step with input xs and stage values vs: the
+  // cotangent of its output (ybuf) to that of its input (obuf, assigned) and
+  // to vbar_j (assigned; touched[j] set for each).
+  auto make_fin = [&](const double2* ybuf, double2* obuf, double2* xs, const Ptr& vs, std::vector<char>& touched) {
+    CombAdjBuilder fin(c, ce, ediv, msk, ybuf, c->hb);
+    fin.term(obuf, xs, 1.0, 0, tb.p[S], true);
     for (int j = 0; j < S; ++j)
       if (tb.b[j] != 0.0) {
         fin.term(vb(j), vs(j), tb.b[j], 1, tb.p[S] - tb.p[j], true);
         touched[j] = 1;
       }
+    return fin;
+  };
+  // head_done: the step's final combination and first kernel already ran in
+  // the last kernel of the step transposed before it; xs_n, vs_n: the step
+  // transposed next, when its stage data are resident (the same tape), whose
+  // final combination and first kernel this step's last kernel runs.
+  auto rev_step = [&](double2* xs, const Ptr& ys, const Ptr& vs, const Ptr& ds, bool head_done = false,
+                      double2* xs_n = nullptr, const Ptr* vs_n = nullptr) -> int {
+    std::vector<char> touched(S, 0);
+    CombAdjBuilder fin = make_fin(xb, xbn, xs, vs, touched);
     // the stages' RHS-VJP passes, chained (pass_args): fin runs in stage
     // S - 1's first kernel, which reads vb(S - 1); stage k's combination and
     // stage k - 1's first kernel in stage k's last
-    if (!touched[S - 1]) HIPCHK(hipMemsetAsync(vb(S - 1), 0, bytes, st));  // a stage no combination reads
-    TRY(acol_g_comb<N>(pass_args(c, a, vb(S - 1), yb, diss, ds(S - 1)), fin.cb, B, st));
+    if (!head_done) {
+      if (!touched[S - 1]) HIPCHK(hipMemsetAsync(vb(S - 1), 0, bytes, st));  // a stage no combination reads
+      TRY(acol_g_comb<N>(pass_args(c, a, vb(S - 1), yb, diss, ds(S - 1)), fin.cb, B, st));
+    }
     const CombAdj none{};
     for (int k = S - 1; k >= 0; --k) {
       CombAdjBuilder st_(c, ce, ediv, msk, yb, c->hb);
@@ -1779,11 +1793,18 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
         }
       const Args ak = pass_args(c, a, vb(k), yb, diss, ds(k));
       TRY(arow<N>(ak, B, st));
-      if (k == 0) {
-        TRY(acol_comb<N>(ak, st_.cb, B, st));
-      } else {
+      if (k > 0) {
         if (!touched[k - 1]) HIPCHK(hipMemsetAsync(vb(k - 1), 0, bytes, st));
         TRY(acol_out_g<N>(ak, st_.cb, none, vb(k - 1), B, st));
+      } else if (xs_n && S > 1) {
+        // the next step's final combination reads this step's input cotangent
+        // (xbn, complete once st_ has run) and writes the other buffer
+        std::vector<char> tn(S, 0);
+        CombAdjBuilder fn = make_fin(xbn, xb, xs_n, *vs_n, tn);
+        if (!tn[S - 1]) HIPCHK(hipMemsetAsync(vb(S - 1), 0, bytes, st));
+        TRY(acol_out_g<N>(ak, st_.cb, fn.cb, vb(S - 1), B, st));
+      } else {
+        TRY(acol_comb<N>(ak, st_.cb, B, st));
       }
     }
     std::swap(xb, xbn);
@@ -1793,12 +1814,20 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
   // v2, w1 = J(xs) v1, w2 = J(ym) v2 and the derived fields d1, dm of xs and
   // ym; ab: the base arguments of the RHS-VJP passes (their column output
   // serves the passes at v1 and v2, which load their point themselves).
-  auto rev_td = [&](const Args& ab, double2* xs, double2* ym, double2* v1, double2* v2, double2* w1, double2* w2,
-                    double2* d1, double2* dm) -> int {
-    // x' = x + h v1 + h^2/6 w1 + h^2/3 w2 (vbar2 is written whole by its VJP pass below)
-    CombAdjBuilder fin(c, ce, 1.0, 0, xb, c->hb);
-    fin.term(xbn, nullptr, 1.0, 0, 0, true).term(vb(0), v1, 1.0, 1, 0, true)
+  // x' = x + h v1 + h^2/6 w1 + h^2/3 w2: the cotangent of x' (ybuf) to that
+  // of x (obuf) and of v1, w1, w2 (all assigned; vbar2 is written whole by
+  // its VJP pass)
+  auto make_fin_td = [&](const double2* ybuf, double2* obuf, double2* v1, double2* w1, double2* w2) {
+    CombAdjBuilder fin(c, ce, 1.0, 0, ybuf, c->hb);
+    fin.term(obuf, nullptr, 1.0, 0, 0, true).term(vb(0), v1, 1.0, 1, 0, true)
       .term(wb(0), w1, 1.0 / 6, 2, 0, true).term(wb(1), w2, 1.0 / 3, 2, 0, true);
+    return fin;
+  };
+  // head_done, next (v1, w1, w2 of the step transposed next, on the same
+  // tape): as for rev_step
+  auto rev_td = [&](const Args& ab, double2* xs, double2* ym, double2* v1, double2* v2, double2* w1, double2* w2,
+                    double2* d1, double2* dm, bool head_done = false, double2* const* next = nullptr) -> int {
+    CombAdjBuilder fin = make_fin_td(xb, xbn, v1, w1, w2);
     // y = x + h/2 v1 + h^2/8 w1 (its transposed combination follows the third pass)
     CombAdjBuilder cy(c, ce, 1.0, 0, yb, c->hb);
     cy.term(xbn, nullptr, 1.0, 0, 0).term(vb(0), v1, 0.5, 1, 0).term(wb(0), w1, 0.125, 2, 0);
@@ -1818,7 +1847,7 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
     p3.x0 = v2;
     p5.x0 = v1;
     const CombAdj none{};
-    TRY(acol_g_comb<N>(p1, fin.cb, B, st));
+    if (!head_done) TRY(acol_g_comb<N>(p1, fin.cb, B, st));
     TRY(arow<N>(p1, B, st));
     TRY(acol_out_g<N>(p1, none, none, vb(1), B, st));
     TRY(arow<N>(p2, B, st));
@@ -1832,7 +1861,14 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
     TRY(arow<N>(p5, B, st));
     TRY(acol_out_g<N>(p5, none, none, vb(0), B, st));
     TRY(arow<N>(p6, B, st));
-    TRY((acol<N, B_OUT>(p6, B, st)));
+    if (next) {
+      // the next step's final combination (reads xbn, complete after this
+      // pass) and its first pass's B_G, which reads wbar2
+      CombAdjBuilder fn = make_fin_td(xbn, xb, next[0], next[1], next[2]);
+      TRY(acol_out_g<N>(p6, none, fn.cb, wb(1), B, st));
+    } else {
+      TRY((acol<N, B_OUT>(p6, B, st)));
+    }
     std::swap(xb, xbn);
     return 0;
   };
@@ -1868,12 +1904,18 @@ static int run_gen_vjp(ecs_ctx* c, const double* f0, double* fbar, double* hbar,
       Args ab = a;
       ab.inter = tp[0].aux();
       for (int t = L - 1; t >= 0; --t) {
-        if (td)
+        if (td) {
+          double2* const nx[3] = {t > 0 ? tap.v(t - 1, 0) : nullptr, t > 0 ? tap.w(t - 1, 0) : nullptr,
+                                  t > 0 ? tap.w(t - 1, 1) : nullptr};
           TRY(rev_td(ab, tap.x(t), tap.y(t, 1), tap.v(t, 0), tap.v(t, 1), tap.w(t, 0), tap.w(t, 1), tap.d(t, 0),
-                     tap.d(t, 1)));
-        else
+                     tap.d(t, 1), t < L - 1, t > 0 ? nx : nullptr));
+        } else {
+          // consecutive steps of one tape chain across the step boundary
+          const Ptr vn = [&](int k) { return tap.v(t - 1, k); };
           TRY(rev_step(tap.x(t), [&](int k) { return tap.y(t, k); }, [&](int k) { return tap.v(t, k); },
-                       [&](int k) { return tap.d(t, k); }));
+                       [&](int k) { return tap.d(t, k); }, t < L - 1 && S > 1, t > 0 ? tap.x(t - 1) : nullptr,
+                       t > 0 ? &vn : nullptr));
+        }
       }
       if (two) HIPCHK(hipEventRecord(c->ev_rev[3 + p], st));
     }
@@ -2085,17 +2127,25 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
   // fields ds(k).  The step's cotangent accumulators start with their first
   // contribution.
   using Ptr = std::function<double2*(int)>;
-  auto rev_step = [&](int m, double2* xs, const Ptr& ys, const Ptr& vs, double2* vr, const Ptr& ds) -> int {
-    std::vector<char> touched(S, 0);
-    CombAdjBuilder fin(c, ce, ediv, 0, xb, c->hb);
+  // the final combination of accepted step m (as make_fin of run_gen_vjp)
+  auto make_fin = [&](int m, const double2* ybuf, double2* obuf, double2* xs, const Ptr& vs,
+                      std::vector<char>& touched) {
+    CombAdjBuilder fin(c, ce, ediv, 0, ybuf, c->hb);
     fin.cb.h = DT(m);
     fin.cb.hscale = HSC(m);
-    fin.term(xbn, xs, 1.0, 0, tb.p[S], true);
+    fin.term(obuf, xs, 1.0, 0, tb.p[S], true);
     for (int j = 0; j < S; ++j)
       if (tb.b[j] != 0.0) {
         fin.term(vb(j), vs(j), tb.b[j], 1, tb.p[S] - tb.p[j], true);
         touched[j] = 1;
       }
+    return fin;
+  };
+  // head_done, xs_n, vs_n (step m - 1, on the same tape): as in run_gen_vjp
+  auto rev_step = [&](int m, double2* xs, const Ptr& ys, const Ptr& vs, double2* vr, const Ptr& ds,
+                      bool head_done = false, double2* xs_n = nullptr, const Ptr* vs_n = nullptr) -> int {
+    std::vector<char> touched(S, 0);
+    CombAdjBuilder fin = make_fin(m, xb, xbn, xs, vs, touched);
     // v1 = P vraw: vbraw = P vbar1, d/dT through T s (the combination that
     // gives stage 0's input once vbar1 is complete)
     CombAdjBuilder pc(c, ce, 1.0, 0, vb(0), c->hb);
@@ -2108,13 +2158,15 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
     // the stages' RHS-VJP passes, chained (pass_args): fin runs in stage
     // S - 1's first kernel; stage k's combination sa, pc when k = 1, and
     // stage k - 1's first kernel in stage k's last
-    if (!touched[S - 1]) HIPCHK(hipMemsetAsync(vb(S - 1), 0, bytes, st));  // a stage no combination reads
-    const Args a0 = pass_args(c, am, gin(S - 1), yb, 0, ds(S - 1));
-    if (S == 1) {
-      TRY(comb_adj_launch<N>(fin.cb, B, st));   // a one-stage table: fin before pc
-      TRY(acol_g_comb<N>(a0, pc.cb, B, st));
-    } else {
-      TRY(acol_g_comb<N>(a0, fin.cb, B, st));
+    if (!head_done) {
+      if (!touched[S - 1]) HIPCHK(hipMemsetAsync(vb(S - 1), 0, bytes, st));  // a stage no combination reads
+      const Args a0 = pass_args(c, am, gin(S - 1), yb, 0, ds(S - 1));
+      if (S == 1) {
+        TRY(comb_adj_launch<N>(fin.cb, B, st));   // a one-stage table: fin before pc
+        TRY(acol_g_comb<N>(a0, pc.cb, B, st));
+      } else {
+        TRY(acol_g_comb<N>(a0, fin.cb, B, st));
+      }
     }
     const CombAdj none{};
     for (int k = S - 1; k >= 0; --k) {
@@ -2129,11 +2181,18 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
         }
       const Args ak = pass_args(c, am, gin(k), yb, 0, ds(k));
       TRY(arow<N>(ak, B, st));
-      if (k == 0) {
-        TRY(acol_comb<N>(ak, sa.cb, B, st));
-      } else {
+      if (k > 0) {
         if (!touched[k - 1]) HIPCHK(hipMemsetAsync(vb(k - 1), 0, bytes, st));
         TRY(acol_out_g<N>(ak, sa.cb, k == 1 ? pc.cb : none, gin(k - 1), B, st));
+      } else if (xs_n && S > 1) {
+        // step m - 1's final combination (reads xbn, complete once sa has run)
+        // and its first kernel
+        std::vector<char> tn(S, 0);
+        CombAdjBuilder fn = make_fin(m - 1, xbn, xb, xs_n, *vs_n, tn);
+        if (!tn[S - 1]) HIPCHK(hipMemsetAsync(vb(S - 1), 0, bytes, st));
+        TRY(acol_out_g<N>(ak, sa.cb, fn.cb, vb(S - 1), B, st));
+      } else {
+        TRY(acol_comb<N>(ak, sa.cb, B, st));
       }
     }
     std::swap(xb, xbn);
@@ -2166,9 +2225,13 @@ static int run_rk43_vjp(ecs_ctx* c, const double* x0f, double* xbar, double* Tba
         HIPCHK(hipEventRecord(c->ev_rev[1 + p], fs));
         HIPCHK(hipStreamWaitEvent(st, c->ev_rev[1 + p], 0));
       }
-      for (int t = L - 1; t >= 0; --t)
+      for (int t = L - 1; t >= 0; --t) {
+        // consecutive steps of one tape chain across the step boundary
+        const Ptr vn = [&](int k) { return tap.v(t - 1, k); };
         TRY(rev_step(t0 + t, tap.x(t), [&](int k) { return tap.y(t, k); }, [&](int k) { return tap.v(t, k); },
-                     tap.vraw(t), [&](int k) { return tap.d(t, k); }));
+                     tap.vraw(t), [&](int k) { return tap.d(t, k); }, t < L - 1 && S > 1,
+                     t > 0 ? tap.x(t - 1) : nullptr, t > 0 ? &vn : nullptr));
+      }
       if (two) HIPCHK(hipEventRecord(c->ev_rev[3 + p], st));
     }
   }
