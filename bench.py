@@ -281,6 +281,46 @@ def newton_default_leg(inp, pd):
             "reference_published_h100_gmres_s": 16.1}
 
 
+def hookstep_leg(inp, pd):
+    """One iteration of the hookstep Newton solver (newton_hookstep_v2.py:95-121:
+    mode Lawson_RK4, inner 64, trust region s = 100) on the headline guess, with
+    32 checkpoints of 32 Lawson-RK4 steps (newton_hookstep_v2.py:49-58): 64
+    JVPs with reorthogonalised Arnoldi, the SVD trust-region solve and its tries."""
+    import torch
+    from ecs_mhd_2d_amd import solvers
+    pdh = dict(pd)
+    pdh.update({"ministeps": 32, "num_checkpoints": 32})
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    new, s_new, log = solvers.newton_hookstep_update(inp, pdh, "Lawson_RK4", 100.0, inner=64)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {"workload": "one hookstep Newton iteration, mode Lawson_RK4, inner 64, s = 100, 1024 steps, "
+                        "same guess as the headline", "wall_s": wall, "fwall_s": log["fwall"],
+            "gmreswall_s": log["gmreswall"], "f_norm": log["f_norm"], "rel_res": log["rel_res"],
+            "s": s_new, "rel_err": log["rel_err"]}
+
+
+def floquet_leg(inp, pd, iters=2):
+    """Block power iteration for Floquet multipliers (floquet.py:57-99: mode TDRK4,
+    32 tangents per block) on the headline guess with 32 checkpoints of 32 TDRK4
+    steps: each iteration one batched JVP of the 32 tangents and a QR of the block."""
+    import torch
+    from ecs_mhd_2d_amd import floquet
+    pdf = dict(pd)
+    pdf.update({"ministeps": 32, "num_checkpoints": 32})
+    floquet.block_power_iteration(inp, pdf, block_size=32, maxit=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    floquet.block_power_iteration(inp, pdf, block_size=32, maxit=iters)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # the call also integrates the orbit once and applies the block once more for R
+    return {"workload": f"Floquet block power iteration, 32 tangents, TDRK4, 1024 steps, {iters} iterations "
+                        "(+ the orbit and the final R = Q^T J Q), same guess as the headline",
+            "wall_s": wall, "block_jvps": iters + 1, "tangent_steps_per_s": 32 * 1024 * (iters + 1) / wall}
+
+
 def guess_generation_leg():
     """The reference's published guess-generation workload (generate_turbulence.py:22-151,
     timings in benchmarking.m:11-59): from 10 N(0,1) random fields at N=128
@@ -631,6 +671,8 @@ def main():
         extras["transpose_matvec"] = transpose_leg(inp, pd, [tans[i] for i in range(3)])
         extras["newton_iteration"] = newton_leg(inp, pd)
         extras["newton_default_setting"] = newton_default_leg(inp, pd)
+        extras["hookstep_iteration"] = hookstep_leg(inp, pd)
+        extras["floquet"] = floquet_leg(inp, pd)
         extras["other_steppers"] = steppers_leg()
         extras["adjoint_gradient"] = gradient_leg()
         extras["guess_generation"] = guess_generation_leg()
