@@ -172,16 +172,17 @@ def test_objective_vjp_dot_test_in_rk43_mode():
     assert abs(lhs - rhs) < 1e-10 * abs(lhs)
 
 
-@pytest.mark.parametrize("mode", ["Lawson_RK43", "TDRK4"])
+@pytest.mark.parametrize("mode", ["Lawson_RK43", "TDRK4", "RK4", "Lawson_RK6", "Lawson_RK4"])
 def test_linearization_vjp_reuses_the_primal(mode):
     """Linearization(x).vjp (the vjp function of one primal, applied to several
-    cotangents as newton.py:168-169 does) computes the same bits as separate
+    cotangents as newton.py:168-169 does; it keeps the reverse pass's
+    checkpoints after its first call) computes the same bits as separate
     objective_RPO_vjp calls, each of which solves the primal again."""
     print(f"mode {mode}", flush=True)
     from ecs_mhd_2d_amd import loss_functions as LF, mhd, utils
     n = 16
     pd = mhd.make_param_dict(n, 1 / 40, 1 / 40)
-    pd.update({"shift_reflect_ny": 2, "rot": True, "num_checkpoints": 2, "ministeps": 5,
+    pd.update({"shift_reflect_ny": 2, "rot": True, "num_checkpoints": 2, "ministeps": 5, "steps": 10,
                "adaptive_dict": {"atol": 1e-7, "checkpoints": 8, "max_steps_per_checkpoint": 32}})
     rng = np.random.default_rng(9)
     inp = {"fields": torch.as_tensor(np.fft.irfft2(spectrum(n, 7))).cuda(), "T": 0.3, "sx": 0.1}
