@@ -660,9 +660,11 @@ def main():
         lib.ecs_tuned_splits(nn, bb, splits)
         variants = (ctypes.c_int * 4)()
         lib.ecs_tuned_variants(nn, bb, variants)
+        rvariants = (ctypes.c_int * 2)()
+        lib.ecs_tuned_replay_variants(nn, bb, rvariants)
         tiles[f"n{nn}_b{bb}"] = {"col_ty": arr[0], "col_tan_ty": arr[1], "row_tx": arr[2], "row_tan_tx": arr[3],
                                  "col_split": splits[0], "col_tan_split": splits[1],
-                                 "variants": list(variants),
+                                 "variants": list(variants), "replay_variants": list(rvariants),
                                  "fft_plan_col": plans[0], "fft_plan_col_tan": plans[1], "fft_plan_row": plans[2],
                                  "fft_plan_row_tan": plans[3]}
 
@@ -699,7 +701,7 @@ def main():
                      "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC_ROW_T_N128_B1,
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of bench.py --probe-roofline, "
                                        "profiles/pmc_traffic_tangent_row_kernel_n128_b1.md",
-                     "kernel": f"row_kernel<128,TX={tiles['n128_b1']['row_tan_tx']},tangent,replay> (batch 1)",
+                     "kernel": f"row_kernel<128,TX={1 << (tiles['n128_b1']['replay_variants'][1] % 3)},tangent,replay> (batch 1)",
                      "avg_ms": row_t_ms,
                      "timing": "HIP events on the launching stream around 200 back-to-back launches of the "
                                "replay instance, median of three rounds",

@@ -77,6 +77,7 @@ _SIGS = {
     "ecs_tuned_plans": (c_int, [c_int, P_int]),
     "ecs_tuned_splits": (c_int, [c_int, c_int, P_int]),
     "ecs_tuned_variants": (c_int, [c_int, c_int, P_int]),
+    "ecs_tuned_replay_variants": (c_int, [c_int, c_int, P_int]),
     "ecs_mgs": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "ecs_mgs_reorth": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p]),
     "ecs_dot": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),

@@ -128,12 +128,13 @@ int tuned(int n, int kind, int B, int dflt) {
   auto it = g_tune.find(tune_key(n, kind, B));
   if (it != g_tune.end()) return it->second;
   // an untuned bucket keeps the default tile within the grid size's fixed plan
-  auto ip = g_plan.find(plan_key(n, kind));
-  const int plan = ip == g_plan.end() ? plan_table(n, kind) : ip->second;
+  auto ip = g_plan.find(plan_key(n, base_kind(kind)));
+  const int plan = ip == g_plan.end() ? plan_table(n, base_kind(kind)) : ip->second;
   return 3 * plan + dflt % 3;
 }
 
 int plan_of(int n, int kind) {
+  kind = base_kind(kind);
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto ip = g_plan.find(plan_key(n, kind));
   return ip == g_plan.end() ? plan_table(n, kind) : ip->second;
@@ -743,6 +744,16 @@ extern "C" int ecs_tuned_variants(int n, int batch, int* out) {
   for (int kind = 0; kind < 4; ++kind) {
     auto it = g_tune.find(tune_key(n, kind, batch));
     out[kind] = it == g_tune.end() ? -1 : it->second;
+  }
+  return 0;
+}
+
+extern "C" int ecs_tuned_replay_variants(int n, int batch, int* out) {
+  if (!out || batch < 1) return fail(1, "ecs_tuned_replay_variants: bad arguments");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (int k = 0; k < 2; ++k) {
+    auto it = g_tune.find(tune_key(n, k == 0 ? K_COLR : K_ROWR, batch));
+    out[k] = it == g_tune.end() ? -1 : it->second;
   }
   return 0;
 }

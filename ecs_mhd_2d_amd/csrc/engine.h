@@ -121,7 +121,7 @@ constexpr bool row_kind_ok() {
 }
 template <int N, int MODE, bool TAN>
 static int col(const Args& a, int B, hipStream_t st) {
-  const int v = tuned(N, TAN ? K_COLT : K_COL, B, DefTile<N>::COL);
+  const int v = tuned(N, !TAN ? K_COL : a.adaptive == AD_TANGENT ? K_COLR : K_COLT, B, DefTile<N>::COL);
   switch (a.adaptive) {
     case AD_FIXED:
       if constexpr (col_kind_ok<MODE, TAN, AD_FIXED>()) return col_var<N, MODE, TAN, AD_FIXED>(v, a, B, st);
@@ -140,7 +140,7 @@ static int col(const Args& a, int B, hipStream_t st) {
 }
 template <int N, int RMODE, bool TAN>
 static int row(const Args& a, int B, hipStream_t st) {
-  const int v = tuned(N, TAN ? K_ROWT : K_ROW, B, DefTile<N>::ROW);
+  const int v = tuned(N, !TAN ? K_ROW : a.adaptive == AD_TANGENT ? K_ROWR : K_ROWT, B, DefTile<N>::ROW);
   switch (a.adaptive) {
     case AD_FIXED:
       if constexpr (row_kind_ok<RMODE, TAN, AD_FIXED>()) return row_var<N, RMODE, TAN, AD_FIXED>(v, a, B, st);
@@ -183,9 +183,17 @@ template <int N>
 static int time_variants(ecs_ctx* c, int B, int kind, const int* vs, int nv, float* t, hipStream_t st) {
   const size_t ce = c->compact_elems() * B * sizeof(double2);
   std::vector<double> dt(B, 1e-3);
-  TRY(upload_scalars(c, 0, dt.data(), B, st));
-  TRY(upload_scalars(c, 1, dt.data(), B, st));
+  for (int slot = 0; slot < 4; ++slot) TRY(upload_scalars(c, slot, dt.data(), B, st));
   Args a = base_args(c);
+  // the replay instances: every candidate active, attempt 0, its step size
+  // read from the schedule (here the B step sizes of scalar slot 0)
+  Args ar = a;
+  ar.adaptive = AD_TANGENT;
+  ar.run_all = 1;
+  ar.next_all = 1;
+  ar.attempt = 0;
+  ar.s_h = c->scal;
+  ar.sstride = 1;
   HIPCHK(hipMemsetAsync(c->x0, 0, ce, st));
   HIPCHK(hipMemsetAsync(c->dx0, 0, ce, st));
   HIPCHK(hipMemsetAsync(c->acc, 0, ce, st));
@@ -209,7 +217,9 @@ static int time_variants(ecs_ctx* c, int B, int kind, const int* vs, int nv, flo
           case K_COL: return col_var<N, C_S2, false, AD_FIXED>(v, a, B, st);
           case K_COLT: return col_var<N, C_S2, true, AD_FIXED>(v, a, B, st);
           case K_ROW: return row_var<N, R_RHS, false, AD_FIXED>(v, a, B, st);
-          default: return row_var<N, R_RHS, true, AD_FIXED>(v, a, B, st);
+          case K_ROWT: return row_var<N, R_RHS, true, AD_FIXED>(v, a, B, st);
+          case K_COLR: return col_var<N, C_S2, true, AD_TANGENT>(v, ar, B, st);
+          default: return row_var<N, R_RHS, true, AD_TANGENT>(v, ar, B, st);
         }
       };
       rc = launch();
@@ -231,7 +241,8 @@ static int time_variants(ecs_ctx* c, int B, int kind, const int* vs, int nv, flo
 
 // Fixes, per kernel family (column, tangent column, row, tangent row), the
 // FFT radix plan of a grid size and picks the tile per batch bucket (powers
-// of two).  The plan decides the arithmetic, so it comes from a fixed table
+// of two), for the tangent families once in the fixed-step instance and once
+// in the tangent replay's.  The plan decides the arithmetic, so it comes from a fixed table
 // (plan_table: the plan that timed fastest on MI355X at batch 1 and 16), and
 // every process, batch size and run computes the same bits; a candidate's
 // result does not depend on the batch it rides in or on timing noise.  Tiles
@@ -243,8 +254,9 @@ static int time_variants(ecs_ctx* c, int B, int kind, const int* vs, int nv, flo
 // chooses it by timing (both plans at batch 1 and 16, the plan with the
 // smaller worst-case slowdown wins), which is how the table was measured;
 // ECSMHD_VARIANTS="N:c,ct,r,rt[;N:...]" fixes the variants (var_* in host.h)
-// of the four families for grid N at every batch, so a profiled run can use
-// the configuration an unprofiled run tuned.
+// of the four families for grid N at every batch (the replay instances take
+// the tangent families' ct and rt), so a profiled run can use the
+// configuration an unprofiled run tuned.
 static bool forced_variants(int n, int v[4]) {
   const char* s = getenv("ECSMHD_VARIANTS");
   while (s && *s) {
@@ -273,6 +285,8 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
       g_plan[plan_key(N, kind)] = var_plan(fv[kind]);
       g_tune[tune_key(N, kind, B)] = fv[kind];
     }
+    g_tune[tune_key(N, K_COLR, B)] = fv[K_COLT];
+    g_tune[tune_key(N, K_ROWR, B)] = fv[K_ROWT];
     return 0;
   }
   static std::mutex run_mu;  // one tuning pass per grid size at a time
@@ -283,7 +297,7 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
   int plan[4] = {-1, -1, -1, -1};
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    if (g_tune.count(tune_key(N, K_ROWT, B))) return 0;
+    if (g_tune.count(tune_key(N, K_ROWR, B))) return 0;
     for (int kind = 0; kind < 4; ++kind) {
       auto it = g_plan.find(plan_key(N, kind));
       if (it != g_plan.end()) plan[kind] = it->second;
@@ -329,13 +343,13 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
         if (!g_tune.count(tune_key(N, kind, PB[p]))) g_tune[tune_key(N, kind, PB[p])] = v;
       }
     }
-    if (g_tune.count(tune_key(N, K_ROWT, B))) return 0;
+    if (g_tune.count(tune_key(N, K_ROWR, B))) return 0;
   }
-  int pick[4];
-  for (int kind = 0; kind < 4; ++kind) {
+  int pick[NKIND];
+  for (int kind = 0; kind < NKIND; ++kind) {
     int vs[VMAX];
     float t[VMAX];
-    const int nv = var_list(kind, N, plan[kind], vs);
+    const int nv = var_list(kind, N, plan[base_kind(kind)], vs);
     TRY(time_variants<N>(c, B, kind, vs, nv, t, st));
     int best = 0;
     for (int i = 1; i < nv; ++i)
@@ -343,7 +357,7 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
     pick[kind] = vs[best];
   }
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  for (int kind = 0; kind < 4; ++kind) g_tune[tune_key(N, kind, B)] = pick[kind];
+  for (int kind = 0; kind < NKIND; ++kind) g_tune[tune_key(N, kind, B)] = pick[kind];
   return 0;
 }
 
@@ -1049,7 +1063,7 @@ static int run_time_kernel(ecs_ctx* c, int B, int which, int reps, double* ms, h
       case 3: return row<N, R_RHS, true>(a, B, st);
       case 4: return row<N, R_RHS, true>(ar, B, st);
       default: {
-        const int tx = var_tile(tuned(N, K_ROWT, B, DefTile<N>::ROW));
+        const int tx = var_tile(tuned(N, K_ROWR, B, DefTile<N>::ROW));
         hipLaunchKernelGGL(empty_kernel<N>, dim3(N / tx, 1, B), dim3(NT), 0, st, ar);
         LAUNCHCHK();
         return 0;

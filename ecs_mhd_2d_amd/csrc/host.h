@@ -192,10 +192,18 @@ int upload_ints(ecs_ctx* c, const int* vals, int B, hipStream_t st);
 // plan once per (N, kernel family), so every batch size computes with the
 // same arithmetic, and the variant within that plan per (N, kernel family,
 // batch bucket).
-enum TuneKind : int { K_COL = 0, K_COLT = 1, K_ROW = 2, K_ROWT = 3 };
+// K_COLR and K_ROWR are the tangent column and row kernels as the tangent
+// replay of an adaptive schedule runs them (stepper kind AD_TANGENT): the
+// same kernels and FFT plan as K_COLT and K_ROWT, with a variant table of
+// their own, timed in that instance (at N = 128 and 256, batch 1 the replay
+// instance and the fixed-step one rank the variants differently).
+enum TuneKind : int { K_COL = 0, K_COLT = 1, K_ROW = 2, K_ROWT = 3, K_COLR = 4, K_ROWR = 5 };
+constexpr int NKIND = 6;
+inline int base_kind(int kind) { return kind == K_COLR ? K_COLT : kind == K_ROWR ? K_ROWT : kind; }
 constexpr int VMAX = 17;  // variant indices of one family lie in [0, VMAX)
 inline bool var_ok(int kind, int n, int v) {
   if (v < 0 || v >= VMAX) return false;
+  kind = base_kind(kind);
   if (kind == K_ROW || kind == K_ROWT) return v < 6;
   if (v == 16) return kind == K_COL && n >= 128;
   return v < 10 || v >= 12 || n >= 128;

@@ -289,6 +289,14 @@ int ecs_tuned_splits(int n, int batch, int* out);
  * indices are accepted by ECSMHD_VARIANTS="N:c,ct,r,rt". */
 int ecs_tuned_variants(int n, int batch, int* out);
 
+/* Raw variant indices (as ecs_tuned_variants) the autotuner chose for the
+ * tangent column (out[0]) and tangent row kernel (out[1]) as the tangent
+ * replay of an adaptive schedule runs them (ecs_lawson_rk43_jvp, the GMRES
+ * matvec; no reference counterpart): tuned in that instance, apart from the
+ * fixed-step tangent kernels; -1 where not tuned yet.  ECSMHD_VARIANTS sets
+ * them to its ct and rt. */
+int ecs_tuned_replay_variants(int n, int batch, int* out);
+
 #ifdef __cplusplus
 }
 #endif
