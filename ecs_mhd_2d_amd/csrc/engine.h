@@ -179,8 +179,13 @@ static int unpack(const double2* comp, double* full, int B, int zero_outside, hi
 // Average time (ms) of variants vs[0..nv) of one kernel family at batch B,
 // timed with HIP events on `st` over back-to-back launches on zeroed
 // workspace (the workspace must hold B candidates).
+// The replay kinds are timed in pairs as the replay runs them, the column
+// kernel then the row kernel, with `partner` the variant of the other one:
+// launched alone, one after another, variants can rank differently (at
+// N = 256, batch 1 the replay's tangent column variants 3 and 14).
 template <int N>
-static int time_variants(ecs_ctx* c, int B, int kind, const int* vs, int nv, float* t, hipStream_t st) {
+static int time_variants(ecs_ctx* c, int B, int kind, const int* vs, int nv, float* t, hipStream_t st,
+                         int partner = 0) {
   const size_t ce = c->compact_elems() * B * sizeof(double2);
   std::vector<double> dt(B, 1e-3);
   for (int slot = 0; slot < 4; ++slot) TRY(upload_scalars(c, slot, dt.data(), B, st));
@@ -218,8 +223,12 @@ static int time_variants(ecs_ctx* c, int B, int kind, const int* vs, int nv, flo
           case K_COLT: return col_var<N, C_S2, true, AD_FIXED>(v, a, B, st);
           case K_ROW: return row_var<N, R_RHS, false, AD_FIXED>(v, a, B, st);
           case K_ROWT: return row_var<N, R_RHS, true, AD_FIXED>(v, a, B, st);
-          case K_COLR: return col_var<N, C_S2, true, AD_TANGENT>(v, ar, B, st);
-          default: return row_var<N, R_RHS, true, AD_TANGENT>(v, ar, B, st);
+          case K_COLR:
+            TRY((col_var<N, C_S2, true, AD_TANGENT>(v, ar, B, st)));
+            return row_var<N, R_RHS, true, AD_TANGENT>(partner, ar, B, st);
+          default:
+            TRY((col_var<N, C_S2, true, AD_TANGENT>(partner, ar, B, st)));
+            return row_var<N, R_RHS, true, AD_TANGENT>(v, ar, B, st);
         }
       };
       rc = launch();
@@ -350,7 +359,10 @@ static int autotune(ecs_ctx* c, int B, hipStream_t st) {
     int vs[VMAX];
     float t[VMAX];
     const int nv = var_list(kind, N, plan[base_kind(kind)], vs);
-    TRY(time_variants<N>(c, B, kind, vs, nv, t, st));
+    // the replay column kernel paired with the fixed-step tangent row pick,
+    // the replay row kernel with the replay column pick
+    const int partner = kind == K_COLR ? pick[K_ROWT] : kind == K_ROWR ? pick[K_COLR] : 0;
+    TRY(time_variants<N>(c, B, kind, vs, nv, t, st, partner));
     int best = 0;
     for (int i = 1; i < nv; ++i)
       if (t[i] < t[best]) best = i;
